@@ -1,0 +1,206 @@
+/*
+ * multiscat_amd.h -- C ABI of libmultiscat_amd.so, the MI355X (gfx950) close-coupling
+ * solver. Plain C types only: ints, doubles and pointers. Complex arrays are interleaved
+ * (re, im) doubles. No call aborts the process; every entry point returns a status.
+ *
+ * Two groups of entry points:
+ *
+ *  (1) The drop-in group. It replaces the f2py module `multiscat_fortran` one for one:
+ *      same functions, same argument meaning, same array layouts (the Fortran side's
+ *      column-major 2-D/3-D arrays and C-ravel-ordered 1-D channel waves), same ierr
+ *      codes 0/1/2. Each declaration cites the reference routine it replaces (paths
+ *      relative to the reference repository root). All arrays are host memory.
+ *
+ *  (2) The batched group (no counterpart in the reference): a potential table held in
+ *      HBM and a batch of incidence conditions that share it, solved together on the
+ *      GPU with either the reference's Gauss-Seidel split or the all-channel
+ *      block-Jacobi (eigenbasis) preconditioner, in FP64 or in mixed precision (FP32
+ *      inner solves with FP64 residual refinement), plus the adjoint solve that gives
+ *      d(loss)/dV_G(z).
+ */
+#ifndef MULTISCAT_AMD_H
+#define MULTISCAT_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. 0..2 keep the reference's ierr meanings
+ * (lib/multiscat_fortran/fortran/multiscat_python_bindings.f90:42-45,58-60). Conditions the
+ * reference ends with `error stop` (process abort) get codes of their own here. */
+enum {
+    MSX_OK = 0,
+    MSX_ERR_DIMS = 1,          /* a dimension is <= 0                                        */
+    MSX_ERR_NOT_CONVERGED = 2, /* GMRES did not reach the tolerance within maxiter           */
+    MSX_ERR_INVALID = 3,       /* null pointer, bad option, inconsistent sizes               */
+    MSX_ERR_NO_SPECULAR = 4,   /* specular channel (0,0) not in the channel list             */
+    MSX_ERR_EIGEN = 5,         /* eigensolver did not converge ("precon 2" in the reference) */
+    MSX_ERR_DEVICE = 6,        /* HIP runtime error or no GPU                                */
+    MSX_ERR_OUT_OF_MEMORY = 7  /* device allocation failed                                   */
+};
+
+/* Human-readable text of the last error on the calling thread ("" if none). */
+const char *msx_last_error(void);
+
+/* Library/device identity: writes the gfx target name (e.g. "gfx950") into buf. */
+int msx_device_info(char *buf, int buflen, int *device_count);
+
+/* ---------------------------------------------------------------------------------------
+ * (1) Drop-in group: mirrors lib/multiscat_fortran/src/multiscat_fortran/__init__.py:39-169
+ *     and the f2py interface lib/multiscat_fortran/fortran/multiscat_python_bindings.pyf:3-97.
+ * ------------------------------------------------------------------------------------- */
+
+/* Replaces run_multiscat_fortran (multiscat_python_bindings.f90:1-61 -> multiscat.f90:20-62
+ * -> multiscat_gmres.f90:105-618). Solves the close-coupled system for the scattered state
+ * with GMRES preconditioned by the block Gauss-Seidel split (I + L^-1 U), left-preconditioned
+ * by (I - L^-1 U) when gmres_preconditioner_flag == 1, tolerance 0.5*10^-nsig on the
+ * preconditioned residual, 3-iteration window, at most 2000 iterations, x0 = 0, source
+ * wave_b[0] at the last node of the specular channel. The arithmetic runs on the GPU in FP64.
+ *   potential_values        complex[nkx*nky*nz]  column-major (nkx, nky, nz)
+ *   perpendicular_kinetic_difference double[nkx*nky] column-major (nkx, nky)
+ *   wave_a, wave_b, wave_c  complex[nkx*nky]      channel idx = ix*nky + iy
+ *   parallel_kinetic_energy double[nz*nz]         column-major (nz, nz)
+ *   scattered_state_dense   complex[nkx*nky*nz]   column-major (nkx, nky, nz), output
+ * Returns the ierr: 0, 1 (bad dims), 2 (not converged; the state is still written). */
+int msx_run_multiscat_fortran(int gmres_preconditioner_flag, int convergence_significant_figures,
+                              int nkx, int nky, int nz, const double *potential_values,
+                              const double *perpendicular_kinetic_difference,
+                              const double *wave_a, const double *wave_b, const double *wave_c,
+                              const double *parallel_kinetic_energy,
+                              double *scattered_state_dense);
+
+/* As msx_run_multiscat_fortran, also returning the iteration count and the final relative
+ * (preconditioned) residual estimate |g_{k+1}|/|g_1| (the reference computes both but does
+ * not return them, multiscat_gmres.f90:590,608). */
+int msx_run_multiscat_fortran_ex(int gmres_preconditioner_flag,
+                                 int convergence_significant_figures, int nkx, int nky, int nz,
+                                 const double *potential_values,
+                                 const double *perpendicular_kinetic_difference,
+                                 const double *wave_a, const double *wave_b,
+                                 const double *wave_c, const double *parallel_kinetic_energy,
+                                 double *scattered_state_dense, int *iterations,
+                                 double *residual);
+
+/* Replaces get_perpendicular_kinetic_difference (multiscat_python_bindings.f90:63-124 ->
+ * scatsub.f90:54-86): e(ix,iy) = |K + G|^2 - |k|^2 over FFT-ordered G; out column-major. */
+int msx_get_perpendicular_kinetic_difference(double incident_kx, double incident_ky,
+                                             double incident_kz, int nx, int ny,
+                                             double unit_cell_ax, double unit_cell_ay,
+                                             double unit_cell_bx, double unit_cell_by,
+                                             double *perpendicular_kinetic_difference);
+
+/* Replaces get_parallel_kinetic_energy (multiscat_python_bindings.f90:126-154 ->
+ * scatsub.f90:134-226): the Lobatto kinetic matrix of an (nz+1)-node rule on [zmin, zmax]
+ * with its first node dropped, nz x nz column-major. */
+int msx_get_parallel_kinetic_energy(double zmin, double zmax, int nz,
+                                    double *parallel_kinetic_energy);
+
+/* Replaces get_lobatto_weights (multiscat_python_bindings.f90:156-187 -> scatsub.f90:88-100):
+ * node_count-point Gauss-Lobatto nodes x on [zmin, zmax] and sqrt of the weights w. */
+int msx_get_lobatto_weights(double zmin, double zmax, int node_count, double *w, double *x);
+
+/* Replaces get_abc_arrays (multiscat_python_bindings.f90:189-231 -> scatsub.f90:102-132,
+ * 307-335): a, b, c boundary waves of an (n_z_points+1)-node rule; channel idx = ix*ny+iy. */
+int msx_get_abc_arrays(double zmin, double zmax, int nx, int ny,
+                       const double *perpendicular_kinetic_difference, int n_z_points,
+                       double *wave_a, double *wave_b, double *wave_c);
+
+/* Replaces debug_build_preconditioner_fortran (multiscat_python_bindings.f90:233-271 ->
+ * multiscat_gmres.f90:226-317): ascending eigenvalues of H0 = T + diag(Re V_00), the factors
+ * f_j = (H0 + e_j)^-1 e_N as (nz, nkx*nky) column-major, and the eigenvectors (nz, nz)
+ * column-major (eigenvector signs are arbitrary, as with any eigensolver). Runs on the GPU. */
+int msx_debug_build_preconditioner(int nkx, int nky, int nz, const double *potential_values,
+                                   const double *perpendicular_kinetic_difference,
+                                   const double *parallel_kinetic_energy, double *eigenvalues,
+                                   double *preconditioner_factors, double *eigenvectors);
+
+/* Replaces debug_apply_upper_block_fortran (multiscat_python_bindings.f90:273-302 ->
+ * multiscat_gmres.f90:338-382,620-639): state_out_j = sum_{i>j} V[G_j-G_i] state_in_i over
+ * the full grid in storage order; states are (nz, nkx*nky) column-major. Runs on the GPU. */
+int msx_debug_apply_upper_block(int nkx, int nky, int nz, const double *potential_values,
+                                const double *state_in, double *state_out);
+
+/* Replaces debug_solve_lower_block_fortran (multiscat_python_bindings.f90:304-342 ->
+ * multiscat_gmres.f90:384-450,641-687): state_out = L^-1 state_in. Runs on the GPU. */
+int msx_debug_solve_lower_block(int nkx, int nky, int nz, const double *potential_values,
+                                const double *wave_c,
+                                const double *perpendicular_kinetic_difference,
+                                const double *parallel_kinetic_energy, const double *state_in,
+                                double *state_out);
+
+/* ---------------------------------------------------------------------------------------
+ * (2) Batched group (extension; no counterpart in the reference).
+ * ------------------------------------------------------------------------------------- */
+
+typedef struct msx_table msx_table; /* a potential table resident in HBM */
+typedef struct msx_batch msx_batch; /* incidence conditions sharing one table */
+
+enum { MSX_METHOD_GAUSS_SEIDEL = 0, MSX_METHOD_JACOBI = 1 };
+enum { MSX_PRECISION_FP64 = 0, MSX_PRECISION_MIXED = 1 };
+
+typedef struct {
+    int method;       /* MSX_METHOD_*                                                       */
+    int precision;    /* MSX_PRECISION_*                                                    */
+    int neumann;      /* 1: left preconditioner (I - M^-1 N), the reference's flag          */
+    double eps;       /* stop when |g_{k+1}| < eps |g_1| for `window` consecutive iterations */
+    int maxiter;      /* iteration budget per condition (reference: 2000)                   */
+    int restart;      /* Krylov dimension before a restart; 0 = maxiter (unrestarted)       */
+    int window;       /* reference: 3                                                       */
+    double inner_eps; /* mixed precision: relative tolerance of each FP32 inner solve       */
+    int max_refine;   /* mixed precision: FP64 refinement sweeps allowed                    */
+} msx_solve_options;
+
+/* Fills `opt` with the reference's settings for a given nsig (GS, FP64, window 3, 2000). */
+void msx_default_options(msx_solve_options *opt, int convergence_significant_figures);
+
+/* Uploads a potential table V (complex, C order (nkx, nky, nz), natural units) and the
+ * kinetic matrix T (nz x nz, symmetric) and computes the eigensystem of
+ * H0 = T + diag(Re V_00) on the GPU in FP64. */
+int msx_table_create(int nkx, int nky, int nz, const double *potential, const double *kinetic,
+                     msx_table **out);
+/* Recomputes H0 and its eigensystem from the table already in HBM (the per-potential set-up
+ * of multiscat_gmres.f90:226-273, done once for every batch that shares the table). */
+int msx_table_prepare(msx_table *table);
+int msx_table_destroy(msx_table *table);
+/* Copies the eigensystem to the host: lam[nz] ascending, Q row-major (Q[z*nz + m]). */
+int msx_table_eigensystem(const msx_table *table, double *lam, double *Q);
+
+/* Builds a batch of n_cond conditions on `table`. Channel lists are concatenated:
+ * condition b owns channels [off_b, off_b + n_channels[b]) with off_b the prefix sum.
+ *   chx, chy  storage indices (0 <= chx < nkx, 0 <= chy < nky) in solve order
+ *   e         channel energies e_G (natural units)
+ *   wave_b, wave_c  complex per channel;  wave_a0 complex per condition (specular a_0)
+ * The specular channel of each condition is its first (0,0) entry; the source is b at
+ * that channel's last node (multiscat.f90:49). */
+int msx_batch_create(msx_table *table, int n_cond, const int *n_channels, const int *chx,
+                     const int *chy, const double *e, const double *wave_b,
+                     const double *wave_c, const double *wave_a0, msx_batch **out);
+int msx_batch_destroy(msx_batch *batch);
+
+/* Solves every condition; the state stays in HBM. Returns MSX_ERR_NOT_CONVERGED if any
+ * condition missed the tolerance (see msx_batch_stats). */
+int msx_batch_solve(msx_batch *batch, const msx_solve_options *opt);
+
+/* Results. psi: complex, concatenated (n_channels[b] x nz) rows; P: per channel
+ * |2i b psi(z_N) + a0 delta_spec|^2 (computed on the GPU in FP64);
+ * iterations/info per condition; residual: final relative preconditioned residual. */
+int msx_batch_state(const msx_batch *batch, double *psi);
+int msx_batch_intensities(const msx_batch *batch, double *P);
+int msx_batch_stats(const msx_batch *batch, int *iterations, int *info, double *residual);
+
+/* Device pointers for callers that keep data on the GPU (e.g. a torch autograd bridge):
+ * the state psi (complex FP64, same layout as msx_batch_state) and the intensities. */
+int msx_batch_device_state(const msx_batch *batch, void **psi_device);
+int msx_batch_total_channels(const msx_batch *batch, long long *total);
+
+/* Adjoint of the intensities: given dL/dP per channel (host, same layout as
+ * msx_batch_intensities), solves A^H lambda = dL/dpsi on the GPU in FP64 and accumulates
+ * dL/dV into grad (host, complex C order (nkx, nky, nz)), summed over the batch. V enters
+ * through the off-diagonal coupling and through Re V_00 in the diagonal blocks. */
+int msx_batch_adjoint(msx_batch *batch, const msx_solve_options *opt, const double *dL_dP,
+                      double *grad_potential);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MULTISCAT_AMD_H */

@@ -1,0 +1,196 @@
+"""Batched GPU solves: a potential table in HBM and many incidence conditions sharing it.
+
+This is the extension of the drop-in boundary (no counterpart in the reference): one call
+solves every condition of a batch with the reference's Gauss-Seidel-split GMRES or with the
+all-channel block-Jacobi (eigenbasis) preconditioner, in FP64 or mixed precision, and the
+adjoint solve gives d(loss)/dV_G(z). Channel subsets follow the reference's SciPy
+``n_channels`` semantics (multiscat/multiscat/_scipy.py:414-443).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+
+@dataclass
+class SolveOptions:
+    """Python face of msx_solve_options. Defaults are the reference's (nsig 5)."""
+
+    method: str = "gauss_seidel"  # or "jacobi"
+    precision: str = "fp64"  # or "mixed"
+    neumann: bool = True
+    eps: float = 0.5e-5
+    maxiter: int = 2000
+    restart: int = 0
+    window: int = 3
+    inner_eps: float = 1e-5
+    max_refine: int = 6
+
+    @classmethod
+    def from_nsig(cls, nsig: int, **kw) -> "SolveOptions":
+        return cls(eps=0.5 * 10.0 ** (-int(nsig)), **kw)
+
+    def native(self) -> _lib.SolveOptions:
+        o = _lib.SolveOptions()
+        methods = {"gauss_seidel": _lib.METHOD_GAUSS_SEIDEL, "jacobi": _lib.METHOD_JACOBI}
+        precisions = {"fp64": _lib.PRECISION_FP64, "mixed": _lib.PRECISION_MIXED}
+        if self.method not in methods:
+            raise ValueError(f"unknown method {self.method!r}")
+        if self.precision not in precisions:
+            raise ValueError(f"unknown precision {self.precision!r}")
+        o.method = methods[self.method]
+        o.precision = precisions[self.precision]
+        o.neumann = 1 if self.neumann else 0
+        o.eps = float(self.eps)
+        o.maxiter = int(self.maxiter)
+        o.restart = int(self.restart)
+        o.window = int(self.window)
+        o.inner_eps = float(self.inner_eps)
+        o.max_refine = int(self.max_refine)
+        return o
+
+
+class Table:
+    """A potential table V (nkx, nky, nz) and kinetic matrix T resident on the GPU."""
+
+    def __init__(self, potential, kinetic):
+        V = np.ascontiguousarray(potential, dtype=np.complex128)
+        T = np.ascontiguousarray(kinetic, dtype=np.float64)
+        if V.ndim != 3 or T.shape != (V.shape[2], V.shape[2]):
+            raise ValueError("potential must be (nkx, nky, nz) and kinetic (nz, nz)")
+        self.shape = V.shape
+        self._h = C.c_void_p()
+        _lib.check(_lib.lib().msx_table_create(V.shape[0], V.shape[1], V.shape[2], _lib.ptr(V),
+                                               _lib.ptr(T), C.byref(self._h)),
+                   "msx_table_create")
+
+    def prepare(self) -> None:
+        _lib.check(_lib.lib().msx_table_prepare(self._h), "msx_table_prepare")
+
+    def eigensystem(self):
+        nz = self.shape[2]
+        lam = np.zeros(nz)
+        Q = np.zeros((nz, nz))
+        _lib.check(_lib.lib().msx_table_eigensystem(self._h, _lib.ptr(lam), _lib.ptr(Q)),
+                   "msx_table_eigensystem")
+        return lam, Q
+
+    def close(self) -> None:
+        if self._h:
+            _lib.lib().msx_table_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Conditions on one table. ``conditions``: iterable of dicts or objects with
+    ``channels`` ((chx, chy) int arrays, solve order), ``e`` (nkx, nky), ``wave_b``,
+    ``wave_c`` (nkx*nky, C-ravel) and ``wave_a`` (its [0] is a_0)."""
+
+    def __init__(self, table: Table, conditions):
+        self.table = table
+        nkx, nky, nz = table.shape
+        chx, chy, e, b, c, a0, counts = [], [], [], [], [], [], []
+        for cond in conditions:
+            get = (lambda k: cond[k]) if isinstance(cond, dict) else (lambda k: getattr(cond, k))
+            cx_, cy_ = (np.asarray(v, dtype=np.int32) for v in get("channels"))
+            ev = np.asarray(get("e"), dtype=np.float64).reshape(nkx, nky)
+            bv = np.asarray(get("wave_b"), dtype=np.complex128).reshape(nkx, nky)
+            cv = np.asarray(get("wave_c"), dtype=np.complex128).reshape(nkx, nky)
+            av = np.asarray(get("wave_a"), dtype=np.complex128).ravel()
+            chx.append(cx_)
+            chy.append(cy_)
+            e.append(ev[cx_, cy_])
+            b.append(bv[cx_, cy_])
+            c.append(cv[cx_, cy_])
+            a0.append(av[0])
+            counts.append(cx_.size)
+        self.counts = np.asarray(counts, dtype=np.int32)
+        self.offsets = np.concatenate(([0], np.cumsum(self.counts)[:-1])).astype(np.int64)
+        self.chx = np.ascontiguousarray(np.concatenate(chx), dtype=np.int32)
+        self.chy = np.ascontiguousarray(np.concatenate(chy), dtype=np.int32)
+        self._e = np.ascontiguousarray(np.concatenate(e))
+        self._b = np.ascontiguousarray(np.concatenate(b))
+        self._c = np.ascontiguousarray(np.concatenate(c))
+        self._a0 = np.ascontiguousarray(np.asarray(a0, dtype=np.complex128))
+        self.n = len(counts)
+        self.nz = nz
+        self._h = C.c_void_p()
+        _lib.check(_lib.lib().msx_batch_create(
+            table._h, self.n, _lib.ptr(self.counts), _lib.ptr(self.chx), _lib.ptr(self.chy),
+            _lib.ptr(self._e), _lib.ptr(self._b), _lib.ptr(self._c), _lib.ptr(self._a0),
+            C.byref(self._h)), "msx_batch_create")
+
+    @property
+    def total_channels(self) -> int:
+        return int(self.counts.sum())
+
+    def solve(self, options: SolveOptions | None = None, *, check: bool = True) -> int:
+        o = (options or SolveOptions()).native()
+        rc = _lib.lib().msx_batch_solve(self._h, C.byref(o))
+        if rc not in (_lib.MSX_OK, _lib.MSX_ERR_NOT_CONVERGED) or (check and rc != _lib.MSX_OK):
+            raise _lib.MsxError(rc, f"msx_batch_solve: {_lib.last_error()}")
+        return rc
+
+    def state(self):
+        """psi per condition: list of (n_channels, nz) complex arrays."""
+        flat = np.zeros((self.total_channels, self.nz), dtype=np.complex128)
+        _lib.check(_lib.lib().msx_batch_state(self._h, _lib.ptr(flat)), "msx_batch_state")
+        return [flat[o:o + n] for o, n in zip(self.offsets, self.counts)]
+
+    def intensities(self):
+        """P per condition: list of (n_channels,) arrays, channel order as given."""
+        P = np.zeros(self.total_channels)
+        _lib.check(_lib.lib().msx_batch_intensities(self._h, _lib.ptr(P)), "msx_batch_intensities")
+        return [P[o:o + n] for o, n in zip(self.offsets, self.counts)]
+
+    def intensities_flat(self):
+        P = np.zeros(self.total_channels)
+        _lib.check(_lib.lib().msx_batch_intensities(self._h, _lib.ptr(P)), "msx_batch_intensities")
+        return P
+
+    def stats(self):
+        it = np.zeros(self.n, dtype=np.int32)
+        info = np.zeros(self.n, dtype=np.int32)
+        res = np.zeros(self.n)
+        _lib.check(_lib.lib().msx_batch_stats(self._h, _lib.ptr(it), _lib.ptr(info),
+                                              _lib.ptr(res)), "msx_batch_stats")
+        return {"iterations": it, "info": info, "residual": res}
+
+    def device_state_ptr(self) -> int:
+        p = C.c_void_p()
+        _lib.check(_lib.lib().msx_batch_device_state(self._h, C.byref(p)),
+                   "msx_batch_device_state")
+        return int(p.value or 0)
+
+    def adjoint(self, dL_dP, options: SolveOptions | None = None):
+        """dL/dV (nkx, nky, nz) complex (d/dRe V + i d/dIm V) for a loss of the intensities."""
+        g = np.ascontiguousarray(np.asarray(dL_dP, dtype=np.float64).ravel())
+        if g.size != self.total_channels:
+            raise ValueError("dL_dP must have one entry per channel of the batch")
+        grad = np.zeros(self.table.shape, dtype=np.complex128)
+        o = (options or SolveOptions()).native()
+        _lib.check(_lib.lib().msx_batch_adjoint(self._h, C.byref(o), _lib.ptr(g), _lib.ptr(grad)),
+                   "msx_batch_adjoint")
+        return grad
+
+    def close(self) -> None:
+        if self._h:
+            _lib.lib().msx_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,769 @@
+// Operator kernels of the close-coupling solve on gfx950 (64-wide wavefronts).
+//
+//   coupling        y_j(z) = sum_i V[G_j - G_i](z) x_i(z) over i>j / i!=j / i<j
+//                   (lib/multiscat_fortran/fortran/multiscat_gmres.f90:620-639, 654-660)
+//   lower solve     x <- L^-1 x: forward substitution over channels with the eigenbasis
+//                   solve and the Sherman-Morrison boundary fix (multiscat_gmres.f90:641-687)
+//   dinv / diag     D^-1 x and D x for all channels at once (multiscat/multiscat/_scipy.py:249-354)
+//   eigensystem     H0 = Q diag(lam) Q^T, ascending (multiscat_gmres.f90:247-253; diagsub.f90)
+//   factors         f_j = (H0 + e_j)^-1 e_N (multiscat_gmres.f90:255-271)
+//   intensities     P_j = |2i b_j psi_j(N) + a_0 delta_spec|^2 (multiscat/multiscat/_multiscat.py:40-60)
+//   gradient        dL/dV_G(z) from the forward and adjoint states
+//
+// Work decomposition used throughout: the 64 lanes of a wavefront run over the node index z
+// (contiguous in memory, so every load is coalesced) and everything that is uniform across
+// the wavefront -- channel indices, G differences, table offsets -- is scalar work.
+#include "msx_internal.h"
+
+namespace msx {
+
+__device__ __forceinline__ int wrap_mod(int d, int n) { return d < 0 ? d + n : d; }
+
+// ---------------------------------------------------------------------------------------
+// Coupling sum. Block = 256 threads = 4 waves; each wave owns RJ consecutive channels j and
+// 64 nodes z; loops over the source channels i. V[(dx*nky+dy)*nz + z] is a coalesced row.
+// ---------------------------------------------------------------------------------------
+template <typename T, int RJ>
+__global__ __launch_bounds__(256) void coupling_kernel(Layout L, const cx<T> *__restrict__ V,
+                                                       const cx<T> *__restrict__ x,
+                                                       cx<T> *__restrict__ y, int mode,
+                                                       const int *__restrict__ active) {
+    const int b = blockIdx.z;
+    if (active && !active[b]) return;
+    const int nc = L.nc[b];
+    const long long off = L.off[b];
+    const int nz = L.nz, nkx = L.nkx, nky = L.nky;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j0 = (blockIdx.y * 4 + wave) * RJ;
+    if (j0 >= nc) return;
+    const int z = blockIdx.x * 64 + lane;
+    const int zl = z < nz ? z : nz - 1;  // clamp loads; stores are guarded
+    const int *chx = L.chx + off, *chy = L.chy + off;
+    const cx<T> *xb = x + off * nz;
+
+    int gxj[RJ], gyj[RJ];
+    cx<T> acc[RJ];
+#pragma unroll
+    for (int r = 0; r < RJ; ++r) {
+        const int j = j0 + r < nc ? j0 + r : nc - 1;
+        gxj[r] = chx[j];
+        gyj[r] = chy[j];
+        acc[r] = mk<T>(0, 0);
+    }
+    const int jlast = (j0 + RJ < nc ? j0 + RJ : nc) - 1;
+    int ibeg = 0, iend = nc;
+    if (mode == COUPLE_UPPER) ibeg = j0 + 1;
+    if (mode == COUPLE_LOWER) iend = jlast;  // i < j <= jlast
+    for (int i = ibeg; i < iend; ++i) {
+        const cx<T> xi = xb[(long long)i * nz + zl];
+        const int gxi = chx[i], gyi = chy[i];
+#pragma unroll
+        for (int r = 0; r < RJ; ++r) {
+            const int j = j0 + r;
+            bool use = j < nc;
+            if (mode == COUPLE_UPPER) use = use && (i > j);
+            else if (mode == COUPLE_LOWER) use = use && (i < j);
+            else use = use && (i != j);
+            if (use) {
+                const int dx = wrap_mod(gxj[r] - gxi, nkx);
+                const int dy = wrap_mod(gyj[r] - gyi, nky);
+                const cx<T> v = V[((long long)dx * nky + dy) * nz + zl];
+                cfma(acc[r], v, xi);
+            }
+        }
+    }
+    if (z < nz) {
+#pragma unroll
+        for (int r = 0; r < RJ; ++r) {
+            const int j = j0 + r;
+            if (j < nc) y[(off + j) * nz + z] = acc[r];
+        }
+    }
+}
+
+template <typename T>
+void launch_coupling(const Layout &L, const cx<T> *V, const cx<T> *x, cx<T> *y, int mode,
+                     const int *active, hipStream_t s) {
+    constexpr int RJ = 4;
+    dim3 grid((L.nz + 63) / 64, (L.max_nc + 4 * RJ - 1) / (4 * RJ), L.n_inst);
+    hipLaunchKernelGGL((coupling_kernel<T, RJ>), grid, dim3(256), 0, s, L, V, x, y, mode, active);
+    MSX_LAUNCH_CHECK();
+}
+template void launch_coupling<double>(const Layout &, const cx<double> *, const cx<double> *,
+                                      cx<double> *, int, const int *, hipStream_t);
+template void launch_coupling<float>(const Layout &, const cx<float> *, const cx<float> *,
+                                     cx<float> *, int, const int *, hipStream_t);
+
+// ---------------------------------------------------------------------------------------
+// Block forward substitution x <- L^-1 x, one workgroup (8 waves) per instance, sequential
+// over channels. Per channel j:
+//   (1) r(z) = x_j(z) - sum_{i<j} V[G_j-G_i](z) psi_i(z)   waves split i, lanes run over z
+//   (2) t_m = sum_z r(z) Q[z][m] / (e_j + lam_m)           waves split z, lanes run over m
+//   (3) g(z) = sum_m QT[m][z] t_m                          waves split m, lanes run over z
+//   (4) psi_j = g + f_j g(N) c_j / (1 - f_j(N) c_j)
+// Partial sums of the waves meet in LDS. psi_i (i < j) are re-read from global memory: they
+// were written by this workgroup, and __syncthreads orders those writes before the reads.
+// ---------------------------------------------------------------------------------------
+constexpr int LS_WAVES = 8;
+constexpr int LS_MAXMZ = 10;  // nz <= 640
+
+__global__ __launch_bounds__(512) void lower_solve_kernel(
+    Layout L, const cx<double> *__restrict__ V, const double *__restrict__ Q,
+    const double *__restrict__ QT, const double *__restrict__ lam, const double *__restrict__ e,
+    const cx<double> *__restrict__ c, const double *__restrict__ fac, cx<double> *x,
+    const cx<double> *base, cx<double> *out, double sign, const int *__restrict__ active) {
+    const int b = blockIdx.x;
+    if (active && !active[b]) return;
+    extern __shared__ double smem[];
+    const int nz = L.nz, nkx = L.nkx, nky = L.nky;
+    cx<double> *red = reinterpret_cast<cx<double> *>(smem);  // [LS_WAVES][nz]
+    cx<double> *rs = red + LS_WAVES * nz;                    // [nz]
+    cx<double> *ts = rs + nz;                                // [nz]
+    __shared__ cx<double> s_fac;
+
+    const int nc = L.nc[b];
+    const long long off = L.off[b];
+    const int *chx = L.chx + off, *chy = L.chy + off;
+    cx<double> *xb = x + off * nz;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int mz = (nz + 63) >> 6;
+    const int zchunk = (nz + LS_WAVES - 1) / LS_WAVES;
+
+    for (int j = 0; j < nc; ++j) {
+        // (1) coupling to the already solved channels
+        cx<double> acc[LS_MAXMZ];
+#pragma unroll
+        for (int m = 0; m < LS_MAXMZ; ++m) acc[m] = mk(0.0, 0.0);
+        const int gxj = chx[j], gyj = chy[j];
+        for (int i = wave; i < j; i += LS_WAVES) {
+            const int dx = wrap_mod(gxj - chx[i], nkx);
+            const int dy = wrap_mod(gyj - chy[i], nky);
+            const cx<double> *vrow = V + ((long long)dx * nky + dy) * nz;
+            const cx<double> *xi = xb + (long long)i * nz;
+#pragma unroll
+            for (int m = 0; m < LS_MAXMZ; ++m) {
+                const int z = lane + 64 * m;
+                if (m < mz && z < nz) cfma(acc[m], vrow[z], xi[z]);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < LS_MAXMZ; ++m) {
+            const int z = lane + 64 * m;
+            if (m < mz && z < nz) red[wave * nz + z] = acc[m];
+        }
+        __syncthreads();
+        for (int z = threadIdx.x; z < nz; z += blockDim.x) {
+            cx<double> r = xb[(long long)j * nz + z];
+#pragma unroll
+            for (int w = 0; w < LS_WAVES; ++w) r = r - red[w * nz + z];
+            rs[z] = r;
+        }
+        __syncthreads();
+        // (2) transform to the eigenbasis of H0 and divide by (e_j + lam)
+        {
+            const int zb = wave * zchunk, ze = min(nz, zb + zchunk);
+#pragma unroll
+            for (int m = 0; m < LS_MAXMZ; ++m) acc[m] = mk(0.0, 0.0);
+            for (int z = zb; z < ze; ++z) {
+                const cx<double> r = rs[z];
+                const double *qrow = Q + (long long)z * nz;
+#pragma unroll
+                for (int m = 0; m < LS_MAXMZ; ++m) {
+                    const int mm = lane + 64 * m;
+                    if (m < mz && mm < nz) {
+                        const double q = qrow[mm];
+                        acc[m].re = fma(r.re, q, acc[m].re);
+                        acc[m].im = fma(r.im, q, acc[m].im);
+                    }
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < LS_MAXMZ; ++m) {
+                const int mm = lane + 64 * m;
+                if (m < mz && mm < nz) red[wave * nz + mm] = acc[m];
+            }
+        }
+        __syncthreads();
+        const double ej = e[off + j];
+        for (int mm = threadIdx.x; mm < nz; mm += blockDim.x) {
+            cx<double> t = mk(0.0, 0.0);
+#pragma unroll
+            for (int w = 0; w < LS_WAVES; ++w) t = t + red[w * nz + mm];
+            ts[mm] = t * (1.0 / (ej + lam[mm]));
+        }
+        __syncthreads();
+        // (3) back to the node basis
+        {
+            const int mb = wave * zchunk, me = min(nz, mb + zchunk);
+#pragma unroll
+            for (int m = 0; m < LS_MAXMZ; ++m) acc[m] = mk(0.0, 0.0);
+            for (int mm = mb; mm < me; ++mm) {
+                const cx<double> t = ts[mm];
+                const double *qtrow = QT + (long long)mm * nz;
+#pragma unroll
+                for (int m = 0; m < LS_MAXMZ; ++m) {
+                    const int z = lane + 64 * m;
+                    if (m < mz && z < nz) {
+                        const double q = qtrow[z];
+                        acc[m].re = fma(t.re, q, acc[m].re);
+                        acc[m].im = fma(t.im, q, acc[m].im);
+                    }
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < LS_MAXMZ; ++m) {
+                const int z = lane + 64 * m;
+                if (m < mz && z < nz) red[wave * nz + z] = acc[m];
+            }
+        }
+        __syncthreads();
+        for (int z = threadIdx.x; z < nz; z += blockDim.x) {
+            cx<double> g = mk(0.0, 0.0);
+#pragma unroll
+            for (int w = 0; w < LS_WAVES; ++w) g = g + red[w * nz + z];
+            rs[z] = g;
+        }
+        __syncthreads();
+        // (4) Sherman-Morrison boundary correction
+        const double *fj = fac + (off + j) * nz;
+        if (threadIdx.x == 0) {
+            const cx<double> cj = c[off + j];
+            const cx<double> den = mk(1.0, 0.0) - cj * fj[nz - 1];
+            s_fac = cdiv(rs[nz - 1] * cj, den);
+        }
+        __syncthreads();
+        const cx<double> f = s_fac;
+        for (int z = threadIdx.x; z < nz; z += blockDim.x) {
+            const cx<double> psi = rs[z] + f * fj[z];
+            const long long idx = (off + j) * nz + z;
+            xb[(long long)j * nz + z] = psi;
+            if (out) out[idx] = base[idx] + psi * sign;
+        }
+        __syncthreads();
+    }
+}
+
+void launch_lower_solve(const Layout &L, const cx<double> *V, const double *Q, const double *QT,
+                        const double *lam, const double *e, const cx<double> *c,
+                        const double *fac, cx<double> *x, const cx<double> *base,
+                        cx<double> *out, double sign, const int *active, hipStream_t s) {
+    if (L.nz > 64 * LS_MAXMZ) throw Error(3, "lower solve supports nz <= 640");
+    const size_t shm = sizeof(cx<double>) * (size_t)L.nz * (LS_WAVES + 2);
+    static bool attr_set = false;
+    if (shm > 64 * 1024 && !attr_set) {
+        MSX_HIP(hipFuncSetAttribute((const void *)lower_solve_kernel,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(lower_solve_kernel, dim3(L.n_inst), dim3(64 * LS_WAVES), shm, s, L, V, Q,
+                       QT, lam, e, c, fac, x, base, out, sign, active);
+    MSX_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------------------
+// All-channel diagonal blocks. inverse = 1: y = D^-1 x with
+//   D_j^-1 x = Q diag(1/(e_j+lam)) Q^T x + f_j (g(N) c_j / (1 - f_j(N) c_j)),
+// inverse = 0: y = D x = Q diag(e_j+lam) Q^T x - c_j x(N) e_N.
+// One block = TJ channels of one instance; the rows live in LDS, lanes run over the output
+// index of each of the two GEMMs with Q.
+// ---------------------------------------------------------------------------------------
+template <typename T, int TJ>
+__global__ __launch_bounds__(256) void diag_kernel(Layout L, const T *__restrict__ Q,
+                                                   const T *__restrict__ QT,
+                                                   const double *__restrict__ lam,
+                                                   const double *__restrict__ e,
+                                                   const cx<T> *__restrict__ c,
+                                                   const T *__restrict__ fac, const cx<T> *x,
+                                                   const cx<T> *base, cx<T> *y,
+                                                   T sign, int inverse,
+                                                   const int *__restrict__ active) {
+    const int b = blockIdx.y;
+    if (active && !active[b]) return;
+    const int nc = L.nc[b];
+    const int j0 = blockIdx.x * TJ;
+    if (j0 >= nc) return;
+    const int nz = L.nz;
+    const long long off = L.off[b];
+    extern __shared__ double smem_raw[];
+    cx<T> *Xs = reinterpret_cast<cx<T> *>(smem_raw);  // [TJ][nz]
+    cx<T> *Ts = Xs + TJ * nz;                           // [TJ][nz]
+    __shared__ double s_e[TJ];
+    __shared__ cx<T> s_c[TJ], s_x0[TJ];
+    const int rows = min(TJ, nc - j0);
+    for (int idx = threadIdx.x; idx < TJ * nz; idx += blockDim.x) {
+        const int r = idx / nz, z = idx - r * nz;
+        Xs[idx] = r < rows ? x[(off + j0 + r) * nz + z] : mk<T>(0, 0);
+    }
+    if (threadIdx.x < TJ) {
+        const int r = threadIdx.x;
+        s_e[r] = r < rows ? e[off + j0 + r] : 0.0;
+        s_c[r] = r < rows ? c[off + j0 + r] : mk<T>(0, 0);
+    }
+    __syncthreads();
+    if (threadIdx.x < TJ) s_x0[threadIdx.x] = Xs[threadIdx.x * nz + nz - 1];
+    for (int m = threadIdx.x; m < nz; m += blockDim.x) {
+        cx<T> acc[TJ];
+#pragma unroll
+        for (int r = 0; r < TJ; ++r) acc[r] = mk<T>(0, 0);
+        for (int z = 0; z < nz; ++z) {
+            const T q = Q[(long long)z * nz + m];
+#pragma unroll
+            for (int r = 0; r < TJ; ++r) {
+                const cx<T> xv = Xs[r * nz + z];
+                acc[r].re = fma(xv.re, q, acc[r].re);
+                acc[r].im = fma(xv.im, q, acc[r].im);
+            }
+        }
+        const double lm = lam[m];
+#pragma unroll
+        for (int r = 0; r < TJ; ++r) {
+            const double d = s_e[r] + lm;
+            Ts[r * nz + m] = acc[r] * (T)(inverse ? 1.0 / d : d);
+        }
+    }
+    __syncthreads();
+    for (int z = threadIdx.x; z < nz; z += blockDim.x) {
+        cx<T> acc[TJ];
+#pragma unroll
+        for (int r = 0; r < TJ; ++r) acc[r] = mk<T>(0, 0);
+        for (int m = 0; m < nz; ++m) {
+            const T q = QT[(long long)m * nz + z];
+#pragma unroll
+            for (int r = 0; r < TJ; ++r) {
+                const cx<T> tv = Ts[r * nz + m];
+                acc[r].re = fma(tv.re, q, acc[r].re);
+                acc[r].im = fma(tv.im, q, acc[r].im);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < TJ; ++r) Xs[r * nz + z] = acc[r];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < rows * nz; idx += blockDim.x) {
+        const int r = idx / nz, z = idx - r * nz;
+        const long long g = (off + j0 + r) * nz + z;
+        cx<T> v = Xs[idx];
+        if (inverse) {
+            const T *fj = fac + (off + j0 + r) * nz;
+            const cx<T> cj = s_c[r];
+            const cx<T> den = mk<T>(1, 0) - cj * fj[nz - 1];
+            const cx<T> f = cdiv(Xs[r * nz + nz - 1] * cj, den);
+            v = v + f * fj[z];
+        } else if (z == nz - 1) {
+            v = v - s_c[r] * s_x0[r];
+        }
+        y[g] = base ? base[g] + v * sign : v;
+    }
+}
+
+template <typename T>
+void launch_dinv_impl(const Layout &L, const T *Q, const T *QT, const double *lam,
+                      const double *e,
+                      const cx<T> *c, const T *fac, const cx<T> *x, const cx<T> *base, cx<T> *y,
+                      T sign, int inverse, const int *active, hipStream_t s) {
+    const size_t row_bytes = sizeof(cx<T>) * (size_t)L.nz * 2;
+    if (row_bytes * 4 > 150 * 1024) throw Error(3, "diag kernel: nz too large");
+    if (row_bytes * 8 <= 64 * 1024) {
+        dim3 grid((L.max_nc + 7) / 8, L.n_inst);
+        hipLaunchKernelGGL((diag_kernel<T, 8>), grid, dim3(256), row_bytes * 8, s, L, Q, QT, lam,
+                           e, c, fac, x, base, y, sign, inverse, active);
+    } else {
+        static bool attr_set = false;
+        if (row_bytes * 4 > 64 * 1024 && !attr_set) {
+            MSX_HIP(hipFuncSetAttribute((const void *)diag_kernel<T, 4>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            attr_set = true;
+        }
+        dim3 grid((L.max_nc + 3) / 4, L.n_inst);
+        hipLaunchKernelGGL((diag_kernel<T, 4>), grid, dim3(256), row_bytes * 4, s, L, Q, QT, lam,
+                           e, c, fac, x, base, y, sign, inverse, active);
+    }
+    MSX_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_dinv(const Layout &L, const T *Q, const T *QT, const double *lam, const double *e,
+                 const cx<T> *c, const T *fac, const cx<T> *x, const cx<T> *base, cx<T> *y,
+                 T sign, const int *active, hipStream_t s) {
+    launch_dinv_impl<T>(L, Q, QT, lam, e, c, fac, x, base, y, sign, 1, active, s);
+}
+template void launch_dinv<double>(const Layout &, const double *, const double *,
+                                  const double *, const double *, const cx<double> *,
+                                  const double *, const cx<double> *, const cx<double> *,
+                                  cx<double> *, double, const int *, hipStream_t);
+template void launch_dinv<float>(const Layout &, const float *, const float *, const double *,
+                                 const double *, const cx<float> *, const float *,
+                                 const cx<float> *, const cx<float> *, cx<float> *, float,
+                                 const int *, hipStream_t);
+
+void launch_apply_diag(const Layout &L, const double *Q, const double *QT, const double *lam,
+                       const double *e, const cx<double> *c, const cx<double> *x,
+                       cx<double> *y, const int *active, hipStream_t s) {
+    launch_dinv_impl<double>(L, Q, QT, lam, e, c, nullptr, x, nullptr, y, 1.0, 0, active, s);
+}
+
+// ---------------------------------------------------------------------------------------
+// Symmetric eigensystem by parallel cyclic Jacobi (one workgroup). Each round applies n/2
+// disjoint rotations chosen by the circle method, so a sweep of n-1 rounds visits every
+// (p, q) pair once. A' = J^T A J: a column pass (A J, and V J for the eigenvectors) then a
+// row pass. Sweeps stop when the off-diagonal Frobenius norm falls below 1e-15 of the
+// whole. Eigenvalues are then ranked ascending and the eigenvectors permuted accordingly.
+// ---------------------------------------------------------------------------------------
+constexpr int EIG_THREADS = 1024;
+constexpr int EIG_MAXPAIRS = 1024;  // n <= 2048
+
+__device__ double block_sum_eig(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    return t;
+}
+
+__global__ __launch_bounds__(EIG_THREADS) void eig_jacobi_kernel(int n, double *A, double *Vm,
+                                                                 double *lam_sorted,
+                                                                 double *Q_sorted,
+                                                                 double *QT_sorted,
+                                                                 int *info) {
+    __shared__ double cs[EIG_MAXPAIRS], sn[EIG_MAXPAIRS];
+    __shared__ int pp[EIG_MAXPAIRS], qq[EIG_MAXPAIRS];
+    __shared__ double red[EIG_THREADS / 64];
+    const int m = n + (n & 1);
+    const int half = m / 2;
+    const long long nn = (long long)n * n;
+    for (long long idx = threadIdx.x; idx < nn; idx += blockDim.x) {
+        const int i = (int)(idx / n), j = (int)(idx - (long long)i * n);
+        Vm[idx] = i == j ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    int converged = 0;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off2 = 0, tot2 = 0;
+        for (long long idx = threadIdx.x; idx < nn; idx += blockDim.x) {
+            const int i = (int)(idx / n), j = (int)(idx - (long long)i * n);
+            const double a = A[idx];
+            tot2 += a * a;
+            if (i != j) off2 += a * a;
+        }
+        off2 = block_sum_eig(off2, red);
+        tot2 = block_sum_eig(tot2, red);
+        if (off2 <= 1e-30 * tot2) {
+            converged = 1;
+            break;
+        }
+        for (int round = 0; round < m - 1; ++round) {
+            for (int k = threadIdx.x; k < half; k += blockDim.x) {
+                int p, q;
+                if (k == 0) {
+                    p = m - 1;
+                    q = round;
+                } else {
+                    p = (round + k) % (m - 1);
+                    q = (round - k + (m - 1)) % (m - 1);
+                }
+                if (p > q) { const int t = p; p = q; q = t; }
+                double c = 1.0, s = 0.0;
+                if (q < n) {
+                    const double apq = A[(long long)p * n + q];
+                    if (apq != 0.0) {
+                        const double app = A[(long long)p * n + p], aqq = A[(long long)q * n + q];
+                        const double theta = (aqq - app) / (2.0 * apq);
+                        double t;
+                        if (fabs(theta) > 1e150) t = 0.5 / theta;
+                        else t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                        c = 1.0 / sqrt(t * t + 1.0);
+                        s = t * c;
+                    }
+                }
+                pp[k] = p; qq[k] = q; cs[k] = c; sn[k] = s;
+            }
+            __syncthreads();
+            // column pass: A <- A J, Vm <- Vm J
+            for (long long idx = threadIdx.x; idx < (long long)n * half; idx += blockDim.x) {
+                const int i = (int)(idx / half), k = (int)(idx - (long long)i * half);
+                const double s = sn[k];
+                if (s == 0.0) continue;
+                const double c = cs[k];
+                const int p = pp[k], q = qq[k];
+                double *ar = A + (long long)i * n;
+                const double ap = ar[p], aq = ar[q];
+                ar[p] = c * ap - s * aq;
+                ar[q] = s * ap + c * aq;
+                double *vr = Vm + (long long)i * n;
+                const double vp = vr[p], vq = vr[q];
+                vr[p] = c * vp - s * vq;
+                vr[q] = s * vp + c * vq;
+            }
+            __syncthreads();
+            // row pass: A <- J^T A
+            for (long long idx = threadIdx.x; idx < (long long)n * half; idx += blockDim.x) {
+                const int k = (int)(idx / n), j = (int)(idx - (long long)k * n);
+                const double s = sn[k];
+                if (s == 0.0) continue;
+                const double c = cs[k];
+                const int p = pp[k], q = qq[k];
+                const double ap = A[(long long)p * n + j], aq = A[(long long)q * n + j];
+                A[(long long)p * n + j] = c * ap - s * aq;
+                A[(long long)q * n + j] = s * ap + c * aq;
+            }
+            __syncthreads();
+        }
+    }
+    // ascending rank of each eigenvalue (ties broken by index), then permute
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double li = A[(long long)i * n + i];
+        int rank = 0;
+        for (int j = 0; j < n; ++j) {
+            const double lj = A[(long long)j * n + j];
+            rank += (lj < li) || (lj == li && j < i);
+        }
+        lam_sorted[rank] = li;
+        for (int z = 0; z < n; ++z) {
+            const double v = Vm[(long long)z * n + i];
+            Q_sorted[(long long)z * n + rank] = v;
+            QT_sorted[(long long)rank * n + z] = v;
+        }
+    }
+    if (threadIdx.x == 0) *info = converged ? 0 : 1;
+}
+
+void launch_eigensystem(int n, double *A, double *Vec, double *lam_sorted, double *Q_sorted,
+                        double *QT_sorted, int *info, hipStream_t s) {
+    if (n > 2 * EIG_MAXPAIRS) throw Error(3, "eigensystem: n too large");
+    hipLaunchKernelGGL(eig_jacobi_kernel, dim3(1), dim3(EIG_THREADS), 0, s, n, A, Vec,
+                       lam_sorted, Q_sorted, QT_sorted, info);
+    MSX_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------------------
+// Preconditioner factors: one block per channel; g_m = Q[N-1][m] / (e_j + lam_m) in LDS,
+// then fac_j(z) = sum_m QT[m][z] g_m with lanes over z.
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void factors_kernel(long long total, int nz,
+                                                      const double *__restrict__ Q,
+                                                      const double *__restrict__ QT,
+                                                      const double *__restrict__ lam,
+                                                      const double *__restrict__ e,
+                                                      double *__restrict__ fac) {
+    extern __shared__ double g[];
+    const long long j = blockIdx.x;
+    if (j >= total) return;
+    const double ej = e[j];
+    for (int m = threadIdx.x; m < nz; m += blockDim.x)
+        g[m] = Q[(long long)(nz - 1) * nz + m] / (ej + lam[m]);
+    __syncthreads();
+    for (int z = threadIdx.x; z < nz; z += blockDim.x) {
+        double acc = 0;
+        for (int m = 0; m < nz; ++m) acc = fma(QT[(long long)m * nz + z], g[m], acc);
+        fac[j * nz + z] = acc;
+    }
+}
+
+void launch_preconditioner_factors_qt(long long total_channels, int nz, const double *Q,
+                                      const double *QT, const double *lam, const double *e,
+                                      double *fac, hipStream_t s) {
+    if (total_channels <= 0) return;
+    hipLaunchKernelGGL(factors_kernel, dim3((unsigned)total_channels), dim3(256),
+                       sizeof(double) * nz, s, total_channels, nz, Q, QT, lam, e, fac);
+    MSX_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------------------
+// Element-wise helpers.
+// ---------------------------------------------------------------------------------------
+__global__ void to_float_kernel(long long n, const cx<double> *__restrict__ src,
+                                cx<float> *__restrict__ dst) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[i] = mk((float)src[i].re, (float)src[i].im);
+}
+__global__ void to_float_real_kernel(long long n, const double *__restrict__ src,
+                                     float *__restrict__ dst) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+static unsigned ew_blocks(long long n) {
+    long long b = (n + 255) / 256;
+    return (unsigned)(b < 8192 ? (b > 0 ? b : 1) : 8192);
+}
+void launch_to_float(long long n, const cx<double> *src, cx<float> *dst, hipStream_t s) {
+    hipLaunchKernelGGL(to_float_kernel, dim3(ew_blocks(n)), dim3(256), 0, s, n, src, dst);
+    MSX_LAUNCH_CHECK();
+}
+void launch_to_float_real(long long n, const double *src, float *dst, hipStream_t s) {
+    hipLaunchKernelGGL(to_float_real_kernel, dim3(ew_blocks(n)), dim3(256), 0, s, n, src, dst);
+    MSX_LAUNCH_CHECK();
+}
+
+__global__ void intensities_kernel(Layout L, const cx<double> *__restrict__ psi,
+                                   const cx<double> *__restrict__ b,
+                                   const cx<double> *__restrict__ a0,
+                                   const int *__restrict__ spec, double *__restrict__ P) {
+    const int inst = blockIdx.y;
+    const int nc = L.nc[inst];
+    const long long off = L.off[inst];
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
+        const long long g = off + j;
+        const cx<double> two_i = mk(0.0, 2.0);
+        cx<double> s = two_i * b[g] * psi[g * L.nz + L.nz - 1];
+        if (j == spec[inst]) s = s + a0[inst];
+        P[g] = norm2(s);
+    }
+}
+
+void launch_intensities(const Layout &L, const cx<double> *psi, const cx<double> *b,
+                        const cx<double> *a0, const int *spec, double *P, hipStream_t s) {
+    dim3 grid((L.max_nc + 255) / 256, L.n_inst);
+    hipLaunchKernelGGL(intensities_kernel, grid, dim3(256), 0, s, L, psi, b, a0, spec, P);
+    MSX_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------------------
+// Potential gradient. For table entry g = (gx, gy) != 0 and node z:
+//   grad[g][z] = conj( -sum_b sum_j lam_j(z) psi_{i(j)}(z) ),  G_i = G_j - g (mod grid)
+// and for g = 0 the real part of -sum_b sum_j lam_j(z) psi_j(z). chmap[b][g] gives the
+// channel of instance b sitting at grid point g (-1 if none). One block per g, lanes over z.
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void gradient_kernel(Layout L, const int *__restrict__ chmap,
+                                                       const cx<double> *__restrict__ lam_adj,
+                                                       const cx<double> *__restrict__ psi,
+                                                       cx<double> *__restrict__ grad) {
+    const int g = blockIdx.y;
+    const int nkx = L.nkx, nky = L.nky, nz = L.nz;
+    const int gx = g / nky, gy = g - gx * nky;
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int zl = z < nz ? z : nz - 1;
+    const int nG = nkx * nky;
+    cx<double> acc = mk(0.0, 0.0);
+    for (int b = 0; b < L.n_inst; ++b) {
+        const int nc = L.nc[b];
+        const long long off = L.off[b];
+        const int *map = chmap + (long long)b * nG;
+        for (int j = 0; j < nc; ++j) {
+            const int sx = wrap_mod(L.chx[off + j] - gx, nkx);
+            const int sy = wrap_mod(L.chy[off + j] - gy, nky);
+            const int i = map[sx * nky + sy];
+            if (i < 0) continue;
+            cfma(acc, lam_adj[(off + j) * nz + zl], psi[(off + i) * nz + zl]);
+        }
+    }
+    if (z < nz) {
+        cx<double> r;
+        if (g == 0) r = mk(-acc.re, 0.0);
+        else r = mk(-acc.re, acc.im);  // conj(-acc)
+        grad[(long long)g * nz + z] = grad[(long long)g * nz + z] + r;
+    }
+}
+
+void launch_potential_gradient_map(const Layout &L, const int *chmap, const cx<double> *lam_adj,
+                                   const cx<double> *psi, cx<double> *grad, hipStream_t s) {
+    dim3 grid((L.nz + 255) / 256, L.nkx * L.nky);
+    hipLaunchKernelGGL(gradient_kernel, grid, dim3(256), 0, s, L, chmap, lam_adj, psi, grad);
+    MSX_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------------------
+// Small set-up kernels.
+// ---------------------------------------------------------------------------------------
+__global__ void build_h0_kernel(int n, const double *__restrict__ T,
+                                const cx<double> *__restrict__ V, double *__restrict__ A) {
+    const long long nn = (long long)n * n;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < nn;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(idx / n), j = (int)(idx - (long long)i * n);
+        // symmetrised T (the eigensolver assumes exact symmetry)
+        double a = 0.5 * (T[(long long)i * n + j] + T[(long long)j * n + i]);
+        if (i == j) a += V[i].re;  // Re V_00(z_i): g = 0 row of the table
+        A[idx] = a;
+    }
+}
+
+void launch_build_h0(int nz, const double *T, const cx<double> *V, double *A, hipStream_t s) {
+    hipLaunchKernelGGL(build_h0_kernel, dim3(ew_blocks((long long)nz * nz)), dim3(256), 0, s, nz,
+                       T, V, A);
+    MSX_LAUNCH_CHECK();
+}
+
+__global__ void flip_table_kernel(int nkx, int nky, int nz, const cx<double> *__restrict__ V,
+                                  cx<double> *__restrict__ Vt) {
+    const long long total = (long long)nkx * nky * nz;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long g = idx / nz;
+        const int z = (int)(idx - g * nz);
+        const int gx = (int)(g / nky), gy = (int)(g - (long long)gx * nky);
+        const int fx = (nkx - gx) % nkx, fy = (nky - gy) % nky;
+        Vt[idx] = V[((long long)fx * nky + fy) * nz + z];
+    }
+}
+
+void launch_flip_table(int nkx, int nky, int nz, const cx<double> *V, cx<double> *Vt,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(flip_table_kernel, dim3(ew_blocks((long long)nkx * nky * nz)), dim3(256),
+                       0, s, nkx, nky, nz, V, Vt);
+    MSX_LAUNCH_CHECK();
+}
+
+// One block per instance: zero the instance's rows, then place the source.
+__global__ void set_source_kernel(Layout L, const int *__restrict__ spec,
+                                  const cx<double> *__restrict__ bsrc, cx<double> *__restrict__ s) {
+    const int b = blockIdx.x;
+    const long long beg = L.off[b] * L.nz, len = (long long)L.nc[b] * L.nz;
+    for (long long i = threadIdx.x; i < len; i += blockDim.x) s[beg + i] = mk(0.0, 0.0);
+    __syncthreads();
+    if (threadIdx.x == 0) s[(L.off[b] + spec[b]) * L.nz + L.nz - 1] = bsrc[b];
+}
+
+void launch_set_source(const Layout &L, const int *spec, const cx<double> *bsrc, cx<double> *s,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(set_source_kernel, dim3(L.n_inst), dim3(256), 0, st, L, spec, bsrc, s);
+    MSX_LAUNCH_CHECK();
+}
+
+__global__ void adjoint_source_kernel(Layout L, const int *__restrict__ spec,
+                                      const cx<double> *__restrict__ psi,
+                                      const cx<double> *__restrict__ b,
+                                      const cx<double> *__restrict__ a0,
+                                      const double *__restrict__ dLdP,
+                                      cx<double> *__restrict__ g) {
+    const int inst = blockIdx.x;
+    const int nz = L.nz, nc = L.nc[inst];
+    const long long off = L.off[inst];
+    for (long long i = threadIdx.x; i < (long long)nc * nz; i += blockDim.x)
+        g[off * nz + i] = mk(0.0, 0.0);
+    __syncthreads();
+    for (int j = threadIdx.x; j < nc; j += blockDim.x) {
+        const long long ch = off + j;
+        const cx<double> two_i = mk(0.0, 2.0);
+        cx<double> S = two_i * b[ch] * psi[ch * nz + nz - 1];
+        if (j == spec[inst]) S = S + a0[inst];
+        const cx<double> four_i = mk(0.0, 4.0 * dLdP[ch]);
+        g[ch * nz + nz - 1] = four_i * b[ch] * conj(S);
+    }
+}
+
+void launch_adjoint_source(const Layout &L, const int *spec, const cx<double> *psi,
+                           const cx<double> *b, const cx<double> *a0, const double *dLdP,
+                           cx<double> *g, hipStream_t st) {
+    hipLaunchKernelGGL(adjoint_source_kernel, dim3(L.n_inst), dim3(256), 0, st, L, spec, psi, b,
+                       a0, dLdP, g);
+    MSX_LAUNCH_CHECK();
+}
+
+__global__ void add_float_kernel(long long n, const cx<float> *__restrict__ d,
+                                 cx<double> *__restrict__ x) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        x[i] = mk(x[i].re + (double)d[i].re, x[i].im + (double)d[i].im);
+}
+
+void launch_add_float(long long n, const cx<float> *d, cx<double> *x, hipStream_t s) {
+    hipLaunchKernelGGL(add_float_kernel, dim3(ew_blocks(n)), dim3(256), 0, s, n, d, x);
+    MSX_LAUNCH_CHECK();
+}
+
+}  // namespace msx

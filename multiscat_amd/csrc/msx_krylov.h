@@ -1,0 +1,58 @@
+// Batched GMRES state and the launchers of msx_krylov.hip.
+#pragma once
+
+#include "msx_internal.h"
+
+namespace msx {
+
+constexpr int KR_MAX_BASIS = 2048;  // longest Krylov cycle the kernels accept
+
+// Where each instance's vector lives inside a concatenated state vector, and how the
+// vector kernels split it: grid = (nchunk, n_inst), chunk elements per block.
+struct KLayout {
+    int n_inst;
+    int kmax;               // restart length (Hessenberg is (kmax+1) x (kmax+1) per instance)
+    int nchunk;             // blocks per instance
+    long long chunk;        // elements per block
+    const long long *start; // [n_inst] first element of each instance
+    const long long *len;   // [n_inst] element count of each instance
+    int *active;            // [n_inst] 1 while the instance iterates in this cycle
+};
+
+// Per-instance GMRES recurrence state (device pointers).
+struct KState {
+    cx<double> *H;   // [n_inst][(kmax+1)^2] column-major
+    cx<double> *g;   // [n_inst][kmax+1]
+    cx<double> *cs;  // [n_inst][kmax+1]
+    cx<double> *sn;  // [n_inst][kmax+1]
+    cx<double> *y;   // [n_inst][kmax+1]
+    double *beta, *beta0, *resid, *hnorm;
+    int *kconv, *iters, *cycle_iters, *converged;
+    double eps;
+    int window;
+};
+
+template <typename T>
+void k_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<T> *w, int with_norm,
+            double *partial, cx<double> *h, int accumulate, hipStream_t s);
+template <typename T>
+void k_axpy(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<double> *h, cx<T> *w,
+            int with_norm, double *partial, cx<double> *hnorm_dst, hipStream_t s);
+template <typename T, typename TX>
+void k_update(const KLayout &K, const cx<T> *const *vptr, const int *kk, const cx<double> *y,
+              cx<TX> *x, const int *mask, hipStream_t s);
+template <typename T>
+void k_scale(const KLayout &K, const cx<T> *src, cx<T> *dst, const double *sv, const int *mask,
+             hipStream_t s);
+template <typename T>
+void k_axpby(const KLayout &K, const cx<T> *a, const cx<T> *c, T alpha, cx<T> *dst,
+             const int *mask, hipStream_t s);
+void k_givens(const KLayout &K, int k, const cx<double> *hcol, const cx<double> *hcol2,
+              const KState &S, int *count, hipStream_t s);
+void k_backsolve(const KLayout &K, const KState &S, const int *mask, hipStream_t s);
+void k_cycle_start(const KLayout &K, const KState &S, const cx<double> *nrm2, int first,
+                   int *count, hipStream_t s);
+void k_reset(const KLayout &K, const KState &S, int *enabled, hipStream_t s);
+void k_reactivate(const KLayout &K, const KState &S, const int *enabled, hipStream_t s);
+
+}  // namespace msx

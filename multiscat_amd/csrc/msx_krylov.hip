@@ -1,0 +1,440 @@
+// Batched GMRES building blocks (Arnoldi with classical Gram-Schmidt run twice, complex
+// Givens rotations, per-instance convergence masks), for FP64 and FP32 Krylov vectors.
+// Reductions always accumulate in FP64; the Hessenberg matrix, rotations and residual
+// recurrence are FP64 whatever the vector precision.
+//
+// The reference's Arnoldi is modified Gram-Schmidt with a full back substitution each step
+// and a 3-iteration convergence window (lib/multiscat_fortran/fortran/multiscat_gmres.f90:549-606);
+// CGS2 produces the same Hessenberg matrix up to rounding and needs two passes over the
+// basis instead of k dependent ones.
+#include "msx_krylov.h"
+
+namespace msx {
+
+constexpr int KR_THREADS = 256;
+
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Block-wide sum of (re, im) pairs; every thread gets the result.
+__device__ inline void block_sum2(double &a, double &b, double *red) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) { red[2 * wave] = a; red[2 * wave + 1] = b; }
+    __syncthreads();
+    a = 0; b = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { a += red[2 * w]; b += red[2 * w + 1]; }
+}
+
+// partial[(b*(kmax+1) + j)*nchunk + chunk] = sum over the chunk of conj(v_j) w   (j < kcount)
+// partial[(b*(kmax+1) + kcount)*nchunk + chunk] = sum |w|^2           (if with_norm)
+template <typename T>
+__global__ __launch_bounds__(KR_THREADS) void dots_kernel(KLayout K, const cx<T> *const *vptr,
+                                                          int kcount, const cx<T> *__restrict__ w,
+                                                          int with_norm,
+                                                          double *__restrict__ partial) {
+    __shared__ double red[2 * (KR_THREADS / 64)];
+    const int b = blockIdx.y;
+    if (K.active && !K.active[b]) return;
+    const long long n = K.len[b], base = K.start[b];
+    const long long cbeg = (long long)blockIdx.x * K.chunk;
+    const long long cend = cbeg + K.chunk < n ? cbeg + K.chunk : n;
+    const long long slot = (long long)b * (K.kmax + 1);
+    for (int j = 0; j < kcount + (with_norm ? 1 : 0); ++j) {
+        double sr = 0, si = 0;
+        if (j < kcount) {
+            const cx<T> *v = vptr[j] + base;
+            for (long long i = cbeg + threadIdx.x; i < cend; i += KR_THREADS) {
+                const cx<T> a = v[i], c = w[base + i];
+                sr += (double)a.re * c.re + (double)a.im * c.im;
+                si += (double)a.re * c.im - (double)a.im * c.re;
+            }
+        } else {
+            for (long long i = cbeg + threadIdx.x; i < cend; i += KR_THREADS) {
+                const cx<T> c = w[base + i];
+                sr += (double)c.re * c.re + (double)c.im * c.im;
+            }
+        }
+        block_sum2(sr, si, red);
+        if (threadIdx.x == 0) {
+            double *p = partial + ((slot + j) * K.nchunk + blockIdx.x) * 2;
+            p[0] = cbeg < n ? sr : 0.0;
+            p[1] = cbeg < n ? si : 0.0;
+        }
+    }
+}
+
+// h[b*(kmax+1) + j] (+)= sum_chunk partial   for first <= j < count
+__global__ void reduce_partials_kernel(KLayout K, int first, int count,
+                                       const double *__restrict__ partial,
+                                       cx<double> *__restrict__ h, int accumulate) {
+    const int b = blockIdx.x;
+    if (K.active && !K.active[b]) return;
+    const long long slot = (long long)b * (K.kmax + 1);
+    for (int j = first + threadIdx.x; j < count; j += blockDim.x) {
+        double sr = 0, si = 0;
+        const double *p = partial + (slot + j) * K.nchunk * 2;
+        for (int c = 0; c < K.nchunk; ++c) { sr += p[2 * c]; si += p[2 * c + 1]; }
+        cx<double> &dst = h[slot + j];
+        if (accumulate) dst = mk(dst.re + sr, dst.im + si);
+        else dst = mk(sr, si);
+    }
+}
+
+// w -= sum_{j<kcount} v_j h_j ; optionally norm partials of the updated w in slot kcount
+template <typename T>
+__global__ __launch_bounds__(KR_THREADS) void axpy_kernel(KLayout K, const cx<T> *const *vptr,
+                                                          int kcount,
+                                                          const cx<double> *__restrict__ h,
+                                                          cx<T> *__restrict__ w, int with_norm,
+                                                          double *__restrict__ partial) {
+    __shared__ cx<T> hs[KR_MAX_BASIS];
+    __shared__ double red[2 * (KR_THREADS / 64)];
+    const int b = blockIdx.y;
+    if (K.active && !K.active[b]) return;
+    const long long n = K.len[b], base = K.start[b];
+    const long long slot = (long long)b * (K.kmax + 1);
+    for (int j = threadIdx.x; j < kcount; j += KR_THREADS) {
+        const cx<double> hv = h[slot + j];
+        hs[j] = mk((T)hv.re, (T)hv.im);
+    }
+    __syncthreads();
+    const long long cbeg = (long long)blockIdx.x * K.chunk;
+    const long long cend = cbeg + K.chunk < n ? cbeg + K.chunk : n;
+    double nrm = 0;
+    for (long long i = cbeg + threadIdx.x; i < cend; i += KR_THREADS) {
+        cx<T> acc = w[base + i];
+        for (int j = 0; j < kcount; ++j) {
+            const cx<T> v = vptr[j][base + i];
+            const cx<T> hj = hs[j];
+            acc.re -= v.re * hj.re - v.im * hj.im;
+            acc.im -= v.re * hj.im + v.im * hj.re;
+        }
+        w[base + i] = acc;
+        nrm += (double)acc.re * acc.re + (double)acc.im * acc.im;
+    }
+    if (with_norm) {
+        double dummy = 0;
+        block_sum2(nrm, dummy, red);
+        if (threadIdx.x == 0) {
+            double *p = partial + ((slot + kcount) * K.nchunk + blockIdx.x) * 2;
+            p[0] = cbeg < n ? nrm : 0.0;
+            p[1] = 0.0;
+        }
+    }
+}
+
+// x += sum_{j<kk_b} v_j y_j  (kk_b per instance)
+template <typename T, typename TX>
+__global__ __launch_bounds__(KR_THREADS) void update_kernel(KLayout K, const cx<T> *const *vptr,
+                                                            const int *__restrict__ kk,
+                                                            const cx<double> *__restrict__ y,
+                                                            cx<TX> *__restrict__ x,
+                                                            const int *__restrict__ mask) {
+    __shared__ cx<double> ys[KR_MAX_BASIS];
+    const int b = blockIdx.y;
+    if (mask && !mask[b]) return;
+    const int kb = kk[b];
+    if (kb <= 0) return;
+    const long long n = K.len[b], base = K.start[b];
+    const long long slot = (long long)b * (K.kmax + 1);
+    for (int j = threadIdx.x; j < kb; j += KR_THREADS) ys[j] = y[slot + j];
+    __syncthreads();
+    const long long cbeg = (long long)blockIdx.x * K.chunk;
+    const long long cend = cbeg + K.chunk < n ? cbeg + K.chunk : n;
+    for (long long i = cbeg + threadIdx.x; i < cend; i += KR_THREADS) {
+        cx<double> acc = mk((double)x[base + i].re, (double)x[base + i].im);
+        for (int j = 0; j < kb; ++j) {
+            const cx<T> v = vptr[j][base + i];
+            const cx<double> yj = ys[j];
+            acc.re += (double)v.re * yj.re - (double)v.im * yj.im;
+            acc.im += (double)v.re * yj.im + (double)v.im * yj.re;
+        }
+        x[base + i] = mk((TX)acc.re, (TX)acc.im);
+    }
+}
+
+// dst = src * (1 / s_b)  for instances with mask set and s_b > 0
+template <typename T>
+__global__ __launch_bounds__(KR_THREADS) void scale_kernel(KLayout K, const cx<T> *src, cx<T> *dst,
+                                                           const double *__restrict__ s,
+                                                           const int *__restrict__ mask) {
+    const int b = blockIdx.y;
+    if (mask && !mask[b]) return;
+    const double sb = s[b];
+    if (!(sb > 0)) return;
+    const T inv = (T)(1.0 / sb);
+    const long long n = K.len[b], base = K.start[b];
+    const long long cbeg = (long long)blockIdx.x * K.chunk;
+    const long long cend = cbeg + K.chunk < n ? cbeg + K.chunk : n;
+    for (long long i = cbeg + threadIdx.x; i < cend; i += KR_THREADS) {
+        const cx<T> v = src[base + i];
+        dst[base + i] = mk(v.re * inv, v.im * inv);
+    }
+}
+
+// dst = a + alpha * c (alpha real), all instances with mask
+template <typename T>
+__global__ __launch_bounds__(KR_THREADS) void axpby_kernel(KLayout K, const cx<T> *__restrict__ a,
+                                                           const cx<T> *__restrict__ c, T alpha,
+                                                           cx<T> *__restrict__ dst,
+                                                           const int *__restrict__ mask) {
+    const int b = blockIdx.y;
+    if (mask && !mask[b]) return;
+    const long long n = K.len[b], base = K.start[b];
+    const long long cbeg = (long long)blockIdx.x * K.chunk;
+    const long long cend = cbeg + K.chunk < n ? cbeg + K.chunk : n;
+    for (long long i = cbeg + threadIdx.x; i < cend; i += KR_THREADS) {
+        const cx<T> u = a[base + i], v = c[base + i];
+        dst[base + i] = mk(u.re + alpha * v.re, u.im + alpha * v.im);
+    }
+}
+
+// LINPACK-style complex Givens rotation (multiscat_gmres.f90:689-716).
+__device__ inline void givens(cx<double> a, cx<double> b, cx<double> &c, cx<double> &s,
+                              double &r) {
+    const double aa = sqrt(norm2(a)), ab = sqrt(norm2(b));
+    const cx<double> rho = aa > ab ? a : b;
+    const double scale = aa + ab;
+    if (scale == 0.0) {
+        c = mk(1.0, 0.0);
+        s = mk(0.0, 0.0);
+        r = 0.0;
+        return;
+    }
+    const cx<double> as = a * (1.0 / scale), bs = b * (1.0 / scale);
+    r = scale * sqrt(norm2(as) + norm2(bs));
+    if (rho.re < 0.0) r = -r;
+    c = conj(a * (1.0 / r));
+    s = b * (1.0 / r);
+}
+
+// One thread per instance: finish column k of the Hessenberg matrix, rotate, update the
+// residual recurrence and the convergence window. Writes the number of instances still
+// active to *count.
+__global__ void givens_kernel(KLayout K, int k, const cx<double> *__restrict__ hcol,
+                              const cx<double> *__restrict__ hcol2, KState S,
+                              int *__restrict__ count) {
+    __shared__ int red[KR_THREADS / 64];
+    int alive = 0;
+    for (int b = threadIdx.x; b < K.n_inst; b += blockDim.x) {
+        if (!K.active[b]) continue;
+        const long long slot = (long long)b * (K.kmax + 1);
+        cx<double> *H = S.H + (long long)b * (K.kmax + 1) * (K.kmax + 1);  // column-major
+#define HH(i, j) H[(long long)(j) * (K.kmax + 1) + (i)]
+        for (int i = 0; i <= k; ++i) HH(i, k) = hcol[slot + i] + hcol2[slot + i];
+        const double hn = sqrt(hcol[slot + k + 1].re);
+        HH(k + 1, k) = mk(hn, 0.0);
+        cx<double> *cs = S.cs + slot, *sn = S.sn + slot, *g = S.g + slot;
+        for (int i = 0; i < k; ++i) {
+            const cx<double> t = cs[i] * HH(i, k) + conj(sn[i]) * HH(i + 1, k);
+            HH(i + 1, k) = conj(cs[i]) * HH(i + 1, k) - sn[i] * HH(i, k);
+            HH(i, k) = t;
+        }
+        double r;
+        givens(HH(k, k), HH(k + 1, k), cs[k], sn[k], r);
+        HH(k, k) = mk(r, 0.0);
+        HH(k + 1, k) = mk(0.0, 0.0);
+        g[k + 1] = mk(0.0, 0.0) - sn[k] * g[k];
+        g[k] = cs[k] * g[k];
+#undef HH
+        const double res = sqrt(norm2(g[k + 1]));
+        const double ref = S.beta0[b] > 2.2250738585072014e-308 ? S.beta0[b] : 2.2250738585072014e-308;
+        S.resid[b] = res / ref;
+        S.kconv[b] = res < S.eps * ref ? S.kconv[b] + 1 : 0;
+        S.cycle_iters[b] = k + 1;
+        S.iters[b] += 1;
+        S.hnorm[b] = hn;
+        if (S.kconv[b] >= S.window || hn == 0.0) {
+            K.active[b] = 0;
+            S.converged[b] = 1;
+        } else {
+            alive += 1;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) alive += __shfl_xor(alive, o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = alive;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+        *count = t;
+    }
+}
+
+// One thread per instance: back substitution R y = g over the cycle's kk columns.
+__global__ void backsolve_kernel(KLayout K, KState S, const int *__restrict__ mask) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= K.n_inst || (mask && !mask[b])) return;
+    const int kk = S.cycle_iters[b];
+    const long long slot = (long long)b * (K.kmax + 1);
+    const cx<double> *H = S.H + (long long)b * (K.kmax + 1) * (K.kmax + 1);
+    cx<double> *y = S.y + slot;
+    for (int j = 0; j < kk; ++j) y[j] = S.g[slot + j];
+    for (int j = kk - 1; j >= 0; --j) {
+        y[j] = cdiv(y[j], H[(long long)j * (K.kmax + 1) + j]);
+        for (int i = 0; i < j; ++i) y[i] = y[i] - H[(long long)j * (K.kmax + 1) + i] * y[j];
+    }
+}
+
+// Start of a cycle: beta = ||r0||, g = (beta, 0, ...), kconv kept, cycle_iters = 0.
+// Instances with beta == 0 are already solved.
+__global__ void cycle_start_kernel(KLayout K, KState S, const cx<double> *__restrict__ nrm2,
+                                   int first_cycle, int *__restrict__ count) {
+    __shared__ int red[KR_THREADS / 64];
+    int alive = 0;
+    for (int b = threadIdx.x; b < K.n_inst; b += blockDim.x) {
+        S.cycle_iters[b] = 0;  // inactive instances must not replay an older cycle's update
+        if (!K.active[b]) continue;
+        const long long slot = (long long)b * (K.kmax + 1);
+        const double beta = sqrt(nrm2[slot].re);
+        S.beta[b] = beta;
+        if (first_cycle) S.beta0[b] = beta;
+        S.g[slot] = mk(beta, 0.0);
+        S.cycle_iters[b] = 0;
+        if (!(beta > 0.0)) {
+            K.active[b] = 0;
+            S.converged[b] = 1;
+            S.resid[b] = 0.0;
+        } else {
+            alive += 1;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) alive += __shfl_xor(alive, o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = alive;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+        *count = t;
+    }
+}
+
+// Start of a solve: remember which instances take part, clear their counters.
+__global__ void gmres_reset_kernel(KLayout K, KState S, int *__restrict__ enabled) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < K.n_inst; b += gridDim.x * blockDim.x) {
+        enabled[b] = K.active[b];
+        S.converged[b] = 0;
+        S.kconv[b] = 0;
+        S.iters[b] = 0;
+        S.cycle_iters[b] = 0;
+        S.resid[b] = 0.0;
+    }
+}
+
+// Restart: instances that take part and have not converged iterate again.
+__global__ void reactivate_kernel(KLayout K, KState S, const int *__restrict__ enabled) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < K.n_inst; b += gridDim.x * blockDim.x)
+        K.active[b] = enabled[b] && !S.converged[b];
+}
+
+void k_reset(const KLayout &K, const KState &S, int *enabled, hipStream_t s) {
+    hipLaunchKernelGGL(gmres_reset_kernel, dim3((K.n_inst + 255) / 256), dim3(256), 0, s, K, S,
+                       enabled);
+    MSX_LAUNCH_CHECK();
+}
+
+void k_reactivate(const KLayout &K, const KState &S, const int *enabled, hipStream_t s) {
+    hipLaunchKernelGGL(reactivate_kernel, dim3((K.n_inst + 255) / 256), dim3(256), 0, s, K, S,
+                       enabled);
+    MSX_LAUNCH_CHECK();
+}
+
+// ---- launchers ----------------------------------------------------------------------------
+static dim3 kgrid(const KLayout &K) { return dim3(K.nchunk, K.n_inst); }
+
+template <typename T>
+void k_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<T> *w,
+            int with_norm, double *partial, cx<double> *h, int accumulate, hipStream_t s) {
+    hipLaunchKernelGGL((dots_kernel<T>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr, kcount, w,
+                       with_norm, partial);
+    MSX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(K.n_inst), dim3(64), 0, s, K, 0,
+                       kcount + (with_norm ? 1 : 0), partial, h, accumulate);
+    MSX_LAUNCH_CHECK();
+}
+
+template <typename T>
+void k_axpy(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<double> *h,
+            cx<T> *w, int with_norm, double *partial, cx<double> *hnorm_dst, hipStream_t s) {
+    if (kcount > KR_MAX_BASIS) throw Error(3, "Krylov dimension exceeds the kernel limit");
+    hipLaunchKernelGGL((axpy_kernel<T>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr, kcount, h, w,
+                       with_norm, partial);
+    MSX_LAUNCH_CHECK();
+    if (with_norm) {
+        // the norm partials sit in slot kcount; reduce only that slot into hnorm_dst
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3(K.n_inst), dim3(64), 0, s, K, kcount,
+                           kcount + 1, partial, hnorm_dst, 0);
+        MSX_LAUNCH_CHECK();
+    }
+}
+
+template <typename T, typename TX>
+void k_update(const KLayout &K, const cx<T> *const *vptr, const int *kk, const cx<double> *y,
+              cx<TX> *x, const int *mask, hipStream_t s) {
+    hipLaunchKernelGGL((update_kernel<T, TX>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr, kk, y, x,
+                       mask);
+    MSX_LAUNCH_CHECK();
+}
+
+template <typename T>
+void k_scale(const KLayout &K, const cx<T> *src, cx<T> *dst, const double *sv, const int *mask,
+             hipStream_t s) {
+    hipLaunchKernelGGL((scale_kernel<T>), kgrid(K), dim3(KR_THREADS), 0, s, K, src, dst, sv, mask);
+    MSX_LAUNCH_CHECK();
+}
+
+template <typename T>
+void k_axpby(const KLayout &K, const cx<T> *a, const cx<T> *c, T alpha, cx<T> *dst,
+             const int *mask, hipStream_t s) {
+    hipLaunchKernelGGL((axpby_kernel<T>), kgrid(K), dim3(KR_THREADS), 0, s, K, a, c, alpha, dst,
+                       mask);
+    MSX_LAUNCH_CHECK();
+}
+
+void k_givens(const KLayout &K, int k, const cx<double> *hcol, const cx<double> *hcol2,
+              const KState &S, int *count, hipStream_t s) {
+    hipLaunchKernelGGL(givens_kernel, dim3(1), dim3(KR_THREADS), 0, s, K, k, hcol, hcol2, S,
+                       count);
+    MSX_LAUNCH_CHECK();
+}
+
+void k_backsolve(const KLayout &K, const KState &S, const int *mask, hipStream_t s) {
+    hipLaunchKernelGGL(backsolve_kernel, dim3((K.n_inst + 63) / 64), dim3(64), 0, s, K, S, mask);
+    MSX_LAUNCH_CHECK();
+}
+
+void k_cycle_start(const KLayout &K, const KState &S, const cx<double> *nrm2, int first,
+                   int *count, hipStream_t s) {
+    hipLaunchKernelGGL(cycle_start_kernel, dim3(1), dim3(KR_THREADS), 0, s, K, S, nrm2, first,
+                       count);
+    MSX_LAUNCH_CHECK();
+}
+
+#define MSX_INST(T)                                                                           \
+    template void k_dots<T>(const KLayout &, const cx<T> *const *, int, const cx<T> *, int,      \
+                            double *, cx<double> *, int, hipStream_t);                          \
+    template void k_axpy<T>(const KLayout &, const cx<T> *const *, int, const cx<double> *,     \
+                            cx<T> *, int, double *, cx<double> *, hipStream_t);                 \
+    template void k_scale<T>(const KLayout &, const cx<T> *, cx<T> *, const double *,           \
+                             const int *, hipStream_t);                                         \
+    template void k_axpby<T>(const KLayout &, const cx<T> *, const cx<T> *, T, cx<T> *,          \
+                             const int *, hipStream_t);
+MSX_INST(double)
+MSX_INST(float)
+template void k_update<double, double>(const KLayout &, const cx<double> *const *, const int *,
+                                       const cx<double> *, cx<double> *, const int *,
+                                       hipStream_t);
+template void k_update<float, float>(const KLayout &, const cx<float> *const *, const int *,
+                                     const cx<double> *, cx<float> *, const int *, hipStream_t);
+template void k_update<float, double>(const KLayout &, const cx<float> *const *, const int *,
+                                      const cx<double> *, cx<double> *, const int *,
+                                      hipStream_t);
+
+}  // namespace msx

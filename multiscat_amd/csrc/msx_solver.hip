@@ -1,0 +1,983 @@
+// GMRES drivers, the potential-table and batch objects, and the C ABI of libmultiscat_amd.
+//
+// Solve structure (reference: lib/multiscat_fortran/fortran/multiscat_gmres.f90:452-618):
+//   Gauss-Seidel split   A = L + U in channel order; GMRES on (I + L^-1 U) psi = L^-1 s,
+//                        left-preconditioned by (I - L^-1 U) when neumann is set.
+//   Block Jacobi         A = D + V; GMRES on (I + D^-1 V) psi = D^-1 s, left-preconditioned
+//                        by (I - D^-1 V) when neumann is set (the order-1 von Neumann series
+//                        of multiscat/multiscat/_scipy_von_neumann.py:40-58).
+//   Mixed precision      block Jacobi with FP32 inner GMRES on the FP64 preconditioned
+//                        residual r = M D^-1 (s - A psi); psi accumulates in FP64.
+// Every vector and matrix lives in HBM; the host only sequences launches and reads one
+// integer (instances still iterating) per Arnoldi step.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <set>
+
+#include "../../include/multiscat_amd.h"
+#include "msx_krylov.h"
+
+using namespace msx;
+
+namespace msx {
+
+thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+template <typename T>
+using Op = std::function<void(const cx<T> *in, cx<T> *out, const int *active)>;
+
+// ---------------------------------------------------------------------------------------
+// Krylov workspace for one precision.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+struct Krylov {
+    int n_inst = 0, kmax = 0;
+    long long total = 0;
+    hipStream_t s = 0;
+    KLayout K{};
+    KState S{};
+    DBuf<long long> start_d, len_d;
+    DBuf<int> active, enabled, kconv, iters, cycle_iters, converged, count;
+    DBuf<double> beta, beta0, resid, hnorm, partial;
+    DBuf<cx<double>> H, g, cs, sn, y, hcol, hcol2;
+    std::vector<DBuf<cx<T>>> blocks;
+    std::vector<cx<T> *> vp;
+    DBuf<cx<T> *> vp_d;
+    DBuf<cx<T>> w;
+    int *count_h = nullptr;
+
+    Krylov(const std::vector<long long> &start, const std::vector<long long> &len, int kmax_,
+           hipStream_t stream)
+        : n_inst((int)start.size()), kmax(kmax_), s(stream) {
+        if (kmax > KR_MAX_BASIS - 1) kmax = KR_MAX_BASIS - 1;
+        long long maxlen = 1;
+        for (int b = 0; b < n_inst; ++b) {
+            total = std::max(total, start[b] + len[b]);
+            maxlen = std::max(maxlen, len[b]);
+        }
+        K.n_inst = n_inst;
+        K.kmax = kmax;
+        K.chunk = 2048;
+        K.nchunk = (int)((maxlen + K.chunk - 1) / K.chunk);
+        start_d.alloc(n_inst);
+        len_d.alloc(n_inst);
+        start_d.upload(start.data(), n_inst, s);
+        len_d.upload(len.data(), n_inst, s);
+        K.start = start_d.p;
+        K.len = len_d.p;
+        for (auto *b : {&active, &enabled, &kconv, &iters, &cycle_iters, &converged}) b->alloc(n_inst);
+        count.alloc(1);
+        for (auto *b : {&beta, &beta0, &resid, &hnorm}) b->alloc(n_inst);
+        const size_t slots = (size_t)n_inst * (kmax + 1);
+        partial.alloc(slots * K.nchunk * 2);
+        H.alloc(slots * (kmax + 1));
+        for (auto *b : {&g, &cs, &sn, &y, &hcol, &hcol2}) b->alloc(slots);
+        vp_d.alloc(kmax + 1);
+        w.alloc(total);
+        K.active = active.p;
+        S = KState{H.p, g.p, cs.p, sn.p, y.p, beta.p, beta0.p, resid.p, hnorm.p,
+                   kconv.p, iters.p, cycle_iters.p, converged.p, 0.0, 3};
+        MSX_HIP(hipHostMalloc((void **)&count_h, sizeof(int), hipHostMallocDefault));
+        set_all_active(nullptr);
+    }
+    ~Krylov() {
+        if (count_h) (void)hipHostFree(count_h);
+    }
+    // active[b] = mask ? mask[b] : 1 (mask is a host array)
+    void set_all_active(const int *mask) {
+        std::vector<int> h(n_inst, 1);
+        if (mask)
+            for (int b = 0; b < n_inst; ++b) h[b] = mask[b] ? 1 : 0;
+        active.upload(h.data(), n_inst, s);
+        MSX_HIP(hipStreamSynchronize(s));
+    }
+    void ensure(int nvec) {
+        if (nvec > kmax + 1) nvec = kmax + 1;
+        if ((int)vp.size() >= nvec) return;
+        while ((int)vp.size() < nvec) {
+            const int add = std::min(32, kmax + 1 - (int)vp.size());
+            blocks.emplace_back((size_t)add * total);
+            for (int i = 0; i < add; ++i) vp.push_back(blocks.back().p + (size_t)i * total);
+        }
+        vp_d.upload(vp.data(), vp.size(), s);
+    }
+    int read_count() {
+        MSX_HIP(hipMemcpyAsync(count_h, count.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        MSX_HIP(hipStreamSynchronize(s));
+        return *count_h;
+    }
+    std::vector<int> host_ints(const DBuf<int> &b) {
+        std::vector<int> h(n_inst);
+        b.download(h.data(), n_inst, s);
+        MSX_HIP(hipStreamSynchronize(s));
+        return h;
+    }
+    std::vector<double> host_doubles(const DBuf<double> &b) {
+        std::vector<double> h(n_inst);
+        b.download(h.data(), n_inst, s);
+        MSX_HIP(hipStreamSynchronize(s));
+        return h;
+    }
+    // Per-instance squared norms of an arbitrary vector with this layout (host result).
+    std::vector<double> norms2(const cx<T> *v) {
+        k_dots<T>(K, vp_d.p, 0, v, 1, partial.p, hcol.p, 0, s);
+        std::vector<cx<double>> h((size_t)n_inst * (kmax + 1));
+        hcol.download(h.data(), h.size(), s);
+        MSX_HIP(hipStreamSynchronize(s));
+        std::vector<double> out(n_inst);
+        for (int b = 0; b < n_inst; ++b) out[b] = h[(size_t)b * (kmax + 1)].re;
+        return out;
+    }
+};
+
+// GMRES(kmax) on op x = rhs for the instances active on entry. x accumulates the Krylov
+// corrections (x += V y each cycle). If x_zero, the first residual is rhs itself (x0 = 0,
+// as in the reference); otherwise r0 = rhs - op(x) (needs TX == T). Restarts continue from
+// the current x until maxiter iterations or convergence.
+template <typename T, typename TX>
+void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_zero,
+           double eps, int window, int maxiter, bool allow_restart) {
+    hipStream_t s = kr.s;
+    kr.S.eps = eps;
+    kr.S.window = window;
+    k_reset(kr.K, kr.S, kr.enabled.p, s);
+    kr.ensure(1);
+    int done = 0;
+    bool first = true;
+    for (;;) {
+        cx<T> *v0 = kr.vp[0];
+        if (x_zero && first) {
+            MSX_HIP(hipMemcpyAsync(v0, rhs, sizeof(cx<T>) * kr.total, hipMemcpyDeviceToDevice, s));
+        } else {
+            if constexpr (std::is_same<T, TX>::value) {
+                op(reinterpret_cast<const cx<T> *>(x), kr.w.p, kr.active.p);
+                k_axpby<T>(kr.K, rhs, kr.w.p, T(-1), v0, kr.active.p, s);
+            } else {
+                throw Error(3, "restart of a mixed-precision inner solve");
+            }
+        }
+        k_dots<T>(kr.K, kr.vp_d.p, 0, v0, 1, kr.partial.p, kr.hcol.p, 0, s);
+        k_cycle_start(kr.K, kr.S, kr.hcol.p, first ? 1 : 0, kr.count.p, s);
+        if (kr.read_count() == 0) break;
+        k_scale<T>(kr.K, v0, v0, kr.beta.p, kr.active.p, s);
+        int alive = 1;
+        for (int k = 0; k < kr.kmax && done < maxiter; ++k) {
+            kr.ensure(k + 2);
+            op(kr.vp[k], kr.w.p, kr.active.p);
+            k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol.p, 0, s);
+            k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol.p, kr.w.p, 0, kr.partial.p, nullptr, s);
+            k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol2.p, 0, s);
+            k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol2.p, kr.w.p, 1, kr.partial.p, kr.hcol.p, s);
+            k_givens(kr.K, k, kr.hcol.p, kr.hcol2.p, kr.S, kr.count.p, s);
+            k_scale<T>(kr.K, kr.w.p, kr.vp[k + 1], kr.hnorm.p, kr.active.p, s);
+            ++done;
+            alive = kr.read_count();
+            if (alive == 0) break;
+        }
+        k_backsolve(kr.K, kr.S, nullptr, s);
+        k_update<T, TX>(kr.K, kr.vp_d.p, kr.cycle_iters.p, kr.y.p, x, nullptr, s);
+        first = false;
+        x_zero = false;
+        if (alive == 0 || done >= maxiter || !allow_restart) break;
+        k_reactivate(kr.K, kr.S, kr.enabled.p, s);
+    }
+    MSX_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace msx
+
+// ---------------------------------------------------------------------------------------
+// Potential table.
+// ---------------------------------------------------------------------------------------
+struct msx_table {
+    int nkx = 0, nky = 0, nz = 0;
+    hipStream_t stream = 0;
+    DBuf<cx<double>> V, Vt;
+    DBuf<cx<float>> Vf, Vtf;
+    DBuf<double> T, A, Vm, lam, Q, QT;
+    DBuf<float> Qf, QTf;
+    DBuf<int> info;
+    bool flipped = false;
+    ~msx_table() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    size_t nG() const { return (size_t)nkx * nky; }
+    void prepare() {
+        const size_t n2 = (size_t)nz * nz;
+        launch_build_h0(nz, T.p, V.p, A.p, stream);
+        launch_eigensystem(nz, A.p, Vm.p, lam.p, Q.p, QT.p, info.p, stream);
+        int h_info = 0;
+        info.download(&h_info, 1, stream);
+        MSX_HIP(hipStreamSynchronize(stream));
+        if (h_info != 0) throw Error(MSX_ERR_EIGEN, "eigensolver did not converge");
+        launch_to_float_real((long long)n2, Q.p, Qf.p, stream);
+        launch_to_float_real((long long)n2, QT.p, QTf.p, stream);
+        launch_to_float((long long)(nG() * nz), V.p, Vf.p, stream);
+        flipped = false;
+    }
+    void ensure_flipped() {
+        if (flipped) return;
+        if (!Vt.p) {
+            Vt.alloc(nG() * nz);
+            Vtf.alloc(nG() * nz);
+        }
+        launch_flip_table(nkx, nky, nz, V.p, Vt.p, stream);
+        launch_to_float((long long)(nG() * nz), Vt.p, Vtf.p, stream);
+        flipped = true;
+    }
+};
+
+// ---------------------------------------------------------------------------------------
+// Batch of conditions.
+// ---------------------------------------------------------------------------------------
+struct msx_batch {
+    msx_table *t = nullptr;
+    int n_inst = 0, max_nc = 0;
+    long long total_ch = 0;
+    std::vector<int> nc, spec;
+    std::vector<long long> off;
+    DBuf<int> nc_d, chx_d, chy_d, spec_d, chmap_d;
+    DBuf<long long> off_d;
+    DBuf<double> e_d, fac_d, P_d, dLdP_d;
+    DBuf<float> facf_d;
+    DBuf<cx<double>> c_d, b_d, a0_d, bsrc_d, psi_d, lam_d;
+    DBuf<cx<float>> cf_d;
+    // work vectors and Krylov workspaces (allocated on first use, reused by later solves)
+    DBuf<cx<double>> source, src, rhs, tmp1, tmp2, tmp3;
+    DBuf<cx<float>> f_rhs, f_tmp1, f_tmp2;
+    std::unique_ptr<Krylov<double>> kr64, kr_outer;
+    std::unique_ptr<Krylov<float>> kr32;
+    std::vector<int> iters, info;
+    std::vector<double> resid;
+    bool factors_ready = false, solved = false;
+
+    long long n_elem() const { return total_ch * t->nz; }
+    Layout layout() const {
+        return Layout{t->nkx, t->nky, t->nz, n_inst, max_nc, nc_d.p, off_d.p, chx_d.p, chy_d.p};
+    }
+    std::vector<long long> kstart() const {
+        std::vector<long long> v(n_inst);
+        for (int b = 0; b < n_inst; ++b) v[b] = off[b] * t->nz;
+        return v;
+    }
+    std::vector<long long> klen() const {
+        std::vector<long long> v(n_inst);
+        for (int b = 0; b < n_inst; ++b) v[b] = (long long)nc[b] * t->nz;
+        return v;
+    }
+    template <typename T>
+    static void need(DBuf<T> &b, size_t n) {
+        if (b.n < n) b.alloc(n);
+    }
+    void build_factors() {
+        launch_preconditioner_factors_qt(total_ch, t->nz, t->Q.p, t->QT.p, t->lam.p, e_d.p, fac_d.p,
+                                         t->stream);
+        launch_to_float_real(total_ch * t->nz, fac_d.p, facf_d.p, t->stream);
+        factors_ready = true;
+    }
+};
+
+namespace {
+
+using namespace msx;
+
+// ---- operators -------------------------------------------------------------------------
+struct Ops {
+    msx_batch &B;
+    msx_table &t;
+    Layout L;
+    bool neumann;
+    const cx<double> *V;  // coupling table (flipped for the transposed system)
+    const cx<float> *Vf;
+    hipStream_t s;
+
+    Ops(msx_batch &b, bool neu, bool transposed)
+        : B(b), t(*b.t), L(b.layout()), neumann(neu), s(b.t->stream) {
+        if (transposed) {
+            t.ensure_flipped();
+            V = t.Vt.p;
+            Vf = t.Vtf.p;
+        } else {
+            V = t.V.p;
+            Vf = t.Vf.p;
+        }
+    }
+    // Gauss-Seidel, FP64: out = M (I + L^-1 U) in
+    void gs(const cx<double> *in, cx<double> *out, const int *act) {
+        cx<double> *u = B.tmp1.p;
+        cx<double> *w1 = neumann ? B.tmp2.p : out;
+        launch_coupling<double>(L, V, in, u, COUPLE_UPPER, act, s);
+        launch_lower_solve(L, V, t.Q.p, t.QT.p, t.lam.p, B.e_d.p, B.c_d.p, B.fac_d.p, u, in, w1,
+                           1.0, act, s);
+        if (neumann) {
+            launch_coupling<double>(L, V, w1, u, COUPLE_UPPER, act, s);
+            launch_lower_solve(L, V, t.Q.p, t.QT.p, t.lam.p, B.e_d.p, B.c_d.p, B.fac_d.p, u, w1,
+                               out, -1.0, act, s);
+        }
+    }
+    // Gauss-Seidel right-hand side: rhs = M L^-1 src (src is overwritten with L^-1 src)
+    void gs_rhs(cx<double> *src, cx<double> *rhs) {
+        launch_lower_solve(L, V, t.Q.p, t.QT.p, t.lam.p, B.e_d.p, B.c_d.p, B.fac_d.p, src, nullptr,
+                           nullptr, 0.0, nullptr, s);
+        if (neumann) {
+            cx<double> *u = B.tmp1.p;
+            launch_coupling<double>(L, V, src, u, COUPLE_UPPER, nullptr, s);
+            launch_lower_solve(L, V, t.Q.p, t.QT.p, t.lam.p, B.e_d.p, B.c_d.p, B.fac_d.p, u, src,
+                               rhs, -1.0, nullptr, s);
+        } else {
+            MSX_HIP(hipMemcpyAsync(rhs, src, sizeof(cx<double>) * B.n_elem(),
+                                   hipMemcpyDeviceToDevice, s));
+        }
+    }
+    // Block Jacobi: out = M (I + D^-1 V) in, in FP64 or FP32
+    template <typename T>
+    void jac(const cx<T> *in, cx<T> *out, const int *act, cx<T> *u, cx<T> *w1buf) {
+        const cx<T> *Vt = table<T>();
+        cx<T> *w1 = neumann ? w1buf : out;
+        launch_coupling<T>(L, Vt, in, u, COUPLE_FULL, act, s);
+        launch_dinv<T>(L, q<T>(), qt<T>(), t.lam.p, B.e_d.p, c<T>(), fac<T>(), u, in, w1, T(1), act,
+                       s);
+        if (neumann) {
+            launch_coupling<T>(L, Vt, w1, u, COUPLE_FULL, act, s);
+            launch_dinv<T>(L, q<T>(), qt<T>(), t.lam.p, B.e_d.p, c<T>(), fac<T>(), u, w1, out,
+                           T(-1), act, s);
+        }
+    }
+    // Block Jacobi right-hand side in FP64: rhs = M D^-1 src
+    void jac_rhs(const cx<double> *src, cx<double> *rhs) {
+        cx<double> *r1 = neumann ? B.tmp2.p : rhs;
+        launch_dinv<double>(L, t.Q.p, t.QT.p, t.lam.p, B.e_d.p, B.c_d.p, B.fac_d.p, src, nullptr,
+                            r1, 1.0, nullptr, s);
+        if (neumann) {
+            launch_coupling<double>(L, V, r1, B.tmp1.p, COUPLE_FULL, nullptr, s);
+            launch_dinv<double>(L, t.Q.p, t.QT.p, t.lam.p, B.e_d.p, B.c_d.p, B.fac_d.p, B.tmp1.p,
+                                r1, rhs, -1.0, nullptr, s);
+        }
+    }
+    template <typename T> const cx<T> *table() const;
+    template <typename T> const T *q() const;
+    template <typename T> const T *qt() const;
+    template <typename T> const cx<T> *c() const;
+    template <typename T> const T *fac() const;
+};
+template <> const cx<double> *Ops::table<double>() const { return V; }
+template <> const cx<float> *Ops::table<float>() const { return Vf; }
+template <> const double *Ops::q<double>() const { return t.Q.p; }
+template <> const float *Ops::q<float>() const { return t.Qf.p; }
+template <> const double *Ops::qt<double>() const { return t.QT.p; }
+template <> const float *Ops::qt<float>() const { return t.QTf.p; }
+template <> const cx<double> *Ops::c<double>() const { return B.c_d.p; }
+template <> const cx<float> *Ops::c<float>() const { return B.cf_d.p; }
+template <> const double *Ops::fac<double>() const { return B.fac_d.p; }
+template <> const float *Ops::fac<float>() const { return B.facf_d.p; }
+
+int krylov_dim(const msx_solve_options &o, int n_inst) {
+    int m = o.restart > 0 ? o.restart : o.maxiter;
+    if (o.restart <= 0 && n_inst > 1) m = std::min(m, 256);
+    return std::max(1, std::min(m, KR_MAX_BASIS - 1));
+}
+
+void check_options(const msx_solve_options *o) {
+    if (!o) throw Error(MSX_ERR_INVALID, "null options");
+    if (o->method != MSX_METHOD_GAUSS_SEIDEL && o->method != MSX_METHOD_JACOBI)
+        throw Error(MSX_ERR_INVALID, "unknown method");
+    if (o->precision != MSX_PRECISION_FP64 && o->precision != MSX_PRECISION_MIXED)
+        throw Error(MSX_ERR_INVALID, "unknown precision");
+    if (o->precision == MSX_PRECISION_MIXED && o->method != MSX_METHOD_JACOBI)
+        throw Error(MSX_ERR_INVALID, "mixed precision runs with the block-Jacobi method");
+    if (!(o->eps > 0)) throw Error(MSX_ERR_INVALID, "eps must be positive");
+    if (o->maxiter <= 0) throw Error(MSX_ERR_INVALID, "maxiter must be positive");
+    if (o->window <= 0) throw Error(MSX_ERR_INVALID, "window must be positive");
+}
+
+// Solve A psi = src for all instances into `x` (FP64). transposed: A^T with the flipped table.
+void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *src_in,
+                  cx<double> *x, bool transposed, std::vector<int> &iters,
+                  std::vector<int> &info, std::vector<double> &resid) {
+    msx_table &t = *B.t;
+    hipStream_t s = t.stream;
+    const size_t n = (size_t)B.n_elem();
+    msx_batch::need(B.src, n);
+    msx_batch::need(B.rhs, n);
+    msx_batch::need(B.tmp1, n);
+    msx_batch::need(B.tmp2, n);
+    if (!B.factors_ready) B.build_factors();
+    Ops ops(B, o.neumann == 1, transposed);
+    MSX_HIP(hipMemcpyAsync(B.src.p, src_in, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, s));
+    MSX_HIP(hipMemsetAsync(x, 0, sizeof(cx<double>) * n, s));
+    const int kdim = krylov_dim(o, B.n_inst);
+    iters.assign(B.n_inst, 0);
+    info.assign(B.n_inst, 0);
+    resid.assign(B.n_inst, 0.0);
+
+    if (o.precision == MSX_PRECISION_FP64) {
+        if (!B.kr64 || B.kr64->kmax != std::min(kdim, KR_MAX_BASIS - 1))
+            B.kr64.reset(new Krylov<double>(B.kstart(), B.klen(), kdim, s));
+        Krylov<double> &kr = *B.kr64;
+        kr.set_all_active(nullptr);
+        Op<double> op;
+        if (o.method == MSX_METHOD_GAUSS_SEIDEL) {
+            ops.gs_rhs(B.src.p, B.rhs.p);
+            op = [&](const cx<double> *in, cx<double> *out, const int *act) { ops.gs(in, out, act); };
+        } else {
+            ops.jac_rhs(B.src.p, B.rhs.p);
+            op = [&](const cx<double> *in, cx<double> *out, const int *act) {
+                ops.jac<double>(in, out, act, B.tmp1.p, B.tmp2.p);
+            };
+        }
+        gmres<double, double>(kr, op, B.rhs.p, x, true, o.eps, o.window, o.maxiter, true);
+        iters = kr.host_ints(kr.iters);
+        auto conv = kr.host_ints(kr.converged);
+        resid = kr.host_doubles(kr.resid);
+        for (int b = 0; b < B.n_inst; ++b) info[b] = conv[b] ? 0 : 1;
+        return;
+    }
+
+    // Mixed precision (block Jacobi): FP32 inner GMRES, FP64 residual and accumulation.
+    msx_batch::need(B.tmp3, n);
+    msx_batch::need(B.f_rhs, n);
+    msx_batch::need(B.f_tmp1, n);
+    msx_batch::need(B.f_tmp2, n);
+    ops.jac_rhs(B.src.p, B.rhs.p);  // rhs64 = M D^-1 s
+    if (!B.kr_outer) B.kr_outer.reset(new Krylov<double>(B.kstart(), B.klen(), 1, s));
+    Krylov<double> &outer = *B.kr_outer;
+    outer.set_all_active(nullptr);
+    const int inner_dim = std::max(2, std::min(kdim, 200));
+    if (!B.kr32 || B.kr32->kmax != inner_dim)
+        B.kr32.reset(new Krylov<float>(B.kstart(), B.klen(), inner_dim, s));
+    Krylov<float> &inner = *B.kr32;
+    Op<float> op32 = [&](const cx<float> *in, cx<float> *out, const int *act) {
+        ops.jac<float>(in, out, act, B.f_tmp1.p, B.f_tmp2.p);
+    };
+    const std::vector<double> rhs_n2 = outer.norms2(B.rhs.p);
+    std::vector<int> act(B.n_inst, 1);
+    const double inner_eps = o.inner_eps > 0 ? o.inner_eps : 1e-5;
+    const int max_refine = o.max_refine > 0 ? o.max_refine : 6;
+    cx<double> *r64 = B.tmp3.p;
+    for (int sweep = 0;; ++sweep) {
+        if (sweep == 0) {
+            MSX_HIP(hipMemcpyAsync(r64, B.rhs.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, s));
+        } else {
+            // r = rhs64 - M (I + D^-1 V) x  in FP64 (tmp1/tmp2 are the FP64 work vectors)
+            // (B.src is free here: the source was consumed by jac_rhs above)
+            outer.set_all_active(act.data());
+            ops.jac<double>(x, B.src.p, outer.active.p, B.tmp1.p, B.tmp2.p);
+            k_axpby<double>(outer.K, B.rhs.p, B.src.p, -1.0, r64, outer.active.p, s);
+        }
+        const std::vector<double> r_n2 = outer.norms2(r64);
+        int n_act = 0;
+        for (int b = 0; b < B.n_inst; ++b) {
+            if (!act[b]) continue;
+            const double rel = rhs_n2[b] > 0 ? std::sqrt(r_n2[b] / rhs_n2[b]) : 0.0;
+            resid[b] = rel;
+            if (rel < o.eps) act[b] = 0;
+            else ++n_act;
+        }
+        if (n_act == 0 || sweep >= max_refine) break;
+        launch_to_float((long long)n, r64, B.f_rhs.p, s);
+        inner.set_all_active(act.data());
+        gmres<float, double>(inner, op32, B.f_rhs.p, x, true, inner_eps, 1,
+                             std::min(o.maxiter, inner_dim), false);
+        auto it = inner.host_ints(inner.iters);
+        for (int b = 0; b < B.n_inst; ++b) iters[b] += it[b];
+    }
+    for (int b = 0; b < B.n_inst; ++b) info[b] = act[b] ? 1 : 0;
+}
+
+int fail(const Error &e) {
+    set_last_error(e.what());
+    return e.code;
+}
+
+// Column-major (nkx, nky, nz) complex -> row-major [g][z] with g = ix*nky + iy.
+void table_from_fortran(int nkx, int nky, int nz, const double *src, std::vector<cx<double>> &dst) {
+    dst.resize((size_t)nkx * nky * nz);
+    for (int k = 0; k < nz; ++k)
+        for (int iy = 0; iy < nky; ++iy)
+            for (int ix = 0; ix < nkx; ++ix) {
+                const size_t f = (size_t)ix + (size_t)nkx * ((size_t)iy + (size_t)nky * k);
+                dst[((size_t)ix * nky + iy) * nz + k] = mk(src[2 * f], src[2 * f + 1]);
+            }
+}
+
+void transpose_square(int n, const double *src, std::vector<double> &dst) {
+    dst.resize((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) dst[(size_t)i * n + j] = src[(size_t)j * n + i];
+}
+
+// Full-grid storage-order batch of one condition (the Fortran path's channel set).
+msx_batch *full_grid_batch(msx_table *t, const double *e_col, const double *wave_b,
+                           const double *wave_c, const double *wave_a) {
+    const int nkx = t->nkx, nky = t->nky, nc = nkx * nky;
+    std::vector<int> chx(nc), chy(nc);
+    std::vector<double> e(nc), b(2 * (size_t)nc), c(2 * (size_t)nc);
+    for (int g = 0; g < nc; ++g) {
+        const int ix = g / nky, iy = g % nky;
+        chx[g] = ix;
+        chy[g] = iy;
+        e[g] = e_col ? e_col[ix + (size_t)nkx * iy] : 0.0;
+        b[2 * g] = wave_b ? wave_b[2 * g] : 0.0;
+        b[2 * g + 1] = wave_b ? wave_b[2 * g + 1] : 0.0;
+        c[2 * g] = wave_c ? wave_c[2 * g] : 0.0;
+        c[2 * g + 1] = wave_c ? wave_c[2 * g + 1] : 0.0;
+    }
+    double a0[2] = {wave_a ? wave_a[0] : 0.0, wave_a ? wave_a[1] : 0.0};
+    msx_batch *B = nullptr;
+    int rc = msx_batch_create(t, 1, &nc, chx.data(), chy.data(), e.data(), b.data(), c.data(), a0,
+                              &B);
+    if (rc != MSX_OK) throw Error(rc, msx_last_error());
+    return B;
+}
+
+struct TableGuard {
+    msx_table *t = nullptr;
+    ~TableGuard() { if (t) msx_table_destroy(t); }
+};
+struct BatchGuard {
+    msx_batch *b = nullptr;
+    ~BatchGuard() { if (b) msx_batch_destroy(b); }
+};
+
+}  // namespace
+
+// =======================================================================================
+// C ABI
+// =======================================================================================
+extern "C" {
+
+const char *msx_last_error(void) { return g_last_error.c_str(); }
+
+int msx_device_info(char *buf, int buflen, int *device_count) {
+    try {
+        int n = 0;
+        MSX_HIP(hipGetDeviceCount(&n));
+        if (device_count) *device_count = n;
+        if (n <= 0) throw Error(MSX_ERR_DEVICE, "no GPU visible");
+        hipDeviceProp_t p;
+        MSX_HIP(hipGetDeviceProperties(&p, 0));
+        if (buf && buflen > 0) {
+            std::strncpy(buf, p.gcnArchName, buflen - 1);
+            buf[buflen - 1] = 0;
+        }
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+void msx_default_options(msx_solve_options *opt, int nsig) {
+    if (!opt) return;
+    opt->method = MSX_METHOD_GAUSS_SEIDEL;
+    opt->precision = MSX_PRECISION_FP64;
+    opt->neumann = 1;
+    opt->eps = 0.5 * std::pow(10.0, -(double)nsig);  // multiscat.f90:35
+    opt->maxiter = 2000;                             // multiscat_gmres.f90:7,124
+    opt->restart = 0;
+    opt->window = 3;                                 // multiscat_gmres.f90:125
+    opt->inner_eps = 1e-5;
+    opt->max_refine = 6;
+}
+
+int msx_table_create(int nkx, int nky, int nz, const double *potential, const double *kinetic,
+                     msx_table **out) {
+    try {
+        if (!out || !potential || !kinetic) throw Error(MSX_ERR_INVALID, "null argument");
+        *out = nullptr;
+        if (nkx <= 0 || nky <= 0 || nz <= 0) throw Error(MSX_ERR_DIMS, "non-positive dimension");
+        if (nz < 2) throw Error(MSX_ERR_INVALID, "nz must be at least 2");
+        std::unique_ptr<msx_table> t(new msx_table);
+        t->nkx = nkx;
+        t->nky = nky;
+        t->nz = nz;
+        MSX_HIP(hipStreamCreate(&t->stream));
+        const size_t nG = (size_t)nkx * nky, n2 = (size_t)nz * nz;
+        t->V.alloc(nG * nz);
+        t->Vf.alloc(nG * nz);
+        t->V.upload(reinterpret_cast<const cx<double> *>(potential), nG * nz, t->stream);
+        t->T.alloc(n2);
+        t->T.upload(kinetic, n2, t->stream);
+        for (auto *b : {&t->A, &t->Vm, &t->Q, &t->QT}) b->alloc(n2);
+        t->lam.alloc(nz);
+        t->Qf.alloc(n2);
+        t->QTf.alloc(n2);
+        t->info.alloc(1);
+        t->prepare();
+        *out = t.release();
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+int msx_table_prepare(msx_table *t) {
+    try {
+        if (!t) throw Error(MSX_ERR_INVALID, "null table");
+        t->prepare();
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+int msx_table_destroy(msx_table *t) {
+    delete t;
+    return MSX_OK;
+}
+
+int msx_table_eigensystem(const msx_table *t, double *lam, double *Q) {
+    try {
+        if (!t || !lam || !Q) throw Error(MSX_ERR_INVALID, "null argument");
+        t->lam.download(lam, t->nz, t->stream);
+        t->Q.download(Q, (size_t)t->nz * t->nz, t->stream);
+        MSX_HIP(hipStreamSynchronize(t->stream));
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+int msx_batch_create(msx_table *t, int n_cond, const int *n_channels, const int *chx,
+                     const int *chy, const double *e, const double *wave_b, const double *wave_c,
+                     const double *wave_a0, msx_batch **out) {
+    try {
+        if (!t || !out || !n_channels || !chx || !chy || !e || !wave_b || !wave_c || !wave_a0)
+            throw Error(MSX_ERR_INVALID, "null argument");
+        *out = nullptr;
+        if (n_cond <= 0) throw Error(MSX_ERR_DIMS, "no conditions");
+        std::unique_ptr<msx_batch> B(new msx_batch);
+        B->t = t;
+        B->n_inst = n_cond;
+        B->nc.assign(n_channels, n_channels + n_cond);
+        B->off.resize(n_cond);
+        B->spec.resize(n_cond);
+        long long tot = 0;
+        const size_t nG = t->nG();
+        std::vector<int> chmap((size_t)n_cond * nG, -1);
+        for (int b = 0; b < n_cond; ++b) {
+            if (B->nc[b] <= 0) throw Error(MSX_ERR_DIMS, "condition without channels");
+            if ((size_t)B->nc[b] > nG) throw Error(MSX_ERR_INVALID, "more channels than table entries");
+            B->off[b] = tot;
+            B->max_nc = std::max(B->max_nc, B->nc[b]);
+            int sp = -1;
+            for (int j = 0; j < B->nc[b]; ++j) {
+                const int x = chx[tot + j], y = chy[tot + j];
+                if (x < 0 || x >= t->nkx || y < 0 || y >= t->nky)
+                    throw Error(MSX_ERR_INVALID, "channel index outside the table");
+                int &slot = chmap[(size_t)b * nG + (size_t)x * t->nky + y];
+                if (slot >= 0) throw Error(MSX_ERR_INVALID, "duplicate channel in a condition");
+                slot = j;
+                if (sp < 0 && x == 0 && y == 0) sp = j;
+            }
+            if (sp < 0) throw Error(MSX_ERR_NO_SPECULAR, "specular channel missing");
+            B->spec[b] = sp;
+            tot += B->nc[b];
+        }
+        B->total_ch = tot;
+        hipStream_t s = t->stream;
+        B->nc_d.alloc(n_cond);
+        B->nc_d.upload(B->nc.data(), n_cond, s);
+        B->off_d.alloc(n_cond);
+        B->off_d.upload(B->off.data(), n_cond, s);
+        B->spec_d.alloc(n_cond);
+        B->spec_d.upload(B->spec.data(), n_cond, s);
+        B->chx_d.alloc(tot);
+        B->chx_d.upload(chx, tot, s);
+        B->chy_d.alloc(tot);
+        B->chy_d.upload(chy, tot, s);
+        B->chmap_d.alloc(chmap.size());
+        B->chmap_d.upload(chmap.data(), chmap.size(), s);
+        B->e_d.alloc(tot);
+        B->e_d.upload(e, tot, s);
+        B->c_d.alloc(tot);
+        B->c_d.upload(reinterpret_cast<const cx<double> *>(wave_c), tot, s);
+        B->b_d.alloc(tot);
+        B->b_d.upload(reinterpret_cast<const cx<double> *>(wave_b), tot, s);
+        B->a0_d.alloc(n_cond);
+        B->a0_d.upload(reinterpret_cast<const cx<double> *>(wave_a0), n_cond, s);
+        std::vector<cx<double>> bsrc(n_cond);
+        for (int b = 0; b < n_cond; ++b) {
+            const long long g = B->off[b] + B->spec[b];
+            bsrc[b] = mk(wave_b[2 * g], wave_b[2 * g + 1]);
+        }
+        B->bsrc_d.alloc(n_cond);
+        B->bsrc_d.upload(bsrc.data(), n_cond, s);
+        B->cf_d.alloc(tot);
+        {
+            std::vector<cx<float>> cf(tot);
+            for (long long i = 0; i < tot; ++i)
+                cf[i] = mk((float)wave_c[2 * i], (float)wave_c[2 * i + 1]);
+            B->cf_d.upload(cf.data(), tot, s);
+            MSX_HIP(hipStreamSynchronize(s));
+        }
+        B->fac_d.alloc((size_t)tot * t->nz);
+        B->facf_d.alloc((size_t)tot * t->nz);
+        B->psi_d.alloc((size_t)tot * t->nz);
+        B->P_d.alloc(tot);
+        MSX_HIP(hipStreamSynchronize(s));
+        *out = B.release();
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+int msx_batch_destroy(msx_batch *b) {
+    delete b;
+    return MSX_OK;
+}
+
+int msx_batch_solve(msx_batch *B, const msx_solve_options *opt) {
+    try {
+        if (!B) throw Error(MSX_ERR_INVALID, "null batch");
+        check_options(opt);
+        msx_table &t = *B->t;
+        B->build_factors();  // the table's eigensystem may have been re-prepared
+        msx_batch::need(B->src, (size_t)B->n_elem());
+        const Layout L = B->layout();
+        msx_batch::need(B->source, (size_t)B->n_elem());
+        launch_set_source(L, B->spec_d.p, B->bsrc_d.p, B->source.p, t.stream);
+        solve_system(*B, *opt, B->source.p, B->psi_d.p, false, B->iters, B->info, B->resid);
+        launch_intensities(L, B->psi_d.p, B->b_d.p, B->a0_d.p, B->spec_d.p, B->P_d.p, t.stream);
+        MSX_HIP(hipStreamSynchronize(t.stream));
+        B->solved = true;
+        for (int b = 0; b < B->n_inst; ++b)
+            if (B->info[b]) return MSX_ERR_NOT_CONVERGED;
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+int msx_batch_state(const msx_batch *B, double *psi) {
+    try {
+        if (!B || !psi) throw Error(MSX_ERR_INVALID, "null argument");
+        B->psi_d.download(reinterpret_cast<cx<double> *>(psi), (size_t)B->n_elem(), B->t->stream);
+        MSX_HIP(hipStreamSynchronize(B->t->stream));
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+int msx_batch_intensities(const msx_batch *B, double *P) {
+    try {
+        if (!B || !P) throw Error(MSX_ERR_INVALID, "null argument");
+        B->P_d.download(P, (size_t)B->total_ch, B->t->stream);
+        MSX_HIP(hipStreamSynchronize(B->t->stream));
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+int msx_batch_stats(const msx_batch *B, int *iterations, int *info, double *residual) {
+    if (!B) {
+        set_last_error("null batch");
+        return MSX_ERR_INVALID;
+    }
+    for (int b = 0; b < B->n_inst; ++b) {
+        if (iterations) iterations[b] = b < (int)B->iters.size() ? B->iters[b] : 0;
+        if (info) info[b] = b < (int)B->info.size() ? B->info[b] : 0;
+        if (residual) residual[b] = b < (int)B->resid.size() ? B->resid[b] : 0.0;
+    }
+    return MSX_OK;
+}
+
+int msx_batch_device_state(const msx_batch *B, void **psi_device) {
+    if (!B || !psi_device) {
+        set_last_error("null argument");
+        return MSX_ERR_INVALID;
+    }
+    *psi_device = B->psi_d.p;
+    return MSX_OK;
+}
+
+int msx_batch_total_channels(const msx_batch *B, long long *total) {
+    if (!B || !total) {
+        set_last_error("null argument");
+        return MSX_ERR_INVALID;
+    }
+    *total = B->total_ch;
+    return MSX_OK;
+}
+
+int msx_batch_adjoint(msx_batch *B, const msx_solve_options *opt, const double *dL_dP,
+                      double *grad_potential) {
+    try {
+        if (!B || !dL_dP || !grad_potential) throw Error(MSX_ERR_INVALID, "null argument");
+        if (!B->solved) throw Error(MSX_ERR_INVALID, "adjoint before a forward solve");
+        check_options(opt);
+        msx_table &t = *B->t;
+        hipStream_t s = t.stream;
+        const Layout L = B->layout();
+        const size_t n = (size_t)B->n_elem();
+        msx_batch::need(B->dLdP_d, (size_t)B->total_ch);
+        B->dLdP_d.upload(dL_dP, (size_t)B->total_ch, s);
+        DBuf<cx<double>> g(n), lam(n);
+        launch_adjoint_source(L, B->spec_d.p, B->psi_d.p, B->b_d.p, B->a0_d.p, B->dLdP_d.p, g.p, s);
+        std::vector<int> it, inf;
+        std::vector<double> res;
+        solve_system(*B, *opt, g.p, lam.p, true, it, inf, res);
+        const size_t nG = t.nG();
+        DBuf<cx<double>> grad(nG * t.nz);
+        MSX_HIP(hipMemsetAsync(grad.p, 0, sizeof(cx<double>) * nG * t.nz, s));
+        launch_potential_gradient_map(L, B->chmap_d.p, lam.p, B->psi_d.p, grad.p, s);
+        grad.download(reinterpret_cast<cx<double> *>(grad_potential), nG * t.nz, s);
+        MSX_HIP(hipStreamSynchronize(s));
+        for (int b = 0; b < B->n_inst; ++b)
+            if (inf[b]) return MSX_ERR_NOT_CONVERGED;
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+// ---- drop-in group ----------------------------------------------------------------------
+
+int msx_run_multiscat_fortran_ex(int flag, int nsig, int nkx, int nky, int nz, const double *V,
+                                 const double *e, const double *wave_a, const double *wave_b,
+                                 const double *wave_c, const double *T, double *out,
+                                 int *iterations, double *residual) {
+    try {
+        if (nkx <= 0 || nky <= 0 || nz <= 0) return MSX_ERR_DIMS;  // bindings.f90:42-45
+        if (!V || !e || !wave_a || !wave_b || !wave_c || !T || !out)
+            throw Error(MSX_ERR_INVALID, "null argument");
+        std::vector<cx<double>> Vr;
+        table_from_fortran(nkx, nky, nz, V, Vr);
+        std::vector<double> Tr;
+        transpose_square(nz, T, Tr);
+        TableGuard tg;
+        int rc = msx_table_create(nkx, nky, nz, reinterpret_cast<const double *>(Vr.data()),
+                                  Tr.data(), &tg.t);
+        if (rc != MSX_OK) return rc;
+        BatchGuard bg;
+        bg.b = full_grid_batch(tg.t, e, wave_b, wave_c, wave_a);
+        msx_solve_options o;
+        msx_default_options(&o, nsig);
+        o.neumann = flag == 1 ? 1 : 0;  // multiscat_gmres.f90:469
+        rc = msx_batch_solve(bg.b, &o);
+        if (rc != MSX_OK && rc != MSX_ERR_NOT_CONVERGED) return rc;
+        const size_t nG = (size_t)nkx * nky;
+        std::vector<cx<double>> psi(nG * nz);
+        int r2 = msx_batch_state(bg.b, reinterpret_cast<double *>(psi.data()));
+        if (r2 != MSX_OK) return r2;
+        // (nkx, nky, nz) column-major output (multiscat.f90:53-59)
+        for (int k = 0; k < nz; ++k)
+            for (int iy = 0; iy < nky; ++iy)
+                for (int ix = 0; ix < nkx; ++ix) {
+                    const size_t f = (size_t)ix + (size_t)nkx * ((size_t)iy + (size_t)nky * k);
+                    const cx<double> v = psi[((size_t)ix * nky + iy) * nz + k];
+                    out[2 * f] = v.re;
+                    out[2 * f + 1] = v.im;
+                }
+        if (iterations) *iterations = bg.b->iters.empty() ? 0 : bg.b->iters[0];
+        if (residual) *residual = bg.b->resid.empty() ? 0.0 : bg.b->resid[0];
+        return rc == MSX_ERR_NOT_CONVERGED ? MSX_ERR_NOT_CONVERGED : MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+int msx_run_multiscat_fortran(int flag, int nsig, int nkx, int nky, int nz, const double *V,
+                              const double *e, const double *wave_a, const double *wave_b,
+                              const double *wave_c, const double *T, double *out) {
+    return msx_run_multiscat_fortran_ex(flag, nsig, nkx, nky, nz, V, e, wave_a, wave_b, wave_c, T,
+                                        out, nullptr, nullptr);
+}
+
+int msx_debug_build_preconditioner(int nkx, int nky, int nz, const double *V, const double *e,
+                                   const double *T, double *eigenvalues, double *factors,
+                                   double *eigenvectors) {
+    try {
+        if (nkx <= 0 || nky <= 0 || nz <= 0) return MSX_ERR_DIMS;
+        if (!V || !e || !T || !eigenvalues || !factors || !eigenvectors)
+            throw Error(MSX_ERR_INVALID, "null argument");
+        std::vector<cx<double>> Vr;
+        table_from_fortran(nkx, nky, nz, V, Vr);
+        std::vector<double> Tr;
+        transpose_square(nz, T, Tr);
+        TableGuard tg;
+        int rc = msx_table_create(nkx, nky, nz, reinterpret_cast<const double *>(Vr.data()),
+                                  Tr.data(), &tg.t);
+        if (rc != MSX_OK) return rc;
+        BatchGuard bg;
+        bg.b = full_grid_batch(tg.t, e, nullptr, nullptr, nullptr);
+        bg.b->build_factors();
+        hipStream_t s = tg.t->stream;
+        tg.t->lam.download(eigenvalues, nz, s);
+        // (nz, nz) column-major: element (z, m) = Q[z][m] = QT[m][z] row-major
+        tg.t->QT.download(eigenvectors, (size_t)nz * nz, s);
+        // (nz, nkx*nky) column-major: element (z, j) = fac[j][z]
+        bg.b->fac_d.download(factors, (size_t)nz * nkx * nky, s);
+        MSX_HIP(hipStreamSynchronize(s));
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+int msx_debug_apply_upper_block(int nkx, int nky, int nz, const double *V, const double *x,
+                                double *y) {
+    try {
+        if (nkx <= 0 || nky <= 0 || nz <= 0) return MSX_ERR_DIMS;
+        if (!V || !x || !y) throw Error(MSX_ERR_INVALID, "null argument");
+        std::vector<cx<double>> Vr;
+        table_from_fortran(nkx, nky, nz, V, Vr);
+        std::vector<double> Tr((size_t)nz * nz, 0.0);
+        for (int i = 0; i < nz; ++i) Tr[(size_t)i * nz + i] = 1.0;
+        TableGuard tg;
+        int rc = msx_table_create(nkx, nky, nz, reinterpret_cast<const double *>(Vr.data()),
+                                  Tr.data(), &tg.t);
+        if (rc != MSX_OK) return rc;
+        BatchGuard bg;
+        bg.b = full_grid_batch(tg.t, nullptr, nullptr, nullptr, nullptr);
+        const size_t n = (size_t)nkx * nky * nz;
+        DBuf<cx<double>> xin(n), yout(n);
+        hipStream_t s = tg.t->stream;
+        xin.upload(reinterpret_cast<const cx<double> *>(x), n, s);
+        launch_coupling<double>(bg.b->layout(), tg.t->V.p, xin.p, yout.p, COUPLE_UPPER, nullptr, s);
+        yout.download(reinterpret_cast<cx<double> *>(y), n, s);
+        MSX_HIP(hipStreamSynchronize(s));
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+int msx_debug_solve_lower_block(int nkx, int nky, int nz, const double *V, const double *wave_c,
+                                const double *e, const double *T, const double *x, double *y) {
+    try {
+        if (nkx <= 0 || nky <= 0 || nz <= 0) return MSX_ERR_DIMS;
+        if (!V || !wave_c || !e || !T || !x || !y) throw Error(MSX_ERR_INVALID, "null argument");
+        std::vector<cx<double>> Vr;
+        table_from_fortran(nkx, nky, nz, V, Vr);
+        std::vector<double> Tr;
+        transpose_square(nz, T, Tr);
+        TableGuard tg;
+        int rc = msx_table_create(nkx, nky, nz, reinterpret_cast<const double *>(Vr.data()),
+                                  Tr.data(), &tg.t);
+        if (rc != MSX_OK) return rc;
+        BatchGuard bg;
+        bg.b = full_grid_batch(tg.t, e, nullptr, wave_c, nullptr);
+        bg.b->build_factors();
+        const size_t n = (size_t)nkx * nky * nz;
+        DBuf<cx<double>> xin(n);
+        hipStream_t s = tg.t->stream;
+        xin.upload(reinterpret_cast<const cx<double> *>(x), n, s);
+        launch_lower_solve(bg.b->layout(), tg.t->V.p, tg.t->Q.p, tg.t->QT.p, tg.t->lam.p,
+                           bg.b->e_d.p, bg.b->c_d.p, bg.b->fac_d.p, xin.p, nullptr, nullptr, 0.0,
+                           nullptr, s);
+        xin.download(reinterpret_cast<cx<double> *>(y), n, s);
+        MSX_HIP(hipStreamSynchronize(s));
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+}  // extern "C"
