@@ -1,0 +1,208 @@
+"""Throughput benchmark of the batched close-coupling solve on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-baselines]
+
+Workload (one "step" = one batch for one potential table): the hexagonal a = 3.16 A table with
+31 x 31 Fourier components (Morse V_00, seeded corrugation, sigma 0.4 A), 150 GLL nodes on
+[-1.5, 8] A, 256 seeded incidence conditions (E ~ U(10, 100) meV, polar ~ U(0, 60) deg,
+azimuth ~ U(0, 60) deg) each with its own channel count (|K+G|^2 <= 6 k^2, clipped to
+[200, 1200]). A step recomputes the table's eigensystem and DFT tables on the GPU and solves all
+256 conditions to eps = 5e-11 with block-Jacobi GMRES in mixed precision (FP32 inner solves
+on the matrix cores, FP64 residual refinement), then forms P = |S|^2 in FP64. Inputs are resident
+in HBM before the timed region.
+
+Reported next to the headline value:
+  * plain FP64 GPU baselines: the drop-in path (reference algorithm, one call per condition,
+    sampled) and the FP64 batched solvers (Gauss-Seidel split and block Jacobi);
+  * accuracy: max |dP| of the mixed solve against the FP64 GPU solve of the whole batch and
+    against the CPU oracle on the sampled conditions;
+  * roofline of the kernel with the largest share of the timed region, from HIP events on the
+    library's stream (algorithmic flops or bytes per launch / average launch time);
+  * cpu_baseline: the FP64 C restatement of the reference algorithm (oracle/) on a sample of
+    the same conditions, one condition per thread.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}  # MI355X dense matrix peaks
+PEAK_HBM_GBS = 8000.0
+METRIC = "S-matrix solves/s at |dP|<=1e-6, mixed+batched (256 uneven conditions)"
+
+
+def _sync():
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--conditions", type=int, default=256)
+    ap.add_argument("--diag", default="auto", choices=["auto", "vector"])
+    ap.add_argument("--skip-baselines", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: every box here has a single MI355X and no "
+              "multi-GPU figure is produced (see DESIGN.md for how the batch would shard).",
+              file=sys.stderr)
+        return 2
+
+    from multiscat_amd import _lib
+    from multiscat_amd import batch as BT
+    from multiscat_amd import workloads as W
+
+    w = W.hex_batch(n_conditions=args.conditions)
+    counts = np.array(w.channel_counts())
+    nkx, nky, nz = w.potential.shape
+    tab = BT.Table(w.potential, w.kinetic)
+    bt = BT.Batch(tab, w.conditions)
+    eps = 5e-11
+    mixed = BT.SolveOptions(method="jacobi", precision="mixed", eps=eps, diag=args.diag)
+
+    def step(opts):
+        tab.prepare()
+        bt.solve(opts)
+
+    for _ in range(args.warmup):
+        step(mixed)
+    _lib.profile_enable(True)
+    _sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(mixed)
+    _sync()
+    elapsed = time.perf_counter() - t0
+    prof = _lib.profile_report()
+    _lib.profile_enable(False)
+    P_mixed = bt.intensities_flat()
+    st = bt.stats()
+    value = args.conditions * args.steps / elapsed
+
+    # roofline of the dominant instrumented kernel
+    kern = max(prof.items(), key=lambda kv: kv[1]["ms"]) if prof else None
+    roofline = None
+    if kern:
+        name, d = kern
+        avg_s = d["ms"] / 1e3 / max(d["count"], 1)
+        per_launch_flops = d["flops"] / max(d["count"], 1)
+        per_launch_bytes = d["bytes"] / max(d["count"], 1)
+        prec = "f32" if name.endswith("f32") else "f64"
+        if name.startswith("arnoldi"):
+            ach = per_launch_bytes / avg_s / 1e9
+            roofline = {"kernel": name, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS,
+                        "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": None}
+        else:
+            ach = per_launch_flops / avg_s / 1e12
+            roofline = {"kernel": name, "bound": "mfma", "achieved": ach,
+                        "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
+                        "frac": ach / PEAK_TFLOPS[prec], "traffic": None}
+        roofline.update({"launches": d["count"], "avg_launch_us": avg_s * 1e6,
+                         "algorithmic_flops_per_launch": per_launch_flops,
+                         "algorithmic_bytes_per_launch": per_launch_bytes,
+                         "share_of_step": d["ms"] / 1e3 / elapsed})
+    kernels = {k: {"ms_per_step": v["ms"] / args.steps, "launches_per_step": v["count"] / args.steps}
+               for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 inner (c64, MFMA) + fp64 refinement (c128)", "data": "synthetic",
+        "config": {"workload": "hex31x31_nz150_256cond_uneven", "conditions": args.conditions,
+                   "table": [nkx, nky], "nz": nz, "channels_min": int(counts.min()),
+                   "channels_mean": float(counts.mean()), "channels_max": int(counts.max()),
+                   "eps": eps, "method": "block-Jacobi GMRES (Neumann), DFT coupling"},
+        "iterations": {"mean": float(st["iterations"].mean()), "max": int(st["iterations"].max())},
+        "roofline": roofline,
+        "kernels": kernels,
+    }
+
+    if not args.skip_baselines:
+        base = {}
+        fp64 = {}
+        for label, opts in (("jacobi_fp64_batched", BT.SolveOptions(method="jacobi", eps=eps, diag=args.diag)),
+                            ("gauss_seidel_fp64_batched", BT.SolveOptions(eps=eps))):
+            step(opts)  # warm
+            _sync()
+            t0 = time.perf_counter()
+            step(opts)
+            _sync()
+            dt = time.perf_counter() - t0
+            base[label] = {"solves_per_s": args.conditions / dt}
+            fp64[label] = bt.intensities_flat()
+        ref64 = fp64["jacobi_fp64_batched"]
+        dP_fp64 = float(np.abs(P_mixed - ref64).max())
+        dP_methods = float(np.abs(fp64["gauss_seidel_fp64_batched"] - ref64).max())
+        # plain FP64 drop-in path: reference algorithm, one call per condition (sampled)
+        sample = list(range(min(4, args.conditions)))
+        t0 = time.perf_counter()
+        for i in sample:
+            single = BT.Batch(tab, [w.conditions[i]])
+            single.solve(BT.SolveOptions(eps=eps))
+            single.close()
+        dt = time.perf_counter() - t0
+        base["plain_fp64_single_call"] = {"solves_per_s": len(sample) / dt,
+                                          "sample": f"{len(sample)} conditions, one call each"}
+        out["gpu_baselines"] = base
+        out["speedup_vs_plain_fp64_single_call"] = value / base["plain_fp64_single_call"]["solves_per_s"]
+        out["speedup_vs_fp64_batched_gs"] = value / base["gauss_seidel_fp64_batched"]["solves_per_s"]
+        out["speedup_vs_fp64_batched_jacobi"] = value / base["jacobi_fp64_batched"]["solves_per_s"]
+        out["accuracy"] = {"max_abs_dP_mixed_vs_fp64_gpu": dP_fp64,
+                           "max_abs_dP_gs_vs_jacobi_fp64": dP_methods}
+
+        # CPU baseline: the oracle (C restatement of the reference GMRES), threads over conditions
+        try:
+            from oracle import oracle
+
+            threads = max(1, min(args.cpu_sample, os.cpu_count() or 1, 16))
+            sample = list(range(min(args.cpu_sample, args.conditions)))
+
+            def run(i):
+                c = w.conditions[i]
+                psi = oracle.solve(1, 0, w.potential, c.e, c.wave_a, c.wave_b, c.wave_c, w.kinetic,
+                                   channels=c.channels, eps=eps)
+                return oracle.intensities(psi, c.wave_a, c.wave_b)[c.channels[0], c.channels[1]]
+
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(max_workers=threads) as ex:
+                Ps = list(ex.map(run, sample))
+            dt = time.perf_counter() - t0
+            off = np.concatenate(([0], np.cumsum(counts)))
+            dP_oracle = max(float(np.abs(P_mixed[off[i]:off[i + 1]] - Ps[k]).max())
+                            for k, i in enumerate(sample))
+            out["cpu_baseline"] = {"value": len(sample) / dt, "unit": "solves/s", "cores": threads,
+                                   "kind": "port",
+                                   "sample": f"first {len(sample)} of the {args.conditions} "
+                                             f"conditions, FP64 GS-GMRES restatement, "
+                                             f"{threads} threads x 1 condition"}
+            out["accuracy"]["max_abs_dP_mixed_vs_oracle_sample"] = dP_oracle
+        except Exception as exc:  # the oracle is test infrastructure; report its absence
+            out["cpu_baseline"] = {"value": None, "unit": "solves/s", "cores": 0, "kind": "port",
+                                   "sample": f"unavailable: {exc}"}
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

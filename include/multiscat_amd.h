@@ -128,6 +128,24 @@ int msx_debug_solve_lower_block(int nkx, int nky, int nz, const double *potentia
                                 const double *parallel_kinetic_energy, const double *state_in,
                                 double *state_out);
 
+/* Debug face of the channel coupling in the style of the debug group above (extension):
+ * y_j = sum_i V[G_j - G_i] x_i over i > j (mode 0), i != j (mode 1) or i < j (mode 2) for an
+ * explicit channel list; precision 0 = FP64, 1 = FP32 (x, y still passed as FP64);
+ * algorithm 0 = direct sum, 1 = DFT form (mode 1 only). V is C order (nkx, nky, nz) here,
+ * states are (nc, nz) row-major. */
+int msx_debug_apply_coupling(int nkx, int nky, int nz, const double *potential, int nc,
+                             const int *chx, const int *chy, const double *state_in,
+                             double *state_out, int mode, int precision, int algorithm);
+
+/* Debug face of the diagonal blocks (extension): y_j = D_j^-1 x_j (inverse = 1) or D_j x_j
+ * (inverse = 0) with D_j = T + diag(Re V_00) + e_j - c_j e_N e_N^T, for nc channels
+ * (multiscat/multiscat/_scipy.py:249-354). potential00: complex[nz] (V_00(z)); kinetic:
+ * nz x nz row-major; states (nc, nz) row-major complex. variant 0 = matrix-core GEMMs
+ * (nz <= 320), 1 = vector ALU; precision 0 = FP64, 1 = FP32. */
+int msx_debug_apply_diagonal(int nz, const double *potential00, const double *kinetic, int nc,
+                             const double *e, const double *wave_c, const double *state_in,
+                             double *state_out, int inverse, int precision, int variant);
+
 /* ---------------------------------------------------------------------------------------
  * (2) Batched group (extension; no counterpart in the reference).
  * ------------------------------------------------------------------------------------- */
@@ -148,6 +166,9 @@ typedef struct {
     int window;       /* reference: 3                                                       */
     double inner_eps; /* mixed precision: relative tolerance of each FP32 inner solve       */
     int max_refine;   /* mixed precision: FP64 refinement sweeps allowed                    */
+    int coupling;     /* full coupling: 0 = DFT form on the matrix cores when the table fits
+                         (<= 32 x 32), 1 = direct sum                                      */
+    int diag;         /* all-channel D^-1: 0 = matrix-core GEMMs when nz <= 320, 1 = vector */
 } msx_solve_options;
 
 /* Fills `opt` with the reference's settings for a given nsig (GS, FP64, window 3, 2000). */
@@ -199,6 +220,16 @@ int msx_batch_total_channels(const msx_batch *batch, long long *total);
  * through the off-diagonal coupling and through Re V_00 in the diagonal blocks. */
 int msx_batch_adjoint(msx_batch *batch, const msx_solve_options *opt, const double *dL_dP,
                       double *grad_potential);
+
+/* Instrumentation (extension): when enabled, the main kernel launches are bracketed by HIP
+ * events on the library's stream and carry their algorithmic work (flops, bytes) counted on
+ * the host. msx_profile_read sums them per kernel name; msx_profile_names lists the names
+ * seen (comma separated). */
+int msx_profile_enable(int on);
+int msx_profile_reset(void);
+int msx_profile_read(const char *kernel, long long *count, double *total_ms, double *flops,
+                     double *bytes);
+int msx_profile_names(char *buf, int buflen);
 
 #ifdef __cplusplus
 }

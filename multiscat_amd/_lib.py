@@ -32,11 +32,13 @@ EXPORTED = (
     "msx_run_multiscat_fortran_ex", "msx_get_perpendicular_kinetic_difference",
     "msx_get_parallel_kinetic_energy", "msx_get_lobatto_weights", "msx_get_abc_arrays",
     "msx_debug_build_preconditioner", "msx_debug_apply_upper_block",
-    "msx_debug_solve_lower_block", "msx_default_options", "msx_table_create",
+    "msx_debug_solve_lower_block", "msx_debug_apply_coupling", "msx_debug_apply_diagonal",
+    "msx_default_options", "msx_table_create",
     "msx_table_prepare", "msx_table_destroy", "msx_table_eigensystem", "msx_batch_create",
     "msx_batch_destroy", "msx_batch_solve", "msx_batch_state", "msx_batch_intensities",
     "msx_batch_stats", "msx_batch_device_state", "msx_batch_total_channels",
-    "msx_batch_adjoint",
+    "msx_batch_adjoint", "msx_profile_enable", "msx_profile_reset", "msx_profile_read",
+    "msx_profile_names",
 )
 
 
@@ -53,6 +55,8 @@ class SolveOptions(C.Structure):
         ("window", C.c_int),
         ("inner_eps", C.c_double),
         ("max_refine", C.c_int),
+        ("coupling", C.c_int),
+        ("diag", C.c_int),
     ]
 
 
@@ -84,6 +88,8 @@ def _declare(lib):
         "msx_debug_build_preconditioner": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp]),
         "msx_debug_apply_upper_block": (i32, [i32, i32, i32, vp, vp, vp]),
         "msx_debug_solve_lower_block": (i32, [i32, i32, i32, vp, vp, vp, vp, vp, vp]),
+        "msx_debug_apply_coupling": (i32, [i32, i32, i32, vp, i32, vp, vp, vp, vp, i32, i32, i32]),
+        "msx_debug_apply_diagonal": (i32, [i32, vp, vp, i32, vp, vp, vp, vp, i32, i32, i32]),
         "msx_default_options": (None, [C.POINTER(SolveOptions), i32]),
         "msx_table_create": (i32, [i32, i32, i32, vp, vp, pp]),
         "msx_table_prepare": (i32, [vp]),
@@ -98,6 +104,11 @@ def _declare(lib):
         "msx_batch_device_state": (i32, [vp, pp]),
         "msx_batch_total_channels": (i32, [vp, C.POINTER(ll)]),
         "msx_batch_adjoint": (i32, [vp, C.POINTER(SolveOptions), vp, vp]),
+        "msx_profile_enable": (i32, [i32]),
+        "msx_profile_reset": (i32, []),
+        "msx_profile_read": (i32, [C.c_char_p, C.POINTER(ll), C.POINTER(f64), C.POINTER(f64),
+                                   C.POINTER(f64)]),
+        "msx_profile_names": (i32, [C.c_char_p, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -137,6 +148,23 @@ def default_options(nsig: int) -> SolveOptions:
     o = SolveOptions()
     lib().msx_default_options(C.byref(o), int(nsig))
     return o
+
+
+def profile_enable(on: bool = True) -> None:
+    lib().msx_profile_enable(1 if on else 0)
+    lib().msx_profile_reset()
+
+
+def profile_report() -> dict:
+    """{kernel: {"count", "ms", "flops", "bytes"}} summed over the instrumented launches."""
+    buf = C.create_string_buffer(4096)
+    lib().msx_profile_names(buf, 4096)
+    out = {}
+    for name in filter(None, buf.value.decode().split(",")):
+        n, ms, fl, by = C.c_longlong(0), C.c_double(0), C.c_double(0), C.c_double(0)
+        lib().msx_profile_read(name.encode(), C.byref(n), C.byref(ms), C.byref(fl), C.byref(by))
+        out[name] = {"count": n.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
+    return out
 
 
 def device_info():

@@ -30,6 +30,8 @@ class SolveOptions:
     window: int = 3
     inner_eps: float = 1e-5
     max_refine: int = 6
+    coupling: str = "auto"  # or "direct"
+    diag: str = "auto"  # or "vector"
 
     @classmethod
     def from_nsig(cls, nsig: int, **kw) -> "SolveOptions":
@@ -52,6 +54,12 @@ class SolveOptions:
         o.window = int(self.window)
         o.inner_eps = float(self.inner_eps)
         o.max_refine = int(self.max_refine)
+        if self.coupling not in ("auto", "direct"):
+            raise ValueError(f"unknown coupling {self.coupling!r}")
+        o.coupling = 1 if self.coupling == "direct" else 0
+        if self.diag not in ("auto", "vector"):
+            raise ValueError(f"unknown diag {self.diag!r}")
+        o.diag = 1 if self.diag == "vector" else 0
         return o
 
 

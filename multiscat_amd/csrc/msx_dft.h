@@ -1,0 +1,35 @@
+// DFT (convolution) form of the full channel coupling: tables and launchers (msx_dft.hip).
+#pragma once
+
+#include "msx_internal.h"
+
+namespace msx {
+
+// Device tables of one precision: F_x, F_y (np x np, zero padded) and
+// Vhat[z][kx][ky] = (F_x V'(z) F_y)[kx][ky] / (nkx nky), V' = V with V_00 = 0.
+template <typename T>
+struct DftTables {
+    int np;
+    const cx<T> *Fx, *Fy, *Vhat;
+};
+
+struct DftStore {
+    int np = 0;  // 0: table too large for the DFT kernel
+    DBuf<cx<double>> Fx, Fy, Vhat, tmp;
+    DBuf<cx<float>> Fxf, Fyf, Vhatf;
+    DftTables<double> d64() const { return {np, Fx.p, Fy.p, Vhat.p}; }
+    DftTables<float> d32() const { return {np, Fxf.p, Fyf.p, Vhatf.p}; }
+};
+
+// Padded transform size (16 or 32) for an nkx x nky table, 0 if larger than 32.
+int dft_padded_size(int nkx, int nky);
+
+// Builds the DFT matrices and Vhat from the FP64 table V[g][z] (both precisions).
+void build_dft_tables(int nkx, int nky, int nz, const cx<double> *V, DftStore &st, hipStream_t s);
+
+// y = full off-diagonal coupling of x (compact channel layout), every active instance.
+template <typename T>
+void launch_dft_coupling(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
+                         const int *active, hipStream_t s);
+
+}  // namespace msx

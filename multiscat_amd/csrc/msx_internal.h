@@ -86,6 +86,19 @@ void set_last_error(const std::string &msg);
 
 #define MSX_LAUNCH_CHECK() MSX_HIP(hipGetLastError())
 
+// Raise a kernel's dynamic-LDS limit to `bytes` (needed above 64 KiB). The limit covers the
+// dynamic part only; static __shared__ arrays of the kernel count on top of it.
+inline void set_dynamic_lds(const void *kernel, size_t bytes) {
+    if (bytes <= 64 * 1024) return;
+    hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // do not leave a sticky error for the next launch check
+        throw Error(6, "kernel needs " + std::to_string(bytes) +
+                           " bytes of LDS, more than the device grants: " + hipGetErrorString(e));
+    }
+}
+
 // ---- device buffers -------------------------------------------------------------------
 template <typename T>
 struct DBuf {
@@ -161,6 +174,23 @@ template <typename T>
 void launch_dinv(const Layout &L, const T *Q, const T *QT, const double *lam, const double *e,
                  const cx<T> *c, const T *fac, const cx<T> *x, const cx<T> *base, cx<T> *y,
                  T sign, const int *active, hipStream_t s);
+
+// Eigensystem zero-padded to nzp (a multiple of 16) for the matrix-core kernels.
+template <typename T>
+struct EigTables {
+    int nzp;
+    const T *Qp, *QTp;   // [nzp][nzp]
+    const double *lam;   // [nz]
+};
+
+// MFMA form of the diagonal blocks (inverse = 1: D^-1, inverse = 0: D), nz <= 320.
+bool diag_mfma_supported(int nz);
+template <typename T>
+void launch_diag_blocks(const Layout &L, const EigTables<T> &E, const double *e, const cx<T> *c,
+                        const T *fac, const cx<T> *x, const cx<T> *base, cx<T> *y, T sign,
+                        int inverse, const int *active, hipStream_t s);
+void launch_pad_square(int n, int np, const double *src, double *dst, hipStream_t s);
+void launch_pad_square(int n, int np, const double *src, float *dst, hipStream_t s);
 
 // y = D x (the diagonal blocks themselves: (H0 + e_j) x_j - c_j x_j(N) e_N).
 void launch_apply_diag(const Layout &L, const double *Q, const double *QT, const double *lam,

@@ -249,12 +249,7 @@ void launch_lower_solve(const Layout &L, const cx<double> *V, const double *Q, c
                         cx<double> *out, double sign, const int *active, hipStream_t s) {
     if (L.nz > 64 * LS_MAXMZ) throw Error(3, "lower solve supports nz <= 640");
     const size_t shm = sizeof(cx<double>) * (size_t)L.nz * (LS_WAVES + 2);
-    static bool attr_set = false;
-    if (shm > 64 * 1024 && !attr_set) {
-        MSX_HIP(hipFuncSetAttribute((const void *)lower_solve_kernel,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr_set = true;
-    }
+    set_dynamic_lds((const void *)lower_solve_kernel, shm);
     hipLaunchKernelGGL(lower_solve_kernel, dim3(L.n_inst), dim3(64 * LS_WAVES), shm, s, L, V, Q,
                        QT, lam, e, c, fac, x, base, out, sign, active);
     MSX_LAUNCH_CHECK();
@@ -368,12 +363,7 @@ void launch_dinv_impl(const Layout &L, const T *Q, const T *QT, const double *la
         hipLaunchKernelGGL((diag_kernel<T, 8>), grid, dim3(256), row_bytes * 8, s, L, Q, QT, lam,
                            e, c, fac, x, base, y, sign, inverse, active);
     } else {
-        static bool attr_set = false;
-        if (row_bytes * 4 > 64 * 1024 && !attr_set) {
-            MSX_HIP(hipFuncSetAttribute((const void *)diag_kernel<T, 4>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            attr_set = true;
-        }
+        set_dynamic_lds((const void *)diag_kernel<T, 4>, row_bytes * 4);
         dim3 grid((L.max_nc + 3) / 4, L.n_inst);
         hipLaunchKernelGGL((diag_kernel<T, 4>), grid, dim3(256), row_bytes * 4, s, L, Q, QT, lam,
                            e, c, fac, x, base, y, sign, inverse, active);
@@ -400,142 +390,6 @@ void launch_apply_diag(const Layout &L, const double *Q, const double *QT, const
                        const double *e, const cx<double> *c, const cx<double> *x,
                        cx<double> *y, const int *active, hipStream_t s) {
     launch_dinv_impl<double>(L, Q, QT, lam, e, c, nullptr, x, nullptr, y, 1.0, 0, active, s);
-}
-
-// ---------------------------------------------------------------------------------------
-// Symmetric eigensystem by parallel cyclic Jacobi (one workgroup). Each round applies n/2
-// disjoint rotations chosen by the circle method, so a sweep of n-1 rounds visits every
-// (p, q) pair once. A' = J^T A J: a column pass (A J, and V J for the eigenvectors) then a
-// row pass. Sweeps stop when the off-diagonal Frobenius norm falls below 1e-15 of the
-// whole. Eigenvalues are then ranked ascending and the eigenvectors permuted accordingly.
-// ---------------------------------------------------------------------------------------
-constexpr int EIG_THREADS = 1024;
-constexpr int EIG_MAXPAIRS = 1024;  // n <= 2048
-
-__device__ double block_sum_eig(double v, double *red) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) red[wave] = v;
-    __syncthreads();
-    double t = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
-    return t;
-}
-
-__global__ __launch_bounds__(EIG_THREADS) void eig_jacobi_kernel(int n, double *A, double *Vm,
-                                                                 double *lam_sorted,
-                                                                 double *Q_sorted,
-                                                                 double *QT_sorted,
-                                                                 int *info) {
-    __shared__ double cs[EIG_MAXPAIRS], sn[EIG_MAXPAIRS];
-    __shared__ int pp[EIG_MAXPAIRS], qq[EIG_MAXPAIRS];
-    __shared__ double red[EIG_THREADS / 64];
-    const int m = n + (n & 1);
-    const int half = m / 2;
-    const long long nn = (long long)n * n;
-    for (long long idx = threadIdx.x; idx < nn; idx += blockDim.x) {
-        const int i = (int)(idx / n), j = (int)(idx - (long long)i * n);
-        Vm[idx] = i == j ? 1.0 : 0.0;
-    }
-    __syncthreads();
-    int converged = 0;
-    for (int sweep = 0; sweep < 60; ++sweep) {
-        double off2 = 0, tot2 = 0;
-        for (long long idx = threadIdx.x; idx < nn; idx += blockDim.x) {
-            const int i = (int)(idx / n), j = (int)(idx - (long long)i * n);
-            const double a = A[idx];
-            tot2 += a * a;
-            if (i != j) off2 += a * a;
-        }
-        off2 = block_sum_eig(off2, red);
-        tot2 = block_sum_eig(tot2, red);
-        if (off2 <= 1e-30 * tot2) {
-            converged = 1;
-            break;
-        }
-        for (int round = 0; round < m - 1; ++round) {
-            for (int k = threadIdx.x; k < half; k += blockDim.x) {
-                int p, q;
-                if (k == 0) {
-                    p = m - 1;
-                    q = round;
-                } else {
-                    p = (round + k) % (m - 1);
-                    q = (round - k + (m - 1)) % (m - 1);
-                }
-                if (p > q) { const int t = p; p = q; q = t; }
-                double c = 1.0, s = 0.0;
-                if (q < n) {
-                    const double apq = A[(long long)p * n + q];
-                    if (apq != 0.0) {
-                        const double app = A[(long long)p * n + p], aqq = A[(long long)q * n + q];
-                        const double theta = (aqq - app) / (2.0 * apq);
-                        double t;
-                        if (fabs(theta) > 1e150) t = 0.5 / theta;
-                        else t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                        c = 1.0 / sqrt(t * t + 1.0);
-                        s = t * c;
-                    }
-                }
-                pp[k] = p; qq[k] = q; cs[k] = c; sn[k] = s;
-            }
-            __syncthreads();
-            // column pass: A <- A J, Vm <- Vm J
-            for (long long idx = threadIdx.x; idx < (long long)n * half; idx += blockDim.x) {
-                const int i = (int)(idx / half), k = (int)(idx - (long long)i * half);
-                const double s = sn[k];
-                if (s == 0.0) continue;
-                const double c = cs[k];
-                const int p = pp[k], q = qq[k];
-                double *ar = A + (long long)i * n;
-                const double ap = ar[p], aq = ar[q];
-                ar[p] = c * ap - s * aq;
-                ar[q] = s * ap + c * aq;
-                double *vr = Vm + (long long)i * n;
-                const double vp = vr[p], vq = vr[q];
-                vr[p] = c * vp - s * vq;
-                vr[q] = s * vp + c * vq;
-            }
-            __syncthreads();
-            // row pass: A <- J^T A
-            for (long long idx = threadIdx.x; idx < (long long)n * half; idx += blockDim.x) {
-                const int k = (int)(idx / n), j = (int)(idx - (long long)k * n);
-                const double s = sn[k];
-                if (s == 0.0) continue;
-                const double c = cs[k];
-                const int p = pp[k], q = qq[k];
-                const double ap = A[(long long)p * n + j], aq = A[(long long)q * n + j];
-                A[(long long)p * n + j] = c * ap - s * aq;
-                A[(long long)q * n + j] = s * ap + c * aq;
-            }
-            __syncthreads();
-        }
-    }
-    // ascending rank of each eigenvalue (ties broken by index), then permute
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const double li = A[(long long)i * n + i];
-        int rank = 0;
-        for (int j = 0; j < n; ++j) {
-            const double lj = A[(long long)j * n + j];
-            rank += (lj < li) || (lj == li && j < i);
-        }
-        lam_sorted[rank] = li;
-        for (int z = 0; z < n; ++z) {
-            const double v = Vm[(long long)z * n + i];
-            Q_sorted[(long long)z * n + rank] = v;
-            QT_sorted[(long long)rank * n + z] = v;
-        }
-    }
-    if (threadIdx.x == 0) *info = converged ? 0 : 1;
-}
-
-void launch_eigensystem(int n, double *A, double *Vec, double *lam_sorted, double *Q_sorted,
-                        double *QT_sorted, int *info, hipStream_t s) {
-    if (n > 2 * EIG_MAXPAIRS) throw Error(3, "eigensystem: n too large");
-    hipLaunchKernelGGL(eig_jacobi_kernel, dim3(1), dim3(EIG_THREADS), 0, s, n, A, Vec,
-                       lam_sorted, Q_sorted, QT_sorted, info);
-    MSX_LAUNCH_CHECK();
 }
 
 // ---------------------------------------------------------------------------------------

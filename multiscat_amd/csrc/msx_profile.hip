@@ -1,0 +1,131 @@
+// Launch-level instrumentation (see msx_profile.h) and its C ABI.
+#include "msx_profile.h"
+
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/multiscat_amd.h"
+
+namespace msx {
+
+namespace {
+
+struct Rec {
+    std::string name;
+    hipEvent_t a, b;
+    double flops, bytes;
+};
+
+std::mutex g_mu;
+bool g_on = false;
+std::vector<Rec> g_recs;
+std::vector<hipEvent_t> g_pool;
+
+struct Sum {
+    long long count = 0;
+    double ms = 0, flops = 0, bytes = 0;
+};
+std::map<std::string, Sum> g_sums;
+
+hipEvent_t take_event() {
+    if (!g_pool.empty()) {
+        hipEvent_t e = g_pool.back();
+        g_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return e;
+}
+
+// Fold finished records into the sums (caller holds the lock).
+void drain() {
+    for (auto &r : g_recs) {
+        float ms = 0;
+        if (r.a && r.b && hipEventSynchronize(r.b) == hipSuccess &&
+            hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            Sum &s = g_sums[r.name];
+            s.count += 1;
+            s.ms += ms;
+            s.flops += r.flops;
+            s.bytes += r.bytes;
+        } else {
+            (void)hipGetLastError();
+        }
+        if (r.a) g_pool.push_back(r.a);
+        if (r.b) g_pool.push_back(r.b);
+    }
+    g_recs.clear();
+}
+
+}  // namespace
+
+bool profiling() { return g_on; }
+
+ProfScope::ProfScope(const char *name, hipStream_t s, double flops, double bytes) : s_(s) {
+    if (!g_on) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Rec r{name, take_event(), take_event(), flops, bytes};
+    if (r.a) (void)hipEventRecord(r.a, s);
+    g_recs.push_back(r);
+    slot_ = (int)g_recs.size() - 1;
+}
+
+ProfScope::~ProfScope() {
+    if (slot_ < 0) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (slot_ < (int)g_recs.size() && g_recs[slot_].b) (void)hipEventRecord(g_recs[slot_].b, s_);
+    if (g_recs.size() > 4096) drain();
+}
+
+}  // namespace msx
+
+extern "C" {
+
+int msx_profile_enable(int on) {
+    std::lock_guard<std::mutex> lk(msx::g_mu);
+    msx::g_on = on != 0;
+    return MSX_OK;
+}
+
+int msx_profile_reset(void) {
+    std::lock_guard<std::mutex> lk(msx::g_mu);
+    msx::drain();
+    msx::g_sums.clear();
+    return MSX_OK;
+}
+
+int msx_profile_read(const char *kernel, long long *count, double *total_ms, double *flops,
+                     double *bytes) {
+    if (!kernel) return MSX_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(msx::g_mu);
+    msx::drain();
+    auto it = msx::g_sums.find(kernel);
+    const msx::Sum s = it == msx::g_sums.end() ? msx::Sum{} : it->second;
+    if (count) *count = s.count;
+    if (total_ms) *total_ms = s.ms;
+    if (flops) *flops = s.flops;
+    if (bytes) *bytes = s.bytes;
+    return MSX_OK;
+}
+
+int msx_profile_names(char *buf, int buflen) {
+    if (!buf || buflen <= 0) return MSX_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(msx::g_mu);
+    msx::drain();
+    std::string all;
+    for (auto &kv : msx::g_sums) {
+        if (!all.empty()) all += ",";
+        all += kv.first;
+    }
+    std::strncpy(buf, all.c_str(), buflen - 1);
+    buf[buflen - 1] = 0;
+    return MSX_OK;
+}
+
+}  // extern "C"

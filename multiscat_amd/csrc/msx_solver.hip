@@ -18,7 +18,9 @@
 #include <set>
 
 #include "../../include/multiscat_amd.h"
+#include "msx_dft.h"
 #include "msx_krylov.h"
+#include "msx_profile.h"
 
 using namespace msx;
 
@@ -49,10 +51,13 @@ struct Krylov {
     DBuf<cx<T> *> vp_d;
     DBuf<cx<T>> w;
     int *count_h = nullptr;
+    std::vector<long long> len_h;
+    std::vector<int> host_active;
 
     Krylov(const std::vector<long long> &start, const std::vector<long long> &len, int kmax_,
            hipStream_t stream)
-        : n_inst((int)start.size()), kmax(kmax_), s(stream) {
+        : n_inst((int)start.size()), kmax(kmax_), s(stream), len_h(len),
+          host_active(start.size(), 1) {
         if (kmax > KR_MAX_BASIS - 1) kmax = KR_MAX_BASIS - 1;
         long long maxlen = 1;
         for (int b = 0; b < n_inst; ++b) {
@@ -94,6 +99,7 @@ struct Krylov {
             for (int b = 0; b < n_inst; ++b) h[b] = mask[b] ? 1 : 0;
         active.upload(h.data(), n_inst, s);
         MSX_HIP(hipStreamSynchronize(s));
+        host_active = h;
     }
     void ensure(int nvec) {
         if (nvec > kmax + 1) nvec = kmax + 1;
@@ -107,8 +113,16 @@ struct Krylov {
     }
     int read_count() {
         MSX_HIP(hipMemcpyAsync(count_h, count.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        if (profiling()) active.download(host_active.data(), n_inst, s);
         MSX_HIP(hipStreamSynchronize(s));
         return *count_h;
+    }
+    // elements of the instances still iterating (host mirror, kept while profiling)
+    double active_elements() const {
+        double n = 0;
+        for (int b = 0; b < n_inst; ++b)
+            if (host_active[b]) n += (double)len_h[b];
+        return n;
     }
     std::vector<int> host_ints(const DBuf<int> &b) {
         std::vector<int> h(n_inst);
@@ -168,10 +182,22 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
         for (int k = 0; k < kr.kmax && done < maxiter; ++k) {
             kr.ensure(k + 2);
             op(kr.vp[k], kr.w.p, kr.active.p);
-            k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol.p, 0, s);
-            k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol.p, kr.w.p, 0, kr.partial.p, nullptr, s);
-            k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol2.p, 0, s);
-            k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol2.p, kr.w.p, 1, kr.partial.p, kr.hcol.p, s);
+            {
+                // CGS2: two passes of (k+1 dot products, then w -= V h); HBM traffic of the
+                // four vector kernels is 4 (k+1) + 6 vectors of the iterating instances
+                double fl = 0, by = 0;
+                if (profiling()) {
+                    const double n = kr.active_elements();
+                    fl = 32.0 * (k + 1) * n;
+                    by = (4.0 * (k + 1) + 6.0) * n * sizeof(cx<T>);
+                }
+                ProfScope ps(sizeof(T) == 4 ? "arnoldi_cgs2_f32" : "arnoldi_cgs2_f64", s, fl, by);
+                k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol.p, 0, s);
+                k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol.p, kr.w.p, 0, kr.partial.p, nullptr, s);
+                k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol2.p, 0, s);
+                k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol2.p, kr.w.p, 1, kr.partial.p, kr.hcol.p,
+                          s);
+            }
             k_givens(kr.K, k, kr.hcol.p, kr.hcol2.p, kr.S, kr.count.p, s);
             k_scale<T>(kr.K, kr.w.p, kr.vp[k + 1], kr.hnorm.p, kr.active.p, s);
             ++done;
@@ -198,25 +224,49 @@ struct msx_table {
     hipStream_t stream = 0;
     DBuf<cx<double>> V, Vt;
     DBuf<cx<float>> Vf, Vtf;
-    DBuf<double> T, A, Vm, lam, Q, QT;
-    DBuf<float> Qf, QTf;
+    DBuf<double> T, A, Vm, lam, Q, QT, Qp, QTp;
+    DBuf<float> Qf, QTf, Qpf, QTpf;
     DBuf<int> info;
+    int nzp = 0;          // nz rounded up to a multiple of 16 (matrix-core tiles)
+    DftStore dft, dft_t;  // convolution form of the coupling (table and flipped table)
     bool flipped = false;
     ~msx_table() {
         if (stream) (void)hipStreamDestroy(stream);
     }
     size_t nG() const { return (size_t)nkx * nky; }
+    EigTables<double> eig64() const { return {nzp, Qp.p, QTp.p, lam.p}; }
+    EigTables<float> eig32() const { return {nzp, Qpf.p, QTpf.p, lam.p}; }
     void prepare() {
         const size_t n2 = (size_t)nz * nz;
         launch_build_h0(nz, T.p, V.p, A.p, stream);
-        launch_eigensystem(nz, A.p, Vm.p, lam.p, Q.p, QT.p, info.p, stream);
+        {
+            // tridiagonalisation (4/3 n^3), accumulation of Q_H (4/3 n^3), QL rotations on Z
+            // (about 6 n^3 for about 1.5 sweeps per eigenvalue)
+            const double n = nz;
+            ProfScope ps("eigensystem_f64", stream, (4.0 / 3.0 + 4.0 / 3.0 + 6.0) * n * n * n,
+                         5.0 * 8.0 * n * n);
+            launch_eigensystem(nz, A.p, Vm.p, lam.p, Q.p, QT.p, info.p, stream);
+        }
         int h_info = 0;
         info.download(&h_info, 1, stream);
         MSX_HIP(hipStreamSynchronize(stream));
         if (h_info != 0) throw Error(MSX_ERR_EIGEN, "eigensolver did not converge");
         launch_to_float_real((long long)n2, Q.p, Qf.p, stream);
         launch_to_float_real((long long)n2, QT.p, QTf.p, stream);
+        nzp = (nz + 15) / 16 * 16;
+        const size_t p2 = (size_t)nzp * nzp;
+        if (Qp.n != p2) {
+            Qp.alloc(p2);
+            QTp.alloc(p2);
+            Qpf.alloc(p2);
+            QTpf.alloc(p2);
+        }
+        launch_pad_square(nz, nzp, Q.p, Qp.p, stream);
+        launch_pad_square(nz, nzp, QT.p, QTp.p, stream);
+        launch_pad_square(nz, nzp, Q.p, Qpf.p, stream);
+        launch_pad_square(nz, nzp, QT.p, QTpf.p, stream);
         launch_to_float((long long)(nG() * nz), V.p, Vf.p, stream);
+        build_dft_tables(nkx, nky, nz, V.p, dft, stream);
         flipped = false;
     }
     void ensure_flipped() {
@@ -227,6 +277,7 @@ struct msx_table {
         }
         launch_flip_table(nkx, nky, nz, V.p, Vt.p, stream);
         launch_to_float((long long)(nG() * nz), Vt.p, Vtf.p, stream);
+        build_dft_tables(nkx, nky, nz, Vt.p, dft_t, stream);
         flipped = true;
     }
 };
@@ -293,83 +344,159 @@ struct Ops {
     bool neumann;
     const cx<double> *V;  // coupling table (flipped for the transposed system)
     const cx<float> *Vf;
+    const DftStore *dft;  // its convolution form
+    bool direct;          // force the direct sum for the full coupling
+    bool vector_diag = false;  // force the vector-ALU D^-1 kernel
     hipStream_t s;
 
-    Ops(msx_batch &b, bool neu, bool transposed)
+    Ops(msx_batch &b, bool neu, bool transposed, bool force_direct)
         : B(b), t(*b.t), L(b.layout()), neumann(neu), s(b.t->stream) {
         if (transposed) {
             t.ensure_flipped();
             V = t.Vt.p;
             Vf = t.Vtf.p;
+            dft = &t.dft_t;
         } else {
             V = t.V.p;
             Vf = t.Vf.p;
+            dft = &t.dft;
         }
+        direct = force_direct || dft->np == 0;
+    }
+    // full off-diagonal coupling y = V x (DFT form on the matrix cores when it applies)
+    // ---- algorithmic work of one launch over the instances it processes (profiling) ----
+    const std::vector<int> *host_active = nullptr;  // mirror of the Krylov mask, if any
+    template <typename F>
+    double over_active(const int *act, F f) const {
+        double sum = 0;
+        for (int b = 0; b < B.n_inst; ++b)
+            if (!act || !host_active || (*host_active)[b]) sum += f(B.nc[b]);
+        return sum;
+    }
+    // full coupling: direct sum 8 nc (nc-1) nz flops; DFT form 2 complex 2-D transforms of
+    // the grid per node (matrix products, 8 flops per complex MAC) plus the pointwise product
+    void couple_full_work(const int *act, size_t s_cx, bool dft_form, double &fl, double &by) {
+        const double nz = L.nz, nkx = L.nkx, nky = L.nky, nG = nkx * nky;
+        if (dft_form)
+            fl = over_active(act, [&](int) { return nz * (16.0 * nG * (nkx + nky) + 6.0 * nG); });
+        else
+            fl = over_active(act, [&](int nc) { return 8.0 * nc * (nc - 1.0) * nz; });
+        by = over_active(act, [&](int nc) { return 2.0 * nc * nz * s_cx; }) + nG * nz * s_cx;
+    }
+    void couple_full(const cx<double> *in, cx<double> *out, const int *act) {
+        double fl = 0, by = 0;
+        if (profiling()) couple_full_work(act, sizeof(cx<double>), !direct, fl, by);
+        ProfScope ps(direct ? "coupling_direct_f64" : "coupling_dft_f64", s, fl, by);
+        if (direct) launch_coupling<double>(L, V, in, out, COUPLE_FULL, act, s);
+        else launch_dft_coupling<double>(L, dft->d64(), in, out, act, s);
+    }
+    void couple_full(const cx<float> *in, cx<float> *out, const int *act) {
+        double fl = 0, by = 0;
+        if (profiling()) couple_full_work(act, sizeof(cx<float>), !direct, fl, by);
+        ProfScope ps(direct ? "coupling_direct_f32" : "coupling_dft_f32", s, fl, by);
+        if (direct) launch_coupling<float>(L, Vf, in, out, COUPLE_FULL, act, s);
+        else launch_dft_coupling<float>(L, dft->d32(), in, out, act, s);
+    }
+    void upper(const cx<double> *in, cx<double> *out, const int *act) {
+        double fl = 0, by = 0;
+        if (profiling()) {
+            fl = over_active(act, [&](int nc) { return 4.0 * nc * (nc - 1.0) * L.nz; });
+            by = over_active(act, [&](int nc) { return 32.0 * nc * L.nz; }) +
+                 16.0 * L.nkx * L.nky * L.nz;
+        }
+        ProfScope ps("coupling_upper_f64", s, fl, by);
+        launch_coupling<double>(L, V, in, out, COUPLE_UPPER, act, s);
+    }
+    // L^-1: the lower coupling sum (4 nc (nc-1) nz) and two real nz x nz products per channel
+    void lower(cx<double> *x, const cx<double> *base, cx<double> *out, double sign,
+               const int *act) {
+        double fl = 0, by = 0;
+        if (profiling()) {
+            const double nz = L.nz;
+            fl = over_active(act, [&](int nc) { return 4.0 * nc * (nc - 1.0) * nz + 8.0 * nc * nz * nz; });
+            by = over_active(act, [&](int nc) { return 48.0 * nc * nz; }) + 16.0 * nz * nz;
+        }
+        ProfScope ps("lower_solve_f64", s, fl, by);
+        launch_lower_solve(L, V, t.Q.p, t.QT.p, t.lam.p, B.e_d.p, B.c_d.p, B.fac_d.p, x, base, out,
+                           sign, act, s);
     }
     // Gauss-Seidel, FP64: out = M (I + L^-1 U) in
     void gs(const cx<double> *in, cx<double> *out, const int *act) {
         cx<double> *u = B.tmp1.p;
         cx<double> *w1 = neumann ? B.tmp2.p : out;
-        launch_coupling<double>(L, V, in, u, COUPLE_UPPER, act, s);
-        launch_lower_solve(L, V, t.Q.p, t.QT.p, t.lam.p, B.e_d.p, B.c_d.p, B.fac_d.p, u, in, w1,
-                           1.0, act, s);
+        upper(in, u, act);
+        lower(u, in, w1, 1.0, act);
         if (neumann) {
-            launch_coupling<double>(L, V, w1, u, COUPLE_UPPER, act, s);
-            launch_lower_solve(L, V, t.Q.p, t.QT.p, t.lam.p, B.e_d.p, B.c_d.p, B.fac_d.p, u, w1,
-                               out, -1.0, act, s);
+            upper(w1, u, act);
+            lower(u, w1, out, -1.0, act);
         }
     }
     // Gauss-Seidel right-hand side: rhs = M L^-1 src (src is overwritten with L^-1 src)
     void gs_rhs(cx<double> *src, cx<double> *rhs) {
-        launch_lower_solve(L, V, t.Q.p, t.QT.p, t.lam.p, B.e_d.p, B.c_d.p, B.fac_d.p, src, nullptr,
-                           nullptr, 0.0, nullptr, s);
+        lower(src, nullptr, nullptr, 0.0, nullptr);
         if (neumann) {
             cx<double> *u = B.tmp1.p;
-            launch_coupling<double>(L, V, src, u, COUPLE_UPPER, nullptr, s);
-            launch_lower_solve(L, V, t.Q.p, t.QT.p, t.lam.p, B.e_d.p, B.c_d.p, B.fac_d.p, u, src,
-                               rhs, -1.0, nullptr, s);
+            upper(src, u, nullptr);
+            lower(u, src, rhs, -1.0, nullptr);
         } else {
             MSX_HIP(hipMemcpyAsync(rhs, src, sizeof(cx<double>) * B.n_elem(),
                                    hipMemcpyDeviceToDevice, s));
         }
     }
     // Block Jacobi: out = M (I + D^-1 V) in, in FP64 or FP32
+    // out = base + sign * D^-1 in (base may be null): matrix cores when nz <= 320
+    template <typename T>
+    void dinv(const cx<T> *in, const cx<T> *base, cx<T> *out, T sign, const int *act) {
+        const bool mfma = !vector_diag && diag_mfma_supported(L.nz);
+        double fl = 0, by = 0;
+        if (profiling()) {  // two real nz x nz products on the re and im rows of each channel
+            const double nz = L.nz;
+            fl = over_active(act, [&](int nc) { return nc * (8.0 * nz * nz + 12.0 * nz); });
+            by = over_active(act, [&](int nc) {
+                     return nc * nz * (3.0 * sizeof(cx<T>) + sizeof(T));
+                 }) + 2.0 * nz * nz * sizeof(T);
+        }
+        const char *name = sizeof(T) == 4 ? (mfma ? "diag_mfma_f32" : "diag_vector_f32")
+                                          : (mfma ? "diag_mfma_f64" : "diag_vector_f64");
+        ProfScope ps(name, s, fl, by);
+        if (mfma)
+            launch_diag_blocks<T>(L, eig<T>(), B.e_d.p, c<T>(), fac<T>(), in, base, out, sign, 1,
+                                  act, s);
+        else
+            launch_dinv<T>(L, q<T>(), qt<T>(), t.lam.p, B.e_d.p, c<T>(), fac<T>(), in, base, out,
+                           sign, act, s);
+    }
     template <typename T>
     void jac(const cx<T> *in, cx<T> *out, const int *act, cx<T> *u, cx<T> *w1buf) {
-        const cx<T> *Vt = table<T>();
         cx<T> *w1 = neumann ? w1buf : out;
-        launch_coupling<T>(L, Vt, in, u, COUPLE_FULL, act, s);
-        launch_dinv<T>(L, q<T>(), qt<T>(), t.lam.p, B.e_d.p, c<T>(), fac<T>(), u, in, w1, T(1), act,
-                       s);
+        couple_full(in, u, act);
+        dinv<T>(u, in, w1, T(1), act);
         if (neumann) {
-            launch_coupling<T>(L, Vt, w1, u, COUPLE_FULL, act, s);
-            launch_dinv<T>(L, q<T>(), qt<T>(), t.lam.p, B.e_d.p, c<T>(), fac<T>(), u, w1, out,
-                           T(-1), act, s);
+            couple_full(w1, u, act);
+            dinv<T>(u, w1, out, T(-1), act);
         }
     }
     // Block Jacobi right-hand side in FP64: rhs = M D^-1 src
     void jac_rhs(const cx<double> *src, cx<double> *rhs) {
         cx<double> *r1 = neumann ? B.tmp2.p : rhs;
-        launch_dinv<double>(L, t.Q.p, t.QT.p, t.lam.p, B.e_d.p, B.c_d.p, B.fac_d.p, src, nullptr,
-                            r1, 1.0, nullptr, s);
+        dinv<double>(src, nullptr, r1, 1.0, nullptr);
         if (neumann) {
-            launch_coupling<double>(L, V, r1, B.tmp1.p, COUPLE_FULL, nullptr, s);
-            launch_dinv<double>(L, t.Q.p, t.QT.p, t.lam.p, B.e_d.p, B.c_d.p, B.fac_d.p, B.tmp1.p,
-                                r1, rhs, -1.0, nullptr, s);
+            couple_full(r1, B.tmp1.p, nullptr);
+            dinv<double>(B.tmp1.p, r1, rhs, -1.0, nullptr);
         }
     }
-    template <typename T> const cx<T> *table() const;
     template <typename T> const T *q() const;
     template <typename T> const T *qt() const;
+    template <typename T> EigTables<T> eig() const;
     template <typename T> const cx<T> *c() const;
     template <typename T> const T *fac() const;
 };
-template <> const cx<double> *Ops::table<double>() const { return V; }
-template <> const cx<float> *Ops::table<float>() const { return Vf; }
 template <> const double *Ops::q<double>() const { return t.Q.p; }
 template <> const float *Ops::q<float>() const { return t.Qf.p; }
 template <> const double *Ops::qt<double>() const { return t.QT.p; }
 template <> const float *Ops::qt<float>() const { return t.QTf.p; }
+template <> EigTables<double> Ops::eig<double>() const { return t.eig64(); }
+template <> EigTables<float> Ops::eig<float>() const { return t.eig32(); }
 template <> const cx<double> *Ops::c<double>() const { return B.c_d.p; }
 template <> const cx<float> *Ops::c<float>() const { return B.cf_d.p; }
 template <> const double *Ops::fac<double>() const { return B.fac_d.p; }
@@ -390,6 +517,8 @@ void check_options(const msx_solve_options *o) {
     if (o->precision == MSX_PRECISION_MIXED && o->method != MSX_METHOD_JACOBI)
         throw Error(MSX_ERR_INVALID, "mixed precision runs with the block-Jacobi method");
     if (!(o->eps > 0)) throw Error(MSX_ERR_INVALID, "eps must be positive");
+    if (o->coupling != 0 && o->coupling != 1) throw Error(MSX_ERR_INVALID, "unknown coupling");
+    if (o->diag != 0 && o->diag != 1) throw Error(MSX_ERR_INVALID, "unknown diag kernel");
     if (o->maxiter <= 0) throw Error(MSX_ERR_INVALID, "maxiter must be positive");
     if (o->window <= 0) throw Error(MSX_ERR_INVALID, "window must be positive");
 }
@@ -406,7 +535,8 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     msx_batch::need(B.tmp1, n);
     msx_batch::need(B.tmp2, n);
     if (!B.factors_ready) B.build_factors();
-    Ops ops(B, o.neumann == 1, transposed);
+    Ops ops(B, o.neumann == 1, transposed, o.coupling == 1);
+    ops.vector_diag = o.diag == 1;
     MSX_HIP(hipMemcpyAsync(B.src.p, src_in, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, s));
     MSX_HIP(hipMemsetAsync(x, 0, sizeof(cx<double>) * n, s));
     const int kdim = krylov_dim(o, B.n_inst);
@@ -429,7 +559,9 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
                 ops.jac<double>(in, out, act, B.tmp1.p, B.tmp2.p);
             };
         }
+        ops.host_active = &kr.host_active;
         gmres<double, double>(kr, op, B.rhs.p, x, true, o.eps, o.window, o.maxiter, true);
+        ops.host_active = nullptr;
         iters = kr.host_ints(kr.iters);
         auto conv = kr.host_ints(kr.converged);
         resid = kr.host_doubles(kr.resid);
@@ -465,6 +597,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
             // r = rhs64 - M (I + D^-1 V) x  in FP64 (tmp1/tmp2 are the FP64 work vectors)
             // (B.src is free here: the source was consumed by jac_rhs above)
             outer.set_all_active(act.data());
+            ops.host_active = &outer.host_active;
             ops.jac<double>(x, B.src.p, outer.active.p, B.tmp1.p, B.tmp2.p);
             k_axpby<double>(outer.K, B.rhs.p, B.src.p, -1.0, r64, outer.active.p, s);
         }
@@ -480,6 +613,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         if (n_act == 0 || sweep >= max_refine) break;
         launch_to_float((long long)n, r64, B.f_rhs.p, s);
         inner.set_all_active(act.data());
+        ops.host_active = &inner.host_active;
         gmres<float, double>(inner, op32, B.f_rhs.p, x, true, inner_eps, 1,
                              std::min(o.maxiter, inner_dim), false);
         auto it = inner.host_ints(inner.iters);
@@ -581,6 +715,8 @@ void msx_default_options(msx_solve_options *opt, int nsig) {
     opt->window = 3;                                 // multiscat_gmres.f90:125
     opt->inner_eps = 1e-5;
     opt->max_refine = 6;
+    opt->coupling = 0;
+    opt->diag = 0;
 }
 
 int msx_table_create(int nkx, int nky, int nz, const double *potential, const double *kinetic,
@@ -942,6 +1078,135 @@ int msx_debug_apply_upper_block(int nkx, int nky, int nz, const double *V, const
         xin.upload(reinterpret_cast<const cx<double> *>(x), n, s);
         launch_coupling<double>(bg.b->layout(), tg.t->V.p, xin.p, yout.p, COUPLE_UPPER, nullptr, s);
         yout.download(reinterpret_cast<cx<double> *>(y), n, s);
+        MSX_HIP(hipStreamSynchronize(s));
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+int msx_debug_apply_coupling(int nkx, int nky, int nz, const double *V, int nc, const int *chx,
+                             const int *chy, const double *x, double *y, int mode,
+                             int precision, int algorithm) {
+    try {
+        if (nkx <= 0 || nky <= 0 || nz <= 0 || nc <= 0) return MSX_ERR_DIMS;
+        if (!V || !chx || !chy || !x || !y) throw Error(MSX_ERR_INVALID, "null argument");
+        if (mode < 0 || mode > 2 || precision < 0 || precision > 1 || algorithm < 0 || algorithm > 1)
+            throw Error(MSX_ERR_INVALID, "bad mode/precision/algorithm");
+        std::vector<double> Tr((size_t)nz * nz, 0.0);
+        for (int i = 0; i < nz; ++i) Tr[(size_t)i * nz + i] = 1.0;
+        TableGuard tg;
+        int rc = msx_table_create(nkx, nky, nz, V, Tr.data(), &tg.t);
+        if (rc != MSX_OK) return rc;
+        // the specular channel is not needed for a bare coupling: build the layout directly
+        msx_table &t = *tg.t;
+        hipStream_t s = t.stream;
+        DBuf<int> nc_d(1), chx_d(nc), chy_d(nc);
+        DBuf<long long> off_d(1);
+        long long zero = 0;
+        nc_d.upload(&nc, 1, s);
+        off_d.upload(&zero, 1, s);
+        chx_d.upload(chx, nc, s);
+        chy_d.upload(chy, nc, s);
+        const Layout L{nkx, nky, nz, 1, nc, nc_d.p, off_d.p, chx_d.p, chy_d.p};
+        const size_t n = (size_t)nc * nz;
+        DBuf<cx<double>> xin(n), yout(n);
+        xin.upload(reinterpret_cast<const cx<double> *>(x), n, s);
+        if (algorithm == 1 && mode != COUPLE_FULL)
+            throw Error(MSX_ERR_INVALID, "the DFT form exists for the full coupling only");
+        if (algorithm == 1 && t.dft.np == 0) throw Error(MSX_ERR_INVALID, "table too large for the DFT form");
+        if (precision == 0) {
+            if (algorithm == 0) launch_coupling<double>(L, t.V.p, xin.p, yout.p, mode, nullptr, s);
+            else launch_dft_coupling<double>(L, t.dft.d64(), xin.p, yout.p, nullptr, s);
+        } else {
+            DBuf<cx<float>> xf(n), yf(n);
+            launch_to_float((long long)n, xin.p, xf.p, s);
+            if (algorithm == 0) launch_coupling<float>(L, t.Vf.p, xf.p, yf.p, mode, nullptr, s);
+            else launch_dft_coupling<float>(L, t.dft.d32(), xf.p, yf.p, nullptr, s);
+            std::vector<cx<float>> hf(n);
+            yf.download(hf.data(), n, s);
+            MSX_HIP(hipStreamSynchronize(s));
+            for (size_t i = 0; i < n; ++i) {
+                y[2 * i] = hf[i].re;
+                y[2 * i + 1] = hf[i].im;
+            }
+            return MSX_OK;
+        }
+        yout.download(reinterpret_cast<cx<double> *>(y), n, s);
+        MSX_HIP(hipStreamSynchronize(s));
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+int msx_debug_apply_diagonal(int nz, const double *potential00, const double *kinetic, int nc,
+                             const double *e, const double *wave_c, const double *state_in,
+                             double *state_out, int inverse, int precision, int variant) {
+    try {
+        if (nz <= 0 || nc <= 0) return MSX_ERR_DIMS;
+        if (!potential00 || !kinetic || !e || !wave_c || !state_in || !state_out)
+            throw Error(MSX_ERR_INVALID, "null argument");
+        if (precision < 0 || precision > 1 || variant < 0 || variant > 1 || inverse < 0 ||
+            inverse > 1)
+            throw Error(MSX_ERR_INVALID, "bad inverse/precision/variant");
+        if (variant == 0 && !diag_mfma_supported(nz))
+            throw Error(MSX_ERR_INVALID, "matrix-core diagonal blocks need nz <= 320");
+        TableGuard tg;
+        int rc = msx_table_create(1, 1, nz, potential00, kinetic, &tg.t);
+        if (rc != MSX_OK) return rc;
+        msx_table &t = *tg.t;
+        hipStream_t s = t.stream;
+        // channels need no grid position here: a layout of nc rows, all of instance 0
+        std::vector<int> zeros(nc, 0);
+        DBuf<int> nc_d(1), ch_d(nc);
+        DBuf<long long> off_d(1);
+        long long zero = 0;
+        nc_d.upload(&nc, 1, s);
+        off_d.upload(&zero, 1, s);
+        ch_d.upload(zeros.data(), nc, s);
+        const Layout L{1, 1, nz, 1, nc, nc_d.p, off_d.p, ch_d.p, ch_d.p};
+        const size_t n = (size_t)nc * nz;
+        DBuf<double> e_d(nc), fac(n);
+        DBuf<float> facf(n);
+        DBuf<cx<double>> c_d(nc), xin(n), yout(n);
+        e_d.upload(e, nc, s);
+        c_d.upload(reinterpret_cast<const cx<double> *>(wave_c), nc, s);
+        xin.upload(reinterpret_cast<const cx<double> *>(state_in), n, s);
+        launch_preconditioner_factors_qt(nc, nz, t.Q.p, t.QT.p, t.lam.p, e_d.p, fac.p, s);
+        if (precision == 0) {
+            if (variant == 0)
+                launch_diag_blocks<double>(L, t.eig64(), e_d.p, c_d.p, fac.p, xin.p, nullptr,
+                                           yout.p, 1.0, inverse, nullptr, s);
+            else if (inverse)
+                launch_dinv<double>(L, t.Q.p, t.QT.p, t.lam.p, e_d.p, c_d.p, fac.p, xin.p,
+                                    nullptr, yout.p, 1.0, nullptr, s);
+            else
+                launch_apply_diag(L, t.Q.p, t.QT.p, t.lam.p, e_d.p, c_d.p, xin.p, yout.p,
+                                  nullptr, s);
+        } else {
+            DBuf<cx<float>> cf(nc), xf(n), yf(n);
+            launch_to_float(nc, c_d.p, cf.p, s);
+            launch_to_float((long long)n, xin.p, xf.p, s);
+            launch_to_float_real((long long)n, fac.p, facf.p, s);
+            if (variant == 0)
+                launch_diag_blocks<float>(L, t.eig32(), e_d.p, cf.p, facf.p, xf.p, nullptr, yf.p,
+                                          1.0f, inverse, nullptr, s);
+            else if (inverse)
+                launch_dinv<float>(L, t.Qf.p, t.QTf.p, t.lam.p, e_d.p, cf.p, facf.p, xf.p, nullptr,
+                                   yf.p, 1.0f, nullptr, s);
+            else
+                throw Error(MSX_ERR_INVALID, "forward FP32 blocks exist in the matrix-core form only");
+            std::vector<cx<float>> h(n);
+            yf.download(h.data(), n, s);
+            MSX_HIP(hipStreamSynchronize(s));
+            for (size_t i = 0; i < n; ++i) {
+                state_out[2 * i] = h[i].re;
+                state_out[2 * i + 1] = h[i].im;
+            }
+            return MSX_OK;
+        }
+        yout.download(reinterpret_cast<cx<double> *>(state_out), n, s);
         MSX_HIP(hipStreamSynchronize(s));
         return MSX_OK;
     } catch (const Error &e) {

@@ -87,7 +87,10 @@ def _condition(V, a1, a2, z_min, z_max, energy_mev, theta, phi, n_channels, fact
         ix, iy = np.divmod(np.arange(nkx * nky), nky)
         channels = (ix.astype(np.int32), iy.astype(np.int32))
     else:
-        channels = B.lowest_channels(e, n_channels)
+        # the lowest-n set must reach the specular channel (e_00 = -kz^2 is not always the
+        # lowest e_G at oblique incidence)
+        rank_spec = int(np.nonzero(np.argsort(e.ravel(), kind="stable") == 0)[0][0])
+        channels = B.lowest_channels(e, max(int(n_channels), rank_spec + 1))
     return Condition(energy_mev, theta, phi, k, e, a.ravel(), b.ravel(), c.ravel(), channels)
 
 
@@ -157,20 +160,24 @@ def batch_channel_count(e: np.ndarray, k2: float, lo: int, hi: int, ratio: float
 
 
 def hex_batch(n_conditions: int = 256, nz: int = 150, seed: int = 1234, cond_seed: int = 99,
-              nk: int = 31, lo: int = 200, hi: int = 1200) -> Workload:
+              nk: int = 31, lo: int = 200, hi: int = 1200, channel_counts=None) -> Workload:
+    """``channel_counts`` (one per condition) overrides the |K+G|^2 <= 6 k^2 rule."""
     factor = B.mev_to_natural(HE4_MASS_KG)
     Vm, a1, a2, _ = hex_table(nk, nz, 0.4, seed)
     V = Vm * factor
     T = B.kinetic_matrix(nz, Z_MIN, Z_MAX)
     w = Workload("hex_batch", V, T, a1, a2, Z_MIN, Z_MAX)
     rng = np.random.default_rng(cond_seed)
-    for _ in range(n_conditions):
+    for i in range(n_conditions):
         E = float(rng.uniform(10.0, 100.0))
         th = math.radians(float(rng.uniform(0.0, 60.0)))
         ph = math.radians(float(rng.uniform(0.0, 60.0)))
         k = B.momentum_from_angles(th, ph, E * factor)
         e = B.perpendicular_kinetic_difference(k, a1, a2, nk, nk)
-        n = batch_channel_count(e, E * factor, lo, hi)
+        if channel_counts is not None:
+            n = int(channel_counts[i % len(channel_counts)])
+        else:
+            n = batch_channel_count(e, E * factor, lo, hi)
         w.conditions.append(_condition(V, a1, a2, Z_MIN, Z_MAX, E, th, ph, n, factor))
     return w
 

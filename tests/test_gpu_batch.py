@@ -1,0 +1,171 @@
+"""GPU parity of the batched group: uneven batches of conditions sharing one table, solved
+with the Gauss-Seidel split (the reference's algorithm) or the block-Jacobi eigenbasis
+preconditioner, FP64 or mixed precision, against the FP64 oracle on the same inputs.
+
+Bars: FP64 paths |dP| <= 1e-10 against the oracle converged at eps 5e-13; mixed precision
+|dP| <= 1e-6 and open-channel unitarity within 1e-6.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from multiscat_amd import batch as BT
+from multiscat_amd import workloads as W
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_batch():
+    # 6 conditions, 15 x 15 table, nz 40, 40..120 channels each (argsort order subsets)
+    w = W.hex_batch(n_conditions=6, nz=40, nk=15, cond_seed=5,
+                    channel_counts=[40, 75, 120, 60, 97, 150])
+    ref = []
+    for c in w.conditions:
+        psi = oracle.solve(1, 0, w.potential, c.e, c.wave_a, c.wave_b, c.wave_c, w.kinetic,
+                           channels=c.channels, eps=5e-13)
+        ref.append(oracle.intensities(psi, c.wave_a, c.wave_b)[c.channels[0], c.channels[1]])
+    tab = BT.Table(w.potential, w.kinetic)
+    return w, tab, BT.Batch(tab, w.conditions), ref
+
+
+def _maxdiff(Ps, ref):
+    return max(np.abs(a - b).max() for a, b in zip(Ps, ref))
+
+
+def test_counts_are_uneven(small_batch):
+    w, _, bt, _ = small_batch
+    assert len(set(bt.counts.tolist())) > 1
+
+
+@pytest.mark.parametrize("opts", [
+    dict(method="gauss_seidel"),
+    dict(method="gauss_seidel", neumann=False),
+    dict(method="jacobi"),
+    dict(method="jacobi", coupling="direct"),
+    dict(method="jacobi", neumann=False),
+    dict(method="gauss_seidel", restart=8),
+    dict(method="jacobi", restart=20),
+])
+def test_fp64_batch_matches_oracle(small_batch, opts):
+    _, _, bt, ref = small_batch
+    bt.solve(BT.SolveOptions(eps=5e-13, **opts))
+    assert _maxdiff(bt.intensities(), ref) < 1e-10
+    assert (bt.stats()["info"] == 0).all()
+
+
+@pytest.mark.parametrize("coupling", ["auto", "direct"])
+def test_mixed_batch_within_bar(small_batch, coupling):
+    w, _, bt, ref = small_batch
+    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11, coupling=coupling))
+    Ps = bt.intensities()
+    assert _maxdiff(Ps, ref) < 1e-6
+    for c, P in zip(w.conditions, Ps):
+        open_ = c.e[c.channels[0], c.channels[1]] < 0
+        assert abs(P[open_].sum() - 1.0) < 1e-6
+
+
+def test_jacobi_large_nz_uses_vector_blocks():
+    """nz > 320 takes the vector-ALU diagonal-block kernel instead of the MFMA one."""
+    w = W.hex_batch(n_conditions=2, nz=336, nk=7, cond_seed=8, channel_counts=[9, 14])
+    tab = BT.Table(w.potential, w.kinetic)
+    bt = BT.Batch(tab, w.conditions)
+    for method in ("jacobi", "gauss_seidel"):
+        bt.solve(BT.SolveOptions(method=method, eps=5e-13))
+        for c, P in zip(w.conditions, bt.intensities()):
+            psi = oracle.solve(1, 0, w.potential, c.e, c.wave_a, c.wave_b, c.wave_c, w.kinetic,
+                               channels=c.channels, eps=5e-13)
+            ref = oracle.intensities(psi, c.wave_a, c.wave_b)[c.channels[0], c.channels[1]]
+            assert np.abs(P - ref).max() < 1e-10
+
+
+def test_gs_batch_iterations_match_oracle(small_batch):
+    """Same algorithm, same iteration counts as the reference restatement."""
+    w, _, bt, _ = small_batch
+    bt.solve(BT.SolveOptions(eps=0.5e-8))
+    it = bt.stats()["iterations"]
+    for c, k in zip(w.conditions, it):
+        _, info = oracle.solve(1, 8, w.potential, c.e, c.wave_a, c.wave_b, c.wave_c, w.kinetic,
+                               channels=c.channels, return_info=True)
+        assert k == info["iterations"]
+
+
+def test_state_matches_oracle(small_batch):
+    w, _, bt, _ = small_batch
+    bt.solve(BT.SolveOptions(method="jacobi", eps=5e-13))
+    for c, psi in zip(w.conditions, bt.state()):
+        ref = oracle.solve(1, 0, w.potential, c.e, c.wave_a, c.wave_b, c.wave_c, w.kinetic,
+                           channels=c.channels, eps=5e-13)[c.channels[0], c.channels[1]]
+        np.testing.assert_allclose(psi, ref, rtol=0, atol=1e-9 * np.abs(ref).max())
+
+
+def test_not_converged_reports_code_2(small_batch):
+    _, _, bt, _ = small_batch
+    rc = bt.solve(BT.SolveOptions(eps=1e-14, maxiter=2), check=False)
+    assert rc == 2
+    assert (bt.stats()["info"] == 1).all()
+
+
+def test_eigensystem_on_gpu(small_batch):
+    w, tab, _, _ = small_batch
+    lam, Q = tab.eigensystem()
+    lam_ref, _ = oracle.specular_eigensystem(w.potential, w.kinetic)
+    np.testing.assert_allclose(lam, lam_ref, rtol=1e-12, atol=1e-10)
+    H0 = w.kinetic + np.diag(w.potential[0, 0].real)
+    np.testing.assert_allclose(Q.T @ Q, np.eye(lam.size), atol=1e-12)
+    np.testing.assert_allclose(Q.T @ H0 @ Q, np.diag(lam), atol=1e-10 * np.abs(lam).max())
+
+
+@pytest.mark.parametrize("nk,nc", [(15, 120), (31, 300), (11, 121), (7, 20)])
+def test_dft_coupling_matches_direct(nk, nc):
+    """Convolution (DFT on the matrix cores) vs direct sum of the full coupling, and the
+    direct sum vs the oracle, on a random table and state."""
+    import ctypes as C
+
+    from multiscat_amd import _lib
+
+    rng = np.random.default_rng(nk * 1000 + nc)
+    nz = 21
+    V = rng.standard_normal((nk, nk, nz)) + 1j * rng.standard_normal((nk, nk, nz))
+    V = np.ascontiguousarray(V)
+    order = rng.permutation(nk * nk)[:nc]
+    chx, chy = (np.ascontiguousarray(a, dtype=np.int32) for a in np.divmod(order, nk))
+    x = np.ascontiguousarray(rng.standard_normal((nc, nz)) + 1j * rng.standard_normal((nc, nz)))
+    ref = oracle.apply_coupling(V, x, channels=(chx, chy))
+    out = {}
+    for prec in (0, 1):
+        for alg in (0, 1):
+            y = np.zeros_like(x)
+            rc = _lib.lib().msx_debug_apply_coupling(nk, nk, nz, _lib.ptr(V), nc, _lib.ptr(chx),
+                                                     _lib.ptr(chy), _lib.ptr(x), _lib.ptr(y), 1,
+                                                     prec, alg)
+            _lib.check(rc, "msx_debug_apply_coupling")
+            out[prec, alg] = y
+    scale = np.abs(ref).max()
+    np.testing.assert_allclose(out[0, 0], ref, rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(out[0, 1], ref, rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(out[1, 0], ref, rtol=0, atol=1e-5 * scale)
+    np.testing.assert_allclose(out[1, 1], ref, rtol=0, atol=1e-5 * scale)
+    del C
+
+
+def test_upper_and_lower_coupling_modes_match_oracle():
+    from multiscat_amd import _lib
+
+    rng = np.random.default_rng(3)
+    nkx, nky, nz, nc = 9, 7, 33, 50
+    V = np.ascontiguousarray(rng.standard_normal((nkx, nky, nz)) + 1j * rng.standard_normal((nkx, nky, nz)))
+    order = rng.permutation(nkx * nky)[:nc]
+    chx, chy = (np.ascontiguousarray(a, dtype=np.int32) for a in np.divmod(order, nky))
+    x = np.ascontiguousarray(rng.standard_normal((nc, nz)) + 1j * rng.standard_normal((nc, nz)))
+    up = oracle.apply_upper(V, x, channels=(chx, chy))
+    full = oracle.apply_coupling(V, x, channels=(chx, chy))
+    for mode, ref in ((0, up), (2, full - up)):
+        y = np.zeros_like(x)
+        _lib.check(_lib.lib().msx_debug_apply_coupling(nkx, nky, nz, _lib.ptr(V), nc, _lib.ptr(chx),
+                                                       _lib.ptr(chy), _lib.ptr(x), _lib.ptr(y),
+                                                       mode, 0, 0), "coupling")
+        np.testing.assert_allclose(y, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
