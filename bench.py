@@ -58,13 +58,14 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--conditions", type=int, default=256)
-    ap.add_argument("--diag", default="auto", choices=["auto", "vector"])
+    ap.add_argument("--diag", default="auto", choices=["auto", "vector", "mfma"])
     ap.add_argument("--skip-baselines", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: every box here has a single MI355X and no "
-              "multi-GPU figure is produced (see DESIGN.md for how the batch would shard).",
+              "multi-GPU figure is produced; a batch of conditions shards by condition "
+              "across GPUs with no communication.",
               file=sys.stderr)
         return 2
 

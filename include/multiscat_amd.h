@@ -168,7 +168,7 @@ typedef struct {
     int max_refine;   /* mixed precision: FP64 refinement sweeps allowed                    */
     int coupling;     /* full coupling: 0 = DFT form on the matrix cores when the table fits
                          (<= 32 x 32), 1 = direct sum                                      */
-    int diag;         /* all-channel D^-1: 0 = matrix-core GEMMs when nz <= 320, 1 = vector */
+    int diag;         /* all-channel D^-1: 0 = auto (vector), 1 = vector, 2 = matrix-core GEMMs (nz <= 320) */
 } msx_solve_options;
 
 /* Fills `opt` with the reference's settings for a given nsig (GS, FP64, window 3, 2000). */
@@ -185,6 +185,9 @@ int msx_table_prepare(msx_table *table);
 int msx_table_destroy(msx_table *table);
 /* Copies the eigensystem to the host: lam[nz] ascending, Q row-major (Q[z*nz + m]). */
 int msx_table_eigensystem(const msx_table *table, double *lam, double *Q);
+/* Durations (ms) of the four phases of the last eigensolve: tridiagonalisation, accumulation
+ * of the Householder reflectors, QL iteration, reordering. Diagnostics only. */
+int msx_table_eig_phases(const msx_table *table, double *ms);
 
 /* Builds a batch of n_cond conditions on `table`. Channel lists are concatenated:
  * condition b owns channels [off_b, off_b + n_channels[b]) with off_b the prefix sum.

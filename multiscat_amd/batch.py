@@ -31,7 +31,7 @@ class SolveOptions:
     inner_eps: float = 1e-5
     max_refine: int = 6
     coupling: str = "auto"  # or "direct"
-    diag: str = "auto"  # or "vector"
+    diag: str = "auto"  # "auto" (fastest), "vector" or "mfma" (matrix cores, nz <= 320)
 
     @classmethod
     def from_nsig(cls, nsig: int, **kw) -> "SolveOptions":
@@ -57,9 +57,9 @@ class SolveOptions:
         if self.coupling not in ("auto", "direct"):
             raise ValueError(f"unknown coupling {self.coupling!r}")
         o.coupling = 1 if self.coupling == "direct" else 0
-        if self.diag not in ("auto", "vector"):
+        if self.diag not in ("auto", "vector", "mfma"):
             raise ValueError(f"unknown diag {self.diag!r}")
-        o.diag = 1 if self.diag == "vector" else 0
+        o.diag = {"auto": 0, "vector": 1, "mfma": 2}[self.diag]
         return o
 
 
@@ -87,6 +87,13 @@ class Table:
         _lib.check(_lib.lib().msx_table_eigensystem(self._h, _lib.ptr(lam), _lib.ptr(Q)),
                    "msx_table_eigensystem")
         return lam, Q
+
+    def eig_phases(self) -> np.ndarray:
+        """Durations in ms of the last eigensolve's phases (tridiagonalisation, reflector
+        accumulation, QL iteration, reordering)."""
+        ms = np.zeros(4)
+        _lib.check(_lib.lib().msx_table_eig_phases(self._h, _lib.ptr(ms)), "msx_table_eig_phases")
+        return ms
 
     def close(self) -> None:
         if self._h:

@@ -8,8 +8,9 @@
 //       Computations, sec. 8.3.1), every reflector applied as a parallel rank-2 update;
 //   (2) implicit QL iteration with shifts on the tridiagonal T (Bowdler, Martin, Reinsch and
 //       Wilkinson, Numer. Math. 11 (1968) 293), where one lane runs the scalar bulge chase of a
-//       sweep and records its rotations, and then every lane applies the whole rotation
-//       sequence to its own column of Z (stored transposed, so the updates are coalesced).
+//       sweep and records its rotations while the other wavefronts apply the previous sweep's
+//       rotation sequence, each lane to its own column of Z (stored transposed, so the
+//       updates are coalesced).
 // The eigenvalues are then ranked ascending and the eigenvectors permuted accordingly.
 #include <cfloat>
 
@@ -36,7 +37,8 @@ __global__ __launch_bounds__(EIG_THREADS) void eig_tridiag_ql_kernel(int n, doub
                                                                      double *lam_sorted,
                                                                      double *Q_sorted,
                                                                      double *QT_sorted,
-                                                                     int *info) {
+                                                                     int *info,
+                                                                     long long *ticks) {
     extern __shared__ double es[];
     double *v = es;                      // [n]
     double *w = v + n;                   // [n]
@@ -47,10 +49,11 @@ __global__ __launch_bounds__(EIG_THREADS) void eig_tridiag_ql_kernel(int n, doub
     double *sn = cs + n;                 // [n]
     double *bet = sn + n;                // [n]
     __shared__ double red[EIG_THREADS / 64];
-    __shared__ int s_hi, s_stop, s_done, s_fail;
+    __shared__ int s_fail;
     const int tid = threadIdx.x, NT = blockDim.x;
     const int grp = tid / (NT / EIG_GROUPS), gl = tid % (NT / EIG_GROUPS), GS = NT / EIG_GROUPS;
 
+    if (ticks && tid == 0) ticks[0] = wall_clock64();
     // ---- (1) Householder tridiagonalisation ------------------------------------------
     for (int k = 0; k + 2 < n; ++k) {
         const int m = n - k - 1, r0 = k + 1;
@@ -120,6 +123,7 @@ __global__ __launch_bounds__(EIG_THREADS) void eig_tridiag_ql_kernel(int n, doub
     }
     __syncthreads();
 
+    if (ticks && tid == 0) ticks[1] = wall_clock64();
     // ---- Q_H = H_0 H_1 ... H_{n-3} (row-major in W), accumulated backwards -----------------
     for (long long idx = tid; idx < (long long)n * n; idx += NT) {
         const int i = (int)(idx / n), j = (int)(idx - (long long)i * n);
@@ -161,74 +165,115 @@ __global__ __launch_bounds__(EIG_THREADS) void eig_tridiag_ql_kernel(int n, doub
     }
     __syncthreads();
 
+    if (ticks && tid == 0) ticks[2] = wall_clock64();
     // ---- (2) implicit QL with shifts on (d, e), rotations applied to the rows of Z^T -----
-    for (int l = 0; l < n; ++l) {
-        for (int iter = 0;; ++iter) {
-            if (tid == 0) {
-                int m;
-                for (m = l; m < n - 1; ++m) {
-                    const double dd = fabs(d[m]) + fabs(d[m + 1]);
-                    if (fabs(e[m]) <= DBL_EPSILON * dd) break;
-                }
-                if (m == l) {
-                    s_done = 1;
-                } else if (iter >= 60) {
-                    s_done = 1;
-                    s_fail = 1;
-                } else {
-                    s_done = 0;
-                    double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-                    double r = hypot(g, 1.0);
-                    g = d[m] - d[l] + e[l] / (g + copysign(r, g));
-                    double s = 1.0, c = 1.0, p = 0.0;
-                    int i;
-                    bool underflow = false;
-                    for (i = m - 1; i >= l; --i) {
-                        const double f = s * e[i], b = c * e[i];
-                        r = hypot(f, g);
-                        e[i + 1] = r;
-                        if (r == 0.0) {
-                            d[i + 1] -= p;
-                            e[m] = 0.0;
-                            underflow = true;
-                            break;
-                        }
-                        s = f / r;
-                        c = g / r;
-                        g = d[i + 1] - p;
-                        r = (d[i] - g) * s + 2.0 * c * b;
-                        p = s * r;
-                        d[i + 1] = g + p;
-                        g = c * r - b;
-                        cs[i] = c;
-                        sn[i] = s;
-                    }
-                    s_hi = m - 1;
-                    s_stop = underflow ? i + 1 : l;
-                    if (!underflow) {
-                        d[l] -= p;
-                        e[l] = g;
-                        e[m] = 0.0;
-                    }
-                }
+    // Lane 0 runs the scalar bulge chases; a sweep only reads and writes (d, e), so while lane 0
+    // chases sweep s+1 into one rotation buffer, wavefronts 1.. apply sweep s from the other
+    // buffer to the columns of Z^T. pp is free after the tridiagonalisation and holds the
+    // second buffer.
+    double *cbuf[2] = {cs, pp}, *sbuf[2] = {sn, pp + n};
+    int ql_l = 0, ql_iter = 0;  // lane 0's position in the QL iteration
+    // One sweep of the iteration into (c_out, s_out); false when every eigenvalue has converged.
+    auto ql_sweep = [&](double *c_out, double *s_out, int &hi, int &lo) -> bool {
+        while (ql_l < n) {
+            const int l = ql_l;
+            int m;
+            for (m = l; m < n - 1; ++m) {
+                const double dd = fabs(d[m]) + fabs(d[m + 1]);
+                if (fabs(e[m]) <= DBL_EPSILON * dd) break;
             }
-            __syncthreads();
-            if (s_done) break;
-            const int hi = s_hi, lo = s_stop;
-            for (int k = tid; k < n; k += NT) {
+            if (m == l) {
+                ++ql_l;
+                ql_iter = 0;
+                continue;
+            }
+            if (ql_iter >= 60) {
+                s_fail = 1;
+                ++ql_l;
+                ql_iter = 0;
+                continue;
+            }
+            ++ql_iter;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = hypot(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + copysign(r, g));
+            double s = 1.0, c = 1.0, p = 0.0;
+            double dn = d[m];  // d[i + 1] of the current step (read, not yet rewritten)
+            double ei = e[m - 1], di = d[m - 1];
+            int i;
+            bool underflow = false;
+            for (i = m - 1; i >= l; --i) {
+                const double e_next = i > l ? e[i - 1] : 0.0, d_next = i > l ? d[i - 1] : 0.0;
+                const double f = s * ei, b = c * ei;
+                r = sqrt(f * f + g * g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] = dn - p;
+                    e[m] = 0.0;
+                    underflow = true;
+                    break;
+                }
+                const double rinv = 1.0 / r;
+                s = f * rinv;
+                c = g * rinv;
+                g = dn - p;
+                r = (di - g) * s + 2.0 * c * b;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - b;
+                c_out[i] = c;
+                s_out[i] = s;
+                dn = di;
+                ei = e_next;
+                di = d_next;
+            }
+            hi = m - 1;
+            lo = underflow ? i + 1 : l;
+            if (!underflow) {
+                d[l] -= p;
+                e[l] = g;
+                e[m] = 0.0;
+            }
+            return true;
+        }
+        return false;
+    };
+    __shared__ int s_more[2], s_hi2[2], s_lo2[2];
+    if (tid == 0) s_more[0] = ql_sweep(cbuf[0], sbuf[0], s_hi2[0], s_lo2[0]);
+    __syncthreads();
+    constexpr int QC = 16;  // rows of Z^T loaded ahead per column
+    for (int cur = 0;; cur ^= 1) {
+        if (!s_more[cur]) break;
+        const int hi = s_hi2[cur], lo = s_lo2[cur];
+        if (tid == 0) {
+            s_more[cur ^ 1] = ql_sweep(cbuf[cur ^ 1], sbuf[cur ^ 1], s_hi2[cur ^ 1], s_lo2[cur ^ 1]);
+        } else if (tid >= 64) {
+            const double *cb = cbuf[cur], *sb = sbuf[cur];
+            for (int k = tid - 64; k < n; k += NT - 64) {
                 double zi1 = ZT[(long long)(hi + 1) * n + k];
-                for (int i = hi; i >= lo; --i) {
-                    const double c = cs[i], s = sn[i];
-                    const double zi = ZT[(long long)i * n + k];
-                    ZT[(long long)(i + 1) * n + k] = s * zi + c * zi1;
-                    zi1 = c * zi - s * zi1;
+                for (int i0 = hi; i0 >= lo; i0 -= QC) {
+                    const int cnt = min(QC, i0 - lo + 1);
+                    double z[QC];
+#pragma unroll
+                    for (int t = 0; t < QC; ++t)
+                        if (t < cnt) z[t] = ZT[(long long)(i0 - t) * n + k];
+#pragma unroll
+                    for (int t = 0; t < QC; ++t) {
+                        if (t < cnt) {
+                            const int i = i0 - t;
+                            const double c = cb[i], s = sb[i];
+                            ZT[(long long)(i + 1) * n + k] = s * z[t] + c * zi1;
+                            zi1 = c * z[t] - s * zi1;
+                        }
+                    }
                 }
                 ZT[(long long)lo * n + k] = zi1;
             }
-            __syncthreads();
         }
+        __syncthreads();
     }
 
+    if (ticks && tid == 0) ticks[3] = wall_clock64();
     // ---- ascending order ---------------------------------------------------------------------
     for (int i = tid; i < n; i += NT) {
         const double li = d[i];
@@ -241,16 +286,19 @@ __global__ __launch_bounds__(EIG_THREADS) void eig_tridiag_ql_kernel(int n, doub
             QT_sorted[(long long)rank * n + z] = val;
         }
     }
-    if (tid == 0) *info = s_fail;
+    if (tid == 0) {
+        *info = s_fail;
+        if (ticks) ticks[4] = wall_clock64();
+    }
 }
 
 void launch_eigensystem(int n, double *A, double *Vec, double *lam_sorted, double *Q_sorted,
-                        double *QT_sorted, int *info, hipStream_t s) {
+                        double *QT_sorted, int *info, long long *ticks, hipStream_t s) {
     if (n < 1 || n > 2048) throw Error(3, "eigensystem: 1 <= n <= 2048 required");
     const size_t shm = sizeof(double) * (size_t)n * (7 + EIG_GROUPS);
     set_dynamic_lds((const void *)eig_tridiag_ql_kernel, shm);
     hipLaunchKernelGGL(eig_tridiag_ql_kernel, dim3(1), dim3(EIG_THREADS), shm, s, n, A, Vec,
-                       lam_sorted, Q_sorted, QT_sorted, info);
+                       lam_sorted, Q_sorted, QT_sorted, info, ticks);
     MSX_LAUNCH_CHECK();
 }
 

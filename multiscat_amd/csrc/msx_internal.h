@@ -197,9 +197,11 @@ void launch_apply_diag(const Layout &L, const double *Q, const double *QT, const
                        const double *e, const cx<double> *c, const cx<double> *x,
                        cx<double> *y, const int *active, hipStream_t s);
 
-// Eigensystem of a symmetric n x n matrix (one workgroup, parallel cyclic Jacobi).
+// Eigensystem of a symmetric n x n matrix (one workgroup, Householder + implicit QL).
+// ticks (nullable, 5 entries): wall-clock stamps at the start, after the tridiagonalisation,
+// after the accumulation of Q_H, after the QL iteration and at the end.
 void launch_eigensystem(int n, double *A, double *Vec, double *lam_sorted, double *Q_sorted,
-                        double *QT_sorted, int *info, hipStream_t s);
+                        double *QT_sorted, int *info, long long *ticks, hipStream_t s);
 
 // fac[j][z] = sum_m Q[z][m] Q[N-1][m] / (e_j + lam_m) for all channels.
 void launch_preconditioner_factors_qt(long long total_channels, int nz, const double *Q,

@@ -227,6 +227,7 @@ struct msx_table {
     DBuf<double> T, A, Vm, lam, Q, QT, Qp, QTp;
     DBuf<float> Qf, QTf, Qpf, QTpf;
     DBuf<int> info;
+    DBuf<long long> ticks;  // phase stamps of the last eigensolve
     int nzp = 0;          // nz rounded up to a multiple of 16 (matrix-core tiles)
     DftStore dft, dft_t;  // convolution form of the coupling (table and flipped table)
     bool flipped = false;
@@ -245,7 +246,7 @@ struct msx_table {
             const double n = nz;
             ProfScope ps("eigensystem_f64", stream, (4.0 / 3.0 + 4.0 / 3.0 + 6.0) * n * n * n,
                          5.0 * 8.0 * n * n);
-            launch_eigensystem(nz, A.p, Vm.p, lam.p, Q.p, QT.p, info.p, stream);
+            launch_eigensystem(nz, A.p, Vm.p, lam.p, Q.p, QT.p, info.p, ticks.p, stream);
         }
         int h_info = 0;
         info.download(&h_info, 1, stream);
@@ -518,7 +519,7 @@ void check_options(const msx_solve_options *o) {
         throw Error(MSX_ERR_INVALID, "mixed precision runs with the block-Jacobi method");
     if (!(o->eps > 0)) throw Error(MSX_ERR_INVALID, "eps must be positive");
     if (o->coupling != 0 && o->coupling != 1) throw Error(MSX_ERR_INVALID, "unknown coupling");
-    if (o->diag != 0 && o->diag != 1) throw Error(MSX_ERR_INVALID, "unknown diag kernel");
+    if (o->diag < 0 || o->diag > 2) throw Error(MSX_ERR_INVALID, "unknown diag kernel");
     if (o->maxiter <= 0) throw Error(MSX_ERR_INVALID, "maxiter must be positive");
     if (o->window <= 0) throw Error(MSX_ERR_INVALID, "window must be positive");
 }
@@ -536,7 +537,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     msx_batch::need(B.tmp2, n);
     if (!B.factors_ready) B.build_factors();
     Ops ops(B, o.neumann == 1, transposed, o.coupling == 1);
-    ops.vector_diag = o.diag == 1;
+    ops.vector_diag = o.diag != 2;  // auto: the vector kernel, the faster one at nz <= 320
     MSX_HIP(hipMemcpyAsync(B.src.p, src_in, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, s));
     MSX_HIP(hipMemsetAsync(x, 0, sizeof(cx<double>) * n, s));
     const int kdim = krylov_dim(o, B.n_inst);
@@ -742,6 +743,7 @@ int msx_table_create(int nkx, int nky, int nz, const double *potential, const do
         t->Qf.alloc(n2);
         t->QTf.alloc(n2);
         t->info.alloc(1);
+        t->ticks.alloc(5);
         t->prepare();
         *out = t.release();
         return MSX_OK;
@@ -754,6 +756,21 @@ int msx_table_prepare(msx_table *t) {
     try {
         if (!t) throw Error(MSX_ERR_INVALID, "null table");
         t->prepare();
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+int msx_table_eig_phases(const msx_table *t, double *ms) {
+    try {
+        if (!t || !ms) throw Error(MSX_ERR_INVALID, "null argument");
+        long long tk[5];
+        t->ticks.download(tk, 5, t->stream);
+        MSX_HIP(hipStreamSynchronize(t->stream));
+        int rate_khz = 0;
+        MSX_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, 0));
+        for (int i = 0; i < 4; ++i) ms[i] = rate_khz > 0 ? (double)(tk[i + 1] - tk[i]) / rate_khz : 0.0;
         return MSX_OK;
     } catch (const Error &e) {
         return fail(e);

@@ -49,6 +49,8 @@ def test_counts_are_uneven(small_batch):
     dict(method="jacobi", neumann=False),
     dict(method="gauss_seidel", restart=8),
     dict(method="jacobi", restart=20),
+    dict(method="jacobi", diag="mfma"),
+    dict(method="jacobi", diag="vector"),
 ])
 def test_fp64_batch_matches_oracle(small_batch, opts):
     _, _, bt, ref = small_batch
@@ -57,10 +59,12 @@ def test_fp64_batch_matches_oracle(small_batch, opts):
     assert (bt.stats()["info"] == 0).all()
 
 
-@pytest.mark.parametrize("coupling", ["auto", "direct"])
-def test_mixed_batch_within_bar(small_batch, coupling):
+@pytest.mark.parametrize("coupling,diag", [("auto", "auto"), ("direct", "auto"),
+                                           ("auto", "mfma"), ("auto", "vector")])
+def test_mixed_batch_within_bar(small_batch, coupling, diag):
     w, _, bt, ref = small_batch
-    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11, coupling=coupling))
+    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11, coupling=coupling,
+                             diag=diag))
     Ps = bt.intensities()
     assert _maxdiff(Ps, ref) < 1e-6
     for c, P in zip(w.conditions, Ps):
