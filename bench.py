@@ -52,6 +52,35 @@ def _sync():
         pass
 
 
+def _copy_probe(mib: int) -> int:
+    """20 device-to-device copies of `mib` MiB through torch's elementwise copy kernel: each
+    reads and writes exactly mib MiB, the reference for FETCH_SIZE / WRITE_SIZE scaling."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(20):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_each_way": n * 4, "copies": 20}))
+    return 0
+
+
+def _pmc_traffic(kernel: str):
+    """HBM bytes per launch of a library kernel from the committed profiler summary
+    (profiles/pmc_traffic_hex31x31_nz150_256cond.txt: lines `name fetch_bytes write_bytes`,
+    already scaled by the copy-kernel calibration); None when the kernel is not listed."""
+    path = ROOT / "profiles" / "pmc_traffic_hex31x31_nz150_256cond.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == kernel:
+            return float(parts[1]) + float(parts[2])
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,7 +90,12 @@ def main() -> int:
     ap.add_argument("--diag", default="auto", choices=["auto", "vector", "mfma"])
     ap.add_argument("--skip-baselines", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 20 plain device copies of MIB MiB (calibrates the HBM byte "
+                         "counters of a profiler run against a known size) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        return _copy_probe(args.copy_probe)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: every box here has a single MI355X and no "
               "multi-GPU figure is produced; a batch of conditions shards by condition "
@@ -118,6 +152,7 @@ def main() -> int:
             roofline = {"kernel": name, "bound": "mfma", "achieved": ach,
                         "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
                         "frac": ach / PEAK_TFLOPS[prec], "traffic": None}
+        roofline["traffic"] = _pmc_traffic(name)
         roofline.update({"launches": d["count"], "avg_launch_us": avg_s * 1e6,
                          "algorithmic_flops_per_launch": per_launch_flops,
                          "algorithmic_bytes_per_launch": per_launch_bytes,
@@ -165,6 +200,18 @@ def main() -> int:
         dt = time.perf_counter() - t0
         base["plain_fp64_single_call"] = {"solves_per_s": len(sample) / dt,
                                           "sample": f"{len(sample)} conditions, one call each"}
+        # PCIe-inclusive rate: host arrays in, intensities out (table upload, channel tables,
+        # eigensystem, solve, download of P), for comparison with the HBM-resident value
+        t0 = time.perf_counter()
+        for _ in range(2):
+            tab2 = BT.Table(w.potential, w.kinetic)
+            bt2 = BT.Batch(tab2, w.conditions)
+            bt2.solve(mixed)
+            bt2.intensities_flat()
+            bt2.close()
+            tab2.close()
+        dt = (time.perf_counter() - t0) / 2
+        base["mixed_batched_pcie_inclusive"] = {"solves_per_s": args.conditions / dt}
         out["gpu_baselines"] = base
         out["speedup_vs_plain_fp64_single_call"] = value / base["plain_fp64_single_call"]["solves_per_s"]
         out["speedup_vs_fp64_batched_gs"] = value / base["gauss_seidel_fp64_batched"]["solves_per_s"]

@@ -168,7 +168,7 @@ typedef struct {
     int max_refine;   /* mixed precision: FP64 refinement sweeps allowed                    */
     int coupling;     /* full coupling: 0 = DFT form on the matrix cores when the table fits
                          (<= 32 x 32), 1 = direct sum                                      */
-    int diag;         /* all-channel D^-1: 0 = auto (vector), 1 = vector, 2 = matrix-core GEMMs (nz <= 320) */
+    int diag;         /* all-channel D^-1: 0 = auto (matrix cores when nz <= 320), 1 = vector, 2 = matrix cores */
 } msx_solve_options;
 
 /* Fills `opt` with the reference's settings for a given nsig (GS, FP64, window 3, 2000). */

@@ -537,7 +537,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     msx_batch::need(B.tmp2, n);
     if (!B.factors_ready) B.build_factors();
     Ops ops(B, o.neumann == 1, transposed, o.coupling == 1);
-    ops.vector_diag = o.diag != 2;  // auto: the vector kernel, the faster one at nz <= 320
+    ops.vector_diag = o.diag == 1;  // auto: matrix cores when nz <= 320
     MSX_HIP(hipMemcpyAsync(B.src.p, src_in, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, s));
     MSX_HIP(hipMemsetAsync(x, 0, sizeof(cx<double>) * n, s));
     const int kdim = krylov_dim(o, B.n_inst);
