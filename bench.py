@@ -43,13 +43,10 @@ METRIC = "S-matrix solves/s at |dP|<=1e-6, mixed+batched (256 uneven conditions)
 
 
 def _sync():
-    try:
-        import torch
+    """Device-wide wait through the library (its HIP runtime owns the device here)."""
+    from multiscat_amd import _lib
 
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+    _lib.check(_lib.lib().msx_synchronize(), "msx_synchronize")
 
 
 def _copy_probe(mib: int) -> int:

@@ -45,6 +45,10 @@ const char *msx_last_error(void);
 /* Library/device identity: writes the gfx target name (e.g. "gfx950") into buf. */
 int msx_device_info(char *buf, int buflen, int *device_count);
 
+/* Waits for all work the library queued on the current device (timing helper; every
+ * solve entry point already returns with its results complete). */
+int msx_synchronize(void);
+
 /* ---------------------------------------------------------------------------------------
  * (1) Drop-in group: mirrors lib/multiscat_fortran/src/multiscat_fortran/__init__.py:39-169
  *     and the f2py interface lib/multiscat_fortran/fortran/multiscat_python_bindings.pyf:3-97.
@@ -185,8 +189,9 @@ int msx_table_prepare(msx_table *table);
 int msx_table_destroy(msx_table *table);
 /* Copies the eigensystem to the host: lam[nz] ascending, Q row-major (Q[z*nz + m]). */
 int msx_table_eigensystem(const msx_table *table, double *lam, double *Q);
-/* Durations (ms) of the four phases of the last eigensolve: tridiagonalisation, accumulation
- * of the Householder reflectors, QL iteration, reordering. Diagnostics only. */
+/* Diagnostics of the last eigensolve, ms[7]: durations (ms) of its four phases
+ * (tridiagonalisation, accumulation of the Householder reflectors, QL iteration, reordering),
+ * the time (ms) one lane spent chasing QL bulges, the number of QL sweeps and of rotations. */
 int msx_table_eig_phases(const msx_table *table, double *ms);
 
 /* Builds a batch of n_cond conditions on `table`. Channel lists are concatenated:

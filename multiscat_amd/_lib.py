@@ -28,7 +28,7 @@ PRECISION_MIXED = 1
 
 # Every symbol include/multiscat_amd.h declares.
 EXPORTED = (
-    "msx_last_error", "msx_device_info", "msx_run_multiscat_fortran",
+    "msx_last_error", "msx_device_info", "msx_synchronize", "msx_run_multiscat_fortran",
     "msx_run_multiscat_fortran_ex", "msx_get_perpendicular_kinetic_difference",
     "msx_get_parallel_kinetic_energy", "msx_get_lobatto_weights", "msx_get_abc_arrays",
     "msx_debug_build_preconditioner", "msx_debug_apply_upper_block",
@@ -77,6 +77,7 @@ def _declare(lib):
     sig = {
         "msx_last_error": (C.c_char_p, []),
         "msx_device_info": (i32, [C.c_char_p, i32, C.POINTER(i32)]),
+        "msx_synchronize": (i32, []),
         "msx_run_multiscat_fortran": (i32, [i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp]),
         "msx_run_multiscat_fortran_ex": (
             i32, [i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, C.POINTER(i32),

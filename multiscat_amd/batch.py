@@ -89,9 +89,9 @@ class Table:
         return lam, Q
 
     def eig_phases(self) -> np.ndarray:
-        """Durations in ms of the last eigensolve's phases (tridiagonalisation, reflector
-        accumulation, QL iteration, reordering)."""
-        ms = np.zeros(4)
+        """Last eigensolve: ms of its phases (tridiagonalisation, reflector accumulation, QL
+        iteration, reordering), ms of bulge chasing, number of QL sweeps and rotations."""
+        ms = np.zeros(7)
         _lib.check(_lib.lib().msx_table_eig_phases(self._h, _lib.ptr(ms)), "msx_table_eig_phases")
         return ms
 

@@ -174,13 +174,23 @@ __global__ __launch_bounds__(EIG_THREADS) void eig_tridiag_ql_kernel(int n, doub
     double *cbuf[2] = {cs, pp}, *sbuf[2] = {sn, pp + n};
     int ql_l = 0, ql_iter = 0;  // lane 0's position in the QL iteration
     // One sweep of the iteration into (c_out, s_out); false when every eigenvalue has converged.
+    // Run by all 64 lanes of wavefront 0 (identical ql_l, ql_iter in every lane): the search
+    // for the first negligible e[m], m >= l, tests 64 indices per LDS round trip and takes the
+    // lowest hit of a ballot; lane 0 alone chases the bulge and writes hi, lo.
+    const int lane = tid & 63;
     auto ql_sweep = [&](double *c_out, double *s_out, int &hi, int &lo) -> bool {
         while (ql_l < n) {
             const int l = ql_l;
-            int m;
-            for (m = l; m < n - 1; ++m) {
-                const double dd = fabs(d[m]) + fabs(d[m + 1]);
-                if (fabs(e[m]) <= DBL_EPSILON * dd) break;
+            int m = n - 1;
+            for (int base = l; base < n - 1; base += 64) {
+                const int mm = base + lane;
+                bool small = false;
+                if (mm < n - 1) small = fabs(e[mm]) <= DBL_EPSILON * (fabs(d[mm]) + fabs(d[mm + 1]));
+                const unsigned long long hits = __ballot(small);
+                if (hits) {
+                    m = base + __ffsll(hits) - 1;
+                    break;
+                }
             }
             if (m == l) {
                 ++ql_l;
@@ -194,6 +204,7 @@ __global__ __launch_bounds__(EIG_THREADS) void eig_tridiag_ql_kernel(int n, doub
                 continue;
             }
             ++ql_iter;
+            if (lane != 0) return true;
             double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
             double r = hypot(g, 1.0);
             g = d[m] - d[l] + e[l] / (g + copysign(r, g));
@@ -203,7 +214,8 @@ __global__ __launch_bounds__(EIG_THREADS) void eig_tridiag_ql_kernel(int n, doub
             int i;
             bool underflow = false;
             for (i = m - 1; i >= l; --i) {
-                const double e_next = i > l ? e[i - 1] : 0.0, d_next = i > l ? d[i - 1] : 0.0;
+                const int inext = i > l ? i - 1 : l;  // in range; unused at i == l
+                const double e_next = e[inext], d_next = d[inext];
                 const double f = s * ei, b = c * ei;
                 r = sqrt(f * f + g * g);
                 e[i + 1] = r;
@@ -239,15 +251,32 @@ __global__ __launch_bounds__(EIG_THREADS) void eig_tridiag_ql_kernel(int n, doub
         return false;
     };
     __shared__ int s_more[2], s_hi2[2], s_lo2[2];
-    if (tid == 0) s_more[0] = ql_sweep(cbuf[0], sbuf[0], s_hi2[0], s_lo2[0]);
+    long long chase_cycles = 0, n_sweeps = 0, n_steps = 0;  // wavefront 0's diagnostics
+    auto timed_sweep = [&](int slot) -> int {
+        const long long t0 = ticks ? wall_clock64() : 0;
+        const int more = ql_sweep(cbuf[slot], sbuf[slot], s_hi2[slot], s_lo2[slot]);
+        if (ticks && tid == 0) {
+            chase_cycles += wall_clock64() - t0;
+            if (more) {
+                ++n_sweeps;
+                n_steps += s_hi2[slot] - s_lo2[slot] + 1;
+            }
+        }
+        return more;
+    };
+    if (tid < 64) {
+        const int more = timed_sweep(0);
+        if (tid == 0) s_more[0] = more;
+    }
     __syncthreads();
     constexpr int QC = 16;  // rows of Z^T loaded ahead per column
     for (int cur = 0;; cur ^= 1) {
         if (!s_more[cur]) break;
         const int hi = s_hi2[cur], lo = s_lo2[cur];
-        if (tid == 0) {
-            s_more[cur ^ 1] = ql_sweep(cbuf[cur ^ 1], sbuf[cur ^ 1], s_hi2[cur ^ 1], s_lo2[cur ^ 1]);
-        } else if (tid >= 64) {
+        if (tid < 64) {
+            const int more = timed_sweep(cur ^ 1);
+            if (tid == 0) s_more[cur ^ 1] = more;
+        } else {
             const double *cb = cbuf[cur], *sb = sbuf[cur];
             for (int k = tid - 64; k < n; k += NT - 64) {
                 double zi1 = ZT[(long long)(hi + 1) * n + k];
@@ -288,7 +317,12 @@ __global__ __launch_bounds__(EIG_THREADS) void eig_tridiag_ql_kernel(int n, doub
     }
     if (tid == 0) {
         *info = s_fail;
-        if (ticks) ticks[4] = wall_clock64();
+        if (ticks) {
+            ticks[4] = wall_clock64();
+            ticks[5] = chase_cycles;
+            ticks[6] = n_sweeps;
+            ticks[7] = n_steps;
+        }
     }
 }
 

@@ -198,8 +198,9 @@ void launch_apply_diag(const Layout &L, const double *Q, const double *QT, const
                        cx<double> *y, const int *active, hipStream_t s);
 
 // Eigensystem of a symmetric n x n matrix (one workgroup, Householder + implicit QL).
-// ticks (nullable, 5 entries): wall-clock stamps at the start, after the tridiagonalisation,
-// after the accumulation of Q_H, after the QL iteration and at the end.
+// ticks (nullable, 8 entries): wall-clock stamps at the start, after the tridiagonalisation,
+// after the accumulation of Q_H, after the QL iteration and at the end; then the wall-clock
+// ticks lane 0 spent chasing bulges, the number of QL sweeps and of rotations.
 void launch_eigensystem(int n, double *A, double *Vec, double *lam_sorted, double *Q_sorted,
                         double *QT_sorted, int *info, long long *ticks, hipStream_t s);
 

@@ -687,6 +687,15 @@ extern "C" {
 
 const char *msx_last_error(void) { return g_last_error.c_str(); }
 
+int msx_synchronize(void) {
+    try {
+        MSX_HIP(hipDeviceSynchronize());
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
 int msx_device_info(char *buf, int buflen, int *device_count) {
     try {
         int n = 0;
@@ -743,7 +752,7 @@ int msx_table_create(int nkx, int nky, int nz, const double *potential, const do
         t->Qf.alloc(n2);
         t->QTf.alloc(n2);
         t->info.alloc(1);
-        t->ticks.alloc(5);
+        t->ticks.alloc(8);
         t->prepare();
         *out = t.release();
         return MSX_OK;
@@ -765,12 +774,15 @@ int msx_table_prepare(msx_table *t) {
 int msx_table_eig_phases(const msx_table *t, double *ms) {
     try {
         if (!t || !ms) throw Error(MSX_ERR_INVALID, "null argument");
-        long long tk[5];
-        t->ticks.download(tk, 5, t->stream);
+        long long tk[8];
+        t->ticks.download(tk, 8, t->stream);
         MSX_HIP(hipStreamSynchronize(t->stream));
         int rate_khz = 0;
         MSX_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, 0));
         for (int i = 0; i < 4; ++i) ms[i] = rate_khz > 0 ? (double)(tk[i + 1] - tk[i]) / rate_khz : 0.0;
+        ms[4] = rate_khz > 0 ? (double)tk[5] / rate_khz : 0.0;
+        ms[5] = (double)tk[6];
+        ms[6] = (double)tk[7];
         return MSX_OK;
     } catch (const Error &e) {
         return fail(e);

@@ -123,6 +123,21 @@ def test_eigensystem_on_gpu(small_batch):
     np.testing.assert_allclose(Q.T @ H0 @ Q, np.diag(lam), atol=1e-10 * np.abs(lam).max())
 
 
+@pytest.mark.parametrize("nz", [150, 300])
+def test_eigensystem_on_gpu_large(nz):
+    """Eigensystem of H0 at the production sizes: orthogonality, residual and eigenvalues
+    against numpy.linalg.eigh."""
+    w = W.hex_batch(n_conditions=1, nz=nz, nk=3, cond_seed=2, channel_counts=[5])
+    tab = BT.Table(w.potential, w.kinetic)
+    lam, Q = tab.eigensystem()
+    H0 = w.kinetic + np.diag(w.potential[0, 0].real)
+    lam_ref = np.linalg.eigvalsh(H0)
+    scale = np.abs(lam_ref).max()
+    np.testing.assert_allclose(lam, lam_ref, rtol=0, atol=1e-13 * scale)
+    np.testing.assert_allclose(Q.T @ Q, np.eye(nz), atol=2e-13 * nz ** 0.5)
+    np.testing.assert_allclose(Q.T @ H0 @ Q, np.diag(lam), atol=1e-13 * scale)
+
+
 @pytest.mark.parametrize("nk,nc", [(15, 120), (31, 300), (11, 121), (7, 20)])
 def test_dft_coupling_matches_direct(nk, nc):
     """Convolution (DFT on the matrix cores) vs direct sum of the full coupling, and the
