@@ -10,9 +10,9 @@
 // Each of the four products is a small complex GEMM (NP x NP, NP = 16 or 32 >= nkx, nky,
 // zero padded) computed with 16x16x4 MFMA instructions: four real MFMAs per complex step.
 //
-// One workgroup = one instance and ZG nodes. The state is scattered from the compact
-// channel layout into LDS slabs, each wavefront runs the four GEMMs of its slabs out of LDS
-// (F_x, F_y stay resident in LDS for the whole workgroup), and the result is gathered back.
+// Each wavefront transforms one node at a time in its own LDS slab with F_x and F_y held in
+// registers, so every MFMA reads one operand from LDS; the grid runs over instances fastest so
+// that workgroups resident together share the Vhat slabs of the same nodes in L2.
 // LDS row strides are chosen so that the A-fragment reads (16 rows x 4 columns) and the
 // B-fragment reads (4 rows x 16 columns) of a wavefront hit 64 distinct banks.
 #include "msx_dft.h"
@@ -28,15 +28,17 @@ struct DftGeom {
     static constexpr int PLANE = NP * (SA > SB ? SA : SB);
 };
 
-// C (NT x NT tiles, re/im) = A B.  A: rows x k from (ar, ai) with stride sa; B: k x cols from
-// (br, bi) with stride sb. conj_a / conj_b negate the imaginary part of the operand.
+// C (NT x NT tiles, re/im) = F B with the A operand F held in registers: fr/fi[ti][kk] is
+// F[ti*16 + l16][kk*4 + k4] (lane l16 + 16 k4); B (k x cols) is read from LDS (br, bi) with
+// row stride sb. conj_f negates the imaginary part of F.
 // 16x16x4 operand layout: A[i][k] at lane i + 16 k, B[k][j] at lane j + 16 k; the result
-// element (i, j) sits in lane j + 16 (i / 4), register i % 4.
+// element (i, j) sits in lane j + 16 g, register v with i = Mfma::row(g, v).
 template <typename T, int NP>
-__device__ __forceinline__ void cgemm(const T *ar, const T *ai, int sa, const T *br, const T *bi,
-                                      int sb, bool conj_a, bool conj_b,
-                                      typename Mfma<T>::acc (&cr)[DftGeom<NP>::NT][DftGeom<NP>::NT],
-                                      typename Mfma<T>::acc (&ci)[DftGeom<NP>::NT][DftGeom<NP>::NT]) {
+__device__ __forceinline__ void cgemm_reg_a(const T (&fr)[DftGeom<NP>::NT][NP / 4],
+                                            const T (&fi)[DftGeom<NP>::NT][NP / 4], bool conj_f,
+                                            const T *br, const T *bi, int sb,
+                                            typename Mfma<T>::acc (&cr)[DftGeom<NP>::NT][DftGeom<NP>::NT],
+                                            typename Mfma<T>::acc (&ci)[DftGeom<NP>::NT][DftGeom<NP>::NT]) {
     constexpr int NT = DftGeom<NP>::NT;
     const int lane = threadIdx.x & 63;
     const int l16 = lane & 15, k4 = lane >> 4;
@@ -50,25 +52,64 @@ __device__ __forceinline__ void cgemm(const T *ar, const T *ai, int sa, const T 
 #pragma unroll
     for (int kk = 0; kk < NP / 4; ++kk) {
         const int k = kk * 4 + k4;
-        T xr[NT], xi[NT], yr[NT], yi[NT];
+        T yr[NT], yi[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            yr[t] = br[k * sb + t * 16 + l16];
+            yi[t] = bi[k * sb + t * 16 + l16];
+        }
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti) {
+            const T xr = fr[ti][kk], xi = conj_f ? -fi[ti][kk] : fi[ti][kk];
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj) {
+                cr[ti][tj] = Mfma<T>::op(xr, yr[tj], cr[ti][tj]);
+                cr[ti][tj] = Mfma<T>::op(-xi, yi[tj], cr[ti][tj]);
+                ci[ti][tj] = Mfma<T>::op(xr, yi[tj], ci[ti][tj]);
+                ci[ti][tj] = Mfma<T>::op(xi, yr[tj], ci[ti][tj]);
+            }
+        }
+    }
+}
+
+// C = A F with A (rows x k) read from LDS (ar, ai; row stride sa) and the B operand F in
+// registers: fr/fi[tj][kk] is F[kk*4 + k4][tj*16 + l16]. conj_f negates Im F.
+template <typename T, int NP>
+__device__ __forceinline__ void cgemm_reg_b(const T *ar, const T *ai, int sa,
+                                            const T (&fr)[DftGeom<NP>::NT][NP / 4],
+                                            const T (&fi)[DftGeom<NP>::NT][NP / 4], bool conj_f,
+                                            typename Mfma<T>::acc (&cr)[DftGeom<NP>::NT][DftGeom<NP>::NT],
+                                            typename Mfma<T>::acc (&ci)[DftGeom<NP>::NT][DftGeom<NP>::NT]) {
+    constexpr int NT = DftGeom<NP>::NT;
+    const int lane = threadIdx.x & 63;
+    const int l16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj) {
+            cr[ti][tj] = typename Mfma<T>::acc{0, 0, 0, 0};
+            ci[ti][tj] = typename Mfma<T>::acc{0, 0, 0, 0};
+        }
+#pragma unroll
+    for (int kk = 0; kk < NP / 4; ++kk) {
+        const int k = kk * 4 + k4;
+        T xr[NT], xi[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             xr[t] = ar[(t * 16 + l16) * sa + k];
             xi[t] = ai[(t * 16 + l16) * sa + k];
-            if (conj_a) xi[t] = -xi[t];
-            yr[t] = br[k * sb + t * 16 + l16];
-            yi[t] = bi[k * sb + t * 16 + l16];
-            if (conj_b) yi[t] = -yi[t];
         }
 #pragma unroll
-        for (int ti = 0; ti < NT; ++ti)
+        for (int tj = 0; tj < NT; ++tj) {
+            const T yr = fr[tj][kk], yi = conj_f ? -fi[tj][kk] : fi[tj][kk];
 #pragma unroll
-            for (int tj = 0; tj < NT; ++tj) {
-                cr[ti][tj] = Mfma<T>::op(xr[ti], yr[tj], cr[ti][tj]);
-                cr[ti][tj] = Mfma<T>::op(-xi[ti], yi[tj], cr[ti][tj]);
-                ci[ti][tj] = Mfma<T>::op(xr[ti], yi[tj], ci[ti][tj]);
-                ci[ti][tj] = Mfma<T>::op(xi[ti], yr[tj], ci[ti][tj]);
+            for (int ti = 0; ti < NT; ++ti) {
+                cr[ti][tj] = Mfma<T>::op(xr[ti], yr, cr[ti][tj]);
+                cr[ti][tj] = Mfma<T>::op(-xi[ti], yi, cr[ti][tj]);
+                ci[ti][tj] = Mfma<T>::op(xr[ti], yi, ci[ti][tj]);
+                ci[ti][tj] = Mfma<T>::op(xi[ti], yr, ci[ti][tj]);
             }
+        }
     }
 }
 
@@ -91,67 +132,66 @@ __device__ __forceinline__ void store_tiles(T *dr, T *di, int stride,
             }
 }
 
-template <typename T, int NP, int ZG>
-__global__ __launch_bounds__(256) void dft_coupling_kernel(Layout L, const cx<T> *__restrict__ Fx,
-                                                           const cx<T> *__restrict__ Fy,
-                                                           const cx<T> *__restrict__ Vhat,
-                                                           const cx<T> *__restrict__ x,
-                                                           cx<T> *__restrict__ y,
-                                                           const int *__restrict__ active) {
+// One workgroup = one instance (blockIdx.x, so consecutive workgroups share the Vhat slabs of
+// the same nodes in L2) and zpb nodes (blockIdx.y); each wavefront owns an LDS slab and
+// processes its nodes independently: scatter X(z) onto the grid, the four transforms with
+// F_x / F_y held in registers for the whole kernel, the pointwise product with Vhat(z) in
+// registers after the second, gather Y(z) back to the compact layout. No workgroup barrier.
+template <typename T, int NP, int WPB>
+__global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 3 : 1) void dft_coupling_kernel(Layout L, const cx<T> *__restrict__ Fx,
+                                                                const cx<T> *__restrict__ Fy,
+                                                                const cx<T> *__restrict__ Vhat,
+                                                                const cx<T> *__restrict__ x,
+                                                                cx<T> *__restrict__ y,
+                                                                const int *__restrict__ active,
+                                                                int zpb) {
     using G = DftGeom<NP>;
-    constexpr int NT = G::NT, SA = G::SA, SB = G::SB;
-    const int b = blockIdx.y;
+    constexpr int NT = G::NT, SA = G::SA, SB = G::SB, KK = NP / 4;
+    const int b = blockIdx.x;
     if (active && !active[b]) return;
-    const int nz = L.nz, nky = L.nky;
-    const int z0 = blockIdx.x * ZG;
+    const int nz = L.nz;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int l16 = lane & 15, k4 = lane >> 4;
     extern __shared__ double dft_smem[];
-    T *fxr = reinterpret_cast<T *>(dft_smem);
-    T *fxi = fxr + NP * SA;
-    T *fyr = fxi + NP * SA;
-    T *fyi = fyr + NP * SB;
-    T *slabs = fyi + NP * SB;  // ZG x {re, im} x PLANE
+    T *sr = reinterpret_cast<T *>(dft_smem) + (size_t)wave * 2 * G::PLANE;
+    T *si = sr + G::PLANE;
 
-    for (int idx = threadIdx.x; idx < NP * NP; idx += blockDim.x) {
-        const int r = idx / NP, c = idx - r * NP;
-        const cx<T> fx = Fx[idx], fy = Fy[idx];
-        fxr[r * SA + c] = fx.re;
-        fxi[r * SA + c] = fx.im;
-        fyr[r * SB + c] = fy.re;
-        fyi[r * SB + c] = fy.im;
-    }
-    for (int idx = threadIdx.x; idx < ZG * 2 * G::PLANE; idx += blockDim.x) slabs[idx] = T(0);
-    __syncthreads();
+    T fxr[NT][KK], fxi[NT][KK], fyr[NT][KK], fyi[NT][KK];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int kk = 0; kk < KK; ++kk) {
+            const cx<T> f = Fx[(t * 16 + l16) * NP + kk * 4 + k4];
+            const cx<T> g = Fy[(kk * 4 + k4) * NP + t * 16 + l16];
+            fxr[t][kk] = f.re;
+            fxi[t][kk] = f.im;
+            fyr[t][kk] = g.re;
+            fyi[t][kk] = g.im;
+        }
 
     const int nc = L.nc[b];
     const long long off = L.off[b];
     const int *chx = L.chx + off, *chy = L.chy + off;
-    // scatter: slab (zz) holds X[gx][gy] in the B layout
-    for (int idx = threadIdx.x; idx < nc * ZG; idx += blockDim.x) {
-        const int j = idx / ZG, zz = idx - j * ZG;
-        const int z = z0 + zz;
-        if (z >= nz) continue;
-        const cx<T> v = x[(off + j) * nz + z];
-        T *sr = slabs + (size_t)zz * 2 * G::PLANE;
-        const int p = chx[j] * SB + chy[j];
-        sr[p] = v.re;
-        sr[G::PLANE + p] = v.im;
-    }
-    __syncthreads();
-
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     typename Mfma<T>::acc cr[NT][NT], ci[NT][NT];
-    for (int zz = wave; zz < ZG; zz += 4) {
-        const int z = z0 + zz;
+    for (int zz = wave; zz < zpb; zz += WPB) {
+        const int z = blockIdx.y * zpb + zz;
         if (z >= nz) break;
-        T *sr = slabs + (size_t)zz * 2 * G::PLANE;
-        T *si = sr + G::PLANE;
+        for (int idx = lane; idx < 2 * G::PLANE; idx += 64) sr[idx] = T(0);
+        __builtin_amdgcn_wave_barrier();
+        for (int j = lane; j < nc; j += 64) {
+            const cx<T> v = x[(off + j) * nz + z];
+            const int p = chx[j] * SB + chy[j];
+            sr[p] = v.re;
+            si[p] = v.im;
+        }
+        __builtin_amdgcn_wave_barrier();
         // (1) Y = F_x X
-        cgemm<T, NP>(fxr, fxi, SA, sr, si, SB, false, false, cr, ci);
+        cgemm_reg_a<T, NP>(fxr, fxi, false, sr, si, SB, cr, ci);
         __builtin_amdgcn_wave_barrier();
         store_tiles<T, NP>(sr, si, SA, cr, ci);
         __builtin_amdgcn_wave_barrier();
         // (2) Z = Y F_y, then Z .* Vhat(z)
-        cgemm<T, NP>(sr, si, SA, fyr, fyi, SB, false, false, cr, ci);
+        cgemm_reg_b<T, NP>(sr, si, SA, fyr, fyi, false, cr, ci);
         {
             const cx<T> *vh = Vhat + (size_t)z * NP * NP;
             const int col = lane & 15, g = lane >> 4;
@@ -171,46 +211,42 @@ __global__ __launch_bounds__(256) void dft_coupling_kernel(Layout L, const cx<T>
         store_tiles<T, NP>(sr, si, SB, cr, ci);
         __builtin_amdgcn_wave_barrier();
         // (3) W = conj(F_x) Zhat
-        cgemm<T, NP>(fxr, fxi, SA, sr, si, SB, true, false, cr, ci);
+        cgemm_reg_a<T, NP>(fxr, fxi, true, sr, si, SB, cr, ci);
         __builtin_amdgcn_wave_barrier();
         store_tiles<T, NP>(sr, si, SA, cr, ci);
         __builtin_amdgcn_wave_barrier();
         // (4) out = W conj(F_y)
-        cgemm<T, NP>(sr, si, SA, fyr, fyi, SB, false, true, cr, ci);
+        cgemm_reg_b<T, NP>(sr, si, SA, fyr, fyi, true, cr, ci);
         __builtin_amdgcn_wave_barrier();
         store_tiles<T, NP>(sr, si, SB, cr, ci);
+        __builtin_amdgcn_wave_barrier();
+        for (int j = lane; j < nc; j += 64) {
+            const int p = chx[j] * SB + chy[j];
+            y[(off + j) * nz + z] = mk(sr[p], si[p]);
+        }
+        __builtin_amdgcn_wave_barrier();
     }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < nc * ZG; idx += blockDim.x) {
-        const int j = idx / ZG, zz = idx - j * ZG;
-        const int z = z0 + zz;
-        if (z >= nz) continue;
-        const T *sr = slabs + (size_t)zz * 2 * G::PLANE;
-        const int p = chx[j] * SB + chy[j];
-        y[(off + j) * nz + z] = mk(sr[p], sr[G::PLANE + p]);
-    }
-    (void)nky;
 }
 
-template <typename T, int NP, int ZG>
+template <typename T, int NP, int WPB>
 static void launch_dft_impl(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
                             const int *active, hipStream_t s) {
     using G = DftGeom<NP>;
-    const size_t shm = sizeof(T) * ((size_t)NP * G::SA * 2 + (size_t)NP * G::SB * 2 +
-                                    (size_t)ZG * 2 * G::PLANE);
-    set_dynamic_lds((const void *)dft_coupling_kernel<T, NP, ZG>, shm);
-    dim3 grid((L.nz + ZG - 1) / ZG, L.n_inst);
-    hipLaunchKernelGGL((dft_coupling_kernel<T, NP, ZG>), grid, dim3(256), shm, s, L, D.Fx, D.Fy,
-                       D.Vhat, x, y, active);
+    const size_t shm = sizeof(T) * (size_t)WPB * 2 * G::PLANE;
+    set_dynamic_lds((const void *)dft_coupling_kernel<T, NP, WPB>, shm);
+    const int zpb = 2 * WPB;  // two nodes per wavefront
+    dim3 grid(L.n_inst, (L.nz + zpb - 1) / zpb);
+    hipLaunchKernelGGL((dft_coupling_kernel<T, NP, WPB>), grid, dim3(64 * WPB), shm, s, L, D.Fx,
+                       D.Fy, D.Vhat, x, y, active, zpb);
     MSX_LAUNCH_CHECK();
 }
 
 template <typename T>
 void launch_dft_coupling(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
                          const int *active, hipStream_t s) {
-    constexpr int ZG = sizeof(T) == 4 ? 8 : 4;
-    if (D.np == 16) launch_dft_impl<T, 16, ZG>(L, D, x, y, active, s);
-    else if (D.np == 32) launch_dft_impl<T, 32, ZG>(L, D, x, y, active, s);
+    constexpr int WPB = sizeof(T) == 4 ? 4 : 2;  // LDS per wavefront: 12 KB FP32, 24 KB FP64
+    if (D.np == 16) launch_dft_impl<T, 16, WPB>(L, D, x, y, active, s);
+    else if (D.np == 32) launch_dft_impl<T, 32, WPB>(L, D, x, y, active, s);
     else throw Error(3, "DFT coupling supports tables up to 32 x 32");
 }
 template void launch_dft_coupling<float>(const Layout &, const DftTables<float> &,
