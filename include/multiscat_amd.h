@@ -173,6 +173,9 @@ typedef struct {
     int coupling;     /* full coupling: 0 = DFT form on the matrix cores when the table fits
                          (<= 32 x 32), 1 = direct sum                                      */
     int diag;         /* all-channel D^-1: 0 = auto (matrix cores when nz <= 320), 1 = vector, 2 = matrix cores */
+    int warm_start;   /* 1: GMRES starts from the batch's current state (msx_batch_set_state, or the
+                         last solve) instead of 0; tolerance still relative to the source, as an
+                         initial guess does in the reference's SciPy path (_gmres.py:106-114) */
 } msx_solve_options;
 
 /* Fills `opt` with the reference's settings for a given nsig (GS, FP64, window 3, 2000). */
@@ -214,6 +217,9 @@ int msx_batch_solve(msx_batch *batch, const msx_solve_options *opt);
  * |2i b psi(z_N) + a0 delta_spec|^2 (computed on the GPU in FP64);
  * iterations/info per condition; residual: final relative preconditioned residual. */
 int msx_batch_state(const msx_batch *batch, double *psi);
+/* Sets the state (same layout as msx_batch_state) that a solve with warm_start = 1 starts from
+ * (the reference's ScatteringCondition.initial_state, multiscat/config.py:115-121). */
+int msx_batch_set_state(msx_batch *batch, const double *psi);
 int msx_batch_intensities(const msx_batch *batch, double *P);
 int msx_batch_stats(const msx_batch *batch, int *iterations, int *info, double *residual);
 

@@ -36,7 +36,8 @@ EXPORTED = (
     "msx_default_options", "msx_table_create",
     "msx_table_prepare", "msx_table_destroy", "msx_table_eigensystem", "msx_table_eig_phases",
     "msx_batch_create",
-    "msx_batch_destroy", "msx_batch_solve", "msx_batch_state", "msx_batch_intensities",
+    "msx_batch_destroy", "msx_batch_solve", "msx_batch_state", "msx_batch_set_state",
+    "msx_batch_intensities",
     "msx_batch_stats", "msx_batch_device_state", "msx_batch_total_channels",
     "msx_batch_adjoint", "msx_profile_enable", "msx_profile_reset", "msx_profile_read",
     "msx_profile_names",
@@ -58,6 +59,7 @@ class SolveOptions(C.Structure):
         ("max_refine", C.c_int),
         ("coupling", C.c_int),
         ("diag", C.c_int),
+        ("warm_start", C.c_int),
     ]
 
 
@@ -102,6 +104,7 @@ def _declare(lib):
         "msx_batch_destroy": (i32, [vp]),
         "msx_batch_solve": (i32, [vp, C.POINTER(SolveOptions)]),
         "msx_batch_state": (i32, [vp, vp]),
+        "msx_batch_set_state": (i32, [vp, vp]),
         "msx_batch_intensities": (i32, [vp, vp]),
         "msx_batch_stats": (i32, [vp, vp, vp, vp]),
         "msx_batch_device_state": (i32, [vp, pp]),

@@ -138,6 +138,12 @@ class ScatteringCondition:
     incident_k: tuple[float, float, float]
     potential: ScatteringPotential
     units: UnitSystem = field(default_factory=UnitSystem.si)
+    # Initial guess psi (nkx, nky, nz) for the GMRES solve, as the reference's
+    # ScatteringCondition.initial_state (multiscat/config.py:115-121; zero when None). It is
+    # the state of the DVR basis that get_scattering_state returns, so the solution of a
+    # nearby condition (energy continuation) can be passed on. Used by the "gpu" and "mixed"
+    # backends and by get_scattering_matrices.
+    initial_state: np.ndarray | None = field(default=None, compare=False)
 
     @staticmethod
     def from_angles(*, mass: float, theta: float = 0, phi: float = 0, energy: float,
@@ -171,7 +177,8 @@ class ScatteringCondition:
         energy = hbar ** 2 / (mass * length ** 2)
         kx, ky, kz = self.incident_k
         return ScatteringCondition(self.mass * mass, (kx / length, ky / length, kz / length),
-                                   self.potential.scaled(energy, length), units)
+                                   self.potential.scaled(energy, length), units,
+                                   self.initial_state)
 
     def natural(self) -> "ScatteringCondition":
         """hbar = 1, m = 1/2, lengths in angstrom: energies become k^2 in 1/A^2
@@ -201,7 +208,7 @@ def _intensities_from_state(psi: np.ndarray, inputs: B.SolverInputs) -> np.ndarr
 
 
 def _batch_solve(inputs: B.SolverInputs, config: OptimizationConfig, *, method: str,
-                 precision: str, neumann: bool):
+                 precision: str, neumann: bool, initial_state=None):
     from multiscat_amd import batch as BT
 
     flag, nsig = config.fortran_parameters()
@@ -215,8 +222,11 @@ def _batch_solve(inputs: B.SolverInputs, config: OptimizationConfig, *, method: 
     tab = BT.Table(inputs.potential_values, inputs.parallel_kinetic_energy)
     bt = BT.Batch(tab, [cond])
     try:
+        warm = initial_state is not None
+        if warm:
+            bt.set_state([np.asarray(initial_state, dtype=np.complex128)[chx, chy]])
         bt.solve(BT.SolveOptions(method=method, precision=precision, neumann=neumann, eps=eps,
-                                 maxiter=max(config.max_iterations, 1)))
+                                 maxiter=max(config.max_iterations, 1), warm_start=warm))
         psi = np.zeros(inputs.potential_values.shape, dtype=np.complex128)
         psi[chx, chy] = bt.state()[0]
         return psi
@@ -242,9 +252,11 @@ def _solve_state(condition: ScatteringCondition, config: OptimizationConfig,
             parallel_kinetic_energy=inputs.parallel_kinetic_energy)
     elif backend == "gpu":
         psi = _batch_solve(inputs, config, method="gauss_seidel", precision="fp64",
-                           neumann=config.use_neumann_preconditioner)
+                           neumann=config.use_neumann_preconditioner,
+                           initial_state=condition.initial_state)
     elif backend == "mixed":
-        psi = _batch_solve(inputs, config, method="jacobi", precision="mixed", neumann=True)
+        psi = _batch_solve(inputs, config, method="jacobi", precision="mixed", neumann=True,
+                           initial_state=condition.initial_state)
     else:
         raise ValueError(f"Unknown backend '{backend}'. Expected 'fortran', 'gpu' or 'mixed'.")
     return psi, inputs
@@ -313,9 +325,17 @@ def get_scattering_matrices(conditions: Sequence[ScatteringCondition],
     tab = BT.Table(V0, T0)
     bt = BT.Batch(tab, conds)
     try:
+        warm = any(c.initial_state is not None for c in conditions)
+        if warm:
+            bt.set_state([np.zeros((len(d["channels"][0]), V0.shape[2]), complex)
+                          if c.initial_state is None else
+                          np.asarray(c.initial_state, dtype=np.complex128)[d["channels"][0],
+                                                                           d["channels"][1]]
+                          for c, d in zip(conditions, conds)])
         bt.solve(BT.SolveOptions(method=method, precision=precision,
                                  neumann=config.use_neumann_preconditioner,
-                                 eps=0.5 * 10.0 ** (-nsig), maxiter=max(config.max_iterations, 1)))
+                                 eps=0.5 * 10.0 ** (-nsig), maxiter=max(config.max_iterations, 1),
+                                 warm_start=warm))
         out = []
         for c, P in zip(conds, bt.intensities()):
             full = np.zeros(V0.shape[:2], dtype=np.complex128)

@@ -32,6 +32,7 @@ class SolveOptions:
     max_refine: int = 6
     coupling: str = "auto"  # or "direct"
     diag: str = "auto"  # "auto" (fastest), "vector" or "mfma" (matrix cores, nz <= 320)
+    warm_start: bool = False  # start from the batch's current state (Batch.set_state)
 
     @classmethod
     def from_nsig(cls, nsig: int, **kw) -> "SolveOptions":
@@ -60,6 +61,7 @@ class SolveOptions:
         if self.diag not in ("auto", "vector", "mfma"):
             raise ValueError(f"unknown diag {self.diag!r}")
         o.diag = {"auto": 0, "vector": 1, "mfma": 2}[self.diag]
+        o.warm_start = 1 if self.warm_start else 0
         return o
 
 
@@ -162,6 +164,19 @@ class Batch:
         flat = np.zeros((self.total_channels, self.nz), dtype=np.complex128)
         _lib.check(_lib.lib().msx_batch_state(self._h, _lib.ptr(flat)), "msx_batch_state")
         return [flat[o:o + n] for o, n in zip(self.offsets, self.counts)]
+
+    def set_state(self, psi) -> None:
+        """Initial guess for a solve with warm_start: a list of (n_channels, nz) arrays (one
+        per condition, channel order as given) or one flat array of the whole batch."""
+        if isinstance(psi, (list, tuple)):
+            flat = np.concatenate([np.asarray(p, dtype=np.complex128).reshape(-1, self.nz)
+                                   for p in psi])
+        else:
+            flat = np.asarray(psi, dtype=np.complex128).reshape(-1, self.nz)
+        if flat.shape[0] != self.total_channels:
+            raise ValueError("state must have one row of nz values per channel of the batch")
+        flat = np.ascontiguousarray(flat)
+        _lib.check(_lib.lib().msx_batch_set_state(self._h, _lib.ptr(flat)), "msx_batch_set_state")
 
     def intensities(self):
         """P per condition: list of (n_channels,) arrays, channel order as given."""

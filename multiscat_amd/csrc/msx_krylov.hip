@@ -285,7 +285,8 @@ __global__ void backsolve_kernel(KLayout K, KState S, const int *__restrict__ ma
 // Start of a cycle: beta = ||r0||, g = (beta, 0, ...), kconv kept, cycle_iters = 0.
 // Instances with beta == 0 are already solved.
 __global__ void cycle_start_kernel(KLayout K, KState S, const cx<double> *__restrict__ nrm2,
-                                   int first_cycle, int *__restrict__ count) {
+                                   int first_cycle, int *__restrict__ count,
+                                   const cx<double> *__restrict__ ref_nrm2) {
     __shared__ int red[KR_THREADS / 64];
     int alive = 0;
     for (int b = threadIdx.x; b < K.n_inst; b += blockDim.x) {
@@ -294,13 +295,13 @@ __global__ void cycle_start_kernel(KLayout K, KState S, const cx<double> *__rest
         const long long slot = (long long)b * (K.kmax + 1);
         const double beta = sqrt(nrm2[slot].re);
         S.beta[b] = beta;
-        if (first_cycle) S.beta0[b] = beta;
+        if (first_cycle) S.beta0[b] = ref_nrm2 ? sqrt(ref_nrm2[slot].re) : beta;
         S.g[slot] = mk(beta, 0.0);
         S.cycle_iters[b] = 0;
-        if (!(beta > 0.0)) {
+        if (!(beta > 0.0) || (ref_nrm2 && first_cycle && beta < S.eps * S.beta0[b])) {
             K.active[b] = 0;
             S.converged[b] = 1;
-            S.resid[b] = 0.0;
+            S.resid[b] = S.beta0[b] > 0.0 ? beta / S.beta0[b] : 0.0;
         } else {
             alive += 1;
         }
@@ -411,9 +412,9 @@ void k_backsolve(const KLayout &K, const KState &S, const int *mask, hipStream_t
 }
 
 void k_cycle_start(const KLayout &K, const KState &S, const cx<double> *nrm2, int first,
-                   int *count, hipStream_t s) {
+                   int *count, hipStream_t s, const cx<double> *ref_nrm2) {
     hipLaunchKernelGGL(cycle_start_kernel, dim3(1), dim3(KR_THREADS), 0, s, K, S, nrm2, first,
-                       count);
+                       count, ref_nrm2);
     MSX_LAUNCH_CHECK();
 }
 

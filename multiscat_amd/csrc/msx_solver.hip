@@ -175,7 +175,12 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
             }
         }
         k_dots<T>(kr.K, kr.vp_d.p, 0, v0, 1, kr.partial.p, kr.hcol.p, 0, s);
-        k_cycle_start(kr.K, kr.S, kr.hcol.p, first ? 1 : 0, kr.count.p, s);
+        // with an initial guess the tolerance stays relative to |rhs| (not to the first
+        // residual), so a good guess saves iterations
+        const bool ref_rhs = first && !x_zero;
+        if (ref_rhs) k_dots<T>(kr.K, kr.vp_d.p, 0, rhs, 1, kr.partial.p, kr.hcol2.p, 0, s);
+        k_cycle_start(kr.K, kr.S, kr.hcol.p, first ? 1 : 0, kr.count.p, s,
+                      ref_rhs ? kr.hcol2.p : nullptr);
         if (kr.read_count() == 0) break;
         k_scale<T>(kr.K, v0, v0, kr.beta.p, kr.active.p, s);
         int alive = 1;
@@ -522,6 +527,7 @@ void check_options(const msx_solve_options *o) {
     if (o->diag < 0 || o->diag > 2) throw Error(MSX_ERR_INVALID, "unknown diag kernel");
     if (o->maxiter <= 0) throw Error(MSX_ERR_INVALID, "maxiter must be positive");
     if (o->window <= 0) throw Error(MSX_ERR_INVALID, "window must be positive");
+    if (o->warm_start != 0 && o->warm_start != 1) throw Error(MSX_ERR_INVALID, "warm_start is 0 or 1");
 }
 
 // Solve A psi = src for all instances into `x` (FP64). transposed: A^T with the flipped table.
@@ -539,7 +545,9 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     Ops ops(B, o.neumann == 1, transposed, o.coupling == 1);
     ops.vector_diag = o.diag == 1;  // auto: matrix cores when nz <= 320
     MSX_HIP(hipMemcpyAsync(B.src.p, src_in, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, s));
-    MSX_HIP(hipMemsetAsync(x, 0, sizeof(cx<double>) * n, s));
+    // A warm start (forward solves only) keeps x as the initial guess.
+    const bool warm = o.warm_start == 1 && !transposed;
+    if (!warm) MSX_HIP(hipMemsetAsync(x, 0, sizeof(cx<double>) * n, s));
     const int kdim = krylov_dim(o, B.n_inst);
     iters.assign(B.n_inst, 0);
     info.assign(B.n_inst, 0);
@@ -561,7 +569,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
             };
         }
         ops.host_active = &kr.host_active;
-        gmres<double, double>(kr, op, B.rhs.p, x, true, o.eps, o.window, o.maxiter, true);
+        gmres<double, double>(kr, op, B.rhs.p, x, !warm, o.eps, o.window, o.maxiter, true);
         ops.host_active = nullptr;
         iters = kr.host_ints(kr.iters);
         auto conv = kr.host_ints(kr.converged);
@@ -592,7 +600,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     const int max_refine = o.max_refine > 0 ? o.max_refine : 6;
     cx<double> *r64 = B.tmp3.p;
     for (int sweep = 0;; ++sweep) {
-        if (sweep == 0) {
+        if (sweep == 0 && !warm) {
             MSX_HIP(hipMemcpyAsync(r64, B.rhs.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, s));
         } else {
             // r = rhs64 - M (I + D^-1 V) x  in FP64 (tmp1/tmp2 are the FP64 work vectors)
@@ -727,6 +735,7 @@ void msx_default_options(msx_solve_options *opt, int nsig) {
     opt->max_refine = 6;
     opt->coupling = 0;
     opt->diag = 0;
+    opt->warm_start = 0;
 }
 
 int msx_table_create(int nkx, int nky, int nz, const double *potential, const double *kinetic,
@@ -882,6 +891,7 @@ int msx_batch_create(msx_table *t, int n_cond, const int *n_channels, const int 
         B->fac_d.alloc((size_t)tot * t->nz);
         B->facf_d.alloc((size_t)tot * t->nz);
         B->psi_d.alloc((size_t)tot * t->nz);
+        MSX_HIP(hipMemsetAsync(B->psi_d.p, 0, sizeof(cx<double>) * (size_t)tot * t->nz, t->stream));
         B->P_d.alloc(tot);
         MSX_HIP(hipStreamSynchronize(s));
         *out = B.release();
@@ -912,6 +922,17 @@ int msx_batch_solve(msx_batch *B, const msx_solve_options *opt) {
         B->solved = true;
         for (int b = 0; b < B->n_inst; ++b)
             if (B->info[b]) return MSX_ERR_NOT_CONVERGED;
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+int msx_batch_set_state(msx_batch *B, const double *psi) {
+    try {
+        if (!B || !psi) throw Error(MSX_ERR_INVALID, "null argument");
+        B->psi_d.upload(reinterpret_cast<const cx<double> *>(psi), (size_t)B->n_elem(), B->t->stream);
+        MSX_HIP(hipStreamSynchronize(B->t->stream));
         return MSX_OK;
     } catch (const Error &e) {
         return fail(e);

@@ -1,0 +1,71 @@
+"""Initial guesses (the reference's ScatteringCondition.initial_state, multiscat/config.py:
+115-121, used as GMRES x0 in multiscat/multiscat/_gmres.py:106-114): the tolerance stays
+relative to the source, so the exact solution converges at once, a zero guess reproduces
+the cold solve, and the solution at a nearby energy saves iterations without changing the
+answer."""
+
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import pytest
+
+from multiscat_amd import api
+from multiscat_amd import batch as BT
+from multiscat_amd import basis as B
+from multiscat_amd import workloads as W
+from tests.test_api import MEV, wolken_condition
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("method,precision", [("jacobi", "fp64"), ("gauss_seidel", "fp64"),
+                                              ("jacobi", "mixed")])
+def test_exact_and_zero_initial_guess(method, precision):
+    w = W.hex_batch(n_conditions=5, nz=60, nk=11, cond_seed=4, channel_counts=[30, 55, 80, 41, 66])
+    tab = BT.Table(w.potential, w.kinetic)
+    bt = BT.Batch(tab, w.conditions)
+    opts = dict(method=method, precision=precision, eps=5e-11)
+    bt.solve(BT.SolveOptions(**opts))
+    P0, it0, psi0 = bt.intensities_flat(), bt.stats()["iterations"], bt.state()
+    assert it0.min() > 0
+    bt.set_state(psi0)
+    bt.solve(BT.SolveOptions(warm_start=True, **opts))
+    assert (bt.stats()["iterations"] == 0).all()
+    np.testing.assert_allclose(bt.intensities_flat(), P0, atol=1e-12)
+    bt.set_state([np.zeros_like(p) for p in psi0])
+    bt.solve(BT.SolveOptions(warm_start=True, **opts))
+    np.testing.assert_array_equal(bt.stats()["iterations"], it0)
+    np.testing.assert_allclose(bt.intensities_flat(), P0, atol=1e-13)
+
+
+def test_energy_continuation_saves_iterations():
+    """Wolken problem: the solution at 20 meV as the initial guess at 20.5 meV."""
+    cfg = api.OptimizationConfig(precision=1e-10, max_iterations=2000)
+    c20 = wolken_condition()
+    c205 = api.ScatteringCondition.from_angles(mass=c20.mass, energy=20.5 * MEV,
+                                               theta=np.deg2rad(30), potential=c20.potential)
+    psi20 = api.get_scattering_state(c20, cfg)
+
+    def batch_for(cond):
+        inp = api.solver_inputs(cond)
+        e = inp.perpendicular_kinetic_difference
+        ch = B.lowest_channels(e, None)
+        tab = BT.Table(inp.potential_values, inp.parallel_kinetic_energy)
+        return tab, BT.Batch(tab, [dict(e=e, wave_a=inp.wave_a, wave_b=inp.wave_b,
+                                        wave_c=inp.wave_c, channels=ch)]), ch
+
+    tab, bt, ch = batch_for(c205)
+    opts = dict(method="jacobi", eps=5e-11)
+    bt.solve(BT.SolveOptions(**opts))
+    cold_it, P_cold = int(bt.stats()["iterations"][0]), bt.intensities_flat()
+    bt.set_state([psi20[ch[0], ch[1]]])
+    bt.solve(BT.SolveOptions(warm_start=True, **opts))
+    warm_it = int(bt.stats()["iterations"][0])
+    assert warm_it < cold_it, (warm_it, cold_it)
+    np.testing.assert_allclose(bt.intensities_flat(), P_cold, atol=1e-9)
+    # through the API: the same answer with the guess attached to the condition
+    P_api = api.get_scattering_matrix(dataclasses.replace(c205, initial_state=psi20), cfg,
+                                      backend="gpu").real
+    np.testing.assert_allclose(P_api[ch[0], ch[1]], P_cold, atol=1e-9)
