@@ -7,15 +7,17 @@ Workload (one "step" = one batch for one potential table): the hexagonal a = 3.1
 [-1.5, 8] A, 256 seeded incidence conditions (E ~ U(10, 100) meV, polar ~ U(0, 60) deg,
 azimuth ~ U(0, 60) deg) each with its own channel count (|K+G|^2 <= 6 k^2, clipped to
 [200, 1200]). A step recomputes the table's eigensystem and DFT tables on the GPU and solves all
-256 conditions to eps = 5e-11 with block-Jacobi GMRES in mixed precision (FP32 inner solves
-on the matrix cores, FP64 residual refinement), then forms P = |S|^2 in FP64. Inputs are resident
-in HBM before the timed region.
+256 conditions with block-Jacobi GMRES in mixed precision (FP32 inner solves on the matrix
+cores, FP64 residual refinement) to eps = 5e-8, then forms P = |S|^2 in FP64. Inputs are
+resident in HBM before the timed region. The metric's bar |dP| <= 1e-6 is checked on every run
+against an FP64 solve of the same batch to eps 5e-14 done outside the timed region.
 
 Reported next to the headline value:
-  * plain FP64 GPU baselines: the drop-in path (reference algorithm, one call per condition,
-    sampled) and the FP64 batched solvers (Gauss-Seidel split and block Jacobi);
-  * accuracy: max |dP| of the mixed solve against the FP64 GPU solve of the whole batch and
-    against the CPU oracle on the sampled conditions;
+  * plain FP64 GPU baselines at the reference's default precision (1e-5, i.e. eps 5e-6): the
+    drop-in path (reference algorithm, one call per condition, sampled) and the FP64 batched
+    solvers (Gauss-Seidel split and block Jacobi), each with its own |dP|;
+  * accuracy: max |dP| of the timed solve against the tight FP64 solve and against the CPU
+    oracle on the sampled conditions;
   * roofline of the kernel with the largest share of the timed region, from HIP events on the
     library's stream (algorithmic flops or bytes per launch / average launch time);
   * cpu_baseline: the FP64 C restatement of the reference algorithm (oracle/) on a sample of
@@ -86,7 +88,7 @@ def main() -> int:
     ap.add_argument("--conditions", type=int, default=256)
     ap.add_argument("--diag", default="auto", choices=["auto", "vector", "mfma"])
     ap.add_argument("--skip-baselines", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 20 plain device copies of MIB MiB (calibrates the HBM byte "
                          "counters of a profiler run against a known size) and exit")
@@ -109,12 +111,17 @@ def main() -> int:
     nkx, nky, nz = w.potential.shape
     tab = BT.Table(w.potential, w.kinetic)
     bt = BT.Batch(tab, w.conditions)
-    eps = 5e-11
+    eps = 5e-8        # mixed: |dP| ~ 4e-10 against the tight solve, well inside the bar
+    eps_base = 5e-6   # the reference's default precision 1e-5 (nsig 5) for the FP64 baselines
     mixed = BT.SolveOptions(method="jacobi", precision="mixed", eps=eps, diag=args.diag)
 
     def step(opts):
         tab.prepare()
         bt.solve(opts)
+
+    # the metric's reference answer, untimed
+    step(BT.SolveOptions(method="jacobi", eps=5e-14, diag=args.diag))
+    P_ref = bt.intensities_flat()
 
     for _ in range(args.warmup):
         step(mixed)
@@ -167,6 +174,9 @@ def main() -> int:
                    "channels_mean": float(counts.mean()), "channels_max": int(counts.max()),
                    "eps": eps, "method": "block-Jacobi GMRES (Neumann), DFT coupling"},
         "iterations": {"mean": float(st["iterations"].mean()), "max": int(st["iterations"].max())},
+        "accuracy": {"max_abs_dP_vs_fp64_eps5e-14": float(np.abs(P_mixed - P_ref).max()),
+                     "bar": 1e-6,
+                     "meets_bar": bool(np.abs(P_mixed - P_ref).max() <= 1e-6)},
         "roofline": roofline,
         "kernels": kernels,
     }
@@ -174,28 +184,32 @@ def main() -> int:
     if not args.skip_baselines:
         base = {}
         fp64 = {}
-        for label, opts in (("jacobi_fp64_batched", BT.SolveOptions(method="jacobi", eps=eps, diag=args.diag)),
-                            ("gauss_seidel_fp64_batched", BT.SolveOptions(eps=eps))):
+        for label, opts in (("jacobi_fp64_batched",
+                             BT.SolveOptions(method="jacobi", eps=eps_base, diag=args.diag)),
+                            ("gauss_seidel_fp64_batched", BT.SolveOptions(eps=eps_base))):
             step(opts)  # warm
             _sync()
             t0 = time.perf_counter()
             step(opts)
             _sync()
             dt = time.perf_counter() - t0
-            base[label] = {"solves_per_s": args.conditions / dt}
             fp64[label] = bt.intensities_flat()
-        ref64 = fp64["jacobi_fp64_batched"]
-        dP_fp64 = float(np.abs(P_mixed - ref64).max())
-        dP_methods = float(np.abs(fp64["gauss_seidel_fp64_batched"] - ref64).max())
+            base[label] = {"solves_per_s": args.conditions / dt, "eps": eps_base,
+                           "max_abs_dP": float(np.abs(fp64[label] - P_ref).max())}
         # plain FP64 drop-in path: reference algorithm, one call per condition (sampled)
         sample = list(range(min(4, args.conditions)))
+        off = np.concatenate(([0], np.cumsum(counts)))
+        dP_single = 0.0
         t0 = time.perf_counter()
         for i in sample:
             single = BT.Batch(tab, [w.conditions[i]])
-            single.solve(BT.SolveOptions(eps=eps))
+            single.solve(BT.SolveOptions(eps=eps_base))
+            dP_single = max(dP_single, float(np.abs(single.intensities_flat()
+                                                    - P_ref[off[i]:off[i + 1]]).max()))
             single.close()
         dt = time.perf_counter() - t0
-        base["plain_fp64_single_call"] = {"solves_per_s": len(sample) / dt,
+        base["plain_fp64_single_call"] = {"solves_per_s": len(sample) / dt, "eps": eps_base,
+                                          "max_abs_dP": dP_single,
                                           "sample": f"{len(sample)} conditions, one call each"}
         # PCIe-inclusive rate: host arrays in, intensities out (table upload, channel tables,
         # eigensystem, solve, download of P), for comparison with the HBM-resident value
@@ -213,8 +227,6 @@ def main() -> int:
         out["speedup_vs_plain_fp64_single_call"] = value / base["plain_fp64_single_call"]["solves_per_s"]
         out["speedup_vs_fp64_batched_gs"] = value / base["gauss_seidel_fp64_batched"]["solves_per_s"]
         out["speedup_vs_fp64_batched_jacobi"] = value / base["jacobi_fp64_batched"]["solves_per_s"]
-        out["accuracy"] = {"max_abs_dP_mixed_vs_fp64_gpu": dP_fp64,
-                           "max_abs_dP_gs_vs_jacobi_fp64": dP_methods}
 
         # CPU baseline: the oracle (C restatement of the reference GMRES), threads over conditions
         try:
@@ -226,21 +238,20 @@ def main() -> int:
             def run(i):
                 c = w.conditions[i]
                 psi = oracle.solve(1, 0, w.potential, c.e, c.wave_a, c.wave_b, c.wave_c, w.kinetic,
-                                   channels=c.channels, eps=eps)
+                                   channels=c.channels, eps=eps_base)
                 return oracle.intensities(psi, c.wave_a, c.wave_b)[c.channels[0], c.channels[1]]
 
             t0 = time.perf_counter()
             with ThreadPoolExecutor(max_workers=threads) as ex:
                 Ps = list(ex.map(run, sample))
             dt = time.perf_counter() - t0
-            off = np.concatenate(([0], np.cumsum(counts)))
             dP_oracle = max(float(np.abs(P_mixed[off[i]:off[i + 1]] - Ps[k]).max())
                             for k, i in enumerate(sample))
             out["cpu_baseline"] = {"value": len(sample) / dt, "unit": "solves/s", "cores": threads,
                                    "kind": "port",
                                    "sample": f"first {len(sample)} of the {args.conditions} "
-                                             f"conditions, FP64 GS-GMRES restatement, "
-                                             f"{threads} threads x 1 condition"}
+                                             f"conditions, FP64 GS-GMRES restatement at eps "
+                                             f"{eps_base:g}, {threads} threads x 1 condition"}
             out["accuracy"]["max_abs_dP_mixed_vs_oracle_sample"] = dP_oracle
         except Exception as exc:  # the oracle is test infrastructure; report its absence
             out["cpu_baseline"] = {"value": None, "unit": "solves/s", "cores": 0, "kind": "port",

@@ -578,12 +578,17 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         return;
     }
 
-    // Mixed precision (block Jacobi): FP32 inner GMRES, FP64 residual and accumulation.
+    // Mixed precision (block Jacobi): iterative refinement of (I + D^-1 V) x = D^-1 s. Each
+    // sweep forms the residual r = D^-1 s - x - D^-1 V x in FP64 (one coupling and one D^-1),
+    // stops an instance when |r| < eps |D^-1 s|, and otherwise solves the correction equation
+    // M (I + D^-1 V) d = M r with FP32 GMRES on the matrix cores (M = I - D^-1 V, the Neumann
+    // factor, applied to r in FP32); the corrections accumulate in FP64.
     msx_batch::need(B.tmp3, n);
     msx_batch::need(B.f_rhs, n);
     msx_batch::need(B.f_tmp1, n);
     msx_batch::need(B.f_tmp2, n);
-    ops.jac_rhs(B.src.p, B.rhs.p);  // rhs64 = M D^-1 s
+    cx<double> *d0 = B.rhs.p;  // D^-1 s
+    ops.dinv<double>(B.src.p, nullptr, d0, 1.0, nullptr);
     if (!B.kr_outer) B.kr_outer.reset(new Krylov<double>(B.kstart(), B.klen(), 1, s));
     Krylov<double> &outer = *B.kr_outer;
     outer.set_all_active(nullptr);
@@ -594,21 +599,21 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     Op<float> op32 = [&](const cx<float> *in, cx<float> *out, const int *act) {
         ops.jac<float>(in, out, act, B.f_tmp1.p, B.f_tmp2.p);
     };
-    const std::vector<double> rhs_n2 = outer.norms2(B.rhs.p);
+    const std::vector<double> rhs_n2 = outer.norms2(d0);
     std::vector<int> act(B.n_inst, 1);
     const double inner_eps = o.inner_eps > 0 ? o.inner_eps : 1e-5;
     const int max_refine = o.max_refine > 0 ? o.max_refine : 6;
     cx<double> *r64 = B.tmp3.p;
     for (int sweep = 0;; ++sweep) {
+        outer.set_all_active(act.data());
         if (sweep == 0 && !warm) {
-            MSX_HIP(hipMemcpyAsync(r64, B.rhs.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, s));
+            MSX_HIP(hipMemcpyAsync(r64, d0, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, s));
         } else {
-            // r = rhs64 - M (I + D^-1 V) x  in FP64 (tmp1/tmp2 are the FP64 work vectors)
-            // (B.src is free here: the source was consumed by jac_rhs above)
-            outer.set_all_active(act.data());
+            // r = d0 - (x + D^-1 V x)   (B.src is free: the source was consumed above)
             ops.host_active = &outer.host_active;
-            ops.jac<double>(x, B.src.p, outer.active.p, B.tmp1.p, B.tmp2.p);
-            k_axpby<double>(outer.K, B.rhs.p, B.src.p, -1.0, r64, outer.active.p, s);
+            ops.couple_full(x, B.tmp1.p, outer.active.p);
+            ops.dinv<double>(B.tmp1.p, x, B.src.p, 1.0, outer.active.p);
+            k_axpby<double>(outer.K, d0, B.src.p, -1.0, r64, outer.active.p, s);
         }
         const std::vector<double> r_n2 = outer.norms2(r64);
         int n_act = 0;
@@ -620,7 +625,12 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
             else ++n_act;
         }
         if (n_act == 0 || sweep >= max_refine) break;
-        launch_to_float((long long)n, r64, B.f_rhs.p, s);
+        outer.set_all_active(act.data());
+        ops.host_active = &outer.host_active;
+        // inner right-hand side M r = r - D^-1 V r in FP32
+        launch_to_float((long long)n, r64, B.f_tmp2.p, s);
+        ops.couple_full(B.f_tmp2.p, B.f_tmp1.p, outer.active.p);
+        ops.dinv<float>(B.f_tmp1.p, B.f_tmp2.p, B.f_rhs.p, -1.0f, outer.active.p);
         inner.set_all_active(act.data());
         ops.host_active = &inner.host_active;
         gmres<float, double>(inner, op32, B.f_rhs.p, x, true, inner_eps, 1,
