@@ -243,6 +243,239 @@ static void launch_diag_gemm(const Layout &L, const EigTables<T> &E, const doubl
     MSX_LAUNCH_CHECK();
 }
 
+// FP32 form with Q^T resident in LDS (nzp <= 160: 105 KB). One persistent workgroup per CU
+// stages Q^T once; each of its DL_WAVES wavefronts then loops over (instance, 16-channel)
+// tiles with the same two products as diag_gemm_kernel, reading the A fragments of both
+// products from LDS (the second product reads Q = (Q^T)^T by columns) so that only the
+// channel data, the factors and the result move through the memory hierarchy; the channel
+// data are prefetched two stages ahead. Row stride 164 (= 36 mod 64 words) keeps the 16-row
+// b128 reads of the first product and the 4-row x 16-column b32 reads of the second
+// conflict-free. Work balance: every workgroup scans the active instances' tile counts into
+// LDS, and the wavefronts stride over the compact list of active tiles (a binary search maps
+// a tile to its instance), so each gets the same number of tiles to within one.
+constexpr int DL_WAVES = 8;
+constexpr int DL_STRIDE = 164;
+constexpr int DL_MAX_INST = 4096;  // instances the in-LDS tile scan covers
+
+template <int NT>
+__global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
+    Layout L, EigTables<float> E, const double *__restrict__ e, const cx<float> *__restrict__ c,
+    const float *__restrict__ fac, const cx<float> *x, const cx<float> *base, cx<float> *y,
+    float sign, int inverse, const int *__restrict__ active, int tiles_per_inst) {
+    using M = Mfma<float>;
+    using acc_t = M::acc;
+    constexpr int ZG = NT % 4 == 0 ? 4 : 2;
+    extern __shared__ double dl_smem[];
+    double *lam = dl_smem;                                    // [nzp], 0 past nz
+    float *qt = reinterpret_cast<float *>(dl_smem + 16 * NT);  // [nzp][DL_STRIDE]
+    __shared__ int tile_start[DL_MAX_INST + 1];               // exclusive scan of active tiles
+    __shared__ int wave_sum[DL_WAVES];
+    const int nzp = E.nzp, nz = L.nz, NTL = nzp >> 4;
+    const int n_inst = L.n_inst;
+    {
+        // per-thread serial sums over a contiguous chunk, then a scan across the threads
+        const int per = (n_inst + blockDim.x - 1) / blockDim.x;
+        const int i0 = threadIdx.x * per, i1 = min(n_inst, i0 + per);
+        int local = 0;
+        for (int i = i0; i < i1; ++i)
+            local += (active && !active[i]) ? 0 : (L.nc[i] + 15) / 16;
+        int incl = local;
+        const int ln = threadIdx.x & 63;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o);
+            if (ln >= o) incl += t;
+        }
+        if (ln == 63) wave_sum[threadIdx.x >> 6] = incl;
+        __syncthreads();
+        int before = 0;
+        for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) before += wave_sum[w];
+        int run = before + incl - local;
+        for (int i = i0; i < i1; ++i) {
+            tile_start[i] = run;
+            run += (active && !active[i]) ? 0 : (L.nc[i] + 15) / 16;
+        }
+        if (threadIdx.x == blockDim.x - 1) tile_start[n_inst] = before + incl;
+    }
+    for (int m = threadIdx.x; m < 16 * NT; m += blockDim.x) lam[m] = m < nz ? E.lam[m] : 0.0;
+    for (int idx = threadIdx.x; idx < nzp * nzp / 4; idx += blockDim.x) {
+        const int r = idx / (nzp / 4), c4 = idx - r * (nzp / 4);
+        const float4 v = reinterpret_cast<const float4 *>(E.QTp + (long long)r * nzp)[c4];
+        *reinterpret_cast<float4 *>(qt + r * DL_STRIDE + 4 * c4) = v;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int l16 = lane & 15, g = lane >> 4;
+    auto tile = [&](int t) { return t < NTL ? t : NTL - 1; };
+    const int n_tiles = tile_start[n_inst];
+#pragma unroll 1
+    for (int t = blockIdx.x * DL_WAVES + wave; t < n_tiles; t += gridDim.x * DL_WAVES) {
+        int lo = 0, hi = n_inst - 1;  // last instance with tile_start <= t
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tile_start[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const int b = lo;
+        const int j0 = (t - tile_start[b]) * 16;
+        const int nc = L.nc[b];
+        if (j0 >= nc) continue;  // never taken; also keeps the register allocation spill-free
+        const int jc = j0 + l16;
+        const bool live = jc < nc;
+        const long long cidx = L.off[b] + (live ? jc : 0);
+        const long long row0 = cidx * nz;
+        // per-channel scalars of the later phases, loaded now so that the first product
+        // covers their latency
+        const double ej = e[cidx];
+        const cx<float> cj = c[cidx];
+        const float facN = inverse ? fac[row0 + nz - 1] : 0.0f;
+        const cx<float> xN = inverse ? mk(0.0f, 0.0f) : x[row0 + nz - 1];
+
+        // ---- W^T = Q^T X^T, X prefetched two stages ahead ----
+        acc_t acc1[NT][2];
+#pragma unroll
+        for (int mt = 0; mt < NT; ++mt) acc1[mt][0] = acc1[mt][1] = acc_t{0, 0, 0, 0};
+        cx<float> x0[4], x1[4], x2[4];  // stages s, s + 1, s + 2 in flight
+        auto load_x = [&](int s, cx<float> (&q)[4]) {
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) {
+                const int z = 16 * s + 4 * g + cc;
+                const bool ok = live && z < nz;
+                const cx<float> v = x[row0 + (ok ? z : 0)];
+                q[cc] = ok ? v : mk(0.0f, 0.0f);
+            }
+        };
+        load_x(0, x0);
+        load_x(1, x1);
+#pragma unroll 1
+        for (int s = 0; s < NTL; ++s) {
+            load_x(s + 2, x2);  // past the last block: zeros (z >= nz), unused
+            const int zq = 16 * s + 4 * g;
+            float4 a[NT];
+#pragma unroll
+            for (int mt = 0; mt < NT; ++mt)
+                a[mt] = *reinterpret_cast<const float4 *>(qt + (16 * tile(mt) + l16) * DL_STRIDE + zq);
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+                for (int mt = 0; mt < NT; ++mt) {
+                    const float av = cc == 0 ? a[mt].x : cc == 1 ? a[mt].y : cc == 2 ? a[mt].z : a[mt].w;
+                    acc1[mt][0] = M::op(av, x0[cc].re, acc1[mt][0]);
+                    acc1[mt][1] = M::op(av, x0[cc].im, acc1[mt][1]);
+                }
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) {
+                x0[cc] = x1[cc];
+                x1[cc] = x2[cc];
+            }
+        }
+
+        // ---- scaling and y0(N) = Q[N-1][:] W (Q[N-1][m] = Q^T[m][N-1]) ----
+        float yn0 = 0.f, yn1 = 0.f;
+#pragma unroll
+        for (int mt = 0; mt < NT; ++mt) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = 16 * mt + 4 * g + v;
+                const float qv = qt[(16 * tile(mt) + 4 * g + v) * DL_STRIDE + nz - 1];
+                const double d = ej + lam[m];
+                const float sc = m < nz ? (inverse ? 1.0f / (float)d : (float)d) : 0.f;
+                acc1[mt][0][v] *= sc;
+                acc1[mt][1][v] *= sc;
+                yn0 += qv * acc1[mt][0][v];
+                yn1 += qv * acc1[mt][1][v];
+            }
+        }
+        yn0 += __shfl_xor(yn0, 16);
+        yn0 += __shfl_xor(yn0, 32);
+        yn1 += __shfl_xor(yn1, 16);
+        yn1 += __shfl_xor(yn1, 32);
+        cx<float> corr = mk(0.f, 0.f);
+        if (live) {
+            if (inverse) {
+                const cx<float> den = mk(1.0f, 0.0f) - cj * facN;
+                corr = cdiv(mk(yn0, yn1) * cj, den);
+            } else {
+                corr = cj * xN;
+            }
+        }
+
+        // ---- Y^T = Q W^T: A[i][k] = Q[16 zt + i][m] = Q^T[m][16 zt + i] ----
+#pragma unroll 1
+        for (int zt0 = 0; zt0 < NTL; zt0 += ZG) {
+            acc_t acc2[ZG][2];
+#pragma unroll
+            for (int zg = 0; zg < ZG; ++zg) acc2[zg][0] = acc2[zg][1] = acc_t{0, 0, 0, 0};
+            // this group's factors and base values, loaded ahead of its products
+            float fz[ZG][4];
+            cx<float> bz[ZG][4];
+#pragma unroll
+            for (int zg = 0; zg < ZG; ++zg)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int z = 16 * (zt0 + zg) + 4 * g + v;
+                    const bool ok = live && zt0 + zg < NTL && z < nz;
+                    const long long idx = row0 + (ok ? z : 0);
+                    fz[zg][v] = inverse ? fac[idx] : 0.0f;
+                    bz[zg][v] = base ? base[idx] : mk(0.0f, 0.0f);
+                }
+#pragma unroll
+            for (int mt = 0; mt < NT; ++mt) {
+                float a[ZG][4];
+#pragma unroll
+                for (int zg = 0; zg < ZG; ++zg)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        a[zg][v] = qt[(16 * tile(mt) + 4 * g + v) * DL_STRIDE +
+                                      16 * tile(zt0 + zg) + l16];
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+#pragma unroll
+                    for (int zg = 0; zg < ZG; ++zg) {
+                        acc2[zg][0] = M::op(a[zg][v], acc1[mt][0][v], acc2[zg][0]);
+                        acc2[zg][1] = M::op(a[zg][v], acc1[mt][1][v], acc2[zg][1]);
+                    }
+            }
+#pragma unroll
+            for (int zg = 0; zg < ZG; ++zg) {
+                if (!live || zt0 + zg >= NTL) continue;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int z = 16 * (zt0 + zg) + 4 * g + v;
+                    if (z >= nz) continue;
+                    const long long idx = row0 + z;
+                    cx<float> val = mk(acc2[zg][0][v], acc2[zg][1][v]);
+                    if (inverse) val = val + corr * fz[zg][v];
+                    else if (z == nz - 1) val = val - corr;
+                    y[idx] = base ? bz[zg][v] + val * sign : val;
+                }
+            }
+        }
+    }
+}
+
+template <int NT>
+static void launch_diag_lds(const Layout &L, const EigTables<float> &E, const double *e,
+                            const cx<float> *c, const float *fac, const cx<float> *x,
+                            const cx<float> *base, cx<float> *y, float sign, int inverse,
+                            const int *active, hipStream_t s) {
+    if (E.nzp % 16 != 0 || E.nzp > 16 * NT || E.nzp < L.nz)
+        throw Error(3, "diagonal blocks: padded size does not fit the kernel");
+    const size_t shm = sizeof(double) * 16 * NT + sizeof(float) * (size_t)E.nzp * DL_STRIDE;
+    set_dynamic_lds((const void *)diag_lds_kernel<NT>, shm);
+    static int n_cu = 0;
+    if (n_cu == 0) {
+        int dev = 0;
+        MSX_HIP(hipGetDevice(&dev));
+        MSX_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    if (L.n_inst > DL_MAX_INST) throw Error(3, "diagonal blocks: too many instances for one launch");
+    const long long tiles_max = (long long)L.n_inst * ((L.max_nc + 15) / 16);
+    const int grid = (int)std::min<long long>(n_cu, (tiles_max + DL_WAVES - 1) / DL_WAVES);
+    hipLaunchKernelGGL((diag_lds_kernel<NT>), dim3(std::max(grid, 1)), dim3(64 * DL_WAVES), shm, s,
+                       L, E, e, c, fac, x, base, y, sign, inverse, active, 0);
+    MSX_LAUNCH_CHECK();
+}
+
 bool diag_mfma_supported(int nz) { return nz <= 320; }
 
 template <typename T>
@@ -258,7 +491,9 @@ void launch_diag_blocks(const Layout &L, const EigTables<T> &E, const double *e,
         return;                                                                             \
     }
     if constexpr (sizeof(T) == 4) {
-        MSX_DIAG(2, 4) MSX_DIAG(2, 7) MSX_DIAG(2, 8) MSX_DIAG(2, 10)
+        // Q^T resident in LDS up to nzp = 160
+        if (nt <= 7) return launch_diag_lds<7>(L, E, e, c, fac, x, base, y, sign, inverse, active, s);
+        if (nt <= 10) return launch_diag_lds<10>(L, E, e, c, fac, x, base, y, sign, inverse, active, s);
     } else {
         MSX_DIAG(1, 4) MSX_DIAG(1, 7) MSX_DIAG(1, 8) MSX_DIAG(1, 10)
     }
