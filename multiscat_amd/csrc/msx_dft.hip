@@ -58,15 +58,24 @@ __device__ __forceinline__ void cgemm_reg_a(const T (&fr)[DftGeom<NP>::NT][NP / 
             yr[t] = br[k * sb + t * 16 + l16];
             yi[t] = bi[k * sb + t * 16 + l16];
         }
+        // negations fall on the transient LDS operand, never on the register-resident F:
+        // F y:       cr += fr yr + fi (-yi),  ci += fr yi + fi yr
+        // conj(F) y: cr += fr yr + fi yi,     ci += fr yi + fi (-yr)
+        T yr2[NT], yi2[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            yi2[t] = conj_f ? yi[t] : -yi[t];
+            yr2[t] = conj_f ? -yr[t] : yr[t];
+        }
 #pragma unroll
         for (int ti = 0; ti < NT; ++ti) {
-            const T xr = fr[ti][kk], xi = conj_f ? -fi[ti][kk] : fi[ti][kk];
+            const T xr = fr[ti][kk], xi = fi[ti][kk];
 #pragma unroll
             for (int tj = 0; tj < NT; ++tj) {
                 cr[ti][tj] = Mfma<T>::op(xr, yr[tj], cr[ti][tj]);
-                cr[ti][tj] = Mfma<T>::op(-xi, yi[tj], cr[ti][tj]);
+                cr[ti][tj] = Mfma<T>::op(xi, yi2[tj], cr[ti][tj]);
                 ci[ti][tj] = Mfma<T>::op(xr, yi[tj], ci[ti][tj]);
-                ci[ti][tj] = Mfma<T>::op(xi, yr[tj], ci[ti][tj]);
+                ci[ti][tj] = Mfma<T>::op(xi, yr2[tj], ci[ti][tj]);
             }
         }
     }
@@ -99,14 +108,22 @@ __device__ __forceinline__ void cgemm_reg_b(const T *ar, const T *ai, int sa,
             xr[t] = ar[(t * 16 + l16) * sa + k];
             xi[t] = ai[(t * 16 + l16) * sa + k];
         }
+        // x F:       cr += xr fr + (-xi) fi,  ci += xr fi + xi fr
+        // x conj(F): cr += xr fr + xi fi,     ci += (-xr) fi + xi fr
+        T xn[NT], xm[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            xn[t] = conj_f ? xi[t] : -xi[t];
+            xm[t] = conj_f ? -xr[t] : xr[t];
+        }
 #pragma unroll
         for (int tj = 0; tj < NT; ++tj) {
-            const T yr = fr[tj][kk], yi = conj_f ? -fi[tj][kk] : fi[tj][kk];
+            const T yr = fr[tj][kk], yi = fi[tj][kk];
 #pragma unroll
             for (int ti = 0; ti < NT; ++ti) {
                 cr[ti][tj] = Mfma<T>::op(xr[ti], yr, cr[ti][tj]);
-                cr[ti][tj] = Mfma<T>::op(-xi[ti], yi, cr[ti][tj]);
-                ci[ti][tj] = Mfma<T>::op(xr[ti], yi, ci[ti][tj]);
+                cr[ti][tj] = Mfma<T>::op(xn[ti], yi, cr[ti][tj]);
+                ci[ti][tj] = Mfma<T>::op(xm[ti], yi, ci[ti][tj]);
                 ci[ti][tj] = Mfma<T>::op(xi[ti], yr, ci[ti][tj]);
             }
         }
@@ -137,16 +154,16 @@ __device__ __forceinline__ void store_tiles(T *dr, T *di, int stride,
 // processes its nodes independently: scatter X(z) onto the grid, the four transforms with
 // F_x / F_y held in registers for the whole kernel, the pointwise product with Vhat(z) in
 // registers after the second, gather Y(z) back to the compact layout. No workgroup barrier.
+// The channel values and the Vhat slab of a wavefront's next node are loaded into registers
+// while the current node's transforms run; the channels' grid positions are computed once.
 template <typename T, int NP, int WPB>
-__global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 3 : 1) void dft_coupling_kernel(Layout L, const cx<T> *__restrict__ Fx,
-                                                                const cx<T> *__restrict__ Fy,
-                                                                const cx<T> *__restrict__ Vhat,
-                                                                const cx<T> *__restrict__ x,
-                                                                cx<T> *__restrict__ y,
-                                                                const int *__restrict__ active,
-                                                                int zpb) {
+__global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling_kernel(
+    Layout L, const cx<T> *__restrict__ Fx, const cx<T> *__restrict__ Fy,
+    const cx<T> *__restrict__ Vhat, const cx<T> *__restrict__ x, cx<T> *__restrict__ y,
+    const int *__restrict__ active, int zpb) {
     using G = DftGeom<NP>;
     constexpr int NT = G::NT, SA = G::SA, SB = G::SB, KK = NP / 4;
+    constexpr int KC = NP * NP / 64;  // channels per lane (nc <= NP * NP)
     const int b = blockIdx.x;
     if (active && !active[b]) return;
     const int nz = L.nz;
@@ -171,19 +188,53 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 3 : 1) void dft_coupling
 
     const int nc = L.nc[b];
     const long long off = L.off[b];
-    const int *chx = L.chx + off, *chy = L.chy + off;
+    __shared__ int s_pos[NP * NP];  // slab position of each channel of the instance
+    for (int j = threadIdx.x; j < nc; j += blockDim.x) s_pos[j] = L.chx[off + j] * SB + L.chy[off + j];
+    __syncthreads();
+    const int col = l16, g = k4;
+    constexpr bool PRE_V = false;  // Vhat(z) is loaded just before the second transform
+    cx<T> xv[KC], vh[NT][NT][4];
+    // element offsets fit in 32 bits (the launcher checks), so the loads and stores use a
+    // uniform base and one 32-bit offset register
+    const int row_l = (int)(off + lane) * nz, row_step = 64 * nz, row_0 = (int)off * nz;
+    auto load_x = [&](int z) {
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const int j = lane + 64 * k;
+            xv[k] = x[(j < nc ? row_l + k * row_step : row_0) + z];
+        }
+    };
+    auto load_v = [&](int z) {
+        const cx<T> *vz = Vhat + (size_t)z * NP * NP;
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    vh[ti][tj][v] = vz[(ti * 16 + Mfma<T>::row(g, v)) * NP + tj * 16 + col];
+    };
+    const int z_first = blockIdx.y * zpb + wave;
+    const int z_end = min(nz, (blockIdx.y + 1) * zpb);
+    if (z_first < z_end) {
+        load_x(z_first);
+        if constexpr (PRE_V) load_v(z_first);
+    }
     typename Mfma<T>::acc cr[NT][NT], ci[NT][NT];
-    for (int zz = wave; zz < zpb; zz += WPB) {
-        const int z = blockIdx.y * zpb + zz;
-        if (z >= nz) break;
+    for (int z = z_first; z < z_end; z += WPB) {
         for (int idx = lane; idx < 2 * G::PLANE; idx += 64) sr[idx] = T(0);
         __builtin_amdgcn_wave_barrier();
-        for (int j = lane; j < nc; j += 64) {
-            const cx<T> v = x[(off + j) * nz + z];
-            const int p = chx[j] * SB + chy[j];
-            sr[p] = v.re;
-            si[p] = v.im;
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const int j = lane + 64 * k;
+            if (j < nc) {
+                const int p = s_pos[j];
+                sr[p] = xv[k].re;
+                si[p] = xv[k].im;
+            }
         }
+        const bool more = z + WPB < z_end;
+        if (more) load_x(z + WPB);  // consumed by the scatter above
         __builtin_amdgcn_wave_barrier();
         // (1) Y = F_x X
         cgemm_reg_a<T, NP>(fxr, fxi, false, sr, si, SB, cr, ci);
@@ -191,22 +242,21 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 3 : 1) void dft_coupling
         store_tiles<T, NP>(sr, si, SA, cr, ci);
         __builtin_amdgcn_wave_barrier();
         // (2) Z = Y F_y, then Z .* Vhat(z)
+        if constexpr (!PRE_V) load_v(z);
         cgemm_reg_b<T, NP>(sr, si, SA, fyr, fyi, false, cr, ci);
-        {
-            const cx<T> *vh = Vhat + (size_t)z * NP * NP;
-            const int col = lane & 15, g = lane >> 4;
 #pragma unroll
-            for (int ti = 0; ti < NT; ++ti)
+        for (int ti = 0; ti < NT; ++ti)
 #pragma unroll
-                for (int tj = 0; tj < NT; ++tj)
+            for (int tj = 0; tj < NT; ++tj)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const cx<T> h = vh[(ti * 16 + Mfma<T>::row(g, v)) * NP + tj * 16 + col];
-                        const T a = cr[ti][tj][v], c = ci[ti][tj][v];
-                        cr[ti][tj][v] = a * h.re - c * h.im;
-                        ci[ti][tj][v] = a * h.im + c * h.re;
-                    }
-        }
+                for (int v = 0; v < 4; ++v) {
+                    const cx<T> h = vh[ti][tj][v];
+                    const T a = cr[ti][tj][v], c = ci[ti][tj][v];
+                    cr[ti][tj][v] = a * h.re - c * h.im;
+                    ci[ti][tj][v] = a * h.im + c * h.re;
+                }
+        if constexpr (PRE_V)
+            if (more) load_v(z + WPB);  // this node's slab has been consumed
         __builtin_amdgcn_wave_barrier();
         store_tiles<T, NP>(sr, si, SB, cr, ci);
         __builtin_amdgcn_wave_barrier();
@@ -220,9 +270,13 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 3 : 1) void dft_coupling
         __builtin_amdgcn_wave_barrier();
         store_tiles<T, NP>(sr, si, SB, cr, ci);
         __builtin_amdgcn_wave_barrier();
-        for (int j = lane; j < nc; j += 64) {
-            const int p = chx[j] * SB + chy[j];
-            y[(off + j) * nz + z] = mk(sr[p], si[p]);
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const int j = lane + 64 * k;
+            if (j < nc) {
+                const int p = s_pos[j];
+                y[row_l + k * row_step + z] = mk(sr[p], si[p]);
+            }
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -234,7 +288,10 @@ static void launch_dft_impl(const Layout &L, const DftTables<T> &D, const cx<T> 
     using G = DftGeom<NP>;
     const size_t shm = sizeof(T) * (size_t)WPB * 2 * G::PLANE;
     set_dynamic_lds((const void *)dft_coupling_kernel<T, NP, WPB>, shm);
-    const int zpb = 2 * WPB;  // two nodes per wavefront
+    const long long tot = L.total > 0 ? L.total : (long long)L.max_nc * L.n_inst;
+    if ((long long)L.nz * (tot + 64) >= (1LL << 31))
+        throw Error(3, "DFT coupling: batch state beyond 2^31 elements");
+    const int zpb = 4 * WPB;  // four nodes per wavefront
     dim3 grid(L.n_inst, (L.nz + zpb - 1) / zpb);
     hipLaunchKernelGGL((dft_coupling_kernel<T, NP, WPB>), grid, dim3(64 * WPB), shm, s, L, D.Fx,
                        D.Fy, D.Vhat, x, y, active, zpb);

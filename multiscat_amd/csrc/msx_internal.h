@@ -149,6 +149,7 @@ struct Layout {
     const int *nc;       // [n_inst]
     const long long *off;  // [n_inst] channel offset of each instance
     const int *chx, *chy;  // [total channels]
+    long long total = 0;   // total channels of the batch (0: not given, at most n_inst * max_nc)
 };
 
 // ---- kernel launchers (msx_kernels.hip) ------------------------------------------------

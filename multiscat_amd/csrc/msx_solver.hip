@@ -314,7 +314,8 @@ struct msx_batch {
 
     long long n_elem() const { return total_ch * t->nz; }
     Layout layout() const {
-        return Layout{t->nkx, t->nky, t->nz, n_inst, max_nc, nc_d.p, off_d.p, chx_d.p, chy_d.p};
+        return Layout{t->nkx, t->nky, t->nz, n_inst, max_nc, nc_d.p, off_d.p, chx_d.p, chy_d.p,
+                      (long long)(n_elem() / t->nz)};
     }
     std::vector<long long> kstart() const {
         std::vector<long long> v(n_inst);
