@@ -161,7 +161,17 @@ def main() -> int:
                          "algorithmic_flops_per_launch": per_launch_flops,
                          "algorithmic_bytes_per_launch": per_launch_bytes,
                          "share_of_step": d["ms"] / 1e3 / elapsed})
-    kernels = {k: {"ms_per_step": v["ms"] / args.steps, "launches_per_step": v["count"] / args.steps}
+    def _rate(name, v):
+        secs = v["ms"] / 1e3
+        if name.startswith("arnoldi"):
+            return {"achieved_GB_s": v["bytes"] / secs / 1e9 if secs > 0 else None}
+        prec = "f32" if name.endswith("f32") else "f64"
+        tf = v["flops"] / secs / 1e12 if secs > 0 else None
+        return {"achieved_TFLOP_s": tf,
+                "frac_of_peak": tf / PEAK_TFLOPS[prec] if tf is not None else None}
+
+    kernels = {k: {"ms_per_step": v["ms"] / args.steps, "launches_per_step": v["count"] / args.steps,
+                   **_rate(k, v)}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
 
     out = {

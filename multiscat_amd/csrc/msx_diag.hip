@@ -476,6 +476,71 @@ static void launch_diag_lds(const Layout &L, const EigTables<float> &E, const do
     MSX_LAUNCH_CHECK();
 }
 
+// Preconditioner factors fac[j][z] = ((H0 + e_j)^-1 e_N)(z) = sum_m Q[z][m] Q[N-1][m] / (e_j + lam_m)
+// for every channel j (multiscat_gmres.f90:255-271), FP64 on the matrix cores: the first
+// product of the diagonal blocks is trivial here (Q^T e_N is the last row of Q), so each
+// wavefront builds W[m][j] = Q[N-1][m] / (e_j + lam_m) for 16 channels directly as the B
+// operand of Y^T = Q W (same K order and A-row permutation as diag_gemm_kernel).
+template <int NT>
+__global__ __launch_bounds__(256) void factors_mfma_kernel(long long total, int nz, EigTables<double> E,
+                                                           const double *__restrict__ e,
+                                                           double *__restrict__ fac) {
+    using M = Mfma<double>;
+    using acc_t = M::acc;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int l16 = lane & 15, g = lane >> 4;
+    const long long j0 = ((long long)blockIdx.x * 4 + wave) * 16;
+    if (j0 >= total) return;
+    const long long j = j0 + l16;
+    const bool live = j < total;
+    const double ej = e[live ? j : j0];
+    const int nzp = E.nzp, NTL = nzp >> 4;
+    const int pa = M::perm(l16);
+    auto tile = [&](int t) { return t < NTL ? t : NTL - 1; };
+    double w[NT][4];
+#pragma unroll
+    for (int mt = 0; mt < NT; ++mt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = 16 * mt + 4 * g + v;
+            w[mt][v] = m < nz ? E.Qp[(long long)(nz - 1) * nzp + m] / (ej + E.lam[m]) : 0.0;
+        }
+    for (int zt = 0; zt < NTL; ++zt) {
+        acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int mt = 0; mt < NT; ++mt) {
+            const double *qa = E.Qp + (long long)(16 * zt + pa) * nzp + 16 * tile(mt) + 4 * g;
+            const double2 a01 = reinterpret_cast<const double2 *>(qa)[0];
+            const double2 a23 = reinterpret_cast<const double2 *>(qa)[1];
+            acc = M::op(a01.x, w[mt][0], acc);
+            acc = M::op(a01.y, w[mt][1], acc);
+            acc = M::op(a23.x, w[mt][2], acc);
+            acc = M::op(a23.y, w[mt][3], acc);
+        }
+        if (!live) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int z = 16 * zt + 4 * g + v;
+            if (z < nz) fac[j * nz + z] = acc[v];
+        }
+    }
+}
+
+bool launch_factors_mfma(long long total_channels, int nz, const EigTables<double> &E,
+                         const double *e, double *fac, hipStream_t s) {
+    if (total_channels <= 0) return true;
+    if (E.nzp % 16 != 0 || E.nzp < nz || E.nzp > 320) return false;
+    const unsigned blocks = (unsigned)((total_channels + 63) / 64);
+    if (E.nzp <= 160)
+        hipLaunchKernelGGL(factors_mfma_kernel<10>, dim3(blocks), dim3(256), 0, s, total_channels,
+                           nz, E, e, fac);
+    else
+        hipLaunchKernelGGL(factors_mfma_kernel<20>, dim3(blocks), dim3(256), 0, s, total_channels,
+                           nz, E, e, fac);
+    MSX_LAUNCH_CHECK();
+    return true;
+}
+
 bool diag_mfma_supported(int nz) { return nz <= 320; }
 
 template <typename T>

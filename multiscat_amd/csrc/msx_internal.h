@@ -205,6 +205,10 @@ void launch_apply_diag(const Layout &L, const double *Q, const double *QT, const
 void launch_eigensystem(int n, double *A, double *Vec, double *lam_sorted, double *Q_sorted,
                         double *QT_sorted, int *info, long long *ticks, hipStream_t s);
 
+// The same factors on the matrix cores (FP64) when nzp <= 320; false if the size does not fit.
+bool launch_factors_mfma(long long total_channels, int nz, const EigTables<double> &E,
+                         const double *e, double *fac, hipStream_t s);
+
 // fac[j][z] = sum_m Q[z][m] Q[N-1][m] / (e_j + lam_m) for all channels.
 void launch_preconditioner_factors_qt(long long total_channels, int nz, const double *Q,
                                       const double *QT, const double *lam, const double *e,

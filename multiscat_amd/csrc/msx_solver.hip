@@ -332,8 +332,11 @@ struct msx_batch {
         if (b.n < n) b.alloc(n);
     }
     void build_factors() {
-        launch_preconditioner_factors_qt(total_ch, t->nz, t->Q.p, t->QT.p, t->lam.p, e_d.p, fac_d.p,
-                                         t->stream);
+        ProfScope ps("factors_f64", t->stream, 2.0 * total_ch * t->nz * t->nz,
+                     (double)total_ch * t->nz * 8.0);
+        if (!launch_factors_mfma(total_ch, t->nz, t->eig64(), e_d.p, fac_d.p, t->stream))
+            launch_preconditioner_factors_qt(total_ch, t->nz, t->Q.p, t->QT.p, t->lam.p, e_d.p,
+                                             fac_d.p, t->stream);
         launch_to_float_real(total_ch * t->nz, fac_d.p, facf_d.p, t->stream);
         factors_ready = true;
     }
