@@ -193,9 +193,16 @@ int msx_table_destroy(msx_table *table);
 /* Copies the eigensystem to the host: lam[nz] ascending, Q row-major (Q[z*nz + m]). */
 int msx_table_eigensystem(const msx_table *table, double *lam, double *Q);
 /* Diagnostics of the last eigensolve, ms[7]: durations (ms) of its four phases
- * (tridiagonalisation, accumulation of the Householder reflectors, QL iteration, reordering),
- * the time (ms) one lane spent chasing QL bulges, the number of QL sweeps and of rotations. */
+ * (tridiagonalisation, accumulation of the Householder reflectors, solution of the tridiagonal
+ * problem, back-transformation or reordering), then for the QL solver (nz > 1024) the time (ms)
+ * one lane spent chasing bulges, the number of sweeps and of rotations (zero otherwise). */
 int msx_table_eig_phases(const msx_table *table, double *ms);
+/* Eigensystem of a symmetric n x n matrix A (row-major, host) with the table's GPU solver:
+ * lam[n] ascending, Q row-major (A = Q diag(lam) Q^T). method 0: divide and conquer for
+ * 2 <= n <= 1024, QL above; 1: Householder + QL; 2: Householder + divide and conquer. Replaces
+ * the reference's diagonalisation routine (lib/multiscat_fortran/fortran/diagsub.f90:5-284)
+ * for callers that need it outside a table. */
+int msx_symmetric_eigensystem(int n, const double *A, int method, double *lam, double *Q);
 
 /* Builds a batch of n_cond conditions on `table`. Channel lists are concatenated:
  * condition b owns channels [off_b, off_b + n_channels[b]) with off_b the prefix sum.

@@ -65,6 +65,25 @@ class SolveOptions:
         return o
 
 
+_EIG_METHODS = {"auto": 0, "ql": 1, "divide_conquer": 2}
+
+
+def symmetric_eigensystem(A, method: str = "auto"):
+    """(lam ascending, Q) with A = Q diag(lam) Q^T for a symmetric matrix, on the GPU in FP64:
+    Householder tridiagonalisation, then divide and conquer ("divide_conquer", n <= 1024) or
+    implicit QL ("ql"); "auto" picks divide and conquer where it applies."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    if A.ndim != 2 or A.shape[0] != A.shape[1]:
+        raise ValueError("A must be square")
+    n = A.shape[0]
+    lam = np.zeros(n)
+    Q = np.zeros((n, n))
+    _lib.check(_lib.lib().msx_symmetric_eigensystem(n, _lib.ptr(A), _EIG_METHODS[method],
+                                                    _lib.ptr(lam), _lib.ptr(Q)),
+               "msx_symmetric_eigensystem")
+    return lam, Q
+
+
 class Table:
     """A potential table V (nkx, nky, nz) and kinetic matrix T resident on the GPU."""
 
@@ -91,8 +110,9 @@ class Table:
         return lam, Q
 
     def eig_phases(self) -> np.ndarray:
-        """Last eigensolve: ms of its phases (tridiagonalisation, reflector accumulation, QL
-        iteration, reordering), ms of bulge chasing, number of QL sweeps and rotations."""
+        """Last eigensolve: ms of its phases (tridiagonalisation, reflector accumulation,
+        tridiagonal solve, back-transformation or reordering), then for the QL solver the ms
+        of bulge chasing and the numbers of sweeps and rotations."""
         ms = np.zeros(7)
         _lib.check(_lib.lib().msx_table_eig_phases(self._h, _lib.ptr(ms)), "msx_table_eig_phases")
         return ms

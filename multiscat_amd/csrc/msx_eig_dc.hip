@@ -1,0 +1,416 @@
+// Eigensystem of H0 = Q diag(lam) Q^T by Householder tridiagonalisation and divide and conquer.
+//
+// The reference diagonalises H0 with an EISPACK-style Householder + QL pair
+// (lib/multiscat_fortran/fortran/diagsub.f90:5-284, called from multiscat_gmres.f90:251-253);
+// the eigenpairs are the same up to eigenvector signs. The QL stage is a serial bulge chase; here
+// the tridiagonal (d, e) from the Householder stage is solved by Cuppen's divide and conquer
+// instead, which is parallel at every step:
+//   * tearing: T = diag(T1, T2) + |beta| u u^T at the boundary p of every node of a balanced
+//     binary tree whose leaves are single rows (d_{p-1} and d_p lose |e_{p-1}|), so the leaves
+//     are the 1 x 1 problems (d_i', e_i) (J. J. M. Cuppen, Numer. Math. 36 (1981) 177);
+//   * merge of two solved children: D + rho z z^T with z the last row of Q1 and the first row of
+//     Q2 (times sign(beta)), scaled so that ||z|| = 1 and rho = 2 |beta|; deflation of small
+//     z_i and of close pole pairs by a Givens rotation (tolerance 8 eps max(|D|, rho));
+//   * secular roots 1 + rho sum z_i^2 / (d_i - lam) = 0, one per thread, each kept as
+//     lam = d_o + tau from its nearer pole d_o so that d_i - lam = (d_i - d_o) - tau carries no
+//     cancellation; iteration on the two-pole rational model of the equation (the
+//     "middle way" of R.-C. Li, LAPACK Working Note 89, 1994) inside a bisection bracket;
+//   * eigenvectors from the weights recomputed from the roots (M. Gu and S. C. Eisenstat, SIAM
+//     J. Matrix Anal. Appl. 16 (1995) 172), z_i^2 = prod_k (lam_k - d_i) / (rho prod_{j!=i}
+//     (d_j - d_i)), which makes them orthogonal to working precision; the product
+//     Q-hat U of the children's eigenvectors with the secular eigenvectors runs on the FP64
+//     matrix cores.
+// Every tree level is one launch with one workgroup per node; the back-transformation
+// Q = Q_H S is a final matrix-core product.
+#include <algorithm>
+#include <cfloat>
+
+#include "msx_internal.h"
+#include "msx_mfma.h"
+
+namespace msx {
+
+constexpr int DC_MAX_N = 1024;
+
+bool eig_dc_supported(int n) { return n >= 2 && n <= DC_MAX_N; }
+
+// first row of node k at depth t of the tree over n rows
+__host__ __device__ inline int dc_row(int n, int t, int k) { return (int)(((long long)k * n) >> t); }
+
+// Leaves: every row is its own 1 x 1 problem after the tearing at all n - 1 boundaries.
+__global__ void dc_leaves_kernel(int n, const double *__restrict__ d, const double *__restrict__ e,
+                                 double *__restrict__ lam, double *__restrict__ S) {
+    const long long nn = (long long)n * n;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < nn;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(idx / n), j = (int)(idx - (long long)i * n);
+        S[idx] = i == j ? 1.0 : 0.0;
+        if (i == j) {
+            double v = d[i];
+            if (i > 0) v -= fabs(e[i - 1]);
+            if (i + 1 < n) v -= fabs(e[i]);
+            lam[i] = v;
+        }
+    }
+}
+
+// Root k of g(lam) = 1 + rho sum_i z_i^2 / (dl_i - lam) (K poles dl ascending, rho > 0): the
+// origin pole o and tau with lam = dl[o] + tau.
+__device__ void dc_secular_root(int k, int K, const double *dl, const double *z, double rho,
+                                int &o_out, double &tau_out) {
+    double lo, hi;
+    int o;
+    if (k < K - 1) {
+        // g increases from -inf to +inf on (dl_k, dl_k+1): its sign at the midpoint tells which
+        // pole is nearer
+        const double mid = 0.5 * (dl[k + 1] - dl[k]);
+        double f = 1.0;
+        for (int i = 0; i < K; ++i) f += rho * z[i] * z[i] / ((dl[i] - dl[k]) - mid);
+        if (f >= 0.0) {
+            o = k;
+            lo = 0.0;
+            hi = mid;
+        } else {
+            o = k + 1;
+            lo = -mid;
+            hi = 0.0;
+        }
+    } else {
+        // the last root lies in (dl_K, dl_K + rho ||z||^2]
+        double zz = 0.0;
+        for (int i = 0; i < K; ++i) zz += z[i] * z[i];
+        o = k;
+        lo = 0.0;
+        hi = rho * zz;
+    }
+    double tau = 0.5 * (lo + hi);
+    const double dlo = dl[o];
+    const double Dk = dl[k] - dlo, Dk1 = k < K - 1 ? dl[k + 1] - dlo : 0.0;
+    for (int it = 0; it < 80; ++it) {
+        // psi: poles left of the root (terms <= 0), phi: poles right of it (terms >= 0)
+        double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
+        for (int i = 0; i <= k; ++i) {
+            const double r = 1.0 / ((dl[i] - dlo) - tau);
+            const double term = rho * z[i] * (z[i] * r);
+            psi += term;
+            dpsi += term * r;
+        }
+        for (int i = k + 1; i < K; ++i) {
+            const double r = 1.0 / ((dl[i] - dlo) - tau);
+            const double term = rho * z[i] * (z[i] * r);
+            phi += term;
+            dphi += term * r;
+        }
+        const double g = 1.0 + psi + phi;
+        const double err = 8.0 * (phi - psi + 1.0) + K * (phi - psi);
+        if (fabs(g) <= DBL_EPSILON * err) break;
+        if (g < 0.0) lo = tau;
+        else hi = tau;
+        if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi))) break;
+        // root of the model c0 + b1 / (d1 - eta) + b2 / (d2 - eta) that matches g and g' at tau
+        const double d1 = Dk - tau;
+        double eta = 0.0;
+        bool ok = false;
+        if (k < K - 1) {
+            const double d2 = Dk1 - tau;
+            const double b1 = dpsi * d1 * d1, b2 = dphi * d2 * d2;
+            const double c0 = g - b1 / d1 - b2 / d2;
+            const double A = c0, B = c0 * (d1 + d2) + b1 + b2, C = d1 * d2 * g;
+            const double q = B + copysign(sqrt(fmax(B * B - 4.0 * A * C, 0.0)), B);
+            if (q != 0.0) {
+                eta = 2.0 * C / q;
+                ok = d1 < eta && eta < d2 && lo < tau + eta && tau + eta < hi;
+            }
+            if (!ok && A != 0.0) {
+                eta = q / (2.0 * A);
+                ok = d1 < eta && eta < d2 && lo < tau + eta && tau + eta < hi;
+            }
+        } else {
+            const double b1 = dpsi * d1 * d1;
+            const double c0 = g - b1 / d1;
+            if (c0 > 0.0) {
+                eta = d1 + b1 / c0;
+                ok = lo < tau + eta && tau + eta < hi;
+            }
+        }
+        tau = ok ? tau + eta : 0.5 * (lo + hi);
+    }
+    o_out = o;
+    tau_out = tau;
+}
+
+constexpr int DC_LDS_DOUBLES = 10, DC_LDS_INTS = 7;  // arrays of length m in the merge kernel
+
+// Merge of the two children of node blockIdx.x at depth t. S (n x n, row-major) holds in its
+// diagonal block [a, b) x [a, b) the eigenvectors of the node (columns, ascending eigenvalues
+// in lam[a .. b)); Wq and Wu are n x n workspaces used in the same block.
+__global__ __launch_bounds__(1024) void dc_merge_kernel(int n, int t, const double *__restrict__ e,
+                                                        double *lam, double *S, double *Wq,
+                                                        double *Wu) {
+    const int node = blockIdx.x;
+    const int a = dc_row(n, t, node), b = dc_row(n, t, node + 1);
+    const int m = b - a;
+    if (m < 2) return;  // a single row: its child's eigenpair is already in place
+    const int p = dc_row(n, t + 1, 2 * node + 1);
+    const int m1 = p - a;  // 1 <= m1 < m for every node of two or more rows
+    const int tid = threadIdx.x, NT = blockDim.x;
+    extern __shared__ double dcs[];
+    double *D = dcs, *z = D + m, *dl = z + m, *zz = dl + m, *tau = zz + m, *zh = tau + m;
+    double *inrm = zh + m, *val = inrm + m, *ru = val + m, *rw = ru + m;
+    int *perm = reinterpret_cast<int *>(rw + m), *nd = perm + m, *dfl = nd + m, *org = dfl + m;
+    int *rank = org + m, *rp = rank + m, *rj = rp + m;
+    __shared__ int s_K, s_R;
+    const double beta = e[p - 1];
+    const double rho = 2.0 * fabs(beta), sg = beta >= 0.0 ? 1.0 : -1.0;
+    const double r2 = 0.70710678118654752440;
+    auto at = [&](double *M, int r, int c) -> double & { return M[(long long)(a + r) * n + a + c]; };
+
+    // (1) merged order of the children's eigenvalues (ties: first child first) and z
+    for (int c = tid; c < m; c += NT) {
+        const double v = lam[a + c];
+        int pos;
+        if (c < m1) {
+            int lo = 0, hi = m - m1;
+            while (lo < hi) {
+                const int md = (lo + hi) >> 1;
+                if (lam[p + md] < v) lo = md + 1;
+                else hi = md;
+            }
+            pos = c + lo;
+        } else {
+            int lo = 0, hi = m1;
+            while (lo < hi) {
+                const int md = (lo + hi) >> 1;
+                if (lam[a + md] <= v) lo = md + 1;
+                else hi = md;
+            }
+            pos = (c - m1) + lo;
+        }
+        perm[pos] = c;
+        D[pos] = v;
+        z[pos] = c < m1 ? r2 * at(S, m1 - 1, c) : sg * r2 * at(S, m1, c);
+    }
+    __syncthreads();
+    // (2) Q-hat = diag(Q1, Q2) with its columns in merged order
+    for (long long idx = tid; idx < (long long)m * m; idx += NT) {
+        const int r = (int)(idx / m), q = (int)(idx - (long long)r * m);
+        const int c = perm[q];
+        at(Wq, r, q) = ((r < m1) == (c < m1)) ? at(S, r, c) : 0.0;
+    }
+    // (3) deflation (one thread: the pair test chains along the non-deflated entries)
+    if (tid == 0) {
+        const double tol = 8.0 * DBL_EPSILON * fmax(fmax(fabs(D[0]), fabs(D[m - 1])), rho);
+        int K = 0, nq = 0, R = 0, prev = -1;
+        for (int j = 0; j < m; ++j) {
+            if (rho * fabs(z[j]) <= tol) {
+                dfl[nq++] = j;
+                continue;
+            }
+            if (prev < 0) {
+                prev = j;
+                continue;
+            }
+            const double tt = hypot(z[prev], z[j]);
+            const double u = z[prev] / tt, w = z[j] / tt;
+            if (fabs(u * w * (D[j] - D[prev])) <= tol) {
+                // rotate columns (prev, j) so that z_prev = 0: prev deflates
+                rp[R] = prev;
+                rj[R] = j;
+                ru[R] = u;
+                rw[R] = w;
+                ++R;
+                const double dp = D[prev], dj = D[j];
+                D[prev] = w * w * dp + u * u * dj;
+                D[j] = u * u * dp + w * w * dj;
+                z[j] = tt;
+                z[prev] = 0.0;
+                dfl[nq++] = prev;
+            } else {
+                nd[K++] = prev;
+            }
+            prev = j;
+        }
+        if (prev >= 0) nd[K++] = prev;
+        s_K = K;
+        s_R = R;
+    }
+    __syncthreads();
+    const int K = s_K, R = s_R;
+    // (4) the deflating rotations on the rows of Q-hat, in order
+    for (int r = tid; r < m; r += NT) {
+        double *row = &at(Wq, r, 0);
+        for (int q = 0; q < R; ++q) {
+            const double x = row[rp[q]], y = row[rj[q]];
+            row[rp[q]] = rw[q] * x - ru[q] * y;
+            row[rj[q]] = ru[q] * x + rw[q] * y;
+        }
+    }
+    for (int i = tid; i < K; i += NT) {
+        dl[i] = D[nd[i]];
+        zz[i] = z[nd[i]];
+    }
+    __syncthreads();
+    // (5) secular roots; eigenvalues of the node, non-deflated first, then deflated
+    for (int q = tid; q < K; q += NT) {
+        int o;
+        double tq;
+        dc_secular_root(q, K, dl, zz, rho, o, tq);
+        org[q] = o;
+        tau[q] = tq;
+        val[q] = dl[o] + tq;
+    }
+    for (int q = tid; q < m - K; q += NT) val[K + q] = D[dfl[q]];
+    __syncthreads();
+    // (6) recomputed weights, ranks of the eigenvalues
+    for (int i = tid; i < K; i += NT) {
+        const double di = dl[i];
+        double prod = -((di - dl[org[K - 1]]) - tau[K - 1]) / rho;
+        for (int q = 0; q < K - 1; ++q) {
+            // (lam_q - d_i) / (d_q - d_i) for q < i, / (d_q+1 - d_i) for q >= i: both in (0, 1)
+            const double num = -((di - dl[org[q]]) - tau[q]);
+            const double den = (q < i ? dl[q] : dl[q + 1]) - di;
+            prod *= num / den;
+        }
+        zh[i] = copysign(sqrt(fmax(prod, 0.0)), zz[i]);
+    }
+    for (int s = tid; s < m; s += NT) {
+        const double v = val[s];
+        int r = 0;
+        for (int q = 0; q < m; ++q) r += (val[q] < v) || (val[q] == v && q < s);
+        rank[s] = r;
+    }
+    __syncthreads();
+    // (7) ascending eigenvalues; norms of the secular eigenvectors u_q(i) = zh_i / (d_i - lam_q)
+    for (int s = tid; s < m; s += NT) lam[a + rank[s]] = val[s];
+    for (int q = tid; q < K; q += NT) {
+        const double base = dl[org[q]], tq = tau[q];
+        double ss = 0.0;
+        for (int i = 0; i < K; ++i) {
+            const double u = zh[i] / ((dl[i] - base) - tq);
+            ss += u * u;
+        }
+        inrm[q] = 1.0 / sqrt(ss);
+    }
+    __syncthreads();
+    // (8) U (K x K) into Wu; deflated eigenvectors are columns of Q-hat
+    for (long long idx = tid; idx < (long long)K * K; idx += NT) {
+        const int i = (int)(idx / K), q = (int)(idx - (long long)i * K);
+        at(Wu, i, q) = zh[i] * inrm[q] / ((dl[i] - dl[org[q]]) - tau[q]);
+    }
+    const int nd_cnt = m - K;
+    for (long long idx = tid; idx < (long long)m * nd_cnt; idx += NT) {
+        const int r = (int)(idx / nd_cnt), q = (int)(idx - (long long)r * nd_cnt);
+        at(S, r, rank[K + q]) = at(Wq, r, dfl[q]);
+    }
+    __syncthreads();
+    // (9) S[:, rank_q] = Q-hat[:, nd] U[:, q], 16 x 16 tiles per wavefront on the matrix cores
+    using M = Mfma<double>;
+    const int wave = tid >> 6, lane = tid & 63, l16 = lane & 15, g = lane >> 4, NW = NT >> 6;
+    const int tr = (m + 15) >> 4, tc = (K + 15) >> 4;
+    for (int tile = wave; tile < tr * tc; tile += NW) {
+        const int r0 = (tile / tc) * 16, q0 = (tile % tc) * 16;
+        const int ra = r0 + l16, qb = q0 + l16;
+        const double *arow = &at(Wq, ra < m ? ra : m - 1, 0);
+        M::acc acc = {0.0, 0.0, 0.0, 0.0};
+        for (int i0 = 0; i0 < K; i0 += 4) {
+            const int i = i0 + g;
+            const double av = (ra < m && i < K) ? arow[nd[i]] : 0.0;
+            const double bv = (i < K && qb < K) ? at(Wu, i, qb) : 0.0;
+            acc = M::op(av, bv, acc);
+        }
+        if (qb < K) {
+            const int col = rank[qb];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = r0 + M::row(g, v);
+                if (r < m) at(S, r, col) = acc[v];
+            }
+        }
+    }
+}
+
+// Q = Q_H S (both n x n row-major), Q^T beside it, eigenvalues copied; info = 1 if any
+// eigenvalue is not finite.
+__global__ __launch_bounds__(256) void dc_assemble_kernel(int n, const double *__restrict__ QH,
+                                                          const double *__restrict__ S,
+                                                          const double *__restrict__ lam,
+                                                          double *__restrict__ lam_sorted,
+                                                          double *__restrict__ Q,
+                                                          double *__restrict__ QT, int *info,
+                                                          long long *ticks) {
+    using M = Mfma<double>;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, g = lane >> 4;
+    if (blockIdx.x == 0 && tid == 0 && ticks) ticks[3] = wall_clock64();
+    const int T = (n + 15) >> 4;
+    const int tile = blockIdx.x * 4 + wave;
+    if (tile < T * T) {
+        const int r0 = (tile / T) * 16, c0 = (tile % T) * 16;
+        const int ra = r0 + l16, cb = c0 + l16;
+        const double *arow = QH + (long long)(ra < n ? ra : n - 1) * n;
+        M::acc acc = {0.0, 0.0, 0.0, 0.0};
+        for (int j0 = 0; j0 < n; j0 += 4) {
+            const int j = j0 + g;
+            const double av = (ra < n && j < n) ? arow[j] : 0.0;
+            const double bv = (j < n && cb < n) ? S[(long long)j * n + cb] : 0.0;
+            acc = M::op(av, bv, acc);
+        }
+        if (cb < n) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = r0 + M::row(g, v);
+                if (r < n) {
+                    Q[(long long)r * n + cb] = acc[v];
+                    QT[(long long)cb * n + r] = acc[v];
+                }
+            }
+        }
+    }
+    if (blockIdx.x == 0) {
+        __shared__ int bad;
+        if (tid == 0) bad = 0;
+        __syncthreads();
+        for (int i = tid; i < n; i += blockDim.x) {
+            const double v = lam[i];
+            lam_sorted[i] = v;
+            if (!isfinite(v)) bad = 1;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            *info = bad;
+            if (ticks) ticks[4] = wall_clock64();
+        }
+    }
+}
+
+size_t eig_dc_workspace(int n) { return 3 * (size_t)n * n + 3 * (size_t)n; }
+
+void launch_eigensystem_dc(int n, double *A, double *QH, double *work, double *lam_sorted,
+                           double *Q_sorted, double *QT_sorted, int *info, long long *ticks,
+                           hipStream_t s) {
+    if (!eig_dc_supported(n)) throw Error(3, "divide and conquer: 2 <= n <= 1024 required");
+    const size_t n2 = (size_t)n * n;
+    double *S = work, *Wq = S + n2, *Wu = Wq + n2, *d = Wu + n2, *e = d + n, *lam = e + n;
+    if (ticks) MSX_HIP(hipMemsetAsync(ticks, 0, 8 * sizeof(long long), s));
+    launch_tridiagonalise(n, A, QH, d, e, ticks, s);
+    const unsigned leaf_blocks = (unsigned)std::min<size_t>((n2 + 255) / 256, 4096);
+    hipLaunchKernelGGL(dc_leaves_kernel, dim3(leaf_blocks), dim3(256), 0, s, n, d, e, lam, S);
+    MSX_LAUNCH_CHECK();
+    int L = 0;
+    while ((1 << L) < n) ++L;
+    const size_t per = DC_LDS_DOUBLES * sizeof(double) + DC_LDS_INTS * sizeof(int);
+    for (int t = L - 1; t >= 0; --t) {
+        const int mmax = (n + (1 << t) - 1) >> t;
+        const size_t shm = per * (size_t)mmax;
+        const int threads = mmax <= 32 ? 64 : (mmax <= 128 ? 256 : 1024);
+        set_dynamic_lds((const void *)dc_merge_kernel, shm);
+        hipLaunchKernelGGL(dc_merge_kernel, dim3(1u << t), dim3(threads), shm, s, n, t, e, lam, S,
+                           Wq, Wu);
+        MSX_LAUNCH_CHECK();
+    }
+    const int T = (n + 15) / 16;
+    hipLaunchKernelGGL(dc_assemble_kernel, dim3((T * T + 3) / 4), dim3(256), 0, s, n, QH, S, lam,
+                       lam_sorted, Q_sorted, QT_sorted, info, ticks);
+    MSX_LAUNCH_CHECK();
+}
+
+}  // namespace msx

@@ -205,6 +205,20 @@ void launch_apply_diag(const Layout &L, const double *Q, const double *QT, const
 void launch_eigensystem(int n, double *A, double *Vec, double *lam_sorted, double *Q_sorted,
                         double *QT_sorted, int *info, long long *ticks, hipStream_t s);
 
+// Householder stage alone: tridiagonal (d, e) with e[i] coupling rows i, i + 1, and Q_H
+// (row-major, A = Q_H T Q_H^T); ticks[0..2] as above.
+void launch_tridiagonalise(int n, double *A, double *QH, double *d, double *e, long long *ticks,
+                           hipStream_t s);
+
+// The same eigensystem with the tridiagonal solved by divide and conquer (2 <= n <= 1024):
+// work holds eig_dc_workspace(n) doubles, QH n x n; ticks[3] marks the start of the final
+// back-transformation, ticks[4] its end (ticks[5..7] are zero).
+bool eig_dc_supported(int n);
+size_t eig_dc_workspace(int n);
+void launch_eigensystem_dc(int n, double *A, double *QH, double *work, double *lam_sorted,
+                           double *Q_sorted, double *QT_sorted, int *info, long long *ticks,
+                           hipStream_t s);
+
 // The same factors on the matrix cores (FP64) when nzp <= 320; false if the size does not fit.
 bool launch_factors_mfma(long long total_channels, int nz, const EigTables<double> &E,
                          const double *e, double *fac, hipStream_t s);

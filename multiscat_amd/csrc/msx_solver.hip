@@ -224,12 +224,35 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
 // ---------------------------------------------------------------------------------------
 // Potential table.
 // ---------------------------------------------------------------------------------------
+// Eigensystem of the symmetric n x n matrix A (destroyed; Vm n x n workspace): method 0
+// divide and conquer when n fits it, else QL; 1 QL; 2 divide and conquer.
+static void run_eigensolver(int n, double *A, double *Vm, DBuf<double> &dcw, double *lam,
+                            double *Q, double *QT, int *info, long long *ticks, hipStream_t s,
+                            int method) {
+    const bool dc = method == 2 || (method == 0 && eig_dc_supported(n));
+    const double nn = n;
+    if (dc) {
+        if (!eig_dc_supported(n)) throw Error(MSX_ERR_INVALID, "divide and conquer: 2 <= n <= 1024");
+        if (dcw.n < eig_dc_workspace(n)) dcw.alloc(eig_dc_workspace(n));
+        // tridiagonalisation and Q_H (8/3 n^3), merge products (about 8/3 n^3 over the tree
+        // levels), back-transformation Q_H S (2 n^3)
+        ProfScope ps("eigensystem_f64", s, (8.0 / 3.0 + 8.0 / 3.0 + 2.0) * nn * nn * nn,
+                     5.0 * 8.0 * nn * nn);
+        launch_eigensystem_dc(n, A, Vm, dcw.p, lam, Q, QT, info, ticks, s);
+    } else {
+        // tridiagonalisation and Q_H (8/3 n^3), QL rotations on Z (about 6 n^3 for about 1.5
+        // sweeps per eigenvalue)
+        ProfScope ps("eigensystem_f64", s, (8.0 / 3.0 + 6.0) * nn * nn * nn, 5.0 * 8.0 * nn * nn);
+        launch_eigensystem(n, A, Vm, lam, Q, QT, info, ticks, s);
+    }
+}
+
 struct msx_table {
     int nkx = 0, nky = 0, nz = 0;
     hipStream_t stream = 0;
     DBuf<cx<double>> V, Vt;
     DBuf<cx<float>> Vf, Vtf;
-    DBuf<double> T, A, Vm, lam, Q, QT, Qp, QTp;
+    DBuf<double> T, A, Vm, lam, Q, QT, Qp, QTp, dcw;
     DBuf<float> Qf, QTf, Qpf, QTpf;
     DBuf<int> info;
     DBuf<long long> ticks;  // phase stamps of the last eigensolve
@@ -245,14 +268,7 @@ struct msx_table {
     void prepare() {
         const size_t n2 = (size_t)nz * nz;
         launch_build_h0(nz, T.p, V.p, A.p, stream);
-        {
-            // tridiagonalisation (4/3 n^3), accumulation of Q_H (4/3 n^3), QL rotations on Z
-            // (about 6 n^3 for about 1.5 sweeps per eigenvalue)
-            const double n = nz;
-            ProfScope ps("eigensystem_f64", stream, (4.0 / 3.0 + 4.0 / 3.0 + 6.0) * n * n * n,
-                         5.0 * 8.0 * n * n);
-            launch_eigensystem(nz, A.p, Vm.p, lam.p, Q.p, QT.p, info.p, ticks.p, stream);
-        }
+        run_eigensolver(nz, A.p, Vm.p, dcw, lam.p, Q.p, QT.p, info.p, ticks.p, stream, 0);
         int h_info = 0;
         info.download(&h_info, 1, stream);
         MSX_HIP(hipStreamSynchronize(stream));
@@ -806,6 +822,39 @@ int msx_table_eig_phases(const msx_table *t, double *ms) {
         ms[4] = rate_khz > 0 ? (double)tk[5] / rate_khz : 0.0;
         ms[5] = (double)tk[6];
         ms[6] = (double)tk[7];
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+int msx_symmetric_eigensystem(int n, const double *A, int method, double *lam, double *Q) {
+    try {
+        if (!A || !lam || !Q) throw Error(MSX_ERR_INVALID, "null argument");
+        if (n < 1 || n > 2048) throw Error(MSX_ERR_DIMS, "1 <= n <= 2048 required");
+        if (method < 0 || method > 2) throw Error(MSX_ERR_INVALID, "method must be 0, 1 or 2");
+        if (n == 1) {
+            lam[0] = A[0];
+            Q[0] = 1.0;
+            return MSX_OK;
+        }
+        hipStream_t s;
+        MSX_HIP(hipStreamCreate(&s));
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } guard{s};
+        const size_t n2 = (size_t)n * n;
+        DBuf<double> a(n2), vm(n2), dcw, l(n), q(n2), qt(n2);
+        DBuf<int> info(1);
+        a.upload(A, n2, s);
+        run_eigensolver(n, a.p, vm.p, dcw, l.p, q.p, qt.p, info.p, nullptr, s, method);
+        int h_info = 0;
+        info.download(&h_info, 1, s);
+        l.download(lam, n, s);
+        q.download(Q, n2, s);
+        MSX_HIP(hipStreamSynchronize(s));
+        if (h_info != 0) throw Error(MSX_ERR_EIGEN, "eigensolver did not converge");
         return MSX_OK;
     } catch (const Error &e) {
         return fail(e);
