@@ -38,9 +38,7 @@ __global__ __launch_bounds__(EIG_THREADS) void eig_tridiag_ql_kernel(int n, doub
                                                                      double *Q_sorted,
                                                                      double *QT_sorted,
                                                                      int *info,
-                                                                     long long *ticks,
-                                                                     double *d_out,
-                                                                     double *e_out) {
+                                                                     long long *ticks) {
     extern __shared__ double es[];
     double *v = es;                      // [n]
     double *w = v + n;                   // [n]
@@ -158,14 +156,6 @@ __global__ __launch_bounds__(EIG_THREADS) void eig_tridiag_ql_kernel(int n, doub
             W[(long long)(r0 + i) * n + r0 + j] -= v[i] * w[j];
         }
         __syncthreads();
-    }
-    if (d_out) {  // tridiagonal (d, e) and Q_H (row-major in W) for the divide-and-conquer stage
-        for (int k = tid; k < n; k += NT) {
-            d_out[k] = d[k];
-            e_out[k] = e[k];
-        }
-        if (ticks && tid == 0) ticks[2] = wall_clock64();
-        return;
     }
     // Z^T into A (A's contents are no longer needed)
     double *ZT = A;
@@ -342,17 +332,7 @@ void launch_eigensystem(int n, double *A, double *Vec, double *lam_sorted, doubl
     const size_t shm = sizeof(double) * (size_t)n * (7 + EIG_GROUPS);
     set_dynamic_lds((const void *)eig_tridiag_ql_kernel, shm);
     hipLaunchKernelGGL(eig_tridiag_ql_kernel, dim3(1), dim3(EIG_THREADS), shm, s, n, A, Vec,
-                       lam_sorted, Q_sorted, QT_sorted, info, ticks, nullptr, nullptr);
-    MSX_LAUNCH_CHECK();
-}
-
-void launch_tridiagonalise(int n, double *A, double *QH, double *d, double *e, long long *ticks,
-                           hipStream_t s) {
-    if (n < 1 || n > 2048) throw Error(3, "eigensystem: 1 <= n <= 2048 required");
-    const size_t shm = sizeof(double) * (size_t)n * (7 + EIG_GROUPS);
-    set_dynamic_lds((const void *)eig_tridiag_ql_kernel, shm);
-    hipLaunchKernelGGL(eig_tridiag_ql_kernel, dim3(1), dim3(EIG_THREADS), shm, s, n, A, QH,
-                       nullptr, nullptr, nullptr, nullptr, ticks, d, e);
+                       lam_sorted, Q_sorted, QT_sorted, info, ticks);
     MSX_LAUNCH_CHECK();
 }
 

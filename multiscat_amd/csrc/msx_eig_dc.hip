@@ -5,6 +5,8 @@
 // the eigenpairs are the same up to eigenvector signs. The QL stage is a serial bulge chase; here
 // the tridiagonal (d, e) from the Householder stage is solved by Cuppen's divide and conquer
 // instead, which is parallel at every step:
+//   * Householder reduction A = Q_H T Q_H^T, Q_H = P_{n-1} ... P_2 (reflectors kept, Q_H never
+//     formed);
 //   * tearing: T = diag(T1, T2) + |beta| u u^T at the boundary p of every node of a balanced
 //     binary tree whose leaves are single rows (d_{p-1} and d_p lose |e_{p-1}|), so the leaves
 //     are the 1 x 1 problems (d_i', e_i) (J. J. M. Cuppen, Numer. Math. 36 (1981) 177);
@@ -20,8 +22,9 @@
 //     (d_j - d_i)), which makes them orthogonal to working precision; the product
 //     Q-hat U of the children's eigenvectors with the secular eigenvectors runs on the FP64
 //     matrix cores.
-// Every tree level is one launch with one workgroup per node; the back-transformation
-// Q = Q_H S is a final matrix-core product.
+// Every tree level is one launch with one workgroup per node. The Householder stage runs in
+// EISPACK's tred2 order on the packed lower triangle held in LDS, and the back-transformation
+// applies its reflectors to the tridiagonal eigenvectors one column per lane group.
 #include <algorithm>
 #include <cfloat>
 
@@ -163,7 +166,9 @@ __global__ __launch_bounds__(1024) void dc_merge_kernel(int n, int t, const doub
     const double beta = e[p - 1];
     const double rho = 2.0 * fabs(beta), sg = beta >= 0.0 ? 1.0 : -1.0;
     const double r2 = 0.70710678118654752440;
-    auto at = [&](double *M, int r, int c) -> double & { return M[(long long)(a + r) * n + a + c]; };
+    const int wave = tid >> 6, lane = tid & 63, NW = NT >> 6;
+    // element (r, c) of the node's block of an n x n matrix (n <= 1024: 32-bit offsets)
+    auto at = [&](double *M, int r, int c) -> double & { return M[(a + r) * n + a + c]; };
 
     // (1) merged order of the children's eigenvalues (ties: first child first) and z
     for (int c = tid; c < m; c += NT) {
@@ -192,11 +197,11 @@ __global__ __launch_bounds__(1024) void dc_merge_kernel(int n, int t, const doub
     }
     __syncthreads();
     // (2) Q-hat = diag(Q1, Q2) with its columns in merged order
-    for (long long idx = tid; idx < (long long)m * m; idx += NT) {
-        const int r = (int)(idx / m), q = (int)(idx - (long long)r * m);
-        const int c = perm[q];
-        at(Wq, r, q) = ((r < m1) == (c < m1)) ? at(S, r, c) : 0.0;
-    }
+    for (int r = wave; r < m; r += NW)
+        for (int q = lane; q < m; q += 64) {
+            const int c = perm[q];
+            at(Wq, r, q) = ((r < m1) == (c < m1)) ? at(S, r, c) : 0.0;
+        }
     // (3) deflation (one thread: the pair test chains along the non-deflated entries)
     if (tid == 0) {
         const double tol = 8.0 * DBL_EPSILON * fmax(fmax(fabs(D[0]), fabs(D[m - 1])), rho);
@@ -293,19 +298,17 @@ __global__ __launch_bounds__(1024) void dc_merge_kernel(int n, int t, const doub
     }
     __syncthreads();
     // (8) U (K x K) into Wu; deflated eigenvectors are columns of Q-hat
-    for (long long idx = tid; idx < (long long)K * K; idx += NT) {
-        const int i = (int)(idx / K), q = (int)(idx - (long long)i * K);
-        at(Wu, i, q) = zh[i] * inrm[q] / ((dl[i] - dl[org[q]]) - tau[q]);
+    for (int i = wave; i < K; i += NW) {
+        const double zi = zh[i], di = dl[i];
+        for (int q = lane; q < K; q += 64) at(Wu, i, q) = zi * inrm[q] / ((di - dl[org[q]]) - tau[q]);
     }
     const int nd_cnt = m - K;
-    for (long long idx = tid; idx < (long long)m * nd_cnt; idx += NT) {
-        const int r = (int)(idx / nd_cnt), q = (int)(idx - (long long)r * nd_cnt);
-        at(S, r, rank[K + q]) = at(Wq, r, dfl[q]);
-    }
+    for (int r = wave; r < m; r += NW)
+        for (int q = lane; q < nd_cnt; q += 64) at(S, r, rank[K + q]) = at(Wq, r, dfl[q]);
     __syncthreads();
     // (9) S[:, rank_q] = Q-hat[:, nd] U[:, q], 16 x 16 tiles per wavefront on the matrix cores
     using M = Mfma<double>;
-    const int wave = tid >> 6, lane = tid & 63, l16 = lane & 15, g = lane >> 4, NW = NT >> 6;
+    const int l16 = lane & 15, g = lane >> 4;
     const int tr = (m + 15) >> 4, tc = (K + 15) >> 4;
     for (int tile = wave; tile < tr * tc; tile += NW) {
         const int r0 = (tile / tc) * 16, q0 = (tile % tc) * 16;
@@ -329,39 +332,200 @@ __global__ __launch_bounds__(1024) void dc_merge_kernel(int n, int t, const doub
     }
 }
 
-// Q = Q_H S (both n x n row-major), Q^T beside it, eigenvalues copied; info = 1 if any
-// eigenvalue is not finite.
-__global__ __launch_bounds__(256) void dc_assemble_kernel(int n, const double *__restrict__ QH,
-                                                          const double *__restrict__ S,
-                                                          const double *__restrict__ lam,
-                                                          double *__restrict__ lam_sorted,
-                                                          double *__restrict__ Q,
-                                                          double *__restrict__ QT, int *info,
-                                                          long long *ticks) {
-    using M = Mfma<double>;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, g = lane >> 4;
-    if (blockIdx.x == 0 && tid == 0 && ticks) ticks[3] = wall_clock64();
-    const int T = (n + 15) >> 4;
-    const int tile = blockIdx.x * 4 + wave;
-    if (tile < T * T) {
-        const int r0 = (tile / T) * 16, c0 = (tile % T) * 16;
-        const int ra = r0 + l16, cb = c0 + l16;
-        const double *arow = QH + (long long)(ra < n ? ra : n - 1) * n;
-        M::acc acc = {0.0, 0.0, 0.0, 0.0};
-        for (int j0 = 0; j0 < n; j0 += 4) {
-            const int j = j0 + g;
-            const double av = (ra < n && j < n) ? arow[j] : 0.0;
-            const double bv = (j < n && cb < n) ? S[(long long)j * n + cb] : 0.0;
-            acc = M::op(av, bv, acc);
+// ---- Householder stage -------------------------------------------------------------------
+// tred2 order (EISPACK, Golub & Van Loan sec. 8.3.1): row i = n-1 .. 1 of the lower triangle is
+// reduced by the reflector P_i = I - u u^T / h acting on coordinates 0 .. i-1, so the active
+// block is always a prefix of the packed lower triangle (element (j, k), k <= j, at
+// j (j + 1) / 2 + k). The prefix lives in LDS once it fits (from the start for n up to ~190);
+// u is kept in row i, which the later steps no longer touch.
+__device__ __forceinline__ int tri(int j) { return j * (j + 1) / 2; }  // n <= 1024: fits
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// One reduction step on the packed matrix Am (LDS or global); p is an LDS vector of length n.
+// Three workgroup barriers: the two reductions of the step are computed redundantly by every
+// wavefront (same order, so the same value), u_l = f - g is substituted in registers and
+// written into row i only after the last barrier, and q = p / h - K u is formed where used.
+__device__ __forceinline__ void tred_step(int i, double *Am, double *p, double *h_out,
+                                          double *e_out) {
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const int wave = tid >> 6, lane = tid & 63, NW = NT >> 6;
+    const int l = i - 1;
+    double *x = Am + tri(i);  // row i: x[0 .. l], diagonal x[i]
+    for (int j = tid; j < i; j += NT) p[j] = 0.0;
+    const double f = x[l];
+    double loc = 0.0;
+    for (int k = lane; k < l; k += 64) loc += x[k] * x[k];
+    const double sigma = wave_sum(loc);
+    if (!(sigma > 0.0)) {  // row i is already tridiagonal (always so for i = 1)
+        if (tid == 0) {
+            e_out[l] = f;
+            h_out[i] = 0.0;
         }
-        if (cb < n) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int r = r0 + M::row(g, v);
-                if (r < n) {
-                    Q[(long long)r * n + cb] = acc[v];
-                    QT[(long long)cb * n + r] = acc[v];
+        __syncthreads();
+        return;
+    }
+    const double g = f > 0.0 ? -sqrt(sigma + f * f) : sqrt(sigma + f * f);
+    const double h = sigma + f * f - f * g;  // ||u||^2 / 2 with u = x - g e_l
+    const double ul = f - g, hinv = 1.0 / h;
+    auto u = [&](int k) { return k == l ? ul : x[k]; };
+    __syncthreads();  // p is zero
+    // p = A u over the i x i block: 32 x 32 tiles of the lower triangle; in each tile lanes
+    // 0..31 sum along their row, lanes 32..63 down their column (the upper triangle's part).
+    // Loads are predicated by value (a clamped address times zero), so the loop has no branch.
+    const bool rowl = lane < 32;
+    const int ln = lane & 31;
+    const int T = (i + 31) >> 5, ntile = T * (T + 1) / 2;
+    {
+        int I = 0, J = 0;  // tile coordinates of `tile`, advanced incrementally
+        for (int tile = wave, seen = 0; tile < ntile; tile += NW) {
+            for (; seen < tile; ++seen)
+                if (++J > I) {
+                    ++I;
+                    J = 0;
                 }
+            const int r0 = 32 * I, c0 = 32 * J;
+            const int mine = rowl ? r0 + ln : c0 + ln;
+            const bool diag = I == J;
+            int addr = rowl ? tri(mine) + c0 : tri(r0) + mine;
+            int step = rowl ? 1 : r0 + 1;
+            const int ub = rowl ? c0 : r0;
+            double acc = 0.0;
+#pragma unroll 8
+            for (int t = 0; t < 32; ++t) {
+                const int R = rowl ? mine : r0 + t;
+                const int C = rowl ? c0 + t : mine;
+                const bool ok = mine < i && R < i && (!diag || (rowl ? C <= R : C < R));
+                const double a = Am[ok ? addr : 0];
+                const double uu = u(min(ub + t, l));
+                acc = fma(ok ? a : 0.0, uu, acc);
+                addr += step;
+                step += rowl ? 0 : 1;
+            }
+            if (mine < i) atomicAdd(&p[mine], acc);
+        }
+    }
+    __syncthreads();
+    double kl = 0.0;
+    for (int j = lane; j < i; j += 64) kl += u(j) * p[j];
+    const double K = 0.5 * wave_sum(kl) * hinv * hinv;
+    auto q = [&](int k) { return p[k] * hinv - K * u(k); };
+    // A -= u q^T + q u^T on the lower triangle of the block, by the same tiles: lane owns
+    // column c0 + (lane & 31) and rows r0 + (lane >> 5) + 2 t
+    {
+        int I = 0, J = 0;
+        for (int tile = wave, seen = 0; tile < ntile; tile += NW) {
+            for (; seen < tile; ++seen)
+                if (++J > I) {
+                    ++I;
+                    J = 0;
+                }
+            const int r0 = 32 * I, C = 32 * J + ln;
+            const bool diag = I == J;
+            if (C < i) {
+                const double uc = u(C), qc = q(C);
+#pragma unroll 4
+                for (int t = 0; t < 16; ++t) {
+                    const int R = r0 + (lane >> 5) + 2 * t;
+                    if (R < i && (!diag || C <= R)) Am[tri(R) + C] -= u(R) * qc + q(R) * uc;
+                }
+            }
+        }
+    }
+    if (tid == 0) {
+        e_out[l] = g;
+        h_out[i] = h;
+    }
+    __syncthreads();
+    if (tid == 0) x[l] = ul;  // the reflector is stored in row i, which no later step reads
+}
+
+// A (n x n row-major, symmetric; its lower triangle is read) -> d, e (e[i] couples rows i and
+// i + 1), the reflectors u (rows of the packed matrix Ag) and h (h[i] = 0: no reflector).
+// lds_rows: rows of the packed prefix that fit the dynamic LDS after the n-vector p.
+__global__ __launch_bounds__(1024) void eig_tred_kernel(int n, const double *__restrict__ A,
+                                                        double *Ag, double *h_out, double *d_out,
+                                                        double *e_out, int lds_rows,
+                                                        long long *ticks) {
+    extern __shared__ double tl[];
+    double *p = tl, *Al = tl + n;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    if (ticks && tid == 0) ticks[0] = wall_clock64();
+    const bool start_lds = lds_rows >= n;
+    double *dst0 = start_lds ? Al : Ag;
+    for (int j = tid >> 6; j < n; j += NT >> 6)
+        for (int k = tid & 63; k <= j; k += 64) dst0[tri(j) + k] = A[(long long)j * n + k];
+    __syncthreads();
+    int in_lds = start_lds ? n : 0;  // rows held in LDS
+    for (int i = n - 1; i >= 1; --i) {
+        if (!in_lds && i + 1 <= lds_rows) {  // rows 0 .. i fit from here on
+            const int cnt = tri(i + 1);
+            for (int idx = tid; idx < cnt; idx += NT) Al[idx] = Ag[idx];
+            __syncthreads();
+            in_lds = i + 1;
+        }
+        if (in_lds) tred_step(i, Al, p, h_out, e_out);
+        else tred_step(i, Ag, p, h_out, e_out);
+    }
+    __syncthreads();
+    if (in_lds) {
+        const int cnt = tri(in_lds);
+        for (int idx = tid; idx < cnt; idx += NT) Ag[idx] = Al[idx];
+        __syncthreads();
+    }
+    for (int k = tid; k < n; k += NT) d_out[k] = Ag[tri(k) + k];
+    if (tid == 0) {
+        e_out[n - 1] = 0.0;
+        h_out[0] = 0.0;
+        if (ticks) ticks[1] = ticks[2] = wall_clock64();
+    }
+}
+
+// Q = P_{n-1} ... P_2 S: G lanes own one column of S (R rows each, in registers) and apply the
+// reflectors in order, with no workgroup barrier. Also copies the eigenvalues, sets info = 1
+// if any is not finite, and Q^T.
+template <int G, int R>
+__global__ __launch_bounds__(256) void eig_backtransform_kernel(
+    int n, const double *__restrict__ Ap, const double *__restrict__ h,
+    const double *__restrict__ S, const double *__restrict__ lam, double *__restrict__ lam_sorted,
+    double *__restrict__ Q, double *__restrict__ QT, int *info, long long *ticks) {
+    const int tid = threadIdx.x;
+    if (blockIdx.x == 0 && tid == 0 && ticks) ticks[3] = wall_clock64();
+    const int col = (blockIdx.x * 256 + tid) / G, gl = tid % G;
+    if (col < n) {
+        double y[R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            const int k = gl + G * t;
+            y[t] = k < n ? S[(long long)k * n + col] : 0.0;
+        }
+        for (int i = 2; i < n; ++i) {
+            const double hi = h[i];
+            if (hi == 0.0) continue;
+            const double *u = Ap + tri(i);
+            double uk[R];
+            double s = 0.0;
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                const int k = gl + G * t;
+                uk[t] = k < i ? u[k] : 0.0;
+                s = fma(uk[t], y[t], s);
+            }
+#pragma unroll
+            for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+            const double w = s / hi;
+#pragma unroll
+            for (int t = 0; t < R; ++t) y[t] = fma(-uk[t], w, y[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            const int k = gl + G * t;
+            if (k < n) {
+                Q[(long long)k * n + col] = y[t];
+                QT[(long long)col * n + k] = y[t];
             }
         }
     }
@@ -382,16 +546,39 @@ __global__ __launch_bounds__(256) void dc_assemble_kernel(int n, const double *_
     }
 }
 
-size_t eig_dc_workspace(int n) { return 3 * (size_t)n * n + 3 * (size_t)n; }
+template <int G, int R>
+static void launch_backtransform(int n, const double *Ap, const double *h, const double *S,
+                                 const double *lam, double *lam_sorted, double *Q, double *QT,
+                                 int *info, long long *ticks, hipStream_t s) {
+    const unsigned blocks = (unsigned)(((long long)n * G + 255) / 256);
+    hipLaunchKernelGGL((eig_backtransform_kernel<G, R>), dim3(blocks), dim3(256), 0, s, n, Ap, h,
+                       S, lam, lam_sorted, Q, QT, info, ticks);
+    MSX_LAUNCH_CHECK();
+}
 
-void launch_eigensystem_dc(int n, double *A, double *QH, double *work, double *lam_sorted,
+size_t eig_dc_workspace(int n) {
+    return 3 * (size_t)n * n + 4 * (size_t)n + (size_t)n * (n + 1) / 2;
+}
+
+void launch_eigensystem_dc(int n, const double *A, double *work, double *lam_sorted,
                            double *Q_sorted, double *QT_sorted, int *info, long long *ticks,
                            hipStream_t s) {
     if (!eig_dc_supported(n)) throw Error(3, "divide and conquer: 2 <= n <= 1024 required");
     const size_t n2 = (size_t)n * n;
     double *S = work, *Wq = S + n2, *Wu = Wq + n2, *d = Wu + n2, *e = d + n, *lam = e + n;
+    double *h = lam + n, *Ap = h + n;
     if (ticks) MSX_HIP(hipMemsetAsync(ticks, 0, 8 * sizeof(long long), s));
-    launch_tridiagonalise(n, A, QH, d, e, ticks, s);
+    // Householder stage: the packed prefix of as many rows as fit 150 KiB of LDS beside p
+    {
+        const size_t budget = 150 * 1024 - sizeof(double) * (size_t)n;
+        int rows = 0;
+        while (rows < n && sizeof(double) * (size_t)(rows + 1) * (rows + 2) / 2 <= budget) ++rows;
+        const size_t shm = sizeof(double) * ((size_t)n + (size_t)rows * (rows + 1) / 2);
+        set_dynamic_lds((const void *)eig_tred_kernel, shm);
+        hipLaunchKernelGGL(eig_tred_kernel, dim3(1), dim3(1024), shm, s, n, A, Ap, h, d, e, rows,
+                           ticks);
+        MSX_LAUNCH_CHECK();
+    }
     const unsigned leaf_blocks = (unsigned)std::min<size_t>((n2 + 255) / 256, 4096);
     hipLaunchKernelGGL(dc_leaves_kernel, dim3(leaf_blocks), dim3(256), 0, s, n, d, e, lam, S);
     MSX_LAUNCH_CHECK();
@@ -407,10 +594,13 @@ void launch_eigensystem_dc(int n, double *A, double *QH, double *work, double *l
                            Wq, Wu);
         MSX_LAUNCH_CHECK();
     }
-    const int T = (n + 15) / 16;
-    hipLaunchKernelGGL(dc_assemble_kernel, dim3((T * T + 3) / 4), dim3(256), 0, s, n, QH, S, lam,
-                       lam_sorted, Q_sorted, QT_sorted, info, ticks);
-    MSX_LAUNCH_CHECK();
+    // back-transformation: G lanes per column, R = ceil(n / G) rows per lane
+    if (n <= 64) launch_backtransform<16, 4>(n, Ap, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
+    else if (n <= 128) launch_backtransform<16, 8>(n, Ap, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
+    else if (n <= 192) launch_backtransform<16, 12>(n, Ap, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
+    else if (n <= 256) launch_backtransform<16, 16>(n, Ap, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
+    else if (n <= 512) launch_backtransform<64, 8>(n, Ap, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
+    else launch_backtransform<64, 16>(n, Ap, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
 }
 
 }  // namespace msx

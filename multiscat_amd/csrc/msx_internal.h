@@ -205,17 +205,13 @@ void launch_apply_diag(const Layout &L, const double *Q, const double *QT, const
 void launch_eigensystem(int n, double *A, double *Vec, double *lam_sorted, double *Q_sorted,
                         double *QT_sorted, int *info, long long *ticks, hipStream_t s);
 
-// Householder stage alone: tridiagonal (d, e) with e[i] coupling rows i, i + 1, and Q_H
-// (row-major, A = Q_H T Q_H^T); ticks[0..2] as above.
-void launch_tridiagonalise(int n, double *A, double *QH, double *d, double *e, long long *ticks,
-                           hipStream_t s);
-
-// The same eigensystem with the tridiagonal solved by divide and conquer (2 <= n <= 1024):
-// work holds eig_dc_workspace(n) doubles, QH n x n; ticks[3] marks the start of the final
-// back-transformation, ticks[4] its end (ticks[5..7] are zero).
+// The same eigensystem by Householder tridiagonalisation (packed in LDS) and divide and
+// conquer (2 <= n <= 1024); A is read only (its lower triangle), work holds
+// eig_dc_workspace(n) doubles. ticks: [0] start, [1] = [2] end of the Householder stage,
+// [3] start and [4] end of the back-transformation ([5..7] zero).
 bool eig_dc_supported(int n);
 size_t eig_dc_workspace(int n);
-void launch_eigensystem_dc(int n, double *A, double *QH, double *work, double *lam_sorted,
+void launch_eigensystem_dc(int n, const double *A, double *work, double *lam_sorted,
                            double *Q_sorted, double *QT_sorted, int *info, long long *ticks,
                            hipStream_t s);
 
@@ -246,6 +242,8 @@ void launch_adjoint_source(const Layout &L, const int *spec, const cx<double> *p
 
 // Accumulate a (FP32) correction into an FP64 vector: x += d.
 void launch_add_float(long long n, const cx<float> *d, cx<double> *x, hipStream_t s);
+// dst[i] += src[i] for i < n.
+void launch_add_ints(int n, const int *src, int *dst, hipStream_t s);
 
 // Precision conversions of channel-major state vectors.
 void launch_to_float(long long n, const cx<double> *src, cx<float> *dst, hipStream_t s);

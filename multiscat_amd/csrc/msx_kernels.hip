@@ -620,4 +620,15 @@ void launch_add_float(long long n, const cx<float> *d, cx<double> *x, hipStream_
     MSX_LAUNCH_CHECK();
 }
 
+__global__ void add_ints_kernel(int n, const int *__restrict__ src, int *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] += src[i];
+}
+
+void launch_add_ints(int n, const int *src, int *dst, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(add_ints_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, src, dst);
+    MSX_LAUNCH_CHECK();
+}
+
 }  // namespace msx

@@ -15,8 +15,11 @@ namespace {
 struct Rec {
     std::string name;
     hipEvent_t a, b;
-    double flops, bytes;
+    Work flops, bytes;
+    long long id;
+    bool closed;  // the end event has been recorded
 };
+long long g_next_id = 0;
 
 std::mutex g_mu;
 bool g_on = false;
@@ -43,43 +46,54 @@ hipEvent_t take_event() {
     return e;
 }
 
-// Fold finished records into the sums (caller holds the lock).
+// Fold finished records into the sums (caller holds the lock); records whose work still
+// refers to an active mask that has not arrived stay for a later drain.
 void drain() {
+    std::vector<Rec> keep;
     for (auto &r : g_recs) {
+        if (!r.closed || !r.flops.ready() || !r.bytes.ready()) {
+            keep.push_back(std::move(r));
+            continue;
+        }
         float ms = 0;
         if (r.a && r.b && hipEventSynchronize(r.b) == hipSuccess &&
             hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
             Sum &s = g_sums[r.name];
             s.count += 1;
             s.ms += ms;
-            s.flops += r.flops;
-            s.bytes += r.bytes;
+            s.flops += r.flops.value();
+            s.bytes += r.bytes.value();
         } else {
             (void)hipGetLastError();
         }
         if (r.a) g_pool.push_back(r.a);
         if (r.b) g_pool.push_back(r.b);
     }
-    g_recs.clear();
+    g_recs = std::move(keep);
 }
 
 }  // namespace
 
 bool profiling() { return g_on; }
 
-ProfScope::ProfScope(const char *name, hipStream_t s, double flops, double bytes) : s_(s) {
+ProfScope::ProfScope(const char *name, hipStream_t s, Work flops, Work bytes) : s_(s) {
     if (!g_on) return;
     std::lock_guard<std::mutex> lk(g_mu);
-    Rec r{name, take_event(), take_event(), flops, bytes};
+    Rec r{name, take_event(), take_event(), std::move(flops), std::move(bytes), g_next_id++, false};
     if (r.a) (void)hipEventRecord(r.a, s);
-    g_recs.push_back(r);
-    slot_ = (int)g_recs.size() - 1;
+    id_ = r.id;
+    g_recs.push_back(std::move(r));
 }
 
 ProfScope::~ProfScope() {
-    if (slot_ < 0) return;
+    if (id_ < 0) return;
     std::lock_guard<std::mutex> lk(g_mu);
-    if (slot_ < (int)g_recs.size() && g_recs[slot_].b) (void)hipEventRecord(g_recs[slot_].b, s_);
+    for (auto it = g_recs.rbegin(); it != g_recs.rend(); ++it)
+        if (it->id == id_) {
+            if (it->b) (void)hipEventRecord(it->b, s_);
+            it->closed = true;
+            break;
+        }
     if (g_recs.size() > 4096) drain();
 }
 

@@ -43,21 +43,31 @@ struct Krylov {
     KLayout K{};
     KState S{};
     DBuf<long long> start_d, len_d;
-    DBuf<int> active, enabled, kconv, iters, cycle_iters, converged, count;
+    DBuf<int> enabled, kconv, iters, cycle_iters, converged;
+    // active flags and the count of iterating instances in one buffer, so the host reads
+    // both with a single copy
+    DBuf<int> flags;
+    struct IntView {
+        int *p = nullptr;
+    } active, count;
     DBuf<double> beta, beta0, resid, hnorm, partial;
     DBuf<cx<double>> H, g, cs, sn, y, hcol, hcol2;
     std::vector<DBuf<cx<T>>> blocks;
     std::vector<cx<T> *> vp;
     DBuf<cx<T> *> vp_d;
     DBuf<cx<T>> w;
-    int *count_h = nullptr;
+    int *flags_h = nullptr;  // pinned, two slots of (active flags, count) in flight
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int next_slot = 0;
+    int *mask_h = nullptr;  // pinned, two slots of n_inst flags staged for upload
+    hipEvent_t mask_ev[2] = {nullptr, nullptr};
+    int mask_slot = 0;
     std::vector<long long> len_h;
-    std::vector<int> host_active;
+    std::shared_ptr<ActiveSnap> snap;  // host view of the mask the launches issued now see
 
     Krylov(const std::vector<long long> &start, const std::vector<long long> &len, int kmax_,
            hipStream_t stream)
-        : n_inst((int)start.size()), kmax(kmax_), s(stream), len_h(len),
-          host_active(start.size(), 1) {
+        : n_inst((int)start.size()), kmax(kmax_), s(stream), len_h(len) {
         if (kmax > KR_MAX_BASIS - 1) kmax = KR_MAX_BASIS - 1;
         long long maxlen = 1;
         for (int b = 0; b < n_inst; ++b) {
@@ -74,8 +84,10 @@ struct Krylov {
         len_d.upload(len.data(), n_inst, s);
         K.start = start_d.p;
         K.len = len_d.p;
-        for (auto *b : {&active, &enabled, &kconv, &iters, &cycle_iters, &converged}) b->alloc(n_inst);
-        count.alloc(1);
+        for (auto *b : {&enabled, &kconv, &iters, &cycle_iters, &converged}) b->alloc(n_inst);
+        flags.alloc((size_t)n_inst + 1);
+        active.p = flags.p;
+        count.p = flags.p + n_inst;
         for (auto *b : {&beta, &beta0, &resid, &hnorm}) b->alloc(n_inst);
         const size_t slots = (size_t)n_inst * (kmax + 1);
         partial.alloc(slots * K.nchunk * 2);
@@ -86,20 +98,31 @@ struct Krylov {
         K.active = active.p;
         S = KState{H.p, g.p, cs.p, sn.p, y.p, beta.p, beta0.p, resid.p, hnorm.p,
                    kconv.p, iters.p, cycle_iters.p, converged.p, 0.0, 3};
-        MSX_HIP(hipHostMalloc((void **)&count_h, sizeof(int), hipHostMallocDefault));
+        MSX_HIP(hipHostMalloc((void **)&flags_h, 2 * sizeof(int) * (n_inst + 1), hipHostMallocDefault));
+        MSX_HIP(hipHostMalloc((void **)&mask_h, 2 * sizeof(int) * n_inst, hipHostMallocDefault));
+        for (auto &e : ev) MSX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        for (auto &e : mask_ev) MSX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         set_all_active(nullptr);
     }
     ~Krylov() {
-        if (count_h) (void)hipHostFree(count_h);
+        if (flags_h) (void)hipHostFree(flags_h);
+        if (mask_h) (void)hipHostFree(mask_h);
+        for (auto *arr : {ev, mask_ev})
+            for (int i = 0; i < 2; ++i)
+                if (arr[i]) (void)hipEventDestroy(arr[i]);
     }
-    // active[b] = mask ? mask[b] : 1 (mask is a host array)
+    // active[b] = mask ? mask[b] : 1 (mask is a host array), staged through a pinned slot and
+    // uploaded in stream order without waiting for the device
     void set_all_active(const int *mask) {
-        std::vector<int> h(n_inst, 1);
-        if (mask)
-            for (int b = 0; b < n_inst; ++b) h[b] = mask[b] ? 1 : 0;
-        active.upload(h.data(), n_inst, s);
-        MSX_HIP(hipStreamSynchronize(s));
-        host_active = h;
+        const int sl = mask_slot;
+        mask_slot ^= 1;
+        MSX_HIP(hipEventSynchronize(mask_ev[sl]));  // the upload that last read this slot is done
+        int *h = mask_h + (size_t)sl * n_inst;
+        for (int b = 0; b < n_inst; ++b) h[b] = mask ? (mask[b] ? 1 : 0) : 1;
+        MSX_HIP(hipMemcpyAsync(active.p, h, sizeof(int) * n_inst, hipMemcpyHostToDevice, s));
+        MSX_HIP(hipEventRecord(mask_ev[sl], s));
+        snap = std::make_shared<ActiveSnap>();
+        snap->on.assign(h, h + n_inst);
     }
     void ensure(int nvec) {
         if (nvec > kmax + 1) nvec = kmax + 1;
@@ -111,18 +134,40 @@ struct Krylov {
         }
         vp_d.upload(vp.data(), vp.size(), s);
     }
-    int read_count() {
-        MSX_HIP(hipMemcpyAsync(count_h, count.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        if (profiling()) active.download(host_active.data(), n_inst, s);
-        MSX_HIP(hipStreamSynchronize(s));
-        return *count_h;
+    // The number of iterating instances and their mask, copied to pinned slot `slot` behind
+    // the work queued so far; launches issued after this see a mask not known yet (a new
+    // snapshot, filled by wait()).
+    struct Pending {
+        int slot = -1;
+        std::shared_ptr<ActiveSnap> next;
+    };
+    Pending post() {
+        const int slot = next_slot;
+        next_slot ^= 1;
+        MSX_HIP(hipMemcpyAsync(flags_h + (size_t)slot * (n_inst + 1), flags.p,
+                               sizeof(int) * (n_inst + 1), hipMemcpyDeviceToHost, s));
+        MSX_HIP(hipEventRecord(ev[slot], s));
+        Pending p{slot, std::make_shared<ActiveSnap>()};
+        p.next->ready.store(false, std::memory_order_relaxed);
+        snap = p.next;
+        return p;
     }
-    // elements of the instances still iterating (host mirror, kept while profiling)
-    double active_elements() const {
-        double n = 0;
-        for (int b = 0; b < n_inst; ++b)
-            if (host_active[b]) n += (double)len_h[b];
-        return n;
+    int wait(const Pending &p) {
+        MSX_HIP(hipEventSynchronize(ev[p.slot]));
+        const int *a = flags_h + (size_t)p.slot * (n_inst + 1);
+        p.next->on.assign(a, a + n_inst);
+        p.next->ready.store(true, std::memory_order_release);
+        return a[n_inst];
+    }
+    int read_count() { return wait(post()); }
+    // per-instance work of a launch over the iterating instances (counted against snap)
+    Work per_instance(double per_elem) const {
+        Work w;
+        if (!profiling()) return w;
+        w.per.resize(n_inst);
+        for (int b = 0; b < n_inst; ++b) w.per[b] = per_elem * (double)len_h[b];
+        w.snap = snap;
+        return w;
     }
     std::vector<int> host_ints(const DBuf<int> &b) {
         std::vector<int> h(n_inst);
@@ -181,21 +226,27 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
         if (ref_rhs) k_dots<T>(kr.K, kr.vp_d.p, 0, rhs, 1, kr.partial.p, kr.hcol2.p, 0, s);
         k_cycle_start(kr.K, kr.S, kr.hcol.p, first ? 1 : 0, kr.count.p, s,
                       ref_rhs ? kr.hcol2.p : nullptr);
-        if (kr.read_count() == 0) break;
         k_scale<T>(kr.K, v0, v0, kr.beta.p, kr.active.p, s);
+        // The host learns whether any instance still iterates one iteration late: iteration k
+        // is queued before the count of iteration k - 1 is read, so the GPU never waits for
+        // the host. Every kernel skips inactive instances, so an iteration queued after the
+        // last instance converged does nothing.
         int alive = 1;
+        typename Krylov<T>::Pending prev = kr.post();  // the count after the cycle start
         for (int k = 0; k < kr.kmax && done < maxiter; ++k) {
+            if (prev.slot >= 0 && k + 2 > (int)kr.vp.size()) {
+                // settle before allocating another block of basis vectors
+                alive = kr.wait(prev);
+                prev.slot = -1;
+                if (alive == 0) break;
+            }
             kr.ensure(k + 2);
             op(kr.vp[k], kr.w.p, kr.active.p);
             {
                 // CGS2: two passes of (k+1 dot products, then w -= V h); HBM traffic of the
                 // four vector kernels is 4 (k+1) + 6 vectors of the iterating instances
-                double fl = 0, by = 0;
-                if (profiling()) {
-                    const double n = kr.active_elements();
-                    fl = 32.0 * (k + 1) * n;
-                    by = (4.0 * (k + 1) + 6.0) * n * sizeof(cx<T>);
-                }
+                const Work fl = kr.per_instance(32.0 * (k + 1));
+                const Work by = kr.per_instance((4.0 * (k + 1) + 6.0) * sizeof(cx<T>));
                 ProfScope ps(sizeof(T) == 4 ? "arnoldi_cgs2_f32" : "arnoldi_cgs2_f64", s, fl, by);
                 k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol.p, 0, s);
                 k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol.p, kr.w.p, 0, kr.partial.p, nullptr, s);
@@ -206,9 +257,19 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
             k_givens(kr.K, k, kr.hcol.p, kr.hcol2.p, kr.S, kr.count.p, s);
             k_scale<T>(kr.K, kr.w.p, kr.vp[k + 1], kr.hnorm.p, kr.active.p, s);
             ++done;
-            alive = kr.read_count();
-            if (alive == 0) break;
+            const typename Krylov<T>::Pending cur = kr.post();
+            if (prev.slot >= 0) {
+                alive = kr.wait(prev);
+                if (alive == 0) {  // iteration k had nothing left to do
+                    --done;
+                    alive = kr.wait(cur);
+                    prev.slot = -1;
+                    break;
+                }
+            }
+            prev = cur;
         }
+        if (prev.slot >= 0) alive = kr.wait(prev);
         k_backsolve(kr.K, kr.S, nullptr, s);
         k_update<T, TX>(kr.K, kr.vp_d.p, kr.cycle_iters.p, kr.y.p, x, nullptr, s);
         first = false;
@@ -216,7 +277,6 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
         if (alive == 0 || done >= maxiter || !allow_restart) break;
         k_reactivate(kr.K, kr.S, kr.enabled.p, s);
     }
-    MSX_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace msx
@@ -234,11 +294,11 @@ static void run_eigensolver(int n, double *A, double *Vm, DBuf<double> &dcw, dou
     if (dc) {
         if (!eig_dc_supported(n)) throw Error(MSX_ERR_INVALID, "divide and conquer: 2 <= n <= 1024");
         if (dcw.n < eig_dc_workspace(n)) dcw.alloc(eig_dc_workspace(n));
-        // tridiagonalisation and Q_H (8/3 n^3), merge products (about 8/3 n^3 over the tree
-        // levels), back-transformation Q_H S (2 n^3)
-        ProfScope ps("eigensystem_f64", s, (8.0 / 3.0 + 8.0 / 3.0 + 2.0) * nn * nn * nn,
+        // tridiagonalisation (4/3 n^3), merge products (about 8/3 n^3 over the tree levels),
+        // back-transformation by the reflectors (2 n^3)
+        ProfScope ps("eigensystem_f64", s, (4.0 / 3.0 + 8.0 / 3.0 + 2.0) * nn * nn * nn,
                      5.0 * 8.0 * nn * nn);
-        launch_eigensystem_dc(n, A, Vm, dcw.p, lam, Q, QT, info, ticks, s);
+        launch_eigensystem_dc(n, A, dcw.p, lam, Q, QT, info, ticks, s);
     } else {
         // tridiagonalisation and Q_H (8/3 n^3), QL rotations on Z (about 6 n^3 for about 1.5
         // sweeps per eigenvalue)
@@ -313,7 +373,7 @@ struct msx_batch {
     long long total_ch = 0;
     std::vector<int> nc, spec;
     std::vector<long long> off;
-    DBuf<int> nc_d, chx_d, chy_d, spec_d, chmap_d;
+    DBuf<int> nc_d, chx_d, chy_d, spec_d, chmap_d, iters_acc;
     DBuf<long long> off_d;
     DBuf<double> e_d, fac_d, P_d, dLdP_d;
     DBuf<float> facf_d;
@@ -391,17 +451,20 @@ struct Ops {
     }
     // full off-diagonal coupling y = V x (DFT form on the matrix cores when it applies)
     // ---- algorithmic work of one launch over the instances it processes (profiling) ----
-    const std::vector<int> *host_active = nullptr;  // mirror of the Krylov mask, if any
+    // host view of the Krylov mask the launches see, if any
+    const std::shared_ptr<ActiveSnap> *snap_src = nullptr;
     template <typename F>
-    double over_active(const int *act, F f) const {
-        double sum = 0;
-        for (int b = 0; b < B.n_inst; ++b)
-            if (!act || !host_active || (*host_active)[b]) sum += f(B.nc[b]);
-        return sum;
+    Work over_active(const int *act, F f) const {
+        Work w;
+        if (!profiling()) return w;
+        w.per.resize(B.n_inst);
+        for (int b = 0; b < B.n_inst; ++b) w.per[b] = f(B.nc[b]);
+        if (act && snap_src && *snap_src) w.snap = *snap_src;
+        return w;
     }
     // full coupling: direct sum 8 nc (nc-1) nz flops; DFT form 2 complex 2-D transforms of
     // the grid per node (matrix products, 8 flops per complex MAC) plus the pointwise product
-    void couple_full_work(const int *act, size_t s_cx, bool dft_form, double &fl, double &by) {
+    void couple_full_work(const int *act, size_t s_cx, bool dft_form, Work &fl, Work &by) {
         const double nz = L.nz, nkx = L.nkx, nky = L.nky, nG = nkx * nky;
         if (dft_form)
             fl = over_active(act, [&](int) { return nz * (16.0 * nG * (nkx + nky) + 6.0 * nG); });
@@ -410,21 +473,21 @@ struct Ops {
         by = over_active(act, [&](int nc) { return 2.0 * nc * nz * s_cx; }) + nG * nz * s_cx;
     }
     void couple_full(const cx<double> *in, cx<double> *out, const int *act) {
-        double fl = 0, by = 0;
+        Work fl, by;
         if (profiling()) couple_full_work(act, sizeof(cx<double>), !direct, fl, by);
         ProfScope ps(direct ? "coupling_direct_f64" : "coupling_dft_f64", s, fl, by);
         if (direct) launch_coupling<double>(L, V, in, out, COUPLE_FULL, act, s);
         else launch_dft_coupling<double>(L, dft->d64(), in, out, act, s);
     }
     void couple_full(const cx<float> *in, cx<float> *out, const int *act) {
-        double fl = 0, by = 0;
+        Work fl, by;
         if (profiling()) couple_full_work(act, sizeof(cx<float>), !direct, fl, by);
         ProfScope ps(direct ? "coupling_direct_f32" : "coupling_dft_f32", s, fl, by);
         if (direct) launch_coupling<float>(L, Vf, in, out, COUPLE_FULL, act, s);
         else launch_dft_coupling<float>(L, dft->d32(), in, out, act, s);
     }
     void upper(const cx<double> *in, cx<double> *out, const int *act) {
-        double fl = 0, by = 0;
+        Work fl, by;
         if (profiling()) {
             fl = over_active(act, [&](int nc) { return 4.0 * nc * (nc - 1.0) * L.nz; });
             by = over_active(act, [&](int nc) { return 32.0 * nc * L.nz; }) +
@@ -436,7 +499,7 @@ struct Ops {
     // L^-1: the lower coupling sum (4 nc (nc-1) nz) and two real nz x nz products per channel
     void lower(cx<double> *x, const cx<double> *base, cx<double> *out, double sign,
                const int *act) {
-        double fl = 0, by = 0;
+        Work fl, by;
         if (profiling()) {
             const double nz = L.nz;
             fl = over_active(act, [&](int nc) { return 4.0 * nc * (nc - 1.0) * nz + 8.0 * nc * nz * nz; });
@@ -474,7 +537,7 @@ struct Ops {
     template <typename T>
     void dinv(const cx<T> *in, const cx<T> *base, cx<T> *out, T sign, const int *act) {
         const bool mfma = !vector_diag && diag_mfma_supported(L.nz);
-        double fl = 0, by = 0;
+        Work fl, by;
         if (profiling()) {  // two real nz x nz products on the re and im rows of each channel
             const double nz = L.nz;
             fl = over_active(act, [&](int nc) { return nc * (8.0 * nz * nz + 12.0 * nz); });
@@ -588,9 +651,9 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
                 ops.jac<double>(in, out, act, B.tmp1.p, B.tmp2.p);
             };
         }
-        ops.host_active = &kr.host_active;
+        ops.snap_src = &kr.snap;
         gmres<double, double>(kr, op, B.rhs.p, x, !warm, o.eps, o.window, o.maxiter, true);
-        ops.host_active = nullptr;
+        ops.snap_src = nullptr;
         iters = kr.host_ints(kr.iters);
         auto conv = kr.host_ints(kr.converged);
         resid = kr.host_doubles(kr.resid);
@@ -619,8 +682,11 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     Op<float> op32 = [&](const cx<float> *in, cx<float> *out, const int *act) {
         ops.jac<float>(in, out, act, B.f_tmp1.p, B.f_tmp2.p);
     };
+    // |D^-1 s|^2 per instance: on a cold start it is also the first residual's
     const std::vector<double> rhs_n2 = outer.norms2(d0);
     std::vector<int> act(B.n_inst, 1);
+    msx_batch::need(B.iters_acc, (size_t)B.n_inst);
+    MSX_HIP(hipMemsetAsync(B.iters_acc.p, 0, sizeof(int) * B.n_inst, s));
     const double inner_eps = o.inner_eps > 0 ? o.inner_eps : 1e-5;
     const int max_refine = o.max_refine > 0 ? o.max_refine : 6;
     cx<double> *r64 = B.tmp3.p;
@@ -630,12 +696,12 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
             MSX_HIP(hipMemcpyAsync(r64, d0, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, s));
         } else {
             // r = d0 - (x + D^-1 V x)   (B.src is free: the source was consumed above)
-            ops.host_active = &outer.host_active;
+            ops.snap_src = &outer.snap;
             ops.couple_full(x, B.tmp1.p, outer.active.p);
             ops.dinv<double>(B.tmp1.p, x, B.src.p, 1.0, outer.active.p);
             k_axpby<double>(outer.K, d0, B.src.p, -1.0, r64, outer.active.p, s);
         }
-        const std::vector<double> r_n2 = outer.norms2(r64);
+        const std::vector<double> r_n2 = (sweep == 0 && !warm) ? rhs_n2 : outer.norms2(r64);
         int n_act = 0;
         for (int b = 0; b < B.n_inst; ++b) {
             if (!act[b]) continue;
@@ -646,16 +712,21 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         }
         if (n_act == 0 || sweep >= max_refine) break;
         outer.set_all_active(act.data());
-        ops.host_active = &outer.host_active;
+        ops.snap_src = &outer.snap;
         // inner right-hand side M r = r - D^-1 V r in FP32
         launch_to_float((long long)n, r64, B.f_tmp2.p, s);
         ops.couple_full(B.f_tmp2.p, B.f_tmp1.p, outer.active.p);
         ops.dinv<float>(B.f_tmp1.p, B.f_tmp2.p, B.f_rhs.p, -1.0f, outer.active.p);
         inner.set_all_active(act.data());
-        ops.host_active = &inner.host_active;
+        ops.snap_src = &inner.snap;
         gmres<float, double>(inner, op32, B.f_rhs.p, x, true, inner_eps, 1,
                              std::min(o.maxiter, inner_dim), false);
-        auto it = inner.host_ints(inner.iters);
+        launch_add_ints(B.n_inst, inner.iters.p, B.iters_acc.p, s);
+    }
+    {
+        std::vector<int> it(B.n_inst);
+        B.iters_acc.download(it.data(), B.n_inst, s);
+        MSX_HIP(hipStreamSynchronize(s));
         for (int b = 0; b < B.n_inst; ++b) iters[b] += it[b];
     }
     for (int b = 0; b < B.n_inst; ++b) info[b] = act[b] ? 1 : 0;
