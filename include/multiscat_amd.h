@@ -176,6 +176,8 @@ typedef struct {
     int warm_start;   /* 1: GMRES starts from the batch's current state (msx_batch_set_state, or the
                          last solve) instead of 0; tolerance still relative to the source, as an
                          initial guess does in the reference's SciPy path (_gmres.py:106-114) */
+    int inner_orth;   /* Gram-Schmidt passes of the FP32 inner Arnoldi (mixed precision): 0 = default,
+                         1 = classical single pass, 2 = classical with one reorthogonalisation */
 } msx_solve_options;
 
 /* Fills `opt` with the reference's settings for a given nsig (GS, FP64, window 3, 2000). */

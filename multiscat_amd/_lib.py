@@ -60,6 +60,7 @@ class SolveOptions(C.Structure):
         ("coupling", C.c_int),
         ("diag", C.c_int),
         ("warm_start", C.c_int),
+        ("inner_orth", C.c_int),
     ]
 
 

@@ -33,6 +33,7 @@ class SolveOptions:
     coupling: str = "auto"  # or "direct"
     diag: str = "auto"  # "auto" (fastest), "vector" or "mfma" (matrix cores, nz <= 320)
     warm_start: bool = False  # start from the batch's current state (Batch.set_state)
+    inner_orth: int = 0  # FP32 inner Arnoldi: 0 default, 1 single classical pass, 2 CGS2
 
     @classmethod
     def from_nsig(cls, nsig: int, **kw) -> "SolveOptions":
@@ -62,6 +63,9 @@ class SolveOptions:
             raise ValueError(f"unknown diag {self.diag!r}")
         o.diag = {"auto": 0, "vector": 1, "mfma": 2}[self.diag]
         o.warm_start = 1 if self.warm_start else 0
+        if self.inner_orth not in (0, 1, 2):
+            raise ValueError(f"inner_orth must be 0, 1 or 2, not {self.inner_orth!r}")
+        o.inner_orth = self.inner_orth
         return o
 
 

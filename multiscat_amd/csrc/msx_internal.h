@@ -234,6 +234,15 @@ void launch_flip_table(int nkx, int nky, int nz, const cx<double> *V, cx<double>
 // Source vector: zero everywhere except s[spec_b][N-1] = bsrc_b.
 void launch_set_source(const Layout &L, const int *spec, const cx<double> *bsrc, cx<double> *s,
                        hipStream_t st);
+// d = D^-1 s for that source without touching the other channels' blocks (Sherman-Morrison
+// with the preconditioner factors; d is zero outside the specular channels).
+void launch_specular_dinv(const Layout &L, const int *spec, const cx<double> *bsrc,
+                          const cx<double> *c, const double *fac, cx<double> *d, hipStream_t st);
+
+// Full coupling of a state nonzero only on the specular channels (y_j = V[G_j - G_s] x_s).
+template <typename T>
+void launch_specular_coupling(const Layout &L, const cx<T> *V, const int *spec, const cx<T> *x,
+                              cx<T> *y, const int *active, hipStream_t s);
 
 // Adjoint source g_j(N-1) = 4i dLdP_j b_j conj(S_j), S_j = 2i b_j psi_j(N-1) + a0 delta_spec.
 void launch_adjoint_source(const Layout &L, const int *spec, const cx<double> *psi,

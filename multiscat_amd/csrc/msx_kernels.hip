@@ -13,6 +13,8 @@
 // Work decomposition used throughout: the 64 lanes of a wavefront run over the node index z
 // (contiguous in memory, so every load is coalesced) and everything that is uniform across
 // the wavefront -- channel indices, G differences, table offsets -- is scalar work.
+#include <algorithm>
+
 #include "msx_internal.h"
 
 namespace msx {
@@ -93,6 +95,42 @@ template void launch_coupling<double>(const Layout &, const cx<double> *, const 
                                       cx<double> *, int, const int *, hipStream_t);
 template void launch_coupling<float>(const Layout &, const cx<float> *, const cx<float> *,
                                      cx<float> *, int, const int *, hipStream_t);
+
+// Full coupling of a state that is nonzero only on each instance's specular channel s:
+// y_j(z) = V[G_j - G_s](z) x_s(z) for j != s, y_s = 0. A wavefront per channel, lanes over z.
+template <typename T>
+__global__ __launch_bounds__(256) void specular_coupling_kernel(Layout L, const cx<T> *__restrict__ V,
+                                                                const int *__restrict__ spec,
+                                                                const cx<T> *__restrict__ x,
+                                                                cx<T> *__restrict__ y,
+                                                                const int *__restrict__ active) {
+    const int b = blockIdx.y;
+    if (active && !active[b]) return;
+    const int nc = L.nc[b], nz = L.nz, s = spec[b];
+    const long long off = L.off[b];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int gxs = L.chx[off + s], gys = L.chy[off + s];
+    const cx<T> *xs = x + (off + s) * nz;
+    for (int j = blockIdx.x * 4 + wave; j < nc; j += gridDim.x * 4) {
+        const int dx = wrap_mod(L.chx[off + j] - gxs, L.nkx);
+        const int dy = wrap_mod(L.chy[off + j] - gys, L.nky);
+        const cx<T> *v = V + ((long long)dx * L.nky + dy) * nz;
+        cx<T> *yj = y + (off + j) * nz;
+        for (int z = lane; z < nz; z += 64) yj[z] = j == s ? mk<T>(0, 0) : v[z] * xs[z];
+    }
+}
+
+template <typename T>
+void launch_specular_coupling(const Layout &L, const cx<T> *V, const int *spec, const cx<T> *x,
+                              cx<T> *y, const int *active, hipStream_t s) {
+    const unsigned bx = (unsigned)std::max(1, std::min((L.max_nc + 3) / 4, 64));
+    hipLaunchKernelGGL((specular_coupling_kernel<T>), dim3(bx, L.n_inst), dim3(256), 0, s, L, V,
+                       spec, x, y, active);
+    MSX_LAUNCH_CHECK();
+}
+template void launch_specular_coupling<float>(const Layout &, const cx<float> *, const int *,
+                                              const cx<float> *, cx<float> *, const int *,
+                                              hipStream_t);
 
 // ---------------------------------------------------------------------------------------
 // Block forward substitution x <- L^-1 x, one workgroup (8 waves) per instance, sequential
@@ -575,6 +613,34 @@ __global__ void set_source_kernel(Layout L, const int *__restrict__ spec,
 void launch_set_source(const Layout &L, const int *spec, const cx<double> *bsrc, cx<double> *s,
                        hipStream_t st) {
     hipLaunchKernelGGL(set_source_kernel, dim3(L.n_inst), dim3(256), 0, st, L, spec, bsrc, s);
+    MSX_LAUNCH_CHECK();
+}
+
+// d = D^-1 s for the scattering source s (b at node N of each instance's specular channel, zero
+// elsewhere). D_j = H0 + e_j - c_j e_N e_N^T, so by Sherman-Morrison D_j^-1 e_N =
+// f_j / (1 - c_j f_j(N)) with the preconditioner factor f_j = (H0 + e_j)^-1 e_N: only the
+// specular channel of d is nonzero.
+__global__ void specular_dinv_kernel(Layout L, const int *__restrict__ spec,
+                                     const cx<double> *__restrict__ bsrc,
+                                     const cx<double> *__restrict__ c,
+                                     const double *__restrict__ fac, cx<double> *__restrict__ d) {
+    const int b = blockIdx.y;
+    const long long nz = L.nz, beg = L.off[b] * nz, len = (long long)L.nc[b] * nz;
+    const long long js = L.off[b] + spec[b], sbeg = js * nz;
+    const cx<double> coef = cdiv(bsrc[b], mk(1.0, 0.0) - c[js] * fac[sbeg + nz - 1]);
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < len;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long g = beg + i;
+        d[g] = (g >= sbeg && g < sbeg + nz) ? coef * fac[g] : mk(0.0, 0.0);
+    }
+}
+
+void launch_specular_dinv(const Layout &L, const int *spec, const cx<double> *bsrc,
+                          const cx<double> *c, const double *fac, cx<double> *d, hipStream_t st) {
+    const long long per = (long long)L.max_nc * L.nz;
+    const unsigned bx = (unsigned)std::min<long long>((per + 255) / 256, 64);
+    hipLaunchKernelGGL(specular_dinv_kernel, dim3(std::max(bx, 1u), L.n_inst), dim3(256), 0, st, L,
+                       spec, bsrc, c, fac, d);
     MSX_LAUNCH_CHECK();
 }
 

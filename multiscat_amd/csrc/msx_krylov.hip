@@ -213,7 +213,8 @@ __device__ inline void givens(cx<double> a, cx<double> b, cx<double> &c, cx<doub
     s = b * (1.0 / r);
 }
 
-// One thread per instance: finish column k of the Hessenberg matrix, rotate, update the
+// One thread per instance: finish column k of the Hessenberg matrix (hcol + hcol2, or hcol alone
+// when hcol2 is null), rotate, update the
 // residual recurrence and the convergence window. Writes the number of instances still
 // active to *count.
 __global__ void givens_kernel(KLayout K, int k, const cx<double> *__restrict__ hcol,
@@ -226,7 +227,7 @@ __global__ void givens_kernel(KLayout K, int k, const cx<double> *__restrict__ h
         const long long slot = (long long)b * (K.kmax + 1);
         cx<double> *H = S.H + (long long)b * (K.kmax + 1) * (K.kmax + 1);  // column-major
 #define HH(i, j) H[(long long)(j) * (K.kmax + 1) + (i)]
-        for (int i = 0; i <= k; ++i) HH(i, k) = hcol[slot + i] + hcol2[slot + i];
+        for (int i = 0; i <= k; ++i) HH(i, k) = hcol2 ? hcol[slot + i] + hcol2[slot + i] : hcol[slot + i];
         const double hn = sqrt(hcol[slot + k + 1].re);
         HH(k + 1, k) = mk(hn, 0.0);
         cx<double> *cs = S.cs + slot, *sn = S.sn + slot, *g = S.g + slot;

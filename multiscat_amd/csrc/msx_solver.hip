@@ -199,7 +199,7 @@ struct Krylov {
 // the current x until maxiter iterations or convergence.
 template <typename T, typename TX>
 void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_zero,
-           double eps, int window, int maxiter, bool allow_restart) {
+           double eps, int window, int maxiter, bool allow_restart, int orth_passes = 2) {
     hipStream_t s = kr.s;
     kr.S.eps = eps;
     kr.S.window = window;
@@ -244,17 +244,27 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
             op(kr.vp[k], kr.w.p, kr.active.p);
             {
                 // CGS2: two passes of (k+1 dot products, then w -= V h); HBM traffic of the
-                // four vector kernels is 4 (k+1) + 6 vectors of the iterating instances
-                const Work fl = kr.per_instance(32.0 * (k + 1));
-                const Work by = kr.per_instance((4.0 * (k + 1) + 6.0) * sizeof(cx<T>));
-                ProfScope ps(sizeof(T) == 4 ? "arnoldi_cgs2_f32" : "arnoldi_cgs2_f64", s, fl, by);
+                // four vector kernels is 4 (k+1) + 6 vectors of the iterating instances (one
+                // pass: 2 (k+1) + 3)
+                const double P = orth_passes >= 2 ? 2.0 : 1.0;
+                const Work fl = kr.per_instance(P * 16.0 * (k + 1));
+                const Work by = kr.per_instance(P * (2.0 * (k + 1) + 3.0) * sizeof(cx<T>));
+                const char *nm = orth_passes >= 2
+                                     ? (sizeof(T) == 4 ? "arnoldi_cgs2_f32" : "arnoldi_cgs2_f64")
+                                     : (sizeof(T) == 4 ? "arnoldi_cgs1_f32" : "arnoldi_cgs1_f64");
+                ProfScope ps(nm, s, fl, by);
                 k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol.p, 0, s);
-                k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol.p, kr.w.p, 0, kr.partial.p, nullptr, s);
-                k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol2.p, 0, s);
-                k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol2.p, kr.w.p, 1, kr.partial.p, kr.hcol.p,
-                          s);
+                if (orth_passes >= 2) {
+                    k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol.p, kr.w.p, 0, kr.partial.p, nullptr, s);
+                    k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol2.p, 0, s);
+                    k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol2.p, kr.w.p, 1, kr.partial.p,
+                              kr.hcol.p, s);
+                } else {
+                    k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol.p, kr.w.p, 1, kr.partial.p, kr.hcol.p,
+                              s);
+                }
             }
-            k_givens(kr.K, k, kr.hcol.p, kr.hcol2.p, kr.S, kr.count.p, s);
+            k_givens(kr.K, k, kr.hcol.p, orth_passes >= 2 ? kr.hcol2.p : nullptr, kr.S, kr.count.p, s);
             k_scale<T>(kr.K, kr.w.p, kr.vp[k + 1], kr.hnorm.p, kr.active.p, s);
             ++done;
             const typename Krylov<T>::Pending cur = kr.post();
@@ -565,10 +575,20 @@ struct Ops {
             dinv<T>(u, w1, out, T(-1), act);
         }
     }
+    // D^-1 src in FP64 for every instance; for the scattering source (nonzero only at node N
+    // of the specular channels) by Sherman-Morrison on the specular channels alone
+    void dinv_source(const cx<double> *src, cx<double> *out, bool scatter_source) {
+        if (scatter_source) {
+            ProfScope ps("specular_dinv_f64", s, 0.0, (double)B.n_elem() * sizeof(cx<double>));
+            launch_specular_dinv(L, B.spec_d.p, B.bsrc_d.p, B.c_d.p, B.fac_d.p, out, s);
+        } else {
+            dinv<double>(src, nullptr, out, 1.0, nullptr);
+        }
+    }
     // Block Jacobi right-hand side in FP64: rhs = M D^-1 src
-    void jac_rhs(const cx<double> *src, cx<double> *rhs) {
+    void jac_rhs(const cx<double> *src, cx<double> *rhs, bool scatter_source) {
         cx<double> *r1 = neumann ? B.tmp2.p : rhs;
-        dinv<double>(src, nullptr, r1, 1.0, nullptr);
+        dinv_source(src, r1, scatter_source);
         if (neumann) {
             couple_full(r1, B.tmp1.p, nullptr);
             dinv<double>(B.tmp1.p, r1, rhs, -1.0, nullptr);
@@ -611,12 +631,15 @@ void check_options(const msx_solve_options *o) {
     if (o->maxiter <= 0) throw Error(MSX_ERR_INVALID, "maxiter must be positive");
     if (o->window <= 0) throw Error(MSX_ERR_INVALID, "window must be positive");
     if (o->warm_start != 0 && o->warm_start != 1) throw Error(MSX_ERR_INVALID, "warm_start is 0 or 1");
+    if (o->inner_orth < 0 || o->inner_orth > 2) throw Error(MSX_ERR_INVALID, "inner_orth is 0, 1 or 2");
 }
 
 // Solve A psi = src for all instances into `x` (FP64). transposed: A^T with the flipped table.
+// scatter_source: src_in is the scattering source of launch_set_source (b at node N of the
+// specular channels), which lets D^-1 src skip the other channels.
 void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *src_in,
                   cx<double> *x, bool transposed, std::vector<int> &iters,
-                  std::vector<int> &info, std::vector<double> &resid) {
+                  std::vector<int> &info, std::vector<double> &resid, bool scatter_source) {
     msx_table &t = *B.t;
     hipStream_t s = t.stream;
     const size_t n = (size_t)B.n_elem();
@@ -646,7 +669,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
             ops.gs_rhs(B.src.p, B.rhs.p);
             op = [&](const cx<double> *in, cx<double> *out, const int *act) { ops.gs(in, out, act); };
         } else {
-            ops.jac_rhs(B.src.p, B.rhs.p);
+            ops.jac_rhs(B.src.p, B.rhs.p, scatter_source);
             op = [&](const cx<double> *in, cx<double> *out, const int *act) {
                 ops.jac<double>(in, out, act, B.tmp1.p, B.tmp2.p);
             };
@@ -671,7 +694,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     msx_batch::need(B.f_tmp1, n);
     msx_batch::need(B.f_tmp2, n);
     cx<double> *d0 = B.rhs.p;  // D^-1 s
-    ops.dinv<double>(B.src.p, nullptr, d0, 1.0, nullptr);
+    ops.dinv_source(B.src.p, d0, scatter_source);
     if (!B.kr_outer) B.kr_outer.reset(new Krylov<double>(B.kstart(), B.klen(), 1, s));
     Krylov<double> &outer = *B.kr_outer;
     outer.set_all_active(nullptr);
@@ -713,14 +736,23 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         if (n_act == 0 || sweep >= max_refine) break;
         outer.set_all_active(act.data());
         ops.snap_src = &outer.snap;
-        // inner right-hand side M r = r - D^-1 V r in FP32
+        // inner right-hand side M r = r - D^-1 V r in FP32 (on a cold start the first r = D^-1 s
+        // lives on the specular channels only, and V r is a single-source product)
         launch_to_float((long long)n, r64, B.f_tmp2.p, s);
-        ops.couple_full(B.f_tmp2.p, B.f_tmp1.p, outer.active.p);
+        if (sweep == 0 && !warm && scatter_source && !transposed) {
+            const double fl = 8.0 * (double)B.total_ch * t.nz;
+            const double by = 2.0 * (double)B.total_ch * t.nz * sizeof(cx<float>);
+            ProfScope ps("coupling_specular_f32", s, fl, by);
+            launch_specular_coupling<float>(ops.L, t.Vf.p, B.spec_d.p, B.f_tmp2.p, B.f_tmp1.p,
+                                            outer.active.p, s);
+        } else {
+            ops.couple_full(B.f_tmp2.p, B.f_tmp1.p, outer.active.p);
+        }
         ops.dinv<float>(B.f_tmp1.p, B.f_tmp2.p, B.f_rhs.p, -1.0f, outer.active.p);
         inner.set_all_active(act.data());
         ops.snap_src = &inner.snap;
         gmres<float, double>(inner, op32, B.f_rhs.p, x, true, inner_eps, 1,
-                             std::min(o.maxiter, inner_dim), false);
+                             std::min(o.maxiter, inner_dim), false, o.inner_orth == 1 ? 1 : 2);
         launch_add_ints(B.n_inst, inner.iters.p, B.iters_acc.p, s);
     }
     {
@@ -837,6 +869,7 @@ void msx_default_options(msx_solve_options *opt, int nsig) {
     opt->coupling = 0;
     opt->diag = 0;
     opt->warm_start = 0;
+    opt->inner_orth = 0;
 }
 
 int msx_table_create(int nkx, int nky, int nz, const double *potential, const double *kinetic,
@@ -1050,7 +1083,8 @@ int msx_batch_solve(msx_batch *B, const msx_solve_options *opt) {
         const Layout L = B->layout();
         msx_batch::need(B->source, (size_t)B->n_elem());
         launch_set_source(L, B->spec_d.p, B->bsrc_d.p, B->source.p, t.stream);
-        solve_system(*B, *opt, B->source.p, B->psi_d.p, false, B->iters, B->info, B->resid);
+        solve_system(*B, *opt, B->source.p, B->psi_d.p, false, B->iters, B->info, B->resid,
+                     true);
         launch_intensities(L, B->psi_d.p, B->b_d.p, B->a0_d.p, B->spec_d.p, B->P_d.p, t.stream);
         MSX_HIP(hipStreamSynchronize(t.stream));
         B->solved = true;
@@ -1142,7 +1176,7 @@ int msx_batch_adjoint(msx_batch *B, const msx_solve_options *opt, const double *
         launch_adjoint_source(L, B->spec_d.p, B->psi_d.p, B->b_d.p, B->a0_d.p, B->dLdP_d.p, g.p, s);
         std::vector<int> it, inf;
         std::vector<double> res;
-        solve_system(*B, *opt, g.p, lam.p, true, it, inf, res);
+        solve_system(*B, *opt, g.p, lam.p, true, it, inf, res, false);
         const size_t nG = t.nG();
         DBuf<cx<double>> grad(nG * t.nz);
         MSX_HIP(hipMemsetAsync(grad.p, 0, sizeof(cx<double>) * nG * t.nz, s));

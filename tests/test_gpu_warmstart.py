@@ -36,8 +36,16 @@ def test_exact_and_zero_initial_guess(method, precision):
     np.testing.assert_allclose(bt.intensities_flat(), P0, atol=1e-12)
     bt.set_state([np.zeros_like(p) for p in psi0])
     bt.solve(BT.SolveOptions(warm_start=True, **opts))
-    np.testing.assert_array_equal(bt.stats()["iterations"], it0)
-    np.testing.assert_allclose(bt.intensities_flat(), P0, atol=1e-13)
+    if precision == "fp64":
+        np.testing.assert_array_equal(bt.stats()["iterations"], it0)
+        np.testing.assert_allclose(bt.intensities_flat(), P0, atol=1e-13)
+    else:
+        # a cold mixed solve forms its first FP32 right-hand side with the exact single-source
+        # coupling (the first residual lives on the specular channels), a warm one (which
+        # cannot know its state is zero) through the FP32 convolution: the inner solves differ
+        # by rounding, by at most one iteration, and P agrees to the solve tolerance
+        assert np.abs(bt.stats()["iterations"] - it0).max() <= 1
+        np.testing.assert_allclose(bt.intensities_flat(), P0, atol=1e-10)
 
 
 def test_energy_continuation_saves_iterations():
