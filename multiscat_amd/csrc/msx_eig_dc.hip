@@ -5,7 +5,7 @@
 // the eigenpairs are the same up to eigenvector signs. The QL stage is a serial bulge chase; here
 // the tridiagonal (d, e) from the Householder stage is solved by Cuppen's divide and conquer
 // instead, which is parallel at every step:
-//   * Householder reduction A = Q_H T Q_H^T, Q_H = P_{n-1} ... P_2 (reflectors kept, Q_H never
+//   * Householder reduction A = Q_H T Q_H^T, Q_H = H_0 ... H_{n-3} (reflectors kept, Q_H never
 //     formed);
 //   * tearing: T = diag(T1, T2) + |beta| u u^T at the boundary p of every node of a balanced
 //     binary tree whose leaves are single rows (d_{p-1} and d_p lose |e_{p-1}|), so the leaves
@@ -22,9 +22,9 @@
 //     (d_j - d_i)), which makes them orthogonal to working precision; the product
 //     Q-hat U of the children's eigenvectors with the secular eigenvectors runs on the FP64
 //     matrix cores.
-// Every tree level is one launch with one workgroup per node. The Householder stage runs in
-// EISPACK's tred2 order on the packed lower triangle held in LDS, and the back-transformation
-// applies its reflectors to the tridiagonal eigenvectors one column per lane group.
+// Every tree level is one launch with one workgroup per node. The Householder stage holds its
+// trailing block in LDS once it fits, and the back-transformation applies its reflectors to
+// the tridiagonal eigenvectors one column per lane group.
 #include <algorithm>
 #include <cfloat>
 
@@ -333,163 +333,163 @@ __global__ __launch_bounds__(1024) void dc_merge_kernel(int n, int t, const doub
 }
 
 // ---- Householder stage -------------------------------------------------------------------
-// tred2 order (EISPACK, Golub & Van Loan sec. 8.3.1): row i = n-1 .. 1 of the lower triangle is
-// reduced by the reflector P_i = I - u u^T / h acting on coordinates 0 .. i-1, so the active
-// block is always a prefix of the packed lower triangle (element (j, k), k <= j, at
-// j (j + 1) / 2 + k). The prefix lives in LDS once it fits (from the start for n up to ~190);
-// u is kept in row i, which the later steps no longer touch.
-__device__ __forceinline__ int tri(int j) { return j * (j + 1) / 2; }  // n <= 1024: fits
-
+// Forward order (Golub & Van Loan sec. 8.3.1, the order of LAPACK's dsytrd with the lower
+// triangle and of the reference's routine): row k = 0 .. n-3 is reduced by the reflector
+// H_k = I - u u^T / h acting on coordinates k+1 .. n-1, so the active block is the trailing one.
+// H0 is graded at both ends (the kinetic matrix of the Lobatto basis is largest at the end
+// nodes); reducing from the top keeps the low-lying eigenvectors an order of magnitude closer
+// to LAPACK's than the bottom-up order does. The block is kept in full (both triangles) so
+// that the matrix-vector product and the rank-2 update are plain row sweeps: in place in the
+// input matrix while it is large, then in LDS (odd row stride) once rows k .. n-1 fit. u is
+// kept in row k (columns k+1 .. n-1), which the later steps no longer touch.
 __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
 
-// One reduction step on the packed matrix Am (LDS or global); p is an LDS vector of length n.
-// Three workgroup barriers: the two reductions of the step are computed redundantly by every
-// wavefront (same order, so the same value), u_l = f - g is substituted in registers and
-// written into row i only after the last barrier, and q = p / h - K u is formed where used.
-__device__ __forceinline__ void tred_step(int i, double *Am, double *p, double *h_out,
-                                          double *e_out) {
-    const int tid = threadIdx.x, NT = blockDim.x;
-    const int wave = tid >> 6, lane = tid & 63, NW = NT >> 6;
-    const int l = i - 1;
-    double *x = Am + tri(i);  // row i: x[0 .. l], diagonal x[i]
-    for (int j = tid; j < i; j += NT) p[j] = 0.0;
-    const double f = x[l];
+constexpr int TRED_WAVES = 16;
+
+// One reduction step: row k of the matrix M, whose element (r, c) (r, c >= b0) sits at
+// M[(r - b0) * ld + (c - b0)] (LDS or global). pp: LDS partial sums [TRED_WAVES][64]; q: LDS
+// vector of length >= m. Three workgroup barriers; the two reductions are computed
+// redundantly by the wavefronts that need them (same order, same value), and u_0 is
+// substituted in registers and written into row k after the last barrier.
+__device__ __forceinline__ void tred_step(int n, int k, double *M, int b0, int ld, double *pp,
+                                          double *q, double *h_out, double *e_out) {
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int m = n - k - 1;  // trailing block size, coordinates k+1 .. n-1 as t = 0 .. m-1
+    double *x = M + (long long)(k - b0) * ld + (k + 1 - b0);  // x[t] = A[k][k+1+t]
+    double *B = M + (long long)(k + 1 - b0) * ld + (k + 1 - b0);  // B[r*ld + c] = A22[r][c]
+    const double f = x[0];
     double loc = 0.0;
-    for (int k = lane; k < l; k += 64) loc += x[k] * x[k];
+    for (int t = 1 + lane; t < m; t += 64) loc += x[t] * x[t];
     const double sigma = wave_sum(loc);
-    if (!(sigma > 0.0)) {  // row i is already tridiagonal (always so for i = 1)
+    if (!(sigma > 0.0)) {  // row k is already tridiagonal
         if (tid == 0) {
-            e_out[l] = f;
-            h_out[i] = 0.0;
+            e_out[k] = f;
+            h_out[k] = 0.0;
         }
         __syncthreads();
         return;
     }
     const double g = f > 0.0 ? -sqrt(sigma + f * f) : sqrt(sigma + f * f);
-    const double h = sigma + f * f - f * g;  // ||u||^2 / 2 with u = x - g e_l
-    const double ul = f - g, hinv = 1.0 / h;
-    auto u = [&](int k) { return k == l ? ul : x[k]; };
-    __syncthreads();  // p is zero
-    // p = A u over the i x i block: 32 x 32 tiles of the lower triangle; in each tile lanes
-    // 0..31 sum along their row, lanes 32..63 down their column (the upper triangle's part).
-    // Loads are predicated by value (a clamped address times zero), so the loop has no branch.
-    const bool rowl = lane < 32;
-    const int ln = lane & 31;
-    const int T = (i + 31) >> 5, ntile = T * (T + 1) / 2;
+    const double h = sigma + f * f - f * g;  // ||u||^2 / 2 with u = x - g e_0
+    const double u0 = f - g, hinv = 1.0 / h;
+    auto u = [&](int t) { return t == 0 ? u0 : x[t]; };
+    // p = A22 u: wavefront w takes 64 rows (one per lane) and a K chunk
     {
-        int I = 0, J = 0;  // tile coordinates of `tile`, advanced incrementally
-        for (int tile = wave, seen = 0; tile < ntile; tile += NW) {
-            for (; seen < tile; ++seen)
-                if (++J > I) {
-                    ++I;
-                    J = 0;
-                }
-            const int r0 = 32 * I, c0 = 32 * J;
-            const int mine = rowl ? r0 + ln : c0 + ln;
-            const bool diag = I == J;
-            int addr = rowl ? tri(mine) + c0 : tri(r0) + mine;
-            int step = rowl ? 1 : r0 + 1;
-            const int ub = rowl ? c0 : r0;
+        const int RG = (m + 63) >> 6, KC = TRED_WAVES / RG;
+        const int rg = wave / KC, kc = wave - rg * KC;
+        if (rg < RG) {
+            const int r = 64 * rg + lane;
+            const int cb = (m * kc) / KC, ce = (m * (kc + 1)) / KC;
             double acc = 0.0;
+            if (r < m) {
+                const double *row = B + (long long)r * ld;
 #pragma unroll 8
-            for (int t = 0; t < 32; ++t) {
-                const int R = rowl ? mine : r0 + t;
-                const int C = rowl ? c0 + t : mine;
-                const bool ok = mine < i && R < i && (!diag || (rowl ? C <= R : C < R));
-                const double a = Am[ok ? addr : 0];
-                const double uu = u(min(ub + t, l));
-                acc = fma(ok ? a : 0.0, uu, acc);
-                addr += step;
-                step += rowl ? 0 : 1;
+                for (int c = cb; c < ce; ++c) acc = fma(row[c], u(c), acc);
             }
-            if (mine < i) atomicAdd(&p[mine], acc);
+            pp[wave * 64 + lane] = acc;
         }
     }
     __syncthreads();
-    double kl = 0.0;
-    for (int j = lane; j < i; j += 64) kl += u(j) * p[j];
-    const double K = 0.5 * wave_sum(kl) * hinv * hinv;
-    auto q = [&](int k) { return p[k] * hinv - K * u(k); };
-    // A -= u q^T + q u^T on the lower triangle of the block, by the same tiles: lane owns
-    // column c0 + (lane & 31) and rows r0 + (lane >> 5) + 2 t
+    // wavefront 0: p from the partial sums, K = u^T p / (2 h^2), q = p / h - K u
+    if (wave == 0) {
+        const int RG = (m + 63) >> 6, KC = TRED_WAVES / RG;
+        double kl = 0.0;
+        for (int r = lane; r < m; r += 64) {
+            const int rg = r >> 6;
+            double pr = 0.0;
+            for (int c = 0; c < KC; ++c) pr += pp[(rg * KC + c) * 64 + (r & 63)];
+            q[r] = pr;
+            kl += u(r) * pr;
+        }
+        const double K = 0.5 * wave_sum(kl) * hinv * hinv;
+        for (int r = lane; r < m; r += 64) q[r] = q[r] * hinv - K * u(r);
+    }
+    __syncthreads();
+    // A22 -= u q^T + q u^T, one row per wavefront, lanes along the row; both triangles are
+    // stored, and the update keeps them bitwise equal (products rounded separately, then one
+    // commutative add)
     {
-        int I = 0, J = 0;
-        for (int tile = wave, seen = 0; tile < ntile; tile += NW) {
-            for (; seen < tile; ++seen)
-                if (++J > I) {
-                    ++I;
-                    J = 0;
-                }
-            const int r0 = 32 * I, C = 32 * J + ln;
-            const bool diag = I == J;
-            if (C < i) {
-                const double uc = u(C), qc = q(C);
-#pragma unroll 4
-                for (int t = 0; t < 16; ++t) {
-                    const int R = r0 + (lane >> 5) + 2 * t;
-                    if (R < i && (!diag || C <= R)) Am[tri(R) + C] -= u(R) * qc + q(R) * uc;
+        constexpr int KT = 4;  // columns per lane and pass (256 columns a pass)
+        for (int c0 = 0; c0 < m; c0 += 64 * KT) {
+            double uk[KT], qk[KT];
+#pragma unroll
+            for (int t = 0; t < KT; ++t) {
+                const int c = c0 + lane + 64 * t;
+                uk[t] = c < m ? u(c) : 0.0;
+                qk[t] = c < m ? q[c] : 0.0;
+            }
+            for (int r = wave; r < m; r += TRED_WAVES) {
+                const double ur = u(r), qr = q[r];
+                double *row = B + (long long)r * ld;
+#pragma unroll
+                for (int t = 0; t < KT; ++t) {
+                    const int c = c0 + lane + 64 * t;
+                    if (c < m) row[c] = __dsub_rn(row[c], __dadd_rn(__dmul_rn(ur, qk[t]), __dmul_rn(qr, uk[t])));
                 }
             }
         }
     }
     if (tid == 0) {
-        e_out[l] = g;
-        h_out[i] = h;
+        e_out[k] = g;
+        h_out[k] = h;
     }
     __syncthreads();
-    if (tid == 0) x[l] = ul;  // the reflector is stored in row i, which no later step reads
+    if (tid == 0) x[0] = u0;  // the reflector is stored in row k, which no later step reads
 }
 
-// A (n x n row-major, symmetric; its lower triangle is read) -> d, e (e[i] couples rows i and
-// i + 1), the reflectors u (rows of the packed matrix Ag) and h (h[i] = 0: no reflector).
-// lds_rows: rows of the packed prefix that fit the dynamic LDS after the n-vector p.
-__global__ __launch_bounds__(1024) void eig_tred_kernel(int n, const double *__restrict__ A,
-                                                        double *Ag, double *h_out, double *d_out,
-                                                        double *e_out, int lds_rows,
+// A (n x n row-major, symmetric; overwritten) -> d, e (e[i] couples rows i and i + 1), the
+// reflectors u_k in row k of A (columns k+1 .. n-1) and h (h[k] = 0: no reflector). lds_rows
+// rows of stride lds_ld fit the dynamic LDS after the partial sums and q.
+__global__ __launch_bounds__(1024) void eig_tred_kernel(int n, double *A, double *h_out,
+                                                        double *d_out, double *e_out,
+                                                        int lds_rows, int lds_ld,
                                                         long long *ticks) {
     extern __shared__ double tl[];
-    double *p = tl, *Al = tl + n;
-    const int tid = threadIdx.x, NT = blockDim.x;
+    double *pp = tl, *q = pp + TRED_WAVES * 64, *Al = q + n;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
     if (ticks && tid == 0) ticks[0] = wall_clock64();
-    const bool start_lds = lds_rows >= n;
-    double *dst0 = start_lds ? Al : Ag;
-    for (int j = tid >> 6; j < n; j += NT >> 6)
-        for (int k = tid & 63; k <= j; k += 64) dst0[tri(j) + k] = A[(long long)j * n + k];
-    __syncthreads();
-    int in_lds = start_lds ? n : 0;  // rows held in LDS
-    for (int i = n - 1; i >= 1; --i) {
-        if (!in_lds && i + 1 <= lds_rows) {  // rows 0 .. i fit from here on
-            const int cnt = tri(i + 1);
-            for (int idx = tid; idx < cnt; idx += NT) Al[idx] = Ag[idx];
+    int b0 = -1;  // first row held in LDS (rows and columns b0 .. n-1), -1 while none
+    for (int k = 0; k + 2 < n; ++k) {
+        if (b0 < 0 && n - k <= lds_rows) {  // rows k .. n-1 fit from here on
+            for (int r = k + wave; r < n; r += TRED_WAVES)
+                for (int c = k + lane; c < n; c += 64)
+                    Al[(r - k) * lds_ld + (c - k)] = A[(long long)r * n + c];
             __syncthreads();
-            in_lds = i + 1;
+            b0 = k;
         }
-        if (in_lds) tred_step(i, Al, p, h_out, e_out);
-        else tred_step(i, Ag, p, h_out, e_out);
+        if (b0 >= 0) tred_step(n, k, Al, b0, lds_ld, pp, q, h_out, e_out);
+        else tred_step(n, k, A, 0, n, pp, q, h_out, e_out);
     }
     __syncthreads();
-    if (in_lds) {
-        const int cnt = tri(in_lds);
-        for (int idx = tid; idx < cnt; idx += NT) Ag[idx] = Al[idx];
-        __syncthreads();
-    }
-    for (int k = tid; k < n; k += NT) d_out[k] = Ag[tri(k) + k];
+    // the LDS rows back into A (their reflectors and the final 2 x 2 block), then d and e
+    if (b0 >= 0)
+        for (int r = b0 + wave; r < n; r += TRED_WAVES)
+            for (int c = r + lane; c < n; c += 64)
+                A[(long long)r * n + c] = Al[(r - b0) * lds_ld + (c - b0)];
+    __syncthreads();
+    for (int r = tid; r < n; r += blockDim.x) d_out[r] = A[(long long)r * n + r];
     if (tid == 0) {
+        if (n >= 2) {
+            e_out[n - 2] = A[(long long)(n - 2) * n + n - 1];
+            h_out[n - 2] = 0.0;
+        }
         e_out[n - 1] = 0.0;
-        h_out[0] = 0.0;
+        h_out[n - 1] = 0.0;
         if (ticks) ticks[1] = ticks[2] = wall_clock64();
     }
 }
 
-// Q = P_{n-1} ... P_2 S: G lanes own one column of S (R rows each, in registers) and apply the
-// reflectors in order, with no workgroup barrier. Also copies the eigenvalues, sets info = 1
-// if any is not finite, and Q^T.
+// Q = H_0 H_1 ... H_{n-3} S: G lanes own one column of S (R rows each, in registers) and apply
+// the reflectors from the last to the first, with no workgroup barrier. Also copies the
+// eigenvalues, sets info = 1 if any is not finite, and Q^T.
 template <int G, int R>
 __global__ __launch_bounds__(256) void eig_backtransform_kernel(
-    int n, const double *__restrict__ Ap, const double *__restrict__ h,
+    int n, const double *__restrict__ U, const double *__restrict__ h,
     const double *__restrict__ S, const double *__restrict__ lam, double *__restrict__ lam_sorted,
     double *__restrict__ Q, double *__restrict__ QT, int *info, long long *ticks) {
     const int tid = threadIdx.x;
@@ -499,33 +499,33 @@ __global__ __launch_bounds__(256) void eig_backtransform_kernel(
         double y[R];
 #pragma unroll
         for (int t = 0; t < R; ++t) {
-            const int k = gl + G * t;
-            y[t] = k < n ? S[(long long)k * n + col] : 0.0;
+            const int r = gl + G * t;
+            y[t] = r < n ? S[(long long)r * n + col] : 0.0;
         }
-        for (int i = 2; i < n; ++i) {
-            const double hi = h[i];
-            if (hi == 0.0) continue;
-            const double *u = Ap + tri(i);
-            double uk[R];
+        for (int k = n - 3; k >= 0; --k) {
+            const double hk = h[k];
+            if (hk == 0.0) continue;
+            const double *u = U + (long long)k * n;  // u[r] for r = k+1 .. n-1
+            double ur[R];
             double s = 0.0;
 #pragma unroll
             for (int t = 0; t < R; ++t) {
-                const int k = gl + G * t;
-                uk[t] = k < i ? u[k] : 0.0;
-                s = fma(uk[t], y[t], s);
+                const int r = gl + G * t;
+                ur[t] = (r > k && r < n) ? u[r] : 0.0;
+                s = fma(ur[t], y[t], s);
             }
 #pragma unroll
             for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
-            const double w = s / hi;
+            const double w = s / hk;
 #pragma unroll
-            for (int t = 0; t < R; ++t) y[t] = fma(-uk[t], w, y[t]);
+            for (int t = 0; t < R; ++t) y[t] = fma(-ur[t], w, y[t]);
         }
 #pragma unroll
         for (int t = 0; t < R; ++t) {
-            const int k = gl + G * t;
-            if (k < n) {
-                Q[(long long)k * n + col] = y[t];
-                QT[(long long)col * n + k] = y[t];
+            const int r = gl + G * t;
+            if (r < n) {
+                Q[(long long)r * n + col] = y[t];
+                QT[(long long)col * n + r] = y[t];
             }
         }
     }
@@ -547,36 +547,41 @@ __global__ __launch_bounds__(256) void eig_backtransform_kernel(
 }
 
 template <int G, int R>
-static void launch_backtransform(int n, const double *Ap, const double *h, const double *S,
+static void launch_backtransform(int n, const double *U, const double *h, const double *S,
                                  const double *lam, double *lam_sorted, double *Q, double *QT,
                                  int *info, long long *ticks, hipStream_t s) {
     const unsigned blocks = (unsigned)(((long long)n * G + 255) / 256);
-    hipLaunchKernelGGL((eig_backtransform_kernel<G, R>), dim3(blocks), dim3(256), 0, s, n, Ap, h,
+    hipLaunchKernelGGL((eig_backtransform_kernel<G, R>), dim3(blocks), dim3(256), 0, s, n, U, h,
                        S, lam, lam_sorted, Q, QT, info, ticks);
     MSX_LAUNCH_CHECK();
 }
 
-size_t eig_dc_workspace(int n) {
-    return 3 * (size_t)n * n + 4 * (size_t)n + (size_t)n * (n + 1) / 2;
-}
+size_t eig_dc_workspace(int n) { return 3 * (size_t)n * n + 4 * (size_t)n; }
 
-void launch_eigensystem_dc(int n, const double *A, double *work, double *lam_sorted,
+void launch_eigensystem_dc(int n, double *A, double *work, double *lam_sorted,
                            double *Q_sorted, double *QT_sorted, int *info, long long *ticks,
                            hipStream_t s) {
     if (!eig_dc_supported(n)) throw Error(3, "divide and conquer: 2 <= n <= 1024 required");
     const size_t n2 = (size_t)n * n;
     double *S = work, *Wq = S + n2, *Wu = Wq + n2, *d = Wu + n2, *e = d + n, *lam = e + n;
-    double *h = lam + n, *Ap = h + n;
+    double *h = lam + n;
     if (ticks) MSX_HIP(hipMemsetAsync(ticks, 0, 8 * sizeof(long long), s));
-    // Householder stage: the packed prefix of as many rows as fit 150 KiB of LDS beside p
+    // Householder stage: rows b0 .. n-1 of stride lds_ld (odd) fit 150 KiB of LDS beside the
+    // partial sums and q; the steps before that work in place in A
     {
-        const size_t budget = 150 * 1024 - sizeof(double) * (size_t)n;
-        int rows = 0;
-        while (rows < n && sizeof(double) * (size_t)(rows + 1) * (rows + 2) / 2 <= budget) ++rows;
-        const size_t shm = sizeof(double) * ((size_t)n + (size_t)rows * (rows + 1) / 2);
+        const size_t fixed = sizeof(double) * ((size_t)TRED_WAVES * 64 + n);
+        const size_t budget = 150 * 1024 - fixed;
+        int rows = std::min(n, 1);
+        while (rows < n) {
+            const int r = rows + 1, ld = r | 1;
+            if (sizeof(double) * (size_t)r * ld > budget) break;
+            rows = r;
+        }
+        const int ld = rows | 1;
+        const size_t shm = fixed + sizeof(double) * (size_t)rows * ld;
         set_dynamic_lds((const void *)eig_tred_kernel, shm);
-        hipLaunchKernelGGL(eig_tred_kernel, dim3(1), dim3(1024), shm, s, n, A, Ap, h, d, e, rows,
-                           ticks);
+        hipLaunchKernelGGL(eig_tred_kernel, dim3(1), dim3(64 * TRED_WAVES), shm, s, n, A, h, d, e,
+                           rows, ld, ticks);
         MSX_LAUNCH_CHECK();
     }
     const unsigned leaf_blocks = (unsigned)std::min<size_t>((n2 + 255) / 256, 4096);
@@ -594,13 +599,13 @@ void launch_eigensystem_dc(int n, const double *A, double *work, double *lam_sor
                            Wq, Wu);
         MSX_LAUNCH_CHECK();
     }
-    // back-transformation: G lanes per column, R = ceil(n / G) rows per lane
-    if (n <= 64) launch_backtransform<16, 4>(n, Ap, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
-    else if (n <= 128) launch_backtransform<16, 8>(n, Ap, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
-    else if (n <= 192) launch_backtransform<16, 12>(n, Ap, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
-    else if (n <= 256) launch_backtransform<16, 16>(n, Ap, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
-    else if (n <= 512) launch_backtransform<64, 8>(n, Ap, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
-    else launch_backtransform<64, 16>(n, Ap, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
+    // back-transformation with the reflectors left in A: G lanes per column, R = ceil(n / G)
+    if (n <= 64) launch_backtransform<16, 4>(n, A, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
+    else if (n <= 128) launch_backtransform<16, 8>(n, A, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
+    else if (n <= 192) launch_backtransform<16, 12>(n, A, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
+    else if (n <= 256) launch_backtransform<16, 16>(n, A, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
+    else if (n <= 512) launch_backtransform<64, 8>(n, A, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
+    else launch_backtransform<64, 16>(n, A, h, S, lam, lam_sorted, Q_sorted, QT_sorted, info, ticks, s);
 }
 
 }  // namespace msx

@@ -205,13 +205,13 @@ void launch_apply_diag(const Layout &L, const double *Q, const double *QT, const
 void launch_eigensystem(int n, double *A, double *Vec, double *lam_sorted, double *Q_sorted,
                         double *QT_sorted, int *info, long long *ticks, hipStream_t s);
 
-// The same eigensystem by Householder tridiagonalisation (packed in LDS) and divide and
-// conquer (2 <= n <= 1024); A is read only (its lower triangle), work holds
+// The same eigensystem by Householder tridiagonalisation (in LDS once the active block fits)
+// and divide and conquer (2 <= n <= 1024); A is overwritten, work holds
 // eig_dc_workspace(n) doubles. ticks: [0] start, [1] = [2] end of the Householder stage,
 // [3] start and [4] end of the back-transformation ([5..7] zero).
 bool eig_dc_supported(int n);
 size_t eig_dc_workspace(int n);
-void launch_eigensystem_dc(int n, const double *A, double *work, double *lam_sorted,
+void launch_eigensystem_dc(int n, double *A, double *work, double *lam_sorted,
                            double *Q_sorted, double *QT_sorted, int *info, long long *ticks,
                            hipStream_t s);
 
