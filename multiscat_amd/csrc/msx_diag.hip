@@ -50,7 +50,7 @@ __device__ __forceinline__ void load16(const double *p, double (&a)[CC]) {
 // fragment): the loads of stage s + 1 are issued into the other register buffer before the
 // MFMAs of stage s, and scheduling barriers keep the compiler from sinking them.
 template <typename T, int NR, int NT>
-__global__ __launch_bounds__(64 * DG_WAVES, NT <= 10 ? 2 : 1) void diag_gemm_kernel(
+__global__ __launch_bounds__(64 * DG_WAVES, (NT <= 10 && !(NR == 2 && sizeof(T) == 8)) ? 2 : 1) void diag_gemm_kernel(
     Layout L, EigTables<T> E, const double *__restrict__ e, const cx<T> *__restrict__ c,
     const T *__restrict__ fac, const cx<T> *x, const cx<T> *base, cx<T> *y, T sign, int inverse,
     const int *__restrict__ active) {
@@ -560,7 +560,8 @@ void launch_diag_blocks(const Layout &L, const EigTables<T> &E, const double *e,
         if (nt <= 7) return launch_diag_lds<7>(L, E, e, c, fac, x, base, y, sign, inverse, active, s);
         if (nt <= 10) return launch_diag_lds<10>(L, E, e, c, fac, x, base, y, sign, inverse, active, s);
     } else {
-        MSX_DIAG(1, 4) MSX_DIAG(1, 7) MSX_DIAG(1, 8) MSX_DIAG(1, 10)
+        // 16 channels per wavefront up to nzp = 160: each Q fragment feeds two products
+        MSX_DIAG(2, 4) MSX_DIAG(2, 7) MSX_DIAG(2, 8) MSX_DIAG(2, 10)
     }
     MSX_DIAG(1, 13) MSX_DIAG(1, 16) MSX_DIAG(1, 19) MSX_DIAG(1, 20)
 #undef MSX_DIAG
