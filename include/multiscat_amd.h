@@ -163,7 +163,8 @@ enum { MSX_PRECISION_FP64 = 0, MSX_PRECISION_MIXED = 1 };
 typedef struct {
     int method;       /* MSX_METHOD_*                                                       */
     int precision;    /* MSX_PRECISION_*                                                    */
-    int neumann;      /* 1: left preconditioner (I - M^-1 N), the reference's flag          */
+    int neumann;      /* 1: left preconditioner (I - M^-1 N), the reference's flag; 2 .. 4 (block
+                         Jacobi only): the Neumann series to that degree, sum_i (-D^-1 V)^i  */
     double eps;       /* stop when |g_{k+1}| < eps |g_1| for `window` consecutive iterations */
     int maxiter;      /* iteration budget per condition (reference: 2000)                   */
     int restart;      /* Krylov dimension before a restart; 0 = maxiter (unrestarted)       */

@@ -23,7 +23,7 @@ class SolveOptions:
 
     method: str = "gauss_seidel"  # or "jacobi"
     precision: str = "fp64"  # or "mixed"
-    neumann: bool = True
+    neumann: int = 1  # 0/False none, 1/True the reference's factor, 2..4 higher-degree (Jacobi)
     eps: float = 0.5e-5
     maxiter: int = 2000
     restart: int = 0
@@ -49,7 +49,7 @@ class SolveOptions:
             raise ValueError(f"unknown precision {self.precision!r}")
         o.method = methods[self.method]
         o.precision = precisions[self.precision]
-        o.neumann = 1 if self.neumann else 0
+        o.neumann = int(self.neumann)
         o.eps = float(self.eps)
         o.maxiter = int(self.maxiter)
         o.restart = int(self.restart)

@@ -438,6 +438,7 @@ struct Ops {
     msx_table &t;
     Layout L;
     bool neumann;
+    int order = 1;  // Jacobi: degree of the Neumann factor M = sum_{i<=order} (-D^-1 V)^i
     const cx<double> *V;  // coupling table (flipped for the transposed system)
     const cx<float> *Vf;
     const DftStore *dft;  // its convolution form
@@ -567,12 +568,43 @@ struct Ops {
     }
     template <typename T>
     void jac(const cx<T> *in, cx<T> *out, const int *act, cx<T> *u, cx<T> *w1buf) {
-        cx<T> *w1 = neumann ? w1buf : out;
-        couple_full(in, u, act);
-        dinv<T>(u, in, w1, T(1), act);
-        if (neumann) {
-            couple_full(w1, u, act);
-            dinv<T>(u, w1, out, T(-1), act);
+        if (!neumann || order <= 1) {
+            cx<T> *w1 = neumann ? w1buf : out;
+            couple_full(in, u, act);
+            dinv<T>(u, in, w1, T(1), act);
+            if (neumann) {
+                couple_full(w1, u, act);
+                dinv<T>(u, w1, out, T(-1), act);
+            }
+            return;
+        }
+        // M (I + K) = I - (-K)^(order+1), K = D^-1 V: order + 1 products of K, then one sum
+        const cx<T> *t = in;
+        for (int i = 0; i < order; ++i) {
+            couple_full(t, u, act);
+            dinv<T>(u, nullptr, w1buf, T(1), act);
+            t = w1buf;
+        }
+        couple_full(t, u, act);
+        dinv<T>(u, in, out, (order & 1) ? T(-1) : T(1), act);
+    }
+    // s = M r = sum_{i<=order} (-K)^i r by Horner (s <- r - K s); `first` is the first
+    // coupling product V r (null: computed here). s must not alias r.
+    template <typename T>
+    void neumann_rhs(const cx<T> *r, cx<T> *s_out, const int *act, cx<T> *u, const cx<T> *vr) {
+        if (!neumann) {  // no factor: the right-hand side is r itself
+            MSX_HIP(hipMemcpyAsync(s_out, r, sizeof(cx<T>) * B.n_elem(), hipMemcpyDeviceToDevice, s));
+            return;
+        }
+        const cx<T> *cur = r;
+        for (int i = 0; i < order; ++i) {
+            if (i == 0 && vr) {
+                dinv<T>(vr, r, s_out, T(-1), act);
+            } else {
+                couple_full(cur, u, act);
+                dinv<T>(u, r, s_out, T(-1), act);
+            }
+            cur = s_out;
         }
     }
     // D^-1 src in FP64 for every instance; for the scattering source (nonzero only at node N
@@ -589,10 +621,7 @@ struct Ops {
     void jac_rhs(const cx<double> *src, cx<double> *rhs, bool scatter_source) {
         cx<double> *r1 = neumann ? B.tmp2.p : rhs;
         dinv_source(src, r1, scatter_source);
-        if (neumann) {
-            couple_full(r1, B.tmp1.p, nullptr);
-            dinv<double>(B.tmp1.p, r1, rhs, -1.0, nullptr);
-        }
+        if (neumann) neumann_rhs<double>(r1, rhs, nullptr, B.tmp1.p, nullptr);
     }
     template <typename T> const T *q() const;
     template <typename T> const T *qt() const;
@@ -631,6 +660,9 @@ void check_options(const msx_solve_options *o) {
     if (o->maxiter <= 0) throw Error(MSX_ERR_INVALID, "maxiter must be positive");
     if (o->window <= 0) throw Error(MSX_ERR_INVALID, "window must be positive");
     if (o->warm_start != 0 && o->warm_start != 1) throw Error(MSX_ERR_INVALID, "warm_start is 0 or 1");
+    if (o->neumann < 0 || o->neumann > 4) throw Error(MSX_ERR_INVALID, "neumann is 0 .. 4");
+    if (o->neumann > 1 && o->method != MSX_METHOD_JACOBI)
+        throw Error(MSX_ERR_INVALID, "a Neumann factor of degree > 1 needs the block-Jacobi method");
     if (o->inner_orth < 0 || o->inner_orth > 2) throw Error(MSX_ERR_INVALID, "inner_orth is 0, 1 or 2");
 }
 
@@ -648,7 +680,8 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     msx_batch::need(B.tmp1, n);
     msx_batch::need(B.tmp2, n);
     if (!B.factors_ready) B.build_factors();
-    Ops ops(B, o.neumann == 1, transposed, o.coupling == 1);
+    Ops ops(B, o.neumann >= 1, transposed, o.coupling == 1);
+    ops.order = std::max(1, o.neumann);
     ops.vector_diag = o.diag == 1;  // auto: matrix cores when nz <= 320
     MSX_HIP(hipMemcpyAsync(B.src.p, src_in, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, s));
     // A warm start (forward solves only) keeps x as the initial guess.
@@ -736,10 +769,12 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         if (n_act == 0 || sweep >= max_refine) break;
         outer.set_all_active(act.data());
         ops.snap_src = &outer.snap;
-        // inner right-hand side M r = r - D^-1 V r in FP32 (on a cold start the first r = D^-1 s
-        // lives on the specular channels only, and V r is a single-source product)
+        // inner right-hand side M r in FP32 (M = I - D^-1 V to first degree; on a cold start the
+        // first r = D^-1 s lives on the specular channels only, and V r is a single-source product)
         launch_to_float((long long)n, r64, B.f_tmp2.p, s);
-        if (sweep == 0 && !warm && scatter_source && !transposed) {
+        if (!ops.neumann) {
+            // no Neumann factor: nothing to couple
+        } else if (sweep == 0 && !warm && scatter_source && !transposed) {
             const double fl = 8.0 * (double)B.total_ch * t.nz;
             const double by = 2.0 * (double)B.total_ch * t.nz * sizeof(cx<float>);
             ProfScope ps("coupling_specular_f32", s, fl, by);
@@ -748,7 +783,8 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         } else {
             ops.couple_full(B.f_tmp2.p, B.f_tmp1.p, outer.active.p);
         }
-        ops.dinv<float>(B.f_tmp1.p, B.f_tmp2.p, B.f_rhs.p, -1.0f, outer.active.p);
+        ops.neumann_rhs<float>(B.f_tmp2.p, B.f_rhs.p, outer.active.p, B.f_tmp1.p,
+                               ops.neumann ? B.f_tmp1.p : nullptr);
         inner.set_all_active(act.data());
         ops.snap_src = &inner.snap;
         gmres<float, double>(inner, op32, B.f_rhs.p, x, true, inner_eps, 1,

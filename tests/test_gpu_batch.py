@@ -72,6 +72,21 @@ def test_mixed_batch_within_bar(small_batch, coupling, diag):
         assert abs(P[open_].sum() - 1.0) < 1e-6
 
 
+@pytest.mark.parametrize("degree", [0, 2, 3, 4])
+def test_neumann_degrees(small_batch, degree):
+    """Block Jacobi with no Neumann factor and with the series to degree 2..4: the same
+    intensities as the oracle (FP64) and within the bar (mixed); Gauss-Seidel takes only the
+    reference's 0/1 flag."""
+    _, _, bt, ref = small_batch
+    bt.solve(BT.SolveOptions(method="jacobi", eps=5e-13, neumann=degree))
+    assert _maxdiff(bt.intensities(), ref) < 1e-10
+    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11, neumann=degree))
+    assert _maxdiff(bt.intensities(), ref) < 1e-6
+    if degree > 1:
+        with pytest.raises(RuntimeError, match="block-Jacobi"):
+            bt.solve(BT.SolveOptions(method="gauss_seidel", eps=5e-13, neumann=degree))
+
+
 def test_jacobi_large_nz_uses_vector_blocks():
     """nz > 320 takes the vector-ALU diagonal-block kernel instead of the MFMA one."""
     w = W.hex_batch(n_conditions=2, nz=336, nk=7, cond_seed=8, channel_counts=[9, 14])
