@@ -57,18 +57,44 @@ __global__ void dc_leaves_kernel(int n, const double *__restrict__ d, const doub
     }
 }
 
+// 1 / x from the hardware reciprocal and two Newton steps (about 1 ulp, a third of the
+// instructions of a correctly rounded division); used in the O(K^2) loops of the merge.
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+
+// Lanes per secular root (and per weight and norm): each takes every DC_G-th term and the
+// partial results meet by butterfly shuffles, which give every lane of the group the same
+// bits (each step adds the same two operands), so the group steers one iteration.
+constexpr int DC_G = 4;
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int o = 1; o < DC_G; o <<= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ double group_prod(double v) {
+#pragma unroll
+    for (int o = 1; o < DC_G; o <<= 1) v *= __shfl_xor(v, o);
+    return v;
+}
+
 // Root k of g(lam) = 1 + rho sum_i z_i^2 / (dl_i - lam) (K poles dl ascending, rho > 0): the
-// origin pole o and tau with lam = dl[o] + tau.
+// origin pole o and tau with lam = dl[o] + tau. Called by the DC_G lanes of a group (gl their
+// index in it).
 __device__ void dc_secular_root(int k, int K, const double *dl, const double *z, double rho,
-                                int &o_out, double &tau_out) {
+                                int gl, int &o_out, double &tau_out) {
     double lo, hi;
     int o;
     if (k < K - 1) {
         // g increases from -inf to +inf on (dl_k, dl_k+1): its sign at the midpoint tells which
         // pole is nearer
         const double mid = 0.5 * (dl[k + 1] - dl[k]);
-        double f = 1.0;
-        for (int i = 0; i < K; ++i) f += rho * z[i] * z[i] / ((dl[i] - dl[k]) - mid);
+        double f = 0.0;
+        for (int i = gl; i < K; i += DC_G) f += rho * z[i] * z[i] * rcp_nr((dl[i] - dl[k]) - mid);
+        f = 1.0 + group_sum(f);
         if (f >= 0.0) {
             o = k;
             lo = 0.0;
@@ -81,10 +107,10 @@ __device__ void dc_secular_root(int k, int K, const double *dl, const double *z,
     } else {
         // the last root lies in (dl_K, dl_K + rho ||z||^2]
         double zz = 0.0;
-        for (int i = 0; i < K; ++i) zz += z[i] * z[i];
+        for (int i = gl; i < K; i += DC_G) zz += z[i] * z[i];
         o = k;
         lo = 0.0;
-        hi = rho * zz;
+        hi = rho * group_sum(zz);
     }
     double tau = 0.5 * (lo + hi);
     const double dlo = dl[o];
@@ -92,18 +118,21 @@ __device__ void dc_secular_root(int k, int K, const double *dl, const double *z,
     for (int it = 0; it < 80; ++it) {
         // psi: poles left of the root (terms <= 0), phi: poles right of it (terms >= 0)
         double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
-        for (int i = 0; i <= k; ++i) {
-            const double r = 1.0 / ((dl[i] - dlo) - tau);
+        for (int i = gl; i < K; i += DC_G) {
+            const double r = rcp_nr((dl[i] - dlo) - tau);
             const double term = rho * z[i] * (z[i] * r);
-            psi += term;
-            dpsi += term * r;
+            if (i <= k) {
+                psi += term;
+                dpsi += term * r;
+            } else {
+                phi += term;
+                dphi += term * r;
+            }
         }
-        for (int i = k + 1; i < K; ++i) {
-            const double r = 1.0 / ((dl[i] - dlo) - tau);
-            const double term = rho * z[i] * (z[i] * r);
-            phi += term;
-            dphi += term * r;
-        }
+        psi = group_sum(psi);
+        dpsi = group_sum(dpsi);
+        phi = group_sum(phi);
+        dphi = group_sum(dphi);
         const double g = 1.0 + psi + phi;
         const double err = 8.0 * (phi - psi + 1.0) + K * (phi - psi);
         if (fabs(g) <= DBL_EPSILON * err) break;
@@ -255,28 +284,33 @@ __global__ __launch_bounds__(1024) void dc_merge_kernel(int n, int t, const doub
         zz[i] = z[nd[i]];
     }
     __syncthreads();
-    // (5) secular roots; eigenvalues of the node, non-deflated first, then deflated
-    for (int q = tid; q < K; q += NT) {
+    // (5) secular roots, DC_G lanes per root; eigenvalues of the node, non-deflated first,
+    // then deflated
+    const int gq = tid / DC_G, gl = tid % DC_G, NG = NT / DC_G;
+    for (int q = gq; q < K; q += NG) {
         int o;
         double tq;
-        dc_secular_root(q, K, dl, zz, rho, o, tq);
-        org[q] = o;
-        tau[q] = tq;
-        val[q] = dl[o] + tq;
+        dc_secular_root(q, K, dl, zz, rho, gl, o, tq);
+        if (gl == 0) {
+            org[q] = o;
+            tau[q] = tq;
+            val[q] = dl[o] + tq;
+        }
     }
     for (int q = tid; q < m - K; q += NT) val[K + q] = D[dfl[q]];
     __syncthreads();
-    // (6) recomputed weights, ranks of the eigenvalues
-    for (int i = tid; i < K; i += NT) {
+    // (6) recomputed weights (DC_G lanes each), ranks of the eigenvalues
+    for (int i = gq; i < K; i += NG) {
         const double di = dl[i];
-        double prod = -((di - dl[org[K - 1]]) - tau[K - 1]) / rho;
-        for (int q = 0; q < K - 1; ++q) {
+        double prod = 1.0;
+        for (int q = gl; q < K - 1; q += DC_G) {
             // (lam_q - d_i) / (d_q - d_i) for q < i, / (d_q+1 - d_i) for q >= i: both in (0, 1)
             const double num = -((di - dl[org[q]]) - tau[q]);
             const double den = (q < i ? dl[q] : dl[q + 1]) - di;
-            prod *= num / den;
+            prod *= num * rcp_nr(den);
         }
-        zh[i] = copysign(sqrt(fmax(prod, 0.0)), zz[i]);
+        prod = group_prod(prod) * (-((di - dl[org[K - 1]]) - tau[K - 1]) / rho);
+        if (gl == 0) zh[i] = copysign(sqrt(fmax(prod, 0.0)), zz[i]);
     }
     for (int s = tid; s < m; s += NT) {
         const double v = val[s];
@@ -287,20 +321,21 @@ __global__ __launch_bounds__(1024) void dc_merge_kernel(int n, int t, const doub
     __syncthreads();
     // (7) ascending eigenvalues; norms of the secular eigenvectors u_q(i) = zh_i / (d_i - lam_q)
     for (int s = tid; s < m; s += NT) lam[a + rank[s]] = val[s];
-    for (int q = tid; q < K; q += NT) {
+    for (int q = gq; q < K; q += NG) {
         const double base = dl[org[q]], tq = tau[q];
         double ss = 0.0;
-        for (int i = 0; i < K; ++i) {
-            const double u = zh[i] / ((dl[i] - base) - tq);
+        for (int i = gl; i < K; i += DC_G) {
+            const double u = zh[i] * rcp_nr((dl[i] - base) - tq);
             ss += u * u;
         }
-        inrm[q] = 1.0 / sqrt(ss);
+        ss = group_sum(ss);
+        if (gl == 0) inrm[q] = 1.0 / sqrt(ss);
     }
     __syncthreads();
     // (8) U (K x K) into Wu; deflated eigenvectors are columns of Q-hat
     for (int i = wave; i < K; i += NW) {
         const double zi = zh[i], di = dl[i];
-        for (int q = lane; q < K; q += 64) at(Wu, i, q) = zi * inrm[q] / ((di - dl[org[q]]) - tau[q]);
+        for (int q = lane; q < K; q += 64) at(Wu, i, q) = zi * inrm[q] * rcp_nr((di - dl[org[q]]) - tau[q]);
     }
     const int nd_cnt = m - K;
     for (int r = wave; r < m; r += NW)
@@ -315,6 +350,7 @@ __global__ __launch_bounds__(1024) void dc_merge_kernel(int n, int t, const doub
         const int ra = r0 + l16, qb = q0 + l16;
         const double *arow = &at(Wq, ra < m ? ra : m - 1, 0);
         M::acc acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
         for (int i0 = 0; i0 < K; i0 += 4) {
             const int i = i0 + g;
             const double av = (ra < m && i < K) ? arow[nd[i]] : 0.0;
