@@ -413,36 +413,40 @@ __device__ __forceinline__ void tred_step(int n, int k, double *M, int b0, int l
     const double h = sigma + f * f - f * g;  // ||u||^2 / 2 with u = x - g e_0
     const double u0 = f - g, hinv = 1.0 / h;
     auto u = [&](int t) { return t == 0 ? u0 : x[t]; };
-    // p = A22 u: wavefront w takes 64 rows (one per lane) and a K chunk
+    // p = A22 u = A22^T u (the block is exactly symmetric): wavefront w takes a slice of 64
+    // columns (one per lane, so a row segment is one coalesced read) and a group of rows
+    const int CS = (m + 63) >> 6, RGP = TRED_WAVES / CS;  // column slices, row groups
     {
-        const int RG = (m + 63) >> 6, KC = TRED_WAVES / RG;
-        const int rg = wave / KC, kc = wave - rg * KC;
-        if (rg < RG) {
-            const int r = 64 * rg + lane;
-            const int cb = (m * kc) / KC, ce = (m * (kc + 1)) / KC;
-            double acc = 0.0;
-            if (r < m) {
-                const double *row = B + (long long)r * ld;
-#pragma unroll 8
-                for (int c = cb; c < ce; ++c) acc = fma(row[c], u(c), acc);
+        const int cs = wave % CS, rgp = wave / CS;
+        if (rgp < RGP) {
+            const int c = 64 * cs + lane;
+            const int rb = (m * rgp) / RGP, re = (m * (rgp + 1)) / RGP;
+            double acc0 = 0.0, acc1 = 0.0;
+            if (c < m) {
+                const double *col = B + c;
+                int r = rb;
+                for (; r + 1 < re; r += 2) {
+                    acc0 = fma(col[(long long)r * ld], u(r), acc0);
+                    acc1 = fma(col[(long long)(r + 1) * ld], u(r + 1), acc1);
+                }
+                if (r < re) acc0 = fma(col[(long long)r * ld], u(r), acc0);
             }
-            pp[wave * 64 + lane] = acc;
+            pp[wave * 64 + lane] = acc0 + acc1;
         }
     }
     __syncthreads();
     // wavefront 0: p from the partial sums, K = u^T p / (2 h^2), q = p / h - K u
     if (wave == 0) {
-        const int RG = (m + 63) >> 6, KC = TRED_WAVES / RG;
         double kl = 0.0;
-        for (int r = lane; r < m; r += 64) {
-            const int rg = r >> 6;
-            double pr = 0.0;
-            for (int c = 0; c < KC; ++c) pr += pp[(rg * KC + c) * 64 + (r & 63)];
-            q[r] = pr;
-            kl += u(r) * pr;
+        for (int c = lane; c < m; c += 64) {
+            const int cs = c >> 6;
+            double pc = 0.0;
+            for (int g2 = 0; g2 < RGP; ++g2) pc += pp[(g2 * CS + cs) * 64 + (c & 63)];
+            q[c] = pc;
+            kl += u(c) * pc;
         }
         const double K = 0.5 * wave_sum(kl) * hinv * hinv;
-        for (int r = lane; r < m; r += 64) q[r] = q[r] * hinv - K * u(r);
+        for (int c = lane; c < m; c += 64) q[c] = q[c] * hinv - K * u(c);
     }
     __syncthreads();
     // A22 -= u q^T + q u^T, one row per wavefront, lanes along the row; both triangles are
