@@ -28,8 +28,10 @@ int dft_padded_size(int nkx, int nky);
 void build_dft_tables(int nkx, int nky, int nz, const cx<double> *V, DftStore &st, hipStream_t s);
 
 // y = full off-diagonal coupling of x (compact channel layout), every active instance.
+// active_hint: an upper bound of the number of active instances (-1: unknown), used to
+// spread few instances over more workgroups.
 template <typename T>
 void launch_dft_coupling(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
-                         const int *active, hipStream_t s);
+                         const int *active, hipStream_t s, int active_hint = -1);
 
 }  // namespace msx

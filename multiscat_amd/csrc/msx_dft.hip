@@ -284,14 +284,21 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
 
 template <typename T, int NP, int WPB>
 static void launch_dft_impl(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
-                            const int *active, hipStream_t s) {
+                            const int *active, hipStream_t s, int active_hint) {
     using G = DftGeom<NP>;
     const size_t shm = sizeof(T) * (size_t)WPB * 2 * G::PLANE;
     set_dynamic_lds((const void *)dft_coupling_kernel<T, NP, WPB>, shm);
     const long long tot = L.total > 0 ? L.total : (long long)L.max_nc * L.n_inst;
     if ((long long)L.nz * (tot + 64) >= (1LL << 31))
         throw Error(3, "DFT coupling: batch state beyond 2^31 elements");
-    const int zpb = 4 * WPB;  // four nodes per wavefront
+    // four nodes per wavefront (the next node's data is prefetched during the current one);
+    // one node per wavefront when so few instances iterate that four-node groups would leave
+    // the device short of wavefronts (fewer than about eight per SIMD)
+    int zpb = 4 * WPB;
+    if (active_hint >= 0) {
+        const long long waves = (long long)active_hint * ((L.nz + zpb - 1) / zpb) * WPB;
+        if (waves < 8192) zpb = WPB;
+    }
     dim3 grid(L.n_inst, (L.nz + zpb - 1) / zpb);
     hipLaunchKernelGGL((dft_coupling_kernel<T, NP, WPB>), grid, dim3(64 * WPB), shm, s, L, D.Fx,
                        D.Fy, D.Vhat, x, y, active, zpb);
@@ -300,17 +307,18 @@ static void launch_dft_impl(const Layout &L, const DftTables<T> &D, const cx<T> 
 
 template <typename T>
 void launch_dft_coupling(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
-                         const int *active, hipStream_t s) {
+                         const int *active, hipStream_t s, int active_hint) {
     constexpr int WPB = sizeof(T) == 4 ? 4 : 2;  // LDS per wavefront: 12 KB FP32, 24 KB FP64
-    if (D.np == 16) launch_dft_impl<T, 16, WPB>(L, D, x, y, active, s);
-    else if (D.np == 32) launch_dft_impl<T, 32, WPB>(L, D, x, y, active, s);
+    if (D.np == 16) launch_dft_impl<T, 16, WPB>(L, D, x, y, active, s, active_hint);
+    else if (D.np == 32) launch_dft_impl<T, 32, WPB>(L, D, x, y, active, s, active_hint);
     else throw Error(3, "DFT coupling supports tables up to 32 x 32");
 }
 template void launch_dft_coupling<float>(const Layout &, const DftTables<float> &,
-                                         const cx<float> *, cx<float> *, const int *, hipStream_t);
+                                         const cx<float> *, cx<float> *, const int *, hipStream_t,
+                                         int);
 template void launch_dft_coupling<double>(const Layout &, const DftTables<double> &,
                                           const cx<double> *, cx<double> *, const int *,
-                                          hipStream_t);
+                                          hipStream_t, int);
 
 // ---------------------------------------------------------------------------------------
 // Set-up: DFT matrices (FP64 twiddles, k g reduced mod n) and Vhat(z) = F_x V'(z) F_y / nG.

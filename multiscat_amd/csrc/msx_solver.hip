@@ -64,6 +64,7 @@ struct Krylov {
     int mask_slot = 0;
     std::vector<long long> len_h;
     std::shared_ptr<ActiveSnap> snap;  // host view of the mask the launches issued now see
+    int last_count = -1;  // iterating instances at the last count read (an upper bound later)
 
     Krylov(const std::vector<long long> &start, const std::vector<long long> &len, int kmax_,
            hipStream_t stream)
@@ -123,6 +124,8 @@ struct Krylov {
         MSX_HIP(hipEventRecord(mask_ev[sl], s));
         snap = std::make_shared<ActiveSnap>();
         snap->on.assign(h, h + n_inst);
+        last_count = 0;
+        for (int b = 0; b < n_inst; ++b) last_count += h[b];
     }
     void ensure(int nvec) {
         if (nvec > kmax + 1) nvec = kmax + 1;
@@ -157,7 +160,8 @@ struct Krylov {
         const int *a = flags_h + (size_t)p.slot * (n_inst + 1);
         p.next->on.assign(a, a + n_inst);
         p.next->ready.store(true, std::memory_order_release);
-        return a[n_inst];
+        last_count = a[n_inst];
+        return last_count;
     }
     int read_count() { return wait(post()); }
     // per-instance work of a launch over the iterating instances (counted against snap)
@@ -462,8 +466,10 @@ struct Ops {
     }
     // full off-diagonal coupling y = V x (DFT form on the matrix cores when it applies)
     // ---- algorithmic work of one launch over the instances it processes (profiling) ----
-    // host view of the Krylov mask the launches see, if any
+    // host view of the Krylov mask the launches see, if any, and its latest count
     const std::shared_ptr<ActiveSnap> *snap_src = nullptr;
+    const int *count_src = nullptr;
+    int active_hint(const int *act) const { return act && count_src ? *count_src : -1; }
     template <typename F>
     Work over_active(const int *act, F f) const {
         Work w;
@@ -488,14 +494,14 @@ struct Ops {
         if (profiling()) couple_full_work(act, sizeof(cx<double>), !direct, fl, by);
         ProfScope ps(direct ? "coupling_direct_f64" : "coupling_dft_f64", s, fl, by);
         if (direct) launch_coupling<double>(L, V, in, out, COUPLE_FULL, act, s);
-        else launch_dft_coupling<double>(L, dft->d64(), in, out, act, s);
+        else launch_dft_coupling<double>(L, dft->d64(), in, out, act, s, active_hint(act));
     }
     void couple_full(const cx<float> *in, cx<float> *out, const int *act) {
         Work fl, by;
         if (profiling()) couple_full_work(act, sizeof(cx<float>), !direct, fl, by);
         ProfScope ps(direct ? "coupling_direct_f32" : "coupling_dft_f32", s, fl, by);
         if (direct) launch_coupling<float>(L, Vf, in, out, COUPLE_FULL, act, s);
-        else launch_dft_coupling<float>(L, dft->d32(), in, out, act, s);
+        else launch_dft_coupling<float>(L, dft->d32(), in, out, act, s, active_hint(act));
     }
     void upper(const cx<double> *in, cx<double> *out, const int *act) {
         Work fl, by;
@@ -707,9 +713,9 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
                 ops.jac<double>(in, out, act, B.tmp1.p, B.tmp2.p);
             };
         }
-        ops.snap_src = &kr.snap;
+        ops.snap_src = &kr.snap; ops.count_src = &kr.last_count;
         gmres<double, double>(kr, op, B.rhs.p, x, !warm, o.eps, o.window, o.maxiter, true);
-        ops.snap_src = nullptr;
+        ops.snap_src = nullptr; ops.count_src = nullptr;
         iters = kr.host_ints(kr.iters);
         auto conv = kr.host_ints(kr.converged);
         resid = kr.host_doubles(kr.resid);
@@ -752,7 +758,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
             MSX_HIP(hipMemcpyAsync(r64, d0, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, s));
         } else {
             // r = d0 - (x + D^-1 V x)   (B.src is free: the source was consumed above)
-            ops.snap_src = &outer.snap;
+            ops.snap_src = &outer.snap; ops.count_src = &outer.last_count;
             ops.couple_full(x, B.tmp1.p, outer.active.p);
             ops.dinv<double>(B.tmp1.p, x, B.src.p, 1.0, outer.active.p);
             k_axpby<double>(outer.K, d0, B.src.p, -1.0, r64, outer.active.p, s);
@@ -768,7 +774,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         }
         if (n_act == 0 || sweep >= max_refine) break;
         outer.set_all_active(act.data());
-        ops.snap_src = &outer.snap;
+        ops.snap_src = &outer.snap; ops.count_src = &outer.last_count;
         // inner right-hand side M r in FP32 (M = I - D^-1 V to first degree; on a cold start the
         // first r = D^-1 s lives on the specular channels only, and V r is a single-source product)
         launch_to_float((long long)n, r64, B.f_tmp2.p, s);
@@ -786,7 +792,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         ops.neumann_rhs<float>(B.f_tmp2.p, B.f_rhs.p, outer.active.p, B.f_tmp1.p,
                                ops.neumann ? B.f_tmp1.p : nullptr);
         inner.set_all_active(act.data());
-        ops.snap_src = &inner.snap;
+        ops.snap_src = &inner.snap; ops.count_src = &inner.last_count;
         gmres<float, double>(inner, op32, B.f_rhs.p, x, true, inner_eps, 1,
                              std::min(o.maxiter, inner_dim), false, o.inner_orth == 1 ? 1 : 2);
         launch_add_ints(B.n_inst, inner.iters.p, B.iters_acc.p, s);
