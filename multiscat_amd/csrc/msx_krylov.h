@@ -38,6 +38,11 @@ void k_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<T> 
 template <typename T>
 void k_axpy(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<double> *h, cx<T> *w,
             int with_norm, double *partial, cx<double> *hnorm_dst, hipStream_t s);
+// w -= sum_{j<kcount} v_j h_j, then h2 = V^H w, in one pass over the basis (kcount <= 16;
+// false, with nothing launched, above that).
+template <typename T>
+bool k_axpy_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<double> *h,
+                 cx<T> *w, double *partial, cx<double> *h2, hipStream_t s);
 template <typename T, typename TX>
 void k_update(const KLayout &K, const cx<T> *const *vptr, const int *kk, const cx<double> *y,
               cx<TX> *x, const int *mask, hipStream_t s);

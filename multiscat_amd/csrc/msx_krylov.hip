@@ -128,6 +128,71 @@ __global__ __launch_bounds__(KR_THREADS) void axpy_kernel(KLayout K, const cx<T>
     }
 }
 
+// The first update of CGS2 and the second projection in one pass over the basis:
+//   w <- w - sum_{j<kcount} v_j h_j,  then partial sums of conj(v_j) w (as dots_kernel writes
+// them), with the kcount <= KM basis values of each element held in registers between the two.
+// Same per-thread element order and block reductions as axpy_kernel then dots_kernel, so the
+// results are bitwise those of the two separate passes.
+template <typename T, int KM>
+__global__ __launch_bounds__(KR_THREADS) void axpy_dots_kernel(KLayout K,
+                                                               const cx<T> *const *vptr,
+                                                               int kcount,
+                                                               const cx<double> *__restrict__ h,
+                                                               cx<T> *__restrict__ w,
+                                                               double *__restrict__ partial) {
+    __shared__ cx<T> hs[KM];
+    __shared__ double red[2 * (KR_THREADS / 64)];
+    const int b = blockIdx.y;
+    if (K.active && !K.active[b]) return;
+    const long long n = K.len[b], base = K.start[b];
+    const long long slot = (long long)b * (K.kmax + 1);
+    for (int j = threadIdx.x; j < kcount; j += KR_THREADS) {
+        const cx<double> hv = h[slot + j];
+        hs[j] = mk((T)hv.re, (T)hv.im);
+    }
+    __syncthreads();
+    const cx<T> *vb[KM];
+#pragma unroll
+    for (int j = 0; j < KM; ++j) vb[j] = j < kcount ? vptr[j] + base : nullptr;
+    const long long cbeg = (long long)blockIdx.x * K.chunk;
+    const long long cend = cbeg + K.chunk < n ? cbeg + K.chunk : n;
+    double sr[KM], si[KM];
+#pragma unroll
+    for (int j = 0; j < KM; ++j) sr[j] = si[j] = 0.0;
+    for (long long i = cbeg + threadIdx.x; i < cend; i += KR_THREADS) {
+        cx<T> v[KM];
+#pragma unroll
+        for (int j = 0; j < KM; ++j)
+            if (j < kcount) v[j] = vb[j][i];
+        cx<T> acc = w[base + i];
+#pragma unroll
+        for (int j = 0; j < KM; ++j)
+            if (j < kcount) {
+                const cx<T> hj = hs[j];
+                acc.re -= v[j].re * hj.re - v[j].im * hj.im;
+                acc.im -= v[j].re * hj.im + v[j].im * hj.re;
+            }
+        w[base + i] = acc;
+#pragma unroll
+        for (int j = 0; j < KM; ++j)
+            if (j < kcount) {
+                sr[j] += (double)v[j].re * acc.re + (double)v[j].im * acc.im;
+                si[j] += (double)v[j].re * acc.im - (double)v[j].im * acc.re;
+            }
+    }
+#pragma unroll
+    for (int j = 0; j < KM; ++j) {
+        if (j >= kcount) break;
+        double a = sr[j], c = si[j];
+        block_sum2(a, c, red);
+        if (threadIdx.x == 0) {
+            double *p = partial + ((slot + j) * K.nchunk + blockIdx.x) * 2;
+            p[0] = cbeg < n ? a : 0.0;
+            p[1] = cbeg < n ? c : 0.0;
+        }
+    }
+}
+
 // x += sum_{j<kk_b} v_j y_j  (kk_b per instance)
 template <typename T, typename TX>
 __global__ __launch_bounds__(KR_THREADS) void update_kernel(KLayout K, const cx<T> *const *vptr,
@@ -377,6 +442,24 @@ void k_axpy(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<dou
     }
 }
 
+template <typename T>
+bool k_axpy_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<double> *h,
+                 cx<T> *w, double *partial, cx<double> *h2, hipStream_t s) {
+    if (kcount <= 8)
+        hipLaunchKernelGGL((axpy_dots_kernel<T, 8>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr,
+                           kcount, h, w, partial);
+    else if (kcount <= 16)
+        hipLaunchKernelGGL((axpy_dots_kernel<T, 16>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr,
+                           kcount, h, w, partial);
+    else
+        return false;
+    MSX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(K.n_inst), dim3(64), 0, s, K, 0, kcount,
+                       partial, h2, 0);
+    MSX_LAUNCH_CHECK();
+    return true;
+}
+
 template <typename T, typename TX>
 void k_update(const KLayout &K, const cx<T> *const *vptr, const int *kk, const cx<double> *y,
               cx<TX> *x, const int *mask, hipStream_t s) {
@@ -424,6 +507,9 @@ void k_cycle_start(const KLayout &K, const KState &S, const cx<double> *nrm2, in
                             double *, cx<double> *, int, hipStream_t);                          \
     template void k_axpy<T>(const KLayout &, const cx<T> *const *, int, const cx<double> *,     \
                             cx<T> *, int, double *, cx<double> *, hipStream_t);                 \
+    template bool k_axpy_dots<T>(const KLayout &, const cx<T> *const *, int,                    \
+                                 const cx<double> *, cx<T> *, double *, cx<double> *,           \
+                                 hipStream_t);                                                  \
     template void k_scale<T>(const KLayout &, const cx<T> *, cx<T> *, const double *,           \
                              const int *, hipStream_t);                                         \
     template void k_axpby<T>(const KLayout &, const cx<T> *, const cx<T> *, T, cx<T> *,          \

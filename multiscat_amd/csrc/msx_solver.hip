@@ -248,19 +248,27 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
             op(kr.vp[k], kr.w.p, kr.active.p);
             {
                 // CGS2: two passes of (k+1 dot products, then w -= V h); HBM traffic of the
-                // four vector kernels is 4 (k+1) + 6 vectors of the iterating instances (one
-                // pass: 2 (k+1) + 3)
+                // vector kernels is 4 (k+1) + 6 vectors of the iterating instances, 3 (k+1) + 6
+                // when the first update and the second projection share a pass (k+1 <= 16);
+                // one pass: 2 (k+1) + 3
                 const double P = orth_passes >= 2 ? 2.0 : 1.0;
+                const double vecs = orth_passes >= 2 ? (k + 1 <= 16 ? 3.0 * (k + 1) + 6.0 : 4.0 * (k + 1) + 6.0)
+                                                     : 2.0 * (k + 1) + 3.0;
                 const Work fl = kr.per_instance(P * 16.0 * (k + 1));
-                const Work by = kr.per_instance(P * (2.0 * (k + 1) + 3.0) * sizeof(cx<T>));
+                const Work by = kr.per_instance(vecs * sizeof(cx<T>));
                 const char *nm = orth_passes >= 2
                                      ? (sizeof(T) == 4 ? "arnoldi_cgs2_f32" : "arnoldi_cgs2_f64")
                                      : (sizeof(T) == 4 ? "arnoldi_cgs1_f32" : "arnoldi_cgs1_f64");
                 ProfScope ps(nm, s, fl, by);
                 k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol.p, 0, s);
                 if (orth_passes >= 2) {
-                    k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol.p, kr.w.p, 0, kr.partial.p, nullptr, s);
-                    k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol2.p, 0, s);
+                    // the first update and the second projection share one pass over V
+                    if (!k_axpy_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol.p, kr.w.p, kr.partial.p,
+                                        kr.hcol2.p, s)) {
+                        k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol.p, kr.w.p, 0, kr.partial.p,
+                                  nullptr, s);
+                        k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol2.p, 0, s);
+                    }
                     k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol2.p, kr.w.p, 1, kr.partial.p,
                               kr.hcol.p, s);
                 } else {
