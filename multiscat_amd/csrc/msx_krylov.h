@@ -53,13 +53,15 @@ template <typename T>
 void k_axpby(const KLayout &K, const cx<T> *a, const cx<T> *c, T alpha, cx<T> *dst,
              const int *mask, hipStream_t s);
 void k_givens(const KLayout &K, int k, const cx<double> *hcol, const cx<double> *hcol2,
-              const KState &S, int *count, hipStream_t s);
+              const KState &S, int *count, hipStream_t s,
+              int *host_flags = nullptr);
 void k_backsolve(const KLayout &K, const KState &S, const int *mask, hipStream_t s);
 // Starts a cycle from the residual norms nrm2; on the first cycle the convergence reference
 // beta0 is the residual norm, or sqrt(ref_nrm2) when given (an initial guess: then an
 // instance whose residual is already below eps * beta0 stops at once).
 void k_cycle_start(const KLayout &K, const KState &S, const cx<double> *nrm2, int first,
-                   int *count, hipStream_t s, const cx<double> *ref_nrm2 = nullptr);
+                   int *count, hipStream_t s, const cx<double> *ref_nrm2 = nullptr,
+                   int *host_flags = nullptr);
 void k_reset(const KLayout &K, const KState &S, int *enabled, hipStream_t s);
 void k_reactivate(const KLayout &K, const KState &S, const int *enabled, hipStream_t s);
 

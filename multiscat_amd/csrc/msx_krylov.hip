@@ -278,13 +278,30 @@ __device__ inline void givens(cx<double> a, cx<double> b, cx<double> &c, cx<doub
     s = b * (1.0 / r);
 }
 
+// Block total of `alive` into *count; with host_flags (pinned host memory, n_inst + 1 ints)
+// also the active flags and the count, written straight to the host (no copy launch).
+__device__ void finish_count(const KLayout &K, int alive, int *red, int *count, int *host_flags) {
+    for (int o = 32; o > 0; o >>= 1) alive += __shfl_xor(alive, o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = alive;
+    __syncthreads();
+    int t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    if (threadIdx.x == 0) *count = t;
+    if (host_flags) {
+        for (int b = threadIdx.x; b < K.n_inst; b += blockDim.x) host_flags[b] = K.active[b];
+        if (threadIdx.x == 0) host_flags[K.n_inst] = t;
+        __threadfence_system();
+    }
+}
+
 // One thread per instance: finish column k of the Hessenberg matrix (hcol + hcol2, or hcol alone
 // when hcol2 is null), rotate, update the
 // residual recurrence and the convergence window. Writes the number of instances still
 // active to *count.
 __global__ void givens_kernel(KLayout K, int k, const cx<double> *__restrict__ hcol,
                               const cx<double> *__restrict__ hcol2, KState S,
-                              int *__restrict__ count) {
+                              int *__restrict__ count, int *host_flags) {
     __shared__ int red[KR_THREADS / 64];
     int alive = 0;
     for (int b = threadIdx.x; b < K.n_inst; b += blockDim.x) {
@@ -322,15 +339,7 @@ __global__ void givens_kernel(KLayout K, int k, const cx<double> *__restrict__ h
             alive += 1;
         }
     }
-    for (int o = 32; o > 0; o >>= 1) alive += __shfl_xor(alive, o);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) red[wave] = alive;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int t = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
-        *count = t;
-    }
+    finish_count(K, alive, red, count, host_flags);
 }
 
 // One thread per instance: back substitution R y = g over the cycle's kk columns.
@@ -352,7 +361,7 @@ __global__ void backsolve_kernel(KLayout K, KState S, const int *__restrict__ ma
 // Instances with beta == 0 are already solved.
 __global__ void cycle_start_kernel(KLayout K, KState S, const cx<double> *__restrict__ nrm2,
                                    int first_cycle, int *__restrict__ count,
-                                   const cx<double> *__restrict__ ref_nrm2) {
+                                   const cx<double> *__restrict__ ref_nrm2, int *host_flags) {
     __shared__ int red[KR_THREADS / 64];
     int alive = 0;
     for (int b = threadIdx.x; b < K.n_inst; b += blockDim.x) {
@@ -372,15 +381,7 @@ __global__ void cycle_start_kernel(KLayout K, KState S, const cx<double> *__rest
             alive += 1;
         }
     }
-    for (int o = 32; o > 0; o >>= 1) alive += __shfl_xor(alive, o);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) red[wave] = alive;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int t = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
-        *count = t;
-    }
+    finish_count(K, alive, red, count, host_flags);
 }
 
 // Start of a solve: remember which instances take part, clear their counters.
@@ -484,9 +485,9 @@ void k_axpby(const KLayout &K, const cx<T> *a, const cx<T> *c, T alpha, cx<T> *d
 }
 
 void k_givens(const KLayout &K, int k, const cx<double> *hcol, const cx<double> *hcol2,
-              const KState &S, int *count, hipStream_t s) {
+              const KState &S, int *count, hipStream_t s, int *host_flags) {
     hipLaunchKernelGGL(givens_kernel, dim3(1), dim3(KR_THREADS), 0, s, K, k, hcol, hcol2, S,
-                       count);
+                       count, host_flags);
     MSX_LAUNCH_CHECK();
 }
 
@@ -496,9 +497,9 @@ void k_backsolve(const KLayout &K, const KState &S, const int *mask, hipStream_t
 }
 
 void k_cycle_start(const KLayout &K, const KState &S, const cx<double> *nrm2, int first,
-                   int *count, hipStream_t s, const cx<double> *ref_nrm2) {
+                   int *count, hipStream_t s, const cx<double> *ref_nrm2, int *host_flags) {
     hipLaunchKernelGGL(cycle_start_kernel, dim3(1), dim3(KR_THREADS), 0, s, K, S, nrm2, first,
-                       count, ref_nrm2);
+                       count, ref_nrm2, host_flags);
     MSX_LAUNCH_CHECK();
 }
 

@@ -57,6 +57,7 @@ struct Krylov {
     DBuf<cx<T> *> vp_d;
     DBuf<cx<T>> w;
     int *flags_h = nullptr;  // pinned, two slots of (active flags, count) in flight
+    int *flags_dev = nullptr;  // the same memory as the device addresses it
     hipEvent_t ev[2] = {nullptr, nullptr};
     int next_slot = 0;
     int *mask_h = nullptr;  // pinned, two slots of n_inst flags staged for upload
@@ -100,6 +101,7 @@ struct Krylov {
         S = KState{H.p, g.p, cs.p, sn.p, y.p, beta.p, beta0.p, resid.p, hnorm.p,
                    kconv.p, iters.p, cycle_iters.p, converged.p, 0.0, 3};
         MSX_HIP(hipHostMalloc((void **)&flags_h, 2 * sizeof(int) * (n_inst + 1), hipHostMallocDefault));
+        MSX_HIP(hipHostGetDevicePointer((void **)&flags_dev, flags_h, 0));
         MSX_HIP(hipHostMalloc((void **)&mask_h, 2 * sizeof(int) * n_inst, hipHostMallocDefault));
         for (auto &e : ev) MSX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         for (auto &e : mask_ev) MSX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -144,11 +146,16 @@ struct Krylov {
         int slot = -1;
         std::shared_ptr<ActiveSnap> next;
     };
-    Pending post() {
+    // Device address of the slot the next post() takes: the kernel that ends an iteration
+    // writes the flags and the count there itself (post(true) then only records the event).
+    // Its previous contents were read by the wait() one iteration earlier.
+    int *next_flags() const { return flags_dev + (size_t)next_slot * (n_inst + 1); }
+    Pending post(bool written = false) {
         const int slot = next_slot;
         next_slot ^= 1;
-        MSX_HIP(hipMemcpyAsync(flags_h + (size_t)slot * (n_inst + 1), flags.p,
-                               sizeof(int) * (n_inst + 1), hipMemcpyDeviceToHost, s));
+        if (!written)
+            MSX_HIP(hipMemcpyAsync(flags_h + (size_t)slot * (n_inst + 1), flags.p,
+                                   sizeof(int) * (n_inst + 1), hipMemcpyDeviceToHost, s));
         MSX_HIP(hipEventRecord(ev[slot], s));
         Pending p{slot, std::make_shared<ActiveSnap>()};
         p.next->ready.store(false, std::memory_order_relaxed);
@@ -229,14 +236,14 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
         const bool ref_rhs = first && !x_zero;
         if (ref_rhs) k_dots<T>(kr.K, kr.vp_d.p, 0, rhs, 1, kr.partial.p, kr.hcol2.p, 0, s);
         k_cycle_start(kr.K, kr.S, kr.hcol.p, first ? 1 : 0, kr.count.p, s,
-                      ref_rhs ? kr.hcol2.p : nullptr);
+                      ref_rhs ? kr.hcol2.p : nullptr, kr.next_flags());
         k_scale<T>(kr.K, v0, v0, kr.beta.p, kr.active.p, s);
         // The host learns whether any instance still iterates one iteration late: iteration k
         // is queued before the count of iteration k - 1 is read, so the GPU never waits for
         // the host. Every kernel skips inactive instances, so an iteration queued after the
         // last instance converged does nothing.
         int alive = 1;
-        typename Krylov<T>::Pending prev = kr.post();  // the count after the cycle start
+        typename Krylov<T>::Pending prev = kr.post(true);  // the count after the cycle start
         for (int k = 0; k < kr.kmax && done < maxiter; ++k) {
             if (prev.slot >= 0 && k + 2 > (int)kr.vp.size()) {
                 // settle before allocating another block of basis vectors
@@ -276,10 +283,11 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
                               s);
                 }
             }
-            k_givens(kr.K, k, kr.hcol.p, orth_passes >= 2 ? kr.hcol2.p : nullptr, kr.S, kr.count.p, s);
+            k_givens(kr.K, k, kr.hcol.p, orth_passes >= 2 ? kr.hcol2.p : nullptr, kr.S, kr.count.p, s,
+                     kr.next_flags());
             k_scale<T>(kr.K, kr.w.p, kr.vp[k + 1], kr.hnorm.p, kr.active.p, s);
             ++done;
-            const typename Krylov<T>::Pending cur = kr.post();
+            const typename Krylov<T>::Pending cur = kr.post(true);
             if (prev.slot >= 0) {
                 alive = kr.wait(prev);
                 if (alive == 0) {  // iteration k had nothing left to do
