@@ -686,6 +686,8 @@ void check_options(const msx_solve_options *o) {
     if (o->neumann > 1 && o->method != MSX_METHOD_JACOBI)
         throw Error(MSX_ERR_INVALID, "a Neumann factor of degree > 1 needs the block-Jacobi method");
     if (o->inner_orth < 0 || o->inner_orth > 2) throw Error(MSX_ERR_INVALID, "inner_orth is 0, 1 or 2");
+    if (!(o->inner_eps >= 0 && o->inner_eps < 1))
+        throw Error(MSX_ERR_INVALID, "inner_eps is 0 (automatic) or in (0, 1)");
 }
 
 // Solve A psi = src for all instances into `x` (FP64). transposed: A^T with the flipped table.
@@ -765,7 +767,11 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     std::vector<int> act(B.n_inst, 1);
     msx_batch::need(B.iters_acc, (size_t)B.n_inst);
     MSX_HIP(hipMemsetAsync(B.iters_acc.p, 0, sizeof(int) * B.n_inst, s));
-    const double inner_eps = o.inner_eps > 0 ? o.inner_eps : 1e-5;
+    // unset: loose enough that two refinement sweeps (the residual shrinking by about
+    // inner_eps each) meet eps with a factor ~10 to spare, never tighter than FP32 GMRES
+    // reliably reaches (1e-5) nor looser than 1e-4
+    const double inner_eps =
+        o.inner_eps > 0 ? o.inner_eps : std::min(1e-4, std::max(1e-5, std::sqrt(o.eps) / 3.0));
     const int max_refine = o.max_refine > 0 ? o.max_refine : 6;
     cx<double> *r64 = B.tmp3.p;
     for (int sweep = 0;; ++sweep) {
@@ -922,7 +928,7 @@ void msx_default_options(msx_solve_options *opt, int nsig) {
     opt->maxiter = 2000;                             // multiscat_gmres.f90:7,124
     opt->restart = 0;
     opt->window = 3;                                 // multiscat_gmres.f90:125
-    opt->inner_eps = 1e-5;
+    opt->inner_eps = 0;  // sqrt(eps)/3 within [1e-5, 1e-4]
     opt->max_refine = 6;
     opt->coupling = 0;
     opt->diag = 0;

@@ -87,6 +87,21 @@ def test_neumann_degrees(small_batch, degree):
             bt.solve(BT.SolveOptions(method="gauss_seidel", eps=5e-13, neumann=degree))
 
 
+def test_mixed_inner_tolerance(small_batch):
+    """The automatic FP32 inner tolerance (sqrt(eps)/3 within [1e-5, 1e-4]) and a fixed one
+    give the same intensities to the outer tolerance; out-of-range values are rejected."""
+    _, _, bt, ref = small_batch
+    Ps = {}
+    for ie in (0.0, 1e-5, 1e-4):
+        bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-8, inner_eps=ie))
+        Ps[ie] = bt.intensities()
+        assert _maxdiff(Ps[ie], ref) < 1e-6
+    assert _maxdiff(Ps[0.0], Ps[1e-5]) < 1e-7
+    for bad in (-1e-5, 1.0, float("nan")):
+        with pytest.raises(RuntimeError, match="inner_eps"):
+            bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", inner_eps=bad))
+
+
 def test_jacobi_large_nz_uses_vector_blocks():
     """nz > 320 takes the vector-ALU diagonal-block kernel instead of the MFMA one."""
     w = W.hex_batch(n_conditions=2, nz=336, nk=7, cond_seed=8, channel_counts=[9, 14])
