@@ -12,11 +12,18 @@
 //
 // Each wavefront transforms one node at a time in its own LDS slab with F_x and F_y held in
 // registers, so every MFMA reads one operand from LDS; the grid runs over instances fastest so
-// that workgroups resident together share the Vhat slabs of the same nodes in L2.
+// that workgroups resident together share the Vhat slabs of the same nodes in L2. For a square
+// table F_y = F_x elementwise in the register layout (the DFT matrix is symmetric), so one copy
+// serves both, and the FP32 kernel then forms each complex product from three real ones
+// (T1 = A_r B_r, T2 = A_i B_i, T3 = (A_r + A_i)(B_r + B_i); re = T1 - T2, im = T3 - T1 - T2),
+// 384 MFMAs per node instead of 512. With node-major vectors (Layout::nm) the channels of one
+// node are contiguous, so the loads of X(z) and the stores of Y(z) are whole cache lines.
 // LDS row strides are chosen so that the A-fragment reads (16 rows x 4 columns) and the
 // B-fragment reads (4 rows x 16 columns) of a wavefront hit 64 distinct banks.
 #include "msx_dft.h"
 #include "msx_mfma.h"
+
+#include <type_traits>
 
 namespace msx {
 
@@ -130,6 +137,113 @@ __device__ __forceinline__ void cgemm_reg_b(const T *ar, const T *ai, int sa,
     }
 }
 
+// The three real products of C = F Y (F in registers as the A operand, Y from LDS) or of
+// C = conj(F) Y (CONJ: the sum operand is Y_r - Y_i); combine3 forms C from them.
+template <typename T, int NP, bool CONJ>
+__device__ __forceinline__ void cgemm3_reg_a(const T (&fr)[DftGeom<NP>::NT][NP / 4],
+                                             const T (&fi)[DftGeom<NP>::NT][NP / 4],
+                                             const T *br, const T *bi, int sb,
+                                             typename Mfma<T>::acc (&c1)[DftGeom<NP>::NT][DftGeom<NP>::NT],
+                                             typename Mfma<T>::acc (&c2)[DftGeom<NP>::NT][DftGeom<NP>::NT],
+                                             typename Mfma<T>::acc (&c3)[DftGeom<NP>::NT][DftGeom<NP>::NT]) {
+    constexpr int NT = DftGeom<NP>::NT;
+    const int lane = threadIdx.x & 63;
+    const int l16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj)
+            c1[ti][tj] = c2[ti][tj] = c3[ti][tj] = typename Mfma<T>::acc{0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < NP / 4; ++kk) {
+        const int k = kk * 4 + k4;
+        T yr[NT], yi[NT], ys[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            yr[t] = br[k * sb + t * 16 + l16];
+            yi[t] = bi[k * sb + t * 16 + l16];
+            ys[t] = CONJ ? yr[t] - yi[t] : yr[t] + yi[t];
+        }
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti) {
+            const T xr = fr[ti][kk], xi = fi[ti][kk], xs = xr + xi;
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj) {
+                c1[ti][tj] = Mfma<T>::op(xr, yr[tj], c1[ti][tj]);
+                c2[ti][tj] = Mfma<T>::op(xi, yi[tj], c2[ti][tj]);
+                c3[ti][tj] = Mfma<T>::op(xs, ys[tj], c3[ti][tj]);
+            }
+        }
+    }
+}
+
+// The three real products of C = X F (X from LDS as the A operand, F in registers) or of
+// C = X conj(F) (CONJ: the sum operand is X_r - X_i).
+template <typename T, int NP, bool CONJ>
+__device__ __forceinline__ void cgemm3_reg_b(const T *ar, const T *ai, int sa,
+                                             const T (&fr)[DftGeom<NP>::NT][NP / 4],
+                                             const T (&fi)[DftGeom<NP>::NT][NP / 4],
+                                             typename Mfma<T>::acc (&c1)[DftGeom<NP>::NT][DftGeom<NP>::NT],
+                                             typename Mfma<T>::acc (&c2)[DftGeom<NP>::NT][DftGeom<NP>::NT],
+                                             typename Mfma<T>::acc (&c3)[DftGeom<NP>::NT][DftGeom<NP>::NT]) {
+    constexpr int NT = DftGeom<NP>::NT;
+    const int lane = threadIdx.x & 63;
+    const int l16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj)
+            c1[ti][tj] = c2[ti][tj] = c3[ti][tj] = typename Mfma<T>::acc{0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < NP / 4; ++kk) {
+        const int k = kk * 4 + k4;
+        T xr[NT], xi[NT], xs[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            xr[t] = ar[(t * 16 + l16) * sa + k];
+            xi[t] = ai[(t * 16 + l16) * sa + k];
+            xs[t] = CONJ ? xr[t] - xi[t] : xr[t] + xi[t];
+        }
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj) {
+            const T yr = fr[tj][kk], yi = fi[tj][kk], ysum = yr + yi;
+#pragma unroll
+            for (int ti = 0; ti < NT; ++ti) {
+                c1[ti][tj] = Mfma<T>::op(xr[ti], yr, c1[ti][tj]);
+                c2[ti][tj] = Mfma<T>::op(xi[ti], yi, c2[ti][tj]);
+                c3[ti][tj] = Mfma<T>::op(xs[ti], ysum, c3[ti][tj]);
+            }
+        }
+    }
+}
+
+// (re, im) of the product into (c1, c2): plain re = T1 - T2, im = T3 - T1 - T2;
+// conjugated re = T1 + T2, im = T1 - T2 - T3.
+template <typename T, int NP, bool CONJ>
+__device__ __forceinline__ void combine3(typename Mfma<T>::acc (&c1)[DftGeom<NP>::NT][DftGeom<NP>::NT],
+                                         typename Mfma<T>::acc (&c2)[DftGeom<NP>::NT][DftGeom<NP>::NT],
+                                         const typename Mfma<T>::acc (&c3)[DftGeom<NP>::NT][DftGeom<NP>::NT]) {
+    constexpr int NT = DftGeom<NP>::NT;
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const T a = c1[ti][tj][v], b = c2[ti][tj][v], c = c3[ti][tj][v];
+                const T sum = a + b, dif = a - b;
+                c1[ti][tj][v] = CONJ ? sum : dif;
+                c2[ti][tj][v] = CONJ ? dif - c : c - sum;
+            }
+}
+
+// Compile-time choice between two register arrays of the same shape.
+template <bool FIRST, class A>
+__device__ __forceinline__ A &pick(A &a, A &b) {
+    if constexpr (FIRST) return a;
+    else return b;
+}
+
 template <typename T, int NP>
 __device__ __forceinline__ void store_tiles(T *dr, T *di, int stride,
                                             typename Mfma<T>::acc (&cr)[DftGeom<NP>::NT][DftGeom<NP>::NT],
@@ -154,16 +268,20 @@ __device__ __forceinline__ void store_tiles(T *dr, T *di, int stride,
 // processes its nodes independently: scatter X(z) onto the grid, the four transforms with
 // F_x / F_y held in registers for the whole kernel, the pointwise product with Vhat(z) in
 // registers after the second, gather Y(z) back to the compact layout. No workgroup barrier.
-// The channel values and the Vhat slab of a wavefront's next node are loaded into registers
-// while the current node's transforms run; the channels' grid positions are computed once.
-template <typename T, int NP, int WPB>
+// The channel values of a wavefront's next node are loaded into registers while the current
+// node's transforms run; the channels' grid positions are computed once.
+// SQ: square table (F_y = F_x); NM: node-major vectors.
+template <typename T, int NP, int WPB, bool SQ, bool NM>
 __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling_kernel(
     Layout L, const cx<T> *__restrict__ Fx, const cx<T> *__restrict__ Fy,
     const cx<T> *__restrict__ Vhat, const cx<T> *__restrict__ x, cx<T> *__restrict__ y,
     const int *__restrict__ active, int zpb) {
     using G = DftGeom<NP>;
+    using acc_t = typename Mfma<T>::acc;
     constexpr int NT = G::NT, SA = G::SA, SB = G::SB, KK = NP / 4;
     constexpr int KC = NP * NP / 64;  // channels per lane (nc <= NP * NP)
+    // three real products per complex one: FP32 with one F in registers (register budget)
+    constexpr bool M3 = sizeof(T) == 4 && SQ;
     const int b = blockIdx.x;
     if (active && !active[b]) return;
     const int nz = L.nz;
@@ -173,18 +291,22 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
     T *sr = reinterpret_cast<T *>(dft_smem) + (size_t)wave * 2 * G::PLANE;
     T *si = sr + G::PLANE;
 
-    T fxr[NT][KK], fxi[NT][KK], fyr[NT][KK], fyi[NT][KK];
+    T fxr[NT][KK], fxi[NT][KK], fyr_[NT][KK], fyi_[NT][KK];
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
         for (int kk = 0; kk < KK; ++kk) {
             const cx<T> f = Fx[(t * 16 + l16) * NP + kk * 4 + k4];
-            const cx<T> g = Fy[(kk * 4 + k4) * NP + t * 16 + l16];
             fxr[t][kk] = f.re;
             fxi[t][kk] = f.im;
-            fyr[t][kk] = g.re;
-            fyi[t][kk] = g.im;
+            if constexpr (!SQ) {
+                const cx<T> g = Fy[(kk * 4 + k4) * NP + t * 16 + l16];
+                fyr_[t][kk] = g.re;
+                fyi_[t][kk] = g.im;
+            }
         }
+    auto &fyr = pick<SQ>(fxr, fyr_);
+    auto &fyi = pick<SQ>(fxi, fyi_);
 
     const int nc = L.nc[b];
     const long long off = L.off[b];
@@ -192,17 +314,19 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
     for (int j = threadIdx.x; j < nc; j += blockDim.x) s_pos[j] = L.chx[off + j] * SB + L.chy[off + j];
     __syncthreads();
     const int col = l16, g = k4;
-    constexpr bool PRE_V = false;  // Vhat(z) is loaded just before the second transform
     cx<T> xv[KC], vh[NT][NT][4];
     // element offsets fit in 32 bits (the launcher checks), so the loads and stores use a
-    // uniform base and one 32-bit offset register
+    // uniform base and one 32-bit offset register; lanes past nc read the instance's first
+    // element (a valid address) and store nothing
     const int row_l = (int)(off + lane) * nz, row_step = 64 * nz, row_0 = (int)off * nz;
+    auto vidx = [&](int k, int z) {
+        const int j = lane + 64 * k;
+        if constexpr (NM) return row_0 + z * nc + (j < nc ? j : 0);
+        else return (j < nc ? row_l + k * row_step : row_0) + z;
+    };
     auto load_x = [&](int z) {
 #pragma unroll
-        for (int k = 0; k < KC; ++k) {
-            const int j = lane + 64 * k;
-            xv[k] = x[(j < nc ? row_l + k * row_step : row_0) + z];
-        }
+        for (int k = 0; k < KC; ++k) xv[k] = x[vidx(k, z)];
     };
     auto load_v = [&](int z) {
         const cx<T> *vz = Vhat + (size_t)z * NP * NP;
@@ -214,15 +338,37 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
                 for (int v = 0; v < 4; ++v)
                     vh[ti][tj][v] = vz[(ti * 16 + Mfma<T>::row(g, v)) * NP + tj * 16 + col];
     };
+    // C = F Y, C = Y F (CONJ: with conj(F)) into (cr, ci)
+    acc_t cr[NT][NT], ci[NT][NT], c3[NT][NT];
+    auto left = [&](auto conj_tag) {
+        constexpr bool CONJ = decltype(conj_tag)::value;
+        if constexpr (M3) {
+            cgemm3_reg_a<T, NP, CONJ>(fxr, fxi, sr, si, SB, cr, ci, c3);
+            combine3<T, NP, CONJ>(cr, ci, c3);
+        } else {
+            cgemm_reg_a<T, NP>(fxr, fxi, CONJ, sr, si, SB, cr, ci);
+        }
+    };
+    auto right = [&](auto conj_tag) {
+        constexpr bool CONJ = decltype(conj_tag)::value;
+        if constexpr (M3) {
+            cgemm3_reg_b<T, NP, CONJ>(sr, si, SA, fyr, fyi, cr, ci, c3);
+            combine3<T, NP, CONJ>(cr, ci, c3);
+        } else {
+            cgemm_reg_b<T, NP>(sr, si, SA, fyr, fyi, CONJ, cr, ci);
+        }
+    };
+    using plain_t = std::integral_constant<bool, false>;
+    using conj_t = std::integral_constant<bool, true>;
     const int z_first = blockIdx.y * zpb + wave;
     const int z_end = min(nz, (blockIdx.y + 1) * zpb);
-    if (z_first < z_end) {
-        load_x(z_first);
-        if constexpr (PRE_V) load_v(z_first);
-    }
-    typename Mfma<T>::acc cr[NT][NT], ci[NT][NT];
+    if (z_first < z_end) load_x(z_first);
     for (int z = z_first; z < z_end; z += WPB) {
-        for (int idx = lane; idx < 2 * G::PLANE; idx += 64) sr[idx] = T(0);
+        {
+            acc_t *p4 = reinterpret_cast<acc_t *>(sr);  // both planes, 16-byte stores
+            constexpr int N4 = 2 * G::PLANE * (int)sizeof(T) / (int)sizeof(acc_t);
+            for (int idx = lane; idx < N4; idx += 64) p4[idx] = acc_t{0, 0, 0, 0};
+        }
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
@@ -237,13 +383,13 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
         if (more) load_x(z + WPB);  // consumed by the scatter above
         __builtin_amdgcn_wave_barrier();
         // (1) Y = F_x X
-        cgemm_reg_a<T, NP>(fxr, fxi, false, sr, si, SB, cr, ci);
+        left(plain_t{});
         __builtin_amdgcn_wave_barrier();
         store_tiles<T, NP>(sr, si, SA, cr, ci);
         __builtin_amdgcn_wave_barrier();
-        // (2) Z = Y F_y, then Z .* Vhat(z)
-        if constexpr (!PRE_V) load_v(z);
-        cgemm_reg_b<T, NP>(sr, si, SA, fyr, fyi, false, cr, ci);
+        // (2) Z = Y F_y, then Z .* Vhat(z) (the slab is loaded just before this product)
+        load_v(z);
+        right(plain_t{});
 #pragma unroll
         for (int ti = 0; ti < NT; ++ti)
 #pragma unroll
@@ -255,18 +401,16 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
                     cr[ti][tj][v] = a * h.re - c * h.im;
                     ci[ti][tj][v] = a * h.im + c * h.re;
                 }
-        if constexpr (PRE_V)
-            if (more) load_v(z + WPB);  // this node's slab has been consumed
         __builtin_amdgcn_wave_barrier();
         store_tiles<T, NP>(sr, si, SB, cr, ci);
         __builtin_amdgcn_wave_barrier();
         // (3) W = conj(F_x) Zhat
-        cgemm_reg_a<T, NP>(fxr, fxi, true, sr, si, SB, cr, ci);
+        left(conj_t{});
         __builtin_amdgcn_wave_barrier();
         store_tiles<T, NP>(sr, si, SA, cr, ci);
         __builtin_amdgcn_wave_barrier();
         // (4) out = W conj(F_y)
-        cgemm_reg_b<T, NP>(sr, si, SA, fyr, fyi, true, cr, ci);
+        right(conj_t{});
         __builtin_amdgcn_wave_barrier();
         store_tiles<T, NP>(sr, si, SB, cr, ci);
         __builtin_amdgcn_wave_barrier();
@@ -275,19 +419,19 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
             const int j = lane + 64 * k;
             if (j < nc) {
                 const int p = s_pos[j];
-                y[row_l + k * row_step + z] = mk(sr[p], si[p]);
+                y[vidx(k, z)] = mk(sr[p], si[p]);
             }
         }
         __builtin_amdgcn_wave_barrier();
     }
 }
 
-template <typename T, int NP, int WPB>
+template <typename T, int NP, int WPB, bool SQ, bool NM>
 static void launch_dft_impl(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
                             const int *active, hipStream_t s, int active_hint) {
     using G = DftGeom<NP>;
     const size_t shm = sizeof(T) * (size_t)WPB * 2 * G::PLANE;
-    set_dynamic_lds((const void *)dft_coupling_kernel<T, NP, WPB>, shm);
+    set_dynamic_lds((const void *)dft_coupling_kernel<T, NP, WPB, SQ, NM>, shm);
     const long long tot = L.total > 0 ? L.total : (long long)L.max_nc * L.n_inst;
     if ((long long)L.nz * (tot + 64) >= (1LL << 31))
         throw Error(3, "DFT coupling: batch state beyond 2^31 elements");
@@ -300,17 +444,34 @@ static void launch_dft_impl(const Layout &L, const DftTables<T> &D, const cx<T> 
         if (waves < 8192) zpb = WPB;
     }
     dim3 grid(L.n_inst, (L.nz + zpb - 1) / zpb);
-    hipLaunchKernelGGL((dft_coupling_kernel<T, NP, WPB>), grid, dim3(64 * WPB), shm, s, L, D.Fx,
-                       D.Fy, D.Vhat, x, y, active, zpb);
+    hipLaunchKernelGGL((dft_coupling_kernel<T, NP, WPB, SQ, NM>), grid, dim3(64 * WPB), shm, s, L,
+                       D.Fx, D.Fy, D.Vhat, x, y, active, zpb);
     MSX_LAUNCH_CHECK();
+}
+
+template <typename T, int NP, int WPB>
+static void launch_dft_np(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
+                          const int *active, hipStream_t s, int active_hint) {
+    const bool sq = L.nkx == L.nky;
+    if constexpr (sizeof(T) == 4) {
+        if (L.nm) {
+            if (sq) launch_dft_impl<T, NP, WPB, true, true>(L, D, x, y, active, s, active_hint);
+            else launch_dft_impl<T, NP, WPB, false, true>(L, D, x, y, active, s, active_hint);
+            return;
+        }
+    } else {
+        if (L.nm) throw Error(3, "DFT coupling: node-major vectors are FP32 only");
+    }
+    if (sq) launch_dft_impl<T, NP, WPB, true, false>(L, D, x, y, active, s, active_hint);
+    else launch_dft_impl<T, NP, WPB, false, false>(L, D, x, y, active, s, active_hint);
 }
 
 template <typename T>
 void launch_dft_coupling(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
                          const int *active, hipStream_t s, int active_hint) {
     constexpr int WPB = sizeof(T) == 4 ? 4 : 2;  // LDS per wavefront: 12 KB FP32, 24 KB FP64
-    if (D.np == 16) launch_dft_impl<T, 16, WPB>(L, D, x, y, active, s, active_hint);
-    else if (D.np == 32) launch_dft_impl<T, 32, WPB>(L, D, x, y, active, s, active_hint);
+    if (D.np == 16) launch_dft_np<T, 16, WPB>(L, D, x, y, active, s, active_hint);
+    else if (D.np == 32) launch_dft_np<T, 32, WPB>(L, D, x, y, active, s, active_hint);
     else throw Error(3, "DFT coupling supports tables up to 32 x 32");
 }
 template void launch_dft_coupling<float>(const Layout &, const DftTables<float> &,

@@ -237,6 +237,7 @@ static void launch_diag_gemm(const Layout &L, const EigTables<T> &E, const doubl
     constexpr int CH = NR == 2 ? 16 : 8;
     if (E.nzp % 16 != 0 || E.nzp > 16 * NT || E.nzp < L.nz)
         throw Error(3, "diagonal blocks: padded size does not fit the kernel");
+    if (L.nm) throw Error(3, "diagonal blocks (global-memory Q): channel-major vectors only");
     dim3 grid((L.max_nc + CH * DG_WAVES - 1) / (CH * DG_WAVES), L.n_inst);
     hipLaunchKernelGGL((diag_gemm_kernel<T, NR, NT>), grid, dim3(64 * DG_WAVES), 0, s, L, E, e,
                        c, fac, x, base, y, sign, inverse, active);
@@ -253,11 +254,13 @@ static void launch_diag_gemm(const Layout &L, const EigTables<T> &E, const doubl
 // conflict-free. Work balance: every workgroup scans the active instances' tile counts into
 // LDS, and the wavefronts stride over the compact list of active tiles (a binary search maps
 // a tile to its instance), so each gets the same number of tiles to within one.
+// NM: node-major vectors (Layout::nm), where the 16 channels of a tile at one node are one
+// 128-byte line; the factors stay channel-major.
 constexpr int DL_WAVES = 8;
 constexpr int DL_STRIDE = 164;
 constexpr int DL_MAX_INST = 4096;  // instances the in-LDS tile scan covers
 
-template <int NT>
+template <int NT, bool NM>
 __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
     Layout L, EigTables<float> E, const double *__restrict__ e, const cx<float> *__restrict__ c,
     const float *__restrict__ fac, const cx<float> *x, const cx<float> *base, cx<float> *y,
@@ -322,13 +325,19 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
         const int jc = j0 + l16;
         const bool live = jc < nc;
         const long long cidx = L.off[b] + (live ? jc : 0);
-        const long long row0 = cidx * nz;
+        const long long row0 = cidx * nz;  // channel-major row (factors, and vectors unless NM)
+        const long long vb = L.off[b] * nz + (live ? jc : 0);
+        // vector element of this lane's channel at node z
+        auto vi = [&](int z) -> long long {
+            if constexpr (NM) return vb + (long long)z * nc;
+            else return row0 + z;
+        };
         // per-channel scalars of the later phases, loaded now so that the first product
         // covers their latency
         const double ej = e[cidx];
         const cx<float> cj = c[cidx];
         const float facN = inverse ? fac[row0 + nz - 1] : 0.0f;
-        const cx<float> xN = inverse ? mk(0.0f, 0.0f) : x[row0 + nz - 1];
+        const cx<float> xN = inverse ? mk(0.0f, 0.0f) : x[vi(nz - 1)];
 
         // ---- W^T = Q^T X^T, X prefetched two stages ahead ----
         acc_t acc1[NT][2];
@@ -340,7 +349,7 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
             for (int cc = 0; cc < 4; ++cc) {
                 const int z = 16 * s + 4 * g + cc;
                 const bool ok = live && z < nz;
-                const cx<float> v = x[row0 + (ok ? z : 0)];
+                const cx<float> v = x[vi(ok ? z : 0)];
                 q[cc] = ok ? v : mk(0.0f, 0.0f);
             }
         };
@@ -414,9 +423,8 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
                 for (int v = 0; v < 4; ++v) {
                     const int z = 16 * (zt0 + zg) + 4 * g + v;
                     const bool ok = live && zt0 + zg < NTL && z < nz;
-                    const long long idx = row0 + (ok ? z : 0);
-                    fz[zg][v] = inverse ? fac[idx] : 0.0f;
-                    bz[zg][v] = base ? base[idx] : mk(0.0f, 0.0f);
+                    fz[zg][v] = inverse ? fac[row0 + (ok ? z : 0)] : 0.0f;
+                    bz[zg][v] = base ? base[vi(ok ? z : 0)] : mk(0.0f, 0.0f);
                 }
 #pragma unroll
             for (int mt = 0; mt < NT; ++mt) {
@@ -442,18 +450,17 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
                 for (int v = 0; v < 4; ++v) {
                     const int z = 16 * (zt0 + zg) + 4 * g + v;
                     if (z >= nz) continue;
-                    const long long idx = row0 + z;
                     cx<float> val = mk(acc2[zg][0][v], acc2[zg][1][v]);
                     if (inverse) val = val + corr * fz[zg][v];
                     else if (z == nz - 1) val = val - corr;
-                    y[idx] = base ? bz[zg][v] + val * sign : val;
+                    y[vi(z)] = base ? bz[zg][v] + val * sign : val;
                 }
             }
         }
     }
 }
 
-template <int NT>
+template <int NT, bool NM>
 static void launch_diag_lds(const Layout &L, const EigTables<float> &E, const double *e,
                             const cx<float> *c, const float *fac, const cx<float> *x,
                             const cx<float> *base, cx<float> *y, float sign, int inverse,
@@ -461,7 +468,7 @@ static void launch_diag_lds(const Layout &L, const EigTables<float> &E, const do
     if (E.nzp % 16 != 0 || E.nzp > 16 * NT || E.nzp < L.nz)
         throw Error(3, "diagonal blocks: padded size does not fit the kernel");
     const size_t shm = sizeof(double) * 16 * NT + sizeof(float) * (size_t)E.nzp * DL_STRIDE;
-    set_dynamic_lds((const void *)diag_lds_kernel<NT>, shm);
+    set_dynamic_lds((const void *)diag_lds_kernel<NT, NM>, shm);
     static int n_cu = 0;
     if (n_cu == 0) {
         int dev = 0;
@@ -471,7 +478,7 @@ static void launch_diag_lds(const Layout &L, const EigTables<float> &E, const do
     if (L.n_inst > DL_MAX_INST) throw Error(3, "diagonal blocks: too many instances for one launch");
     const long long tiles_max = (long long)L.n_inst * ((L.max_nc + 15) / 16);
     const int grid = (int)std::min<long long>(n_cu, (tiles_max + DL_WAVES - 1) / DL_WAVES);
-    hipLaunchKernelGGL((diag_lds_kernel<NT>), dim3(std::max(grid, 1)), dim3(64 * DL_WAVES), shm, s,
+    hipLaunchKernelGGL((diag_lds_kernel<NT, NM>), dim3(std::max(grid, 1)), dim3(64 * DL_WAVES), shm, s,
                        L, E, e, c, fac, x, base, y, sign, inverse, active, 0);
     MSX_LAUNCH_CHECK();
 }
@@ -557,8 +564,13 @@ void launch_diag_blocks(const Layout &L, const EigTables<T> &E, const double *e,
     }
     if constexpr (sizeof(T) == 4) {
         // Q^T resident in LDS up to nzp = 160
-        if (nt <= 7) return launch_diag_lds<7>(L, E, e, c, fac, x, base, y, sign, inverse, active, s);
-        if (nt <= 10) return launch_diag_lds<10>(L, E, e, c, fac, x, base, y, sign, inverse, active, s);
+#define MSX_DIAG_LDS(NT_)                                                                         \
+    if (nt <= NT_) {                                                                              \
+        if (L.nm) return launch_diag_lds<NT_, true>(L, E, e, c, fac, x, base, y, sign, inverse, active, s); \
+        return launch_diag_lds<NT_, false>(L, E, e, c, fac, x, base, y, sign, inverse, active, s); \
+    }
+        MSX_DIAG_LDS(7) MSX_DIAG_LDS(10)
+#undef MSX_DIAG_LDS
     } else {
         // 16 channels per wavefront up to nzp = 160: each Q fragment feeds two products
         MSX_DIAG(2, 4) MSX_DIAG(2, 7) MSX_DIAG(2, 8) MSX_DIAG(2, 10)

@@ -150,7 +150,18 @@ struct Layout {
     const long long *off;  // [n_inst] channel offset of each instance
     const int *chx, *chy;  // [total channels]
     long long total = 0;   // total channels of the batch (0: not given, at most n_inst * max_nc)
+    // Order of the state vectors inside each instance's block [off*nz, (off+nc)*nz):
+    // 0 channel-major (element (j, z) at (off + j)*nz + z, the batch layout above), 1
+    // node-major (at off*nz + z*nc + j; the FP32 inner vectors of the mixed solver, so that
+    // the coupling reads and writes one node's channels as whole cache lines).
+    int nm = 0;
 };
+
+// Position of element (channel j, node z) of the instance at channel offset off with nc
+// channels, in either vector order.
+__host__ __device__ inline long long vec_index(const Layout &L, long long off, int nc, int j, int z) {
+    return L.nm ? off * L.nz + (long long)z * nc + j : (off + j) * L.nz + z;
+}
 
 // ---- kernel launchers (msx_kernels.hip) ------------------------------------------------
 // Coupling modes of the per-z channel sum y_j = sum_i V[G_j - G_i] x_i.
@@ -256,6 +267,10 @@ void launch_add_ints(int n, const int *src, int *dst, hipStream_t s);
 
 // Precision conversions of channel-major state vectors.
 void launch_to_float(long long n, const cx<double> *src, cx<float> *dst, hipStream_t s);
+// FP64 channel-major -> FP32 in the vector order of L (a tiled transpose when L.nm); n is the
+// element count of the batch.
+void launch_to_float_vec(const Layout &L, long long n, const cx<double> *src, cx<float> *dst,
+                         hipStream_t s);
 void launch_to_float_real(long long n, const double *src, float *dst, hipStream_t s);
 
 // P_j = |2i b_j psi_j(N) + a0 delta_spec|^2

@@ -87,6 +87,7 @@ template <typename T>
 void launch_coupling(const Layout &L, const cx<T> *V, const cx<T> *x, cx<T> *y, int mode,
                      const int *active, hipStream_t s) {
     constexpr int RJ = 4;
+    if (L.nm) throw Error(3, "direct coupling: channel-major vectors only");
     dim3 grid((L.nz + 63) / 64, (L.max_nc + 4 * RJ - 1) / (4 * RJ), L.n_inst);
     hipLaunchKernelGGL((coupling_kernel<T, RJ>), grid, dim3(256), 0, s, L, V, x, y, mode, active);
     MSX_LAUNCH_CHECK();
@@ -97,7 +98,8 @@ template void launch_coupling<float>(const Layout &, const cx<float> *, const cx
                                      cx<float> *, int, const int *, hipStream_t);
 
 // Full coupling of a state that is nonzero only on each instance's specular channel s:
-// y_j(z) = V[G_j - G_s](z) x_s(z) for j != s, y_s = 0. A wavefront per channel, lanes over z.
+// y_j(z) = V[G_j - G_s](z) x_s(z) for j != s, y_s = 0. A wavefront per channel, lanes over z
+// (channel-major vectors), or lanes over channels (node-major).
 template <typename T>
 __global__ __launch_bounds__(256) void specular_coupling_kernel(Layout L, const cx<T> *__restrict__ V,
                                                                 const int *__restrict__ spec,
@@ -110,6 +112,19 @@ __global__ __launch_bounds__(256) void specular_coupling_kernel(Layout L, const 
     const long long off = L.off[b];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int gxs = L.chx[off + s], gys = L.chy[off + s];
+    if (L.nm) {
+        // node-major vectors: lanes over channels, workgroups over nodes
+        for (int j = threadIdx.x; j < nc; j += blockDim.x) {
+            const int dx = wrap_mod(L.chx[off + j] - gxs, L.nkx);
+            const int dy = wrap_mod(L.chy[off + j] - gys, L.nky);
+            const cx<T> *v = V + ((long long)dx * L.nky + dy) * nz;
+            for (int z = blockIdx.x; z < nz; z += gridDim.x) {
+                const long long row = off * nz + (long long)z * nc;
+                y[row + j] = j == s ? mk<T>(0, 0) : v[z] * x[row + s];
+            }
+        }
+        return;
+    }
     const cx<T> *xs = x + (off + s) * nz;
     for (int j = blockIdx.x * 4 + wave; j < nc; j += gridDim.x * 4) {
         const int dx = wrap_mod(L.chx[off + j] - gxs, L.nkx);
@@ -413,6 +428,7 @@ template <typename T>
 void launch_dinv(const Layout &L, const T *Q, const T *QT, const double *lam, const double *e,
                  const cx<T> *c, const T *fac, const cx<T> *x, const cx<T> *base, cx<T> *y,
                  T sign, const int *active, hipStream_t s) {
+    if (L.nm) throw Error(3, "vector D^-1: channel-major vectors only");
     launch_dinv_impl<T>(L, Q, QT, lam, e, c, fac, x, base, y, sign, 1, active, s);
 }
 template void launch_dinv<double>(const Layout &, const double *, const double *,
@@ -488,6 +504,42 @@ void launch_to_float(long long n, const cx<double> *src, cx<float> *dst, hipStre
 }
 void launch_to_float_real(long long n, const double *src, float *dst, hipStream_t s) {
     hipLaunchKernelGGL(to_float_real_kernel, dim3(ew_blocks(n)), dim3(256), 0, s, n, src, dst);
+    MSX_LAUNCH_CHECK();
+}
+
+// FP64 channel-major -> FP32 node-major, one 32-channel x 32-node tile per workgroup
+// (blockIdx.x: tile, blockIdx.y: instance): reads run along z, writes along j, through LDS.
+constexpr int TT = 32;
+__global__ __launch_bounds__(256) void to_float_nm_kernel(Layout L, const cx<double> *__restrict__ src,
+                                                          cx<float> *__restrict__ dst) {
+    __shared__ cx<float> tile[TT][TT + 1];
+    const int b = blockIdx.y;
+    const int nc = L.nc[b], nz = L.nz;
+    const int tz = (nz + TT - 1) / TT;
+    const int j0 = (blockIdx.x / tz) * TT, z0 = (blockIdx.x % tz) * TT;
+    if (j0 >= nc) return;
+    const long long off = L.off[b];
+    const int tx = threadIdx.x & (TT - 1), ty = threadIdx.x / TT;
+    for (int jj = ty; jj < TT; jj += 256 / TT) {
+        const int j = j0 + jj, z = z0 + tx;
+        if (j < nc && z < nz) {
+            const cx<double> v = src[(off + j) * nz + z];
+            tile[jj][tx] = mk((float)v.re, (float)v.im);
+        }
+    }
+    __syncthreads();
+    for (int zz = ty; zz < TT; zz += 256 / TT) {
+        const int z = z0 + zz, j = j0 + tx;
+        if (j < nc && z < nz) dst[off * nz + (long long)z * nc + j] = tile[tx][zz];
+    }
+}
+
+void launch_to_float_vec(const Layout &L, long long n, const cx<double> *src, cx<float> *dst,
+                         hipStream_t s) {
+    if (!L.nm) return launch_to_float(n, src, dst, s);
+    const unsigned tiles = (unsigned)(((L.max_nc + TT - 1) / TT) * ((L.nz + TT - 1) / TT));
+    hipLaunchKernelGGL(to_float_nm_kernel, dim3(std::max(tiles, 1u), L.n_inst), dim3(256), 0, s, L,
+                       src, dst);
     MSX_LAUNCH_CHECK();
 }
 

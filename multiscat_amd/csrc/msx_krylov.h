@@ -46,6 +46,11 @@ bool k_axpy_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const c
 template <typename T, typename TX>
 void k_update(const KLayout &K, const cx<T> *const *vptr, const int *kk, const cx<double> *y,
               cx<TX> *x, const int *mask, hipStream_t s);
+// k_update for a basis in node-major order (L.nm) and x in the channel-major batch order;
+// kmax: the Krylov dimension that strides y.
+template <typename T>
+void k_update_nm(const Layout &L, int kmax, const cx<T> *const *vptr, const int *kk,
+                 const cx<double> *y, cx<double> *x, hipStream_t s);
 template <typename T>
 void k_scale(const KLayout &K, const cx<T> *src, cx<T> *dst, const double *sv, const int *mask,
              hipStream_t s);

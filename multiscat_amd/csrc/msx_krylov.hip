@@ -9,6 +9,8 @@
 // basis instead of k dependent ones.
 #include "msx_krylov.h"
 
+#include <algorithm>
+
 namespace msx {
 
 constexpr int KR_THREADS = 256;
@@ -220,6 +222,52 @@ __global__ __launch_bounds__(KR_THREADS) void update_kernel(KLayout K, const cx<
             acc.im += (double)v.re * yj.im + (double)v.im * yj.re;
         }
         x[base + i] = mk((TX)acc.re, (TX)acc.im);
+    }
+}
+
+// x += sum_{j<kk_b} v_j y_j with the basis in node-major order (Layout::nm) and x in the
+// channel-major batch order: one 32-channel x 32-node tile per workgroup (blockIdx.x: tile,
+// blockIdx.y: instance). The sums run along channels (the basis vectors' contiguous index),
+// cross over in LDS, and update x along nodes (its contiguous index).
+template <typename T>
+__global__ __launch_bounds__(KR_THREADS) void update_nm_kernel(Layout L, int kmax,
+                                                               const cx<T> *const *vptr,
+                                                               const int *__restrict__ kk,
+                                                               const cx<double> *__restrict__ y,
+                                                               cx<double> *__restrict__ x) {
+    constexpr int TT = 32;
+    __shared__ cx<double> tile[TT][TT + 1];
+    const int b = blockIdx.y;
+    const int kb = kk[b];
+    if (kb <= 0) return;
+    const int nc = L.nc[b], nz = L.nz;
+    const int tz = (nz + TT - 1) / TT;
+    const int j0 = (blockIdx.x / tz) * TT, z0 = (blockIdx.x % tz) * TT;
+    if (j0 >= nc) return;
+    const long long off = L.off[b];
+    const cx<double> *yb = y + (long long)b * (kmax + 1);
+    const int tx = threadIdx.x & (TT - 1), ty = threadIdx.x / TT;
+    for (int zz = ty; zz < TT; zz += KR_THREADS / TT) {
+        const int z = z0 + zz, j = j0 + tx;
+        cx<double> acc = mk(0.0, 0.0);
+        if (z < nz && j < nc) {
+            const long long i = off * nz + (long long)z * nc + j;
+            for (int k = 0; k < kb; ++k) {
+                const cx<T> v = vptr[k][i];
+                const cx<double> yk = yb[k];
+                acc.re += (double)v.re * yk.re - (double)v.im * yk.im;
+                acc.im += (double)v.re * yk.im + (double)v.im * yk.re;
+            }
+        }
+        tile[zz][tx] = acc;
+    }
+    __syncthreads();
+    for (int jj = ty; jj < TT; jj += KR_THREADS / TT) {
+        const int j = j0 + jj, z = z0 + tx;
+        if (j < nc && z < nz) {
+            const long long i = (off + j) * nz + z;
+            x[i] = x[i] + tile[tx][jj];
+        }
     }
 }
 
@@ -460,6 +508,19 @@ bool k_axpy_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const c
     MSX_LAUNCH_CHECK();
     return true;
 }
+
+template <typename T>
+void k_update_nm(const Layout &L, int kmax, const cx<T> *const *vptr, const int *kk,
+                 const cx<double> *y, cx<double> *x, hipStream_t s) {
+    const unsigned tiles = (unsigned)(((L.max_nc + 31) / 32) * ((L.nz + 31) / 32));
+    hipLaunchKernelGGL((update_nm_kernel<T>), dim3(std::max(tiles, 1u), L.n_inst), dim3(KR_THREADS),
+                       0, s, L, kmax, vptr, kk, y, x);
+    MSX_LAUNCH_CHECK();
+}
+template void k_update_nm<float>(const Layout &, int, const cx<float> *const *, const int *,
+                                 const cx<double> *, cx<double> *, hipStream_t);
+template void k_update_nm<double>(const Layout &, int, const cx<double> *const *, const int *,
+                                  const cx<double> *, cx<double> *, hipStream_t);
 
 template <typename T, typename TX>
 void k_update(const KLayout &K, const cx<T> *const *vptr, const int *kk, const cx<double> *y,

@@ -207,10 +207,12 @@ struct Krylov {
 // GMRES(kmax) on op x = rhs for the instances active on entry. x accumulates the Krylov
 // corrections (x += V y each cycle). If x_zero, the first residual is rhs itself (x0 = 0,
 // as in the reference); otherwise r0 = rhs - op(x) (needs TX == T). Restarts continue from
-// the current x until maxiter iterations or convergence.
+// the current x until maxiter iterations or convergence. basis_order: the vector order of the
+// basis when it differs from x's (node-major FP32 basis, channel-major FP64 x).
 template <typename T, typename TX>
 void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_zero,
-           double eps, int window, int maxiter, bool allow_restart, int orth_passes = 2) {
+           double eps, int window, int maxiter, bool allow_restart, int orth_passes = 2,
+           const Layout *basis_order = nullptr) {
     hipStream_t s = kr.s;
     kr.S.eps = eps;
     kr.S.window = window;
@@ -301,7 +303,14 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
         }
         if (prev.slot >= 0) alive = kr.wait(prev);
         k_backsolve(kr.K, kr.S, nullptr, s);
-        k_update<T, TX>(kr.K, kr.vp_d.p, kr.cycle_iters.p, kr.y.p, x, nullptr, s);
+        if (basis_order && basis_order->nm) {
+            if constexpr (std::is_same<TX, double>::value)
+                k_update_nm<T>(*basis_order, kr.kmax, kr.vp_d.p, kr.cycle_iters.p, kr.y.p, x, s);
+            else
+                throw Error(3, "node-major basis: the solution must be FP64");
+        } else {
+            k_update<T, TX>(kr.K, kr.vp_d.p, kr.cycle_iters.p, kr.y.p, x, nullptr, s);
+        }
         first = false;
         x_zero = false;
         if (alive == 0 || done >= maxiter || !allow_restart) break;
@@ -457,6 +466,7 @@ struct Ops {
     msx_batch &B;
     msx_table &t;
     Layout L;
+    Layout L32;  // order of the FP32 vectors (set_fp32_order)
     bool neumann;
     int order = 1;  // Jacobi: degree of the Neumann factor M = sum_{i<=order} (-D^-1 V)^i
     const cx<double> *V;  // coupling table (flipped for the transposed system)
@@ -479,6 +489,17 @@ struct Ops {
             dft = &t.dft;
         }
         direct = force_direct || dft->np == 0;
+        L32 = L;
+    }
+    // FP32 vectors node-major when every FP32 kernel of this configuration reads and writes
+    // that order: the DFT coupling and the diagonal blocks with Q^T in LDS (nzp <= 160)
+    void set_fp32_order() {
+        L32 = L;
+        L32.nm = (!direct && !vector_diag && t.nzp > 0 && t.nzp <= 160) ? 1 : 0;
+    }
+    template <typename T>
+    const Layout &lay() const {
+        return sizeof(T) == 4 ? L32 : L;
     }
     // full off-diagonal coupling y = V x (DFT form on the matrix cores when it applies)
     // ---- algorithmic work of one launch over the instances it processes (profiling) ----
@@ -516,8 +537,8 @@ struct Ops {
         Work fl, by;
         if (profiling()) couple_full_work(act, sizeof(cx<float>), !direct, fl, by);
         ProfScope ps(direct ? "coupling_direct_f32" : "coupling_dft_f32", s, fl, by);
-        if (direct) launch_coupling<float>(L, Vf, in, out, COUPLE_FULL, act, s);
-        else launch_dft_coupling<float>(L, dft->d32(), in, out, act, s, active_hint(act));
+        if (direct) launch_coupling<float>(L32, Vf, in, out, COUPLE_FULL, act, s);
+        else launch_dft_coupling<float>(L32, dft->d32(), in, out, act, s, active_hint(act));
     }
     void upper(const cx<double> *in, cx<double> *out, const int *act) {
         Work fl, by;
@@ -582,11 +603,11 @@ struct Ops {
                                           : (mfma ? "diag_mfma_f64" : "diag_vector_f64");
         ProfScope ps(name, s, fl, by);
         if (mfma)
-            launch_diag_blocks<T>(L, eig<T>(), B.e_d.p, c<T>(), fac<T>(), in, base, out, sign, 1,
-                                  act, s);
+            launch_diag_blocks<T>(lay<T>(), eig<T>(), B.e_d.p, c<T>(), fac<T>(), in, base, out,
+                                  sign, 1, act, s);
         else
-            launch_dinv<T>(L, q<T>(), qt<T>(), t.lam.p, B.e_d.p, c<T>(), fac<T>(), in, base, out,
-                           sign, act, s);
+            launch_dinv<T>(lay<T>(), q<T>(), qt<T>(), t.lam.p, B.e_d.p, c<T>(), fac<T>(), in, base,
+                           out, sign, act, s);
     }
     template <typename T>
     void jac(const cx<T> *in, cx<T> *out, const int *act, cx<T> *u, cx<T> *w1buf) {
@@ -707,6 +728,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     Ops ops(B, o.neumann >= 1, transposed, o.coupling == 1);
     ops.order = std::max(1, o.neumann);
     ops.vector_diag = o.diag == 1;  // auto: matrix cores when nz <= 320
+    ops.set_fp32_order();
     MSX_HIP(hipMemcpyAsync(B.src.p, src_in, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, s));
     // A warm start (forward solves only) keeps x as the initial guess.
     const bool warm = o.warm_start == 1 && !transposed;
@@ -799,14 +821,14 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         ops.snap_src = &outer.snap; ops.count_src = &outer.last_count;
         // inner right-hand side M r in FP32 (M = I - D^-1 V to first degree; on a cold start the
         // first r = D^-1 s lives on the specular channels only, and V r is a single-source product)
-        launch_to_float((long long)n, r64, B.f_tmp2.p, s);
+        launch_to_float_vec(ops.L32, (long long)n, r64, B.f_tmp2.p, s);
         if (!ops.neumann) {
             // no Neumann factor: nothing to couple
         } else if (sweep == 0 && !warm && scatter_source && !transposed) {
             const double fl = 8.0 * (double)B.total_ch * t.nz;
             const double by = 2.0 * (double)B.total_ch * t.nz * sizeof(cx<float>);
             ProfScope ps("coupling_specular_f32", s, fl, by);
-            launch_specular_coupling<float>(ops.L, t.Vf.p, B.spec_d.p, B.f_tmp2.p, B.f_tmp1.p,
+            launch_specular_coupling<float>(ops.L32, t.Vf.p, B.spec_d.p, B.f_tmp2.p, B.f_tmp1.p,
                                             outer.active.p, s);
         } else {
             ops.couple_full(B.f_tmp2.p, B.f_tmp1.p, outer.active.p);
@@ -816,7 +838,8 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         inner.set_all_active(act.data());
         ops.snap_src = &inner.snap; ops.count_src = &inner.last_count;
         gmres<float, double>(inner, op32, B.f_rhs.p, x, true, inner_eps, 1,
-                             std::min(o.maxiter, inner_dim), false, o.inner_orth == 1 ? 1 : 2);
+                             std::min(o.maxiter, inner_dim), false, o.inner_orth == 1 ? 1 : 2,
+                             &ops.L32);
         launch_add_ints(B.n_inst, inner.iters.p, B.iters_acc.p, s);
     }
     {
