@@ -300,9 +300,13 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
         if (threadIdx.x == blockDim.x - 1) tile_start[n_inst] = before + incl;
     }
     for (int m = threadIdx.x; m < 16 * NT; m += blockDim.x) lam[m] = m < nz ? E.lam[m] : 0.0;
-    for (int idx = threadIdx.x; idx < nzp * nzp / 4; idx += blockDim.x) {
-        const int r = idx / (nzp / 4), c4 = idx - r * (nzp / 4);
-        const float4 v = reinterpret_cast<const float4 *>(E.QTp + (long long)r * nzp)[c4];
+    // Q^T rows with the columns past nzp zeroed up to an even number of 16-column blocks
+    // (the first product runs its stages in pairs)
+    const int ncol = 16 * ((NTL + 1) & ~1);
+    for (int idx = threadIdx.x; idx < nzp * ncol / 4; idx += blockDim.x) {
+        const int r = idx / (ncol / 4), c4 = idx - r * (ncol / 4);
+        const float4 v = 4 * c4 < nzp ? reinterpret_cast<const float4 *>(E.QTp + (long long)r * nzp)[c4]
+                                      : make_float4(0.f, 0.f, 0.f, 0.f);
         *reinterpret_cast<float4 *>(qt + r * DL_STRIDE + 4 * c4) = v;
     }
     __syncthreads();
@@ -339,25 +343,23 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
         const float facN = inverse ? fac[row0 + nz - 1] : 0.0f;
         const cx<float> xN = inverse ? mk(0.0f, 0.0f) : x[vi(nz - 1)];
 
-        // ---- W^T = Q^T X^T, X prefetched two stages ahead ----
+        // ---- W^T = Q^T X^T, stages in pairs with the X data of each pair's second stage in
+        // flight during the first: the two register buffers keep fixed names, so each stage
+        // waits only for its own loads ----
         acc_t acc1[NT][2];
 #pragma unroll
         for (int mt = 0; mt < NT; ++mt) acc1[mt][0] = acc1[mt][1] = acc_t{0, 0, 0, 0};
-        cx<float> x0[4], x1[4], x2[4];  // stages s, s + 1, s + 2 in flight
+        // every lane loads from a clamped, valid address and nothing is masked: Q^T is zero
+        // past column nz, so the values loaded for z >= nz add nothing, and a lane past the
+        // instance's last channel only feeds its own output column, which is never stored
         auto load_x = [&](int s, cx<float> (&q)[4]) {
 #pragma unroll
             for (int cc = 0; cc < 4; ++cc) {
                 const int z = 16 * s + 4 * g + cc;
-                const bool ok = live && z < nz;
-                const cx<float> v = x[vi(ok ? z : 0)];
-                q[cc] = ok ? v : mk(0.0f, 0.0f);
+                q[cc] = x[vi(z < nz ? z : 0)];
             }
         };
-        load_x(0, x0);
-        load_x(1, x1);
-#pragma unroll 1
-        for (int s = 0; s < NTL; ++s) {
-            load_x(s + 2, x2);  // past the last block: zeros (z >= nz), unused
+        auto stage = [&](int s, const cx<float> (&q)[4]) {
             const int zq = 16 * s + 4 * g;
             float4 a[NT];
 #pragma unroll
@@ -368,30 +370,46 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
 #pragma unroll
                 for (int mt = 0; mt < NT; ++mt) {
                     const float av = cc == 0 ? a[mt].x : cc == 1 ? a[mt].y : cc == 2 ? a[mt].z : a[mt].w;
-                    acc1[mt][0] = M::op(av, x0[cc].re, acc1[mt][0]);
-                    acc1[mt][1] = M::op(av, x0[cc].im, acc1[mt][1]);
+                    acc1[mt][0] = M::op(av, q[cc].re, acc1[mt][0]);
+                    acc1[mt][1] = M::op(av, q[cc].im, acc1[mt][1]);
                 }
-#pragma unroll
-            for (int cc = 0; cc < 4; ++cc) {
-                x0[cc] = x1[cc];
-                x1[cc] = x2[cc];
-            }
+        };
+        // an unconditional pair body over an even stage count (Q^T is zero in the extra
+        // block): each stage then waits for the four loads of its own buffer only
+        cx<float> xa[4], xb[4];
+        load_x(0, xa);
+        load_x(1, xb);
+#pragma unroll 1
+        for (int s = 0; s < NTL; s += 2) {
+            stage(s, xa);
+            __builtin_amdgcn_sched_barrier(0);
+            load_x(s + 2, xa);  // past the last block: clamped, unused
+            stage(s + 1, xb);
+            __builtin_amdgcn_sched_barrier(0);
+            load_x(s + 3, xb);
         }
 
-        // ---- scaling and y0(N) = Q[N-1][:] W (Q[N-1][m] = Q^T[m][N-1]) ----
+        // ---- scaling and y0(N) = Q[N-1][:] W (Q[N-1][m] = Q^T[m][N-1]); 1 / (e_j + lam_m)
+        // from the FP64 sum by the hardware reciprocal (1 ulp), 0 past nz ----
         float yn0 = 0.f, yn1 = 0.f;
 #pragma unroll
         for (int mt = 0; mt < NT; ++mt) {
+            double lv[4];
+            float qv[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                lv[v] = lam[16 * mt + 4 * g + v];
+                qv[v] = qt[(16 * tile(mt) + 4 * g + v) * DL_STRIDE + nz - 1];
+            }
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int m = 16 * mt + 4 * g + v;
-                const float qv = qt[(16 * tile(mt) + 4 * g + v) * DL_STRIDE + nz - 1];
-                const double d = ej + lam[m];
-                const float sc = m < nz ? (inverse ? 1.0f / (float)d : (float)d) : 0.f;
+                const float df = (float)(ej + lv[v]);
+                const float sc = m < nz ? (inverse ? __builtin_amdgcn_rcpf(df) : df) : 0.f;
                 acc1[mt][0][v] *= sc;
                 acc1[mt][1][v] *= sc;
-                yn0 += qv * acc1[mt][0][v];
-                yn1 += qv * acc1[mt][1][v];
+                yn0 += qv[v] * acc1[mt][0][v];
+                yn1 += qv[v] * acc1[mt][1][v];
             }
         }
         yn0 += __shfl_xor(yn0, 16);
@@ -421,10 +439,11 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
             for (int zg = 0; zg < ZG; ++zg)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
+                    // unpredicated loads at clamped addresses; unused values are never stored
                     const int z = 16 * (zt0 + zg) + 4 * g + v;
-                    const bool ok = live && zt0 + zg < NTL && z < nz;
-                    fz[zg][v] = inverse ? fac[row0 + (ok ? z : 0)] : 0.0f;
-                    bz[zg][v] = base ? base[vi(ok ? z : 0)] : mk(0.0f, 0.0f);
+                    const int zc = zt0 + zg < NTL && z < nz ? z : 0;
+                    fz[zg][v] = inverse ? fac[row0 + zc] : 0.0f;
+                    bz[zg][v] = base ? base[vi(zc)] : mk(0.0f, 0.0f);
                 }
 #pragma unroll
             for (int mt = 0; mt < NT; ++mt) {
