@@ -222,24 +222,25 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
     bool first = true;
     for (;;) {
         cx<T> *v0 = kr.vp[0];
-        if (x_zero && first) {
-            MSX_HIP(hipMemcpyAsync(v0, rhs, sizeof(cx<T>) * kr.total, hipMemcpyDeviceToDevice, s));
-        } else {
+        // the first residual r0: rhs itself for x0 = 0 (scaled straight into v0 below)
+        const cx<T> *r0 = rhs;
+        if (!(x_zero && first)) {
             if constexpr (std::is_same<T, TX>::value) {
                 op(reinterpret_cast<const cx<T> *>(x), kr.w.p, kr.active.p);
                 k_axpby<T>(kr.K, rhs, kr.w.p, T(-1), v0, kr.active.p, s);
+                r0 = v0;
             } else {
                 throw Error(3, "restart of a mixed-precision inner solve");
             }
         }
-        k_dots<T>(kr.K, kr.vp_d.p, 0, v0, 1, kr.partial.p, kr.hcol.p, 0, s);
+        k_dots<T>(kr.K, kr.vp_d.p, 0, r0, 1, kr.partial.p, kr.hcol.p, 0, s);
         // with an initial guess the tolerance stays relative to |rhs| (not to the first
         // residual), so a good guess saves iterations
         const bool ref_rhs = first && !x_zero;
         if (ref_rhs) k_dots<T>(kr.K, kr.vp_d.p, 0, rhs, 1, kr.partial.p, kr.hcol2.p, 0, s);
         k_cycle_start(kr.K, kr.S, kr.hcol.p, first ? 1 : 0, kr.count.p, s,
                       ref_rhs ? kr.hcol2.p : nullptr, kr.next_flags());
-        k_scale<T>(kr.K, v0, v0, kr.beta.p, kr.active.p, s);
+        k_scale<T>(kr.K, r0, v0, kr.beta.p, kr.active.p, s);
         // The host learns whether any instance still iterates one iteration late: iteration k
         // is queued before the count of iteration k - 1 is read, so the GPU never waits for
         // the host. Every kernel skips inactive instances, so an iteration queued after the
@@ -712,8 +713,10 @@ void check_options(const msx_solve_options *o) {
 }
 
 // Solve A psi = src for all instances into `x` (FP64). transposed: A^T with the flipped table.
-// scatter_source: src_in is the scattering source of launch_set_source (b at node N of the
-// specular channels), which lets D^-1 src skip the other channels.
+// scatter_source: the source is the scattering source (b at node N of the specular channels);
+// the block-Jacobi paths then form D^-1 src from the channel tables alone, and src_in (the
+// vector launch_set_source writes) is read only by Gauss-Seidel and may be null otherwise.
+// src_in is only read.
 void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *src_in,
                   cx<double> *x, bool transposed, std::vector<int> &iters,
                   std::vector<int> &info, std::vector<double> &resid, bool scatter_source) {
@@ -729,7 +732,9 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     ops.order = std::max(1, o.neumann);
     ops.vector_diag = o.diag == 1;  // auto: matrix cores when nz <= 320
     ops.set_fp32_order();
-    MSX_HIP(hipMemcpyAsync(B.src.p, src_in, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, s));
+    const cx<double> *src = src_in;
+    if (!src && !(scatter_source && o.method == MSX_METHOD_JACOBI))
+        throw Error(MSX_ERR_INVALID, "solve: no source vector");
     // A warm start (forward solves only) keeps x as the initial guess.
     const bool warm = o.warm_start == 1 && !transposed;
     if (!warm) MSX_HIP(hipMemsetAsync(x, 0, sizeof(cx<double>) * n, s));
@@ -745,10 +750,12 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         kr.set_all_active(nullptr);
         Op<double> op;
         if (o.method == MSX_METHOD_GAUSS_SEIDEL) {
+            // the right-hand side is formed in place (L^-1 src): on a copy, src_in is only read
+            MSX_HIP(hipMemcpyAsync(B.src.p, src, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, s));
             ops.gs_rhs(B.src.p, B.rhs.p);
             op = [&](const cx<double> *in, cx<double> *out, const int *act) { ops.gs(in, out, act); };
         } else {
-            ops.jac_rhs(B.src.p, B.rhs.p, scatter_source);
+            ops.jac_rhs(src, B.rhs.p, scatter_source);
             op = [&](const cx<double> *in, cx<double> *out, const int *act) {
                 ops.jac<double>(in, out, act, B.tmp1.p, B.tmp2.p);
             };
@@ -773,7 +780,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     msx_batch::need(B.f_tmp1, n);
     msx_batch::need(B.f_tmp2, n);
     cx<double> *d0 = B.rhs.p;  // D^-1 s
-    ops.dinv_source(B.src.p, d0, scatter_source);
+    ops.dinv_source(src, d0, scatter_source);
     if (!B.kr_outer) B.kr_outer.reset(new Krylov<double>(B.kstart(), B.klen(), 1, s));
     Krylov<double> &outer = *B.kr_outer;
     outer.set_all_active(nullptr);
@@ -798,10 +805,10 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     cx<double> *r64 = B.tmp3.p;
     for (int sweep = 0;; ++sweep) {
         outer.set_all_active(act.data());
-        if (sweep == 0 && !warm) {
-            MSX_HIP(hipMemcpyAsync(r64, d0, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, s));
-        } else {
-            // r = d0 - (x + D^-1 V x)   (B.src is free: the source was consumed above)
+        // a cold start's first residual is d0 itself
+        const cx<double> *res64 = (sweep == 0 && !warm) ? d0 : r64;
+        if (!(sweep == 0 && !warm)) {
+            // r = d0 - (x + D^-1 V x)   (B.src serves as scratch)
             ops.snap_src = &outer.snap; ops.count_src = &outer.last_count;
             ops.couple_full(x, B.tmp1.p, outer.active.p);
             ops.dinv<double>(B.tmp1.p, x, B.src.p, 1.0, outer.active.p);
@@ -821,7 +828,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         ops.snap_src = &outer.snap; ops.count_src = &outer.last_count;
         // inner right-hand side M r in FP32 (M = I - D^-1 V to first degree; on a cold start the
         // first r = D^-1 s lives on the specular channels only, and V r is a single-source product)
-        launch_to_float_vec(ops.L32, (long long)n, r64, B.f_tmp2.p, s);
+        launch_to_float_vec(ops.L32, (long long)n, res64, B.f_tmp2.p, s);
         if (!ops.neumann) {
             // no Neumann factor: nothing to couple
         } else if (sweep == 0 && !warm && scatter_source && !transposed) {
@@ -1168,10 +1175,15 @@ int msx_batch_solve(msx_batch *B, const msx_solve_options *opt) {
         B->build_factors();  // the table's eigensystem may have been re-prepared
         msx_batch::need(B->src, (size_t)B->n_elem());
         const Layout L = B->layout();
-        msx_batch::need(B->source, (size_t)B->n_elem());
-        launch_set_source(L, B->spec_d.p, B->bsrc_d.p, B->source.p, t.stream);
-        solve_system(*B, *opt, B->source.p, B->psi_d.p, false, B->iters, B->info, B->resid,
-                     true);
+            // the source vector is read only by the Gauss-Seidel right-hand side; block Jacobi
+        // forms D^-1 of the scattering source from the channel tables
+        const cx<double> *src = nullptr;
+        if (opt->method == MSX_METHOD_GAUSS_SEIDEL) {
+            msx_batch::need(B->source, (size_t)B->n_elem());
+            launch_set_source(L, B->spec_d.p, B->bsrc_d.p, B->source.p, t.stream);
+            src = B->source.p;
+        }
+        solve_system(*B, *opt, src, B->psi_d.p, false, B->iters, B->info, B->resid, true);
         launch_intensities(L, B->psi_d.p, B->b_d.p, B->a0_d.p, B->spec_d.p, B->P_d.p, t.stream);
         MSX_HIP(hipStreamSynchronize(t.stream));
         B->solved = true;
