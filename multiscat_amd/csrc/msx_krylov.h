@@ -30,6 +30,7 @@ struct KState {
     int *kconv, *iters, *cycle_iters, *converged;
     double eps;
     int window;
+    const double *eps_inst = nullptr;  // [n_inst] per-instance tolerance, overrides eps if set
 };
 
 template <typename T>

@@ -376,7 +376,8 @@ __global__ void givens_kernel(KLayout K, int k, const cx<double> *__restrict__ h
         const double res = sqrt(norm2(g[k + 1]));
         const double ref = S.beta0[b] > 2.2250738585072014e-308 ? S.beta0[b] : 2.2250738585072014e-308;
         S.resid[b] = res / ref;
-        S.kconv[b] = res < S.eps * ref ? S.kconv[b] + 1 : 0;
+        const double eps = S.eps_inst ? S.eps_inst[b] : S.eps;
+        S.kconv[b] = res < eps * ref ? S.kconv[b] + 1 : 0;
         S.cycle_iters[b] = k + 1;
         S.iters[b] += 1;
         S.hnorm[b] = hn;
@@ -421,7 +422,8 @@ __global__ void cycle_start_kernel(KLayout K, KState S, const cx<double> *__rest
         if (first_cycle) S.beta0[b] = ref_nrm2 ? sqrt(ref_nrm2[slot].re) : beta;
         S.g[slot] = mk(beta, 0.0);
         S.cycle_iters[b] = 0;
-        if (!(beta > 0.0) || (ref_nrm2 && first_cycle && beta < S.eps * S.beta0[b])) {
+        const double eps = S.eps_inst ? S.eps_inst[b] : S.eps;
+        if (!(beta > 0.0) || (ref_nrm2 && first_cycle && beta < eps * S.beta0[b])) {
             K.active[b] = 0;
             S.converged[b] = 1;
             S.resid[b] = S.beta0[b] > 0.0 ? beta / S.beta0[b] : 0.0;

@@ -208,13 +208,15 @@ struct Krylov {
 // corrections (x += V y each cycle). If x_zero, the first residual is rhs itself (x0 = 0,
 // as in the reference); otherwise r0 = rhs - op(x) (needs TX == T). Restarts continue from
 // the current x until maxiter iterations or convergence. basis_order: the vector order of the
-// basis when it differs from x's (node-major FP32 basis, channel-major FP64 x).
+// basis when it differs from x's (node-major FP32 basis, channel-major FP64 x). eps_inst
+// (device, optional): a tolerance per instance instead of eps.
 template <typename T, typename TX>
 void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_zero,
            double eps, int window, int maxiter, bool allow_restart, int orth_passes = 2,
-           const Layout *basis_order = nullptr) {
+           const Layout *basis_order = nullptr, const double *eps_inst = nullptr) {
     hipStream_t s = kr.s;
     kr.S.eps = eps;
+    kr.S.eps_inst = eps_inst;
     kr.S.window = window;
     k_reset(kr.K, kr.S, kr.enabled.p, s);
     kr.ensure(1);
@@ -414,6 +416,7 @@ struct msx_batch {
     std::vector<int> nc, spec;
     std::vector<long long> off;
     DBuf<int> nc_d, chx_d, chy_d, spec_d, chmap_d, iters_acc;
+    DBuf<double> eps_in_d;  // per-instance tolerances of the mixed solver's inner solves
     DBuf<long long> off_d;
     DBuf<double> e_d, fac_d, P_d, dLdP_d;
     DBuf<float> facf_d;
@@ -803,6 +806,10 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         o.inner_eps > 0 ? o.inner_eps : std::min(1e-4, std::max(1e-5, std::sqrt(o.eps) / 3.0));
     const int max_refine = o.max_refine > 0 ? o.max_refine : 6;
     cx<double> *r64 = B.tmp3.p;
+    // per-instance inner tolerances (host copy; uploaded before each inner solve)
+    std::vector<double> eps_in(B.n_inst, inner_eps);
+    msx_batch::need(B.eps_in_d, (size_t)B.n_inst);
+    constexpr double kSweepMargin = 0.03;
     for (int sweep = 0;; ++sweep) {
         outer.set_all_active(act.data());
         // a cold start's first residual is d0 itself
@@ -820,10 +827,18 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
             if (!act[b]) continue;
             const double rel = rhs_n2[b] > 0 ? std::sqrt(r_n2[b] / rhs_n2[b]) : 0.0;
             resid[b] = rel;
+            // the inner solve need only shrink this residual to a small fraction of eps
+            // (relative to |D^-1 s|): never by more than inner_eps, nor by less than a factor
+            // 10. The margin leaves room for the amplification between the inner (Neumann-
+            // preconditioned) and the outer residual: at 1/3 and 1/10 some conditions took a
+            // third sweep, whose nearly empty GMRES costs more than the iterations saved
+            eps_in[b] = rel > 0 ? std::min(0.1, std::max(inner_eps, kSweepMargin * o.eps / rel))
+                                : inner_eps;
             if (rel < o.eps) act[b] = 0;
             else ++n_act;
         }
         if (n_act == 0 || sweep >= max_refine) break;
+        B.eps_in_d.upload(eps_in.data(), (size_t)B.n_inst, s);
         outer.set_all_active(act.data());
         ops.snap_src = &outer.snap; ops.count_src = &outer.last_count;
         // inner right-hand side M r in FP32 (M = I - D^-1 V to first degree; on a cold start the
@@ -846,7 +861,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         ops.snap_src = &inner.snap; ops.count_src = &inner.last_count;
         gmres<float, double>(inner, op32, B.f_rhs.p, x, true, inner_eps, 1,
                              std::min(o.maxiter, inner_dim), false, o.inner_orth == 1 ? 1 : 2,
-                             &ops.L32);
+                             &ops.L32, B.eps_in_d.p);
         launch_add_ints(B.n_inst, inner.iters.p, B.iters_acc.p, s);
     }
     {
