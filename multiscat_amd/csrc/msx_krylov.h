@@ -36,6 +36,8 @@ struct KState {
 template <typename T>
 void k_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<T> *w, int with_norm,
             double *partial, cx<double> *h, int accumulate, hipStream_t s);
+// with_norm: chunk partials of |w|^2 (after the update) in slot kcount, reduced into
+// hnorm_dst[kcount] if hnorm_dst is given
 template <typename T>
 void k_axpy(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<double> *h, cx<T> *w,
             int with_norm, double *partial, cx<double> *hnorm_dst, hipStream_t s);
@@ -58,9 +60,11 @@ void k_scale(const KLayout &K, const cx<T> *src, cx<T> *dst, const double *sv, c
 template <typename T>
 void k_axpby(const KLayout &K, const cx<T> *a, const cx<T> *c, T alpha, cx<T> *dst,
              const int *mask, hipStream_t s);
+// norm_partial (optional): the chunk partials of |w|^2 in slot k + 1 of a partial buffer
+// (k_axpy with_norm and no hnorm_dst), summed here instead of read from hcol[k + 1].
 void k_givens(const KLayout &K, int k, const cx<double> *hcol, const cx<double> *hcol2,
               const KState &S, int *count, hipStream_t s,
-              int *host_flags = nullptr);
+              int *host_flags = nullptr, const double *norm_partial = nullptr);
 void k_backsolve(const KLayout &K, const KState &S, const int *mask, hipStream_t s);
 // Starts a cycle from the residual norms nrm2; on the first cycle the convergence reference
 // beta0 is the residual norm, or sqrt(ref_nrm2) when given (an initial guess: then an

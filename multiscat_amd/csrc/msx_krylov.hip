@@ -349,7 +349,8 @@ __device__ void finish_count(const KLayout &K, int alive, int *red, int *count, 
 // active to *count.
 __global__ void givens_kernel(KLayout K, int k, const cx<double> *__restrict__ hcol,
                               const cx<double> *__restrict__ hcol2, KState S,
-                              int *__restrict__ count, int *host_flags) {
+                              int *__restrict__ count, int *host_flags,
+                              const double *__restrict__ norm_partial) {
     __shared__ int red[KR_THREADS / 64];
     int alive = 0;
     for (int b = threadIdx.x; b < K.n_inst; b += blockDim.x) {
@@ -358,7 +359,26 @@ __global__ void givens_kernel(KLayout K, int k, const cx<double> *__restrict__ h
         cx<double> *H = S.H + (long long)b * (K.kmax + 1) * (K.kmax + 1);  // column-major
 #define HH(i, j) H[(long long)(j) * (K.kmax + 1) + (i)]
         for (int i = 0; i <= k; ++i) HH(i, k) = hcol2 ? hcol[slot + i] + hcol2[slot + i] : hcol[slot + i];
-        const double hn = sqrt(hcol[slot + k + 1].re);
+        // |w|^2: from the chunk partials of slot k + 1 (summed here, in chunk order, as the
+        // reduction kernel would), or already reduced into hcol
+        double n2;
+        if (norm_partial) {
+            // sixteen loads in flight at a time, summed in chunk order
+            const double *p = norm_partial + (slot + k + 1) * K.nchunk * 2;
+            n2 = 0.0;
+            int c = 0;
+            for (; c + 16 <= K.nchunk; c += 16) {
+                double v[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) v[i] = p[2 * (c + i)];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) n2 += v[i];
+            }
+            for (; c < K.nchunk; ++c) n2 += p[2 * c];
+        } else {
+            n2 = hcol[slot + k + 1].re;
+        }
+        const double hn = sqrt(n2);
         HH(k + 1, k) = mk(hn, 0.0);
         cx<double> *cs = S.cs + slot, *sn = S.sn + slot, *g = S.g + slot;
         for (int i = 0; i < k; ++i) {
@@ -485,8 +505,9 @@ void k_axpy(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<dou
     hipLaunchKernelGGL((axpy_kernel<T>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr, kcount, h, w,
                        with_norm, partial);
     MSX_LAUNCH_CHECK();
-    if (with_norm) {
-        // the norm partials sit in slot kcount; reduce only that slot into hnorm_dst
+    if (with_norm && hnorm_dst) {
+        // the norm partials sit in slot kcount; reduce only that slot into hnorm_dst (without
+        // hnorm_dst the consumer sums them itself)
         hipLaunchKernelGGL(reduce_partials_kernel, dim3(K.n_inst), dim3(64), 0, s, K, kcount,
                            kcount + 1, partial, hnorm_dst, 0);
         MSX_LAUNCH_CHECK();
@@ -548,9 +569,10 @@ void k_axpby(const KLayout &K, const cx<T> *a, const cx<T> *c, T alpha, cx<T> *d
 }
 
 void k_givens(const KLayout &K, int k, const cx<double> *hcol, const cx<double> *hcol2,
-              const KState &S, int *count, hipStream_t s, int *host_flags) {
+              const KState &S, int *count, hipStream_t s, int *host_flags,
+              const double *norm_partial) {
     hipLaunchKernelGGL(givens_kernel, dim3(1), dim3(KR_THREADS), 0, s, K, k, hcol, hcol2, S,
-                       count, host_flags);
+                       count, host_flags, norm_partial);
     MSX_LAUNCH_CHECK();
 }
 

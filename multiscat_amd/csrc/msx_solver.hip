@@ -282,14 +282,15 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
                         k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol2.p, 0, s);
                     }
                     k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol2.p, kr.w.p, 1, kr.partial.p,
-                              kr.hcol.p, s);
+                              nullptr, s);
                 } else {
-                    k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol.p, kr.w.p, 1, kr.partial.p, kr.hcol.p,
+                    k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol.p, kr.w.p, 1, kr.partial.p, nullptr,
                               s);
                 }
             }
+            // the Givens kernel sums the norm partials of the last update itself
             k_givens(kr.K, k, kr.hcol.p, orth_passes >= 2 ? kr.hcol2.p : nullptr, kr.S, kr.count.p, s,
-                     kr.next_flags());
+                     kr.next_flags(), kr.partial.p);
             k_scale<T>(kr.K, kr.w.p, kr.vp[k + 1], kr.hnorm.p, kr.active.p, s);
             ++done;
             const typename Krylov<T>::Pending cur = kr.post(true);
