@@ -168,27 +168,30 @@ def test_eigensystem_on_gpu_large(nz):
     np.testing.assert_allclose(Q.T @ H0 @ Q, np.diag(lam), atol=1e-13 * scale)
 
 
-@pytest.mark.parametrize("nk,nc", [(15, 120), (31, 300), (11, 121), (7, 20)])
-def test_dft_coupling_matches_direct(nk, nc):
+@pytest.mark.parametrize("nkx,nky,nc", [(15, 15, 120), (31, 31, 300), (11, 11, 121), (7, 7, 20),
+                                         (15, 9, 100), (31, 20, 400), (9, 13, 60)])
+def test_dft_coupling_matches_direct(nkx, nky, nc):
     """Convolution (DFT on the matrix cores) vs direct sum of the full coupling, and the
-    direct sum vs the oracle, on a random table and state."""
+    direct sum vs the oracle, on a random table and state; square tables take the
+    three-product kernel with one DFT matrix in registers, rectangular ones the four-product
+    kernel with two."""
     import ctypes as C
 
     from multiscat_amd import _lib
 
-    rng = np.random.default_rng(nk * 1000 + nc)
+    rng = np.random.default_rng(nkx * 1000 + nky * 37 + nc)
     nz = 21
-    V = rng.standard_normal((nk, nk, nz)) + 1j * rng.standard_normal((nk, nk, nz))
+    V = rng.standard_normal((nkx, nky, nz)) + 1j * rng.standard_normal((nkx, nky, nz))
     V = np.ascontiguousarray(V)
-    order = rng.permutation(nk * nk)[:nc]
-    chx, chy = (np.ascontiguousarray(a, dtype=np.int32) for a in np.divmod(order, nk))
+    order = rng.permutation(nkx * nky)[:nc]
+    chx, chy = (np.ascontiguousarray(a, dtype=np.int32) for a in np.divmod(order, nky))
     x = np.ascontiguousarray(rng.standard_normal((nc, nz)) + 1j * rng.standard_normal((nc, nz)))
     ref = oracle.apply_coupling(V, x, channels=(chx, chy))
     out = {}
     for prec in (0, 1):
         for alg in (0, 1):
             y = np.zeros_like(x)
-            rc = _lib.lib().msx_debug_apply_coupling(nk, nk, nz, _lib.ptr(V), nc, _lib.ptr(chx),
+            rc = _lib.lib().msx_debug_apply_coupling(nkx, nky, nz, _lib.ptr(V), nc, _lib.ptr(chx),
                                                      _lib.ptr(chy), _lib.ptr(x), _lib.ptr(y), 1,
                                                      prec, alg)
             _lib.check(rc, "msx_debug_apply_coupling")
@@ -199,6 +202,55 @@ def test_dft_coupling_matches_direct(nk, nc):
     np.testing.assert_allclose(out[1, 0], ref, rtol=0, atol=1e-5 * scale)
     np.testing.assert_allclose(out[1, 1], ref, rtol=0, atol=1e-5 * scale)
     del C
+
+
+def _rect_workload(nkx=13, nky=9, nz=40, counts=(30, 64, 90, 45)):
+    """A rectangular cell (a1 = 2.6, a2 = 3.9) with a corrugated table on an nkx x nky grid."""
+    from multiscat_amd import basis as B
+
+    factor = B.mev_to_natural(W.HE4_MASS_KG)
+    a1, a2 = (2.6, 0.0), (0.0, 3.9)
+    z, _ = B.lobatto_rule(nz, W.Z_MIN, W.Z_MAX)
+    ga, gb = B.reciprocal_vectors(a1, a2)
+    mx, my = B.fft_modes(nkx), B.fft_modes(nky)
+    rng = np.random.default_rng(17)
+    V = np.zeros((nkx, nky, nz), dtype=np.complex128)
+    V[0, 0] = W.morse(z, 8.0, 1.2, 0.0)
+    decay = np.exp(-2.4 * z)
+    for ix in range(nkx):
+        for iy in range(nky):
+            if not (mx[ix] > 0 or (mx[ix] == 0 and my[iy] > 0)):
+                continue
+            g = int(mx[ix]) * ga + int(my[iy]) * gb
+            val = complex(rng.normal(0, 0.2), rng.normal(0, 0.2)) * np.exp(-0.16 * float(g @ g)) * decay
+            V[ix, iy] = val
+            V[(-ix) % nkx, (-iy) % nky] = np.conj(val)
+    V *= factor
+    w = W.Workload("rect", V, B.kinetic_matrix(nz, W.Z_MIN, W.Z_MAX), a1, a2, W.Z_MIN, W.Z_MAX)
+    for i, n in enumerate(counts):
+        w.conditions.append(W._condition(V, a1, a2, W.Z_MIN, W.Z_MAX, 15.0 + 9.0 * i,
+                                         np.deg2rad(10.0 * i), np.deg2rad(7.0 * i), n, factor))
+    return w
+
+
+def test_rectangular_table_fp64_and_mixed():
+    """nkx != nky: the FP64 batch (four-product DFT kernel) against the oracle, and the mixed
+    solver (node-major FP32 vectors through the rectangular kernel) within the bar."""
+    w = _rect_workload()
+    assert w.potential.shape[0] != w.potential.shape[1]
+    ref = []
+    for c in w.conditions:
+        psi = oracle.solve(1, 0, w.potential, c.e, c.wave_a, c.wave_b, c.wave_c, w.kinetic,
+                           channels=c.channels, eps=5e-13)
+        ref.append(oracle.intensities(psi, c.wave_a, c.wave_b)[c.channels[0], c.channels[1]])
+    tab = BT.Table(w.potential, w.kinetic)
+    bt = BT.Batch(tab, w.conditions)
+    bt.solve(BT.SolveOptions(method="jacobi", eps=5e-13))
+    assert _maxdiff(bt.intensities(), ref) < 1e-10
+    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11))
+    assert _maxdiff(bt.intensities(), ref) < 1e-6
+    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11, coupling="direct"))
+    assert _maxdiff(bt.intensities(), ref) < 1e-6
 
 
 def test_upper_and_lower_coupling_modes_match_oracle():
