@@ -29,9 +29,12 @@ void build_dft_tables(int nkx, int nky, int nz, const cx<double> *V, DftStore &s
 
 // y = full off-diagonal coupling of x (compact channel layout), every active instance.
 // active_hint: an upper bound of the number of active instances (-1: unknown), used to
-// spread few instances over more workgroups.
+// spread few instances over more workgroups. With x_norm and x_copy: the coupling of
+// x / x_norm[b] (per instance), whose elements are also stored to x_copy as they are scaled
+// (a Krylov vector normalised on its way into the operator).
 template <typename T>
 void launch_dft_coupling(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
-                         const int *active, hipStream_t s, int active_hint = -1);
+                         const int *active, hipStream_t s, int active_hint = -1,
+                         const double *x_norm = nullptr, cx<T> *x_copy = nullptr);
 
 }  // namespace msx

@@ -279,7 +279,8 @@ template <typename T, int NP, int WPB, bool SQ, bool NM>
 __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling_kernel(
     Layout L, const cx<T> *__restrict__ Fx, const cx<T> *__restrict__ Fy,
     const cx<T> *__restrict__ Vhat, const cx<T> *__restrict__ x, cx<T> *__restrict__ y,
-    const int *__restrict__ active, int zpb) {
+    const int *__restrict__ active, int zpb, const double *__restrict__ x_norm,
+    cx<T> *__restrict__ x_copy) {
     using G = DftGeom<NP>;
     using acc_t = typename Mfma<T>::acc;
     constexpr int NT = G::NT, SA = G::SA, SB = G::SB, KK = NP / 4;
@@ -329,6 +330,10 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
         if constexpr (NM) return row_0 + z * nc + (j < nc ? j : 0);
         else return (j < nc ? row_l + k * row_step : row_0) + z;
     };
+    // with x_copy: the input is x / x_norm[b] (a Krylov vector before its normalisation),
+    // scaled where it is scattered and stored to x_copy there (each element is scattered
+    // exactly once; doing it at the load would make the prefetch wait for its data)
+    const T sc = x_copy ? (T)(1.0 / x_norm[b]) : T(1);
     auto load_x = [&](int z) {
 #pragma unroll
         for (int k = 0; k < KC; ++k) xv[k] = x[vidx(k, z)];
@@ -380,8 +385,13 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
             const int j = lane + 64 * k;
             if (j < nc) {
                 const int p = s_pos[j];
-                sr[p] = xv[k].re;
-                si[p] = xv[k].im;
+                cx<T> v = xv[k];
+                if (x_copy) {
+                    v = mk(v.re * sc, v.im * sc);
+                    x_copy[vidx(k, z)] = v;
+                }
+                sr[p] = v.re;
+                si[p] = v.im;
             }
         }
         const bool more = z + WPB < z_end;
@@ -433,7 +443,8 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
 
 template <typename T, int NP, int WPB, bool SQ, bool NM>
 static void launch_dft_impl(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
-                            const int *active, hipStream_t s, int active_hint) {
+                            const int *active, hipStream_t s, int active_hint,
+                            const double *x_norm, cx<T> *x_copy) {
     using G = DftGeom<NP>;
     const size_t shm = sizeof(T) * (size_t)WPB * 2 * G::PLANE;
     set_dynamic_lds((const void *)dft_coupling_kernel<T, NP, WPB, SQ, NM>, shm);
@@ -450,41 +461,45 @@ static void launch_dft_impl(const Layout &L, const DftTables<T> &D, const cx<T> 
     }
     dim3 grid(L.n_inst, (L.nz + zpb - 1) / zpb);
     hipLaunchKernelGGL((dft_coupling_kernel<T, NP, WPB, SQ, NM>), grid, dim3(64 * WPB), shm, s, L,
-                       D.Fx, D.Fy, D.Vhat, x, y, active, zpb);
+                       D.Fx, D.Fy, D.Vhat, x, y, active, zpb, x_norm, x_copy);
     MSX_LAUNCH_CHECK();
 }
 
 template <typename T, int NP, int WPB>
 static void launch_dft_np(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
-                          const int *active, hipStream_t s, int active_hint) {
+                          const int *active, hipStream_t s, int active_hint, const double *x_norm,
+                          cx<T> *x_copy) {
     const bool sq = L.nkx == L.nky;
     if constexpr (sizeof(T) == 4) {
         if (L.nm) {
-            if (sq) launch_dft_impl<T, NP, WPB, true, true>(L, D, x, y, active, s, active_hint);
-            else launch_dft_impl<T, NP, WPB, false, true>(L, D, x, y, active, s, active_hint);
+            if (sq) launch_dft_impl<T, NP, WPB, true, true>(L, D, x, y, active, s, active_hint, x_norm, x_copy);
+            else launch_dft_impl<T, NP, WPB, false, true>(L, D, x, y, active, s, active_hint, x_norm, x_copy);
             return;
         }
     } else {
         if (L.nm) throw Error(3, "DFT coupling: node-major vectors are FP32 only");
     }
-    if (sq) launch_dft_impl<T, NP, WPB, true, false>(L, D, x, y, active, s, active_hint);
-    else launch_dft_impl<T, NP, WPB, false, false>(L, D, x, y, active, s, active_hint);
+    if (sq) launch_dft_impl<T, NP, WPB, true, false>(L, D, x, y, active, s, active_hint, x_norm, x_copy);
+    else launch_dft_impl<T, NP, WPB, false, false>(L, D, x, y, active, s, active_hint, x_norm, x_copy);
 }
 
 template <typename T>
 void launch_dft_coupling(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
-                         const int *active, hipStream_t s, int active_hint) {
+                         const int *active, hipStream_t s, int active_hint, const double *x_norm,
+                         cx<T> *x_copy) {
     constexpr int WPB = sizeof(T) == 4 ? 4 : 2;  // LDS per wavefront: 9 KB FP32, 18 KB FP64
-    if (D.np == 16) launch_dft_np<T, 16, WPB>(L, D, x, y, active, s, active_hint);
-    else if (D.np == 32) launch_dft_np<T, 32, WPB>(L, D, x, y, active, s, active_hint);
+    if ((x_norm == nullptr) != (x_copy == nullptr))
+        throw Error(3, "DFT coupling: x_norm and x_copy go together");
+    if (D.np == 16) launch_dft_np<T, 16, WPB>(L, D, x, y, active, s, active_hint, x_norm, x_copy);
+    else if (D.np == 32) launch_dft_np<T, 32, WPB>(L, D, x, y, active, s, active_hint, x_norm, x_copy);
     else throw Error(3, "DFT coupling supports tables up to 32 x 32");
 }
 template void launch_dft_coupling<float>(const Layout &, const DftTables<float> &,
                                          const cx<float> *, cx<float> *, const int *, hipStream_t,
-                                         int);
+                                         int, const double *, cx<float> *);
 template void launch_dft_coupling<double>(const Layout &, const DftTables<double> &,
                                           const cx<double> *, cx<double> *, const int *,
-                                          hipStream_t, int);
+                                          hipStream_t, int, const double *, cx<double> *);
 
 // ---------------------------------------------------------------------------------------
 // Set-up: DFT matrices (FP64 twiddles, k g reduced mod n) and Vhat(z) = F_x V'(z) F_y / nG.

@@ -31,6 +31,11 @@ void set_last_error(const std::string &msg) { g_last_error = msg; }
 
 template <typename T>
 using Op = std::function<void(const cx<T> *in, cx<T> *out, const int *active)>;
+// The operator applied to in = raw / norm[b] (per instance), also storing that scaled input to
+// `in`: the normalisation of a new Krylov vector folded into the operator's first pass.
+template <typename T>
+using OpScaled = std::function<void(const cx<T> *raw, const double *norm, cx<T> *in, cx<T> *out,
+                                    const int *active)>;
 
 // ---------------------------------------------------------------------------------------
 // Krylov workspace for one precision.
@@ -209,11 +214,14 @@ struct Krylov {
 // as in the reference); otherwise r0 = rhs - op(x) (needs TX == T). Restarts continue from
 // the current x until maxiter iterations or convergence. basis_order: the vector order of the
 // basis when it differs from x's (node-major FP32 basis, channel-major FP64 x). eps_inst
-// (device, optional): a tolerance per instance instead of eps.
+// (device, optional): a tolerance per instance instead of eps. op_scaled (optional): the
+// operator with the normalisation of its input folded in; each new basis vector is then
+// normalised by the next operator call instead of a separate pass.
 template <typename T, typename TX>
 void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_zero,
            double eps, int window, int maxiter, bool allow_restart, int orth_passes = 2,
-           const Layout *basis_order = nullptr, const double *eps_inst = nullptr) {
+           const Layout *basis_order = nullptr, const double *eps_inst = nullptr,
+           const OpScaled<T> *op_scaled = nullptr) {
     hipStream_t s = kr.s;
     kr.S.eps = eps;
     kr.S.eps_inst = eps_inst;
@@ -249,6 +257,7 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
         // last instance converged does nothing.
         int alive = 1;
         typename Krylov<T>::Pending prev = kr.post(true);  // the count after the cycle start
+        bool pending_norm = false;  // vp[k] is still kr.w (unnormalised, norm in kr.hnorm)
         for (int k = 0; k < kr.kmax && done < maxiter; ++k) {
             if (prev.slot >= 0 && k + 2 > (int)kr.vp.size()) {
                 // settle before allocating another block of basis vectors
@@ -257,7 +266,12 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
                 if (alive == 0) break;
             }
             kr.ensure(k + 2);
-            op(kr.vp[k], kr.w.p, kr.active.p);
+            if (pending_norm) {
+                (*op_scaled)(kr.w.p, kr.hnorm.p, kr.vp[k], kr.w.p, kr.active.p);
+                pending_norm = false;
+            } else {
+                op(kr.vp[k], kr.w.p, kr.active.p);
+            }
             {
                 // CGS2: two passes of (k+1 dot products, then w -= V h); HBM traffic of the
                 // vector kernels is 4 (k+1) + 6 vectors of the iterating instances, 3 (k+1) + 6
@@ -291,7 +305,10 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
             // the Givens kernel sums the norm partials of the last update itself
             k_givens(kr.K, k, kr.hcol.p, orth_passes >= 2 ? kr.hcol2.p : nullptr, kr.S, kr.count.p, s,
                      kr.next_flags(), kr.partial.p);
-            k_scale<T>(kr.K, kr.w.p, kr.vp[k + 1], kr.hnorm.p, kr.active.p, s);
+            // v_{k+1} = w / h_{k+1,k}: by the next operator call when it can, else here (a
+            // vector the loop leaves unnormalised is never read: the update uses v_0 .. v_k)
+            if (op_scaled) pending_norm = true;
+            else k_scale<T>(kr.K, kr.w.p, kr.vp[k + 1], kr.hnorm.p, kr.active.p, s);
             ++done;
             const typename Krylov<T>::Pending cur = kr.post(true);
             if (prev.slot >= 0) {
@@ -523,27 +540,32 @@ struct Ops {
     }
     // full coupling: direct sum 8 nc (nc-1) nz flops; DFT form 2 complex 2-D transforms of
     // the grid per node (matrix products, 8 flops per complex MAC) plus the pointwise product
-    void couple_full_work(const int *act, size_t s_cx, bool dft_form, Work &fl, Work &by) {
+    void couple_full_work(const int *act, size_t s_cx, bool dft_form, Work &fl, Work &by,
+                          bool with_copy = false) {
         const double nz = L.nz, nkx = L.nkx, nky = L.nky, nG = nkx * nky;
         if (dft_form)
             fl = over_active(act, [&](int) { return nz * (16.0 * nG * (nkx + nky) + 6.0 * nG); });
         else
             fl = over_active(act, [&](int nc) { return 8.0 * nc * (nc - 1.0) * nz; });
-        by = over_active(act, [&](int nc) { return 2.0 * nc * nz * s_cx; }) + nG * nz * s_cx;
+        const double vecs = with_copy ? 3.0 : 2.0;  // in, out (and the scaled copy of in)
+        by = over_active(act, [&](int nc) { return vecs * nc * nz * s_cx; }) + nG * nz * s_cx;
     }
-    void couple_full(const cx<double> *in, cx<double> *out, const int *act) {
+    // in_norm / in_copy (DFT form only): couple in / in_norm[b] and store that to in_copy
+    void couple_full(const cx<double> *in, cx<double> *out, const int *act,
+                     const double *in_norm = nullptr, cx<double> *in_copy = nullptr) {
         Work fl, by;
-        if (profiling()) couple_full_work(act, sizeof(cx<double>), !direct, fl, by);
+        if (profiling()) couple_full_work(act, sizeof(cx<double>), !direct, fl, by, in_copy != nullptr);
         ProfScope ps(direct ? "coupling_direct_f64" : "coupling_dft_f64", s, fl, by);
         if (direct) launch_coupling<double>(L, V, in, out, COUPLE_FULL, act, s);
-        else launch_dft_coupling<double>(L, dft->d64(), in, out, act, s, active_hint(act));
+        else launch_dft_coupling<double>(L, dft->d64(), in, out, act, s, active_hint(act), in_norm, in_copy);
     }
-    void couple_full(const cx<float> *in, cx<float> *out, const int *act) {
+    void couple_full(const cx<float> *in, cx<float> *out, const int *act,
+                     const double *in_norm = nullptr, cx<float> *in_copy = nullptr) {
         Work fl, by;
-        if (profiling()) couple_full_work(act, sizeof(cx<float>), !direct, fl, by);
+        if (profiling()) couple_full_work(act, sizeof(cx<float>), !direct, fl, by, in_copy != nullptr);
         ProfScope ps(direct ? "coupling_direct_f32" : "coupling_dft_f32", s, fl, by);
         if (direct) launch_coupling<float>(L32, Vf, in, out, COUPLE_FULL, act, s);
-        else launch_dft_coupling<float>(L32, dft->d32(), in, out, act, s, active_hint(act));
+        else launch_dft_coupling<float>(L32, dft->d32(), in, out, act, s, active_hint(act), in_norm, in_copy);
     }
     void upper(const cx<double> *in, cx<double> *out, const int *act) {
         Work fl, by;
@@ -614,11 +636,18 @@ struct Ops {
             launch_dinv<T>(lay<T>(), q<T>(), qt<T>(), t.lam.p, B.e_d.p, c<T>(), fac<T>(), in, base,
                            out, sign, act, s);
     }
+    // raw / raw_norm (DFT form only): in = raw / raw_norm[b] is formed (and stored to in) by
+    // the first coupling; raw may alias out
     template <typename T>
-    void jac(const cx<T> *in, cx<T> *out, const int *act, cx<T> *u, cx<T> *w1buf) {
+    void jac(const cx<T> *in, cx<T> *out, const int *act, cx<T> *u, cx<T> *w1buf,
+             const cx<T> *raw = nullptr, const double *raw_norm = nullptr) {
+        auto first_coupling = [&](const cx<T> *src, cx<T> *dst) {
+            if (raw) couple_full(raw, dst, act, raw_norm, const_cast<cx<T> *>(in));
+            else couple_full(src, dst, act);
+        };
         if (!neumann || order <= 1) {
             cx<T> *w1 = neumann ? w1buf : out;
-            couple_full(in, u, act);
+            first_coupling(in, u);
             dinv<T>(u, in, w1, T(1), act);
             if (neumann) {
                 couple_full(w1, u, act);
@@ -629,7 +658,8 @@ struct Ops {
         // M (I + K) = I - (-K)^(order+1), K = D^-1 V: order + 1 products of K, then one sum
         const cx<T> *t = in;
         for (int i = 0; i < order; ++i) {
-            couple_full(t, u, act);
+            if (i == 0) first_coupling(t, u);
+            else couple_full(t, u, act);
             dinv<T>(u, nullptr, w1buf, T(1), act);
             t = w1buf;
         }
@@ -765,7 +795,14 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
             };
         }
         ops.snap_src = &kr.snap; ops.count_src = &kr.last_count;
-        gmres<double, double>(kr, op, B.rhs.p, x, !warm, o.eps, o.window, o.maxiter, true);
+        // block Jacobi with the DFT coupling: new basis vectors normalised by the operator
+        const OpScaled<double> op_s = [&](const cx<double> *raw, const double *nrm, cx<double> *in,
+                                          cx<double> *out, const int *act) {
+            ops.jac<double>(in, out, act, B.tmp1.p, B.tmp2.p, raw, nrm);
+        };
+        const bool fused = o.method == MSX_METHOD_JACOBI && !ops.direct;
+        gmres<double, double>(kr, op, B.rhs.p, x, !warm, o.eps, o.window, o.maxiter, true, 2,
+                              nullptr, nullptr, fused ? &op_s : nullptr);
         ops.snap_src = nullptr; ops.count_src = nullptr;
         iters = kr.host_ints(kr.iters);
         auto conv = kr.host_ints(kr.converged);
@@ -794,6 +831,10 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     Krylov<float> &inner = *B.kr32;
     Op<float> op32 = [&](const cx<float> *in, cx<float> *out, const int *act) {
         ops.jac<float>(in, out, act, B.f_tmp1.p, B.f_tmp2.p);
+    };
+    const OpScaled<float> op32s = [&](const cx<float> *raw, const double *nrm, cx<float> *in,
+                                      cx<float> *out, const int *act) {
+        ops.jac<float>(in, out, act, B.f_tmp1.p, B.f_tmp2.p, raw, nrm);
     };
     // |D^-1 s|^2 per instance: on a cold start it is also the first residual's
     const std::vector<double> rhs_n2 = outer.norms2(d0);
@@ -862,7 +903,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         ops.snap_src = &inner.snap; ops.count_src = &inner.last_count;
         gmres<float, double>(inner, op32, B.f_rhs.p, x, true, inner_eps, 1,
                              std::min(o.maxiter, inner_dim), false, o.inner_orth == 1 ? 1 : 2,
-                             &ops.L32, B.eps_in_d.p);
+                             &ops.L32, B.eps_in_d.p, ops.direct ? nullptr : &op32s);
         launch_add_ints(B.n_inst, inner.iters.p, B.iters_acc.p, s);
     }
     {
