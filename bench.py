@@ -8,7 +8,7 @@ Workload (one "step" = one batch for one potential table): the hexagonal a = 3.1
 azimuth ~ U(0, 60) deg) each with its own channel count (|K+G|^2 <= 6 k^2, clipped to
 [200, 1200]). A step recomputes the table's eigensystem and DFT tables on the GPU and solves all
 256 conditions with block-Jacobi GMRES in mixed precision (FP32 inner solves on the matrix
-cores, FP64 residual refinement) to eps = 5e-8, then forms P = |S|^2 in FP64. Inputs are
+cores, FP64 residual refinement) to eps = 5e-7, then forms P = |S|^2 in FP64. Inputs are
 resident in HBM before the timed region. The metric's bar |dP| <= 1e-6 is checked on every run
 against an FP64 solve of the same batch to eps 5e-14 done outside the timed region.
 
@@ -111,7 +111,7 @@ def main() -> int:
     nkx, nky, nz = w.potential.shape
     tab = BT.Table(w.potential, w.kinetic)
     bt = BT.Batch(tab, w.conditions)
-    eps = 5e-8        # mixed: |dP| ~ 4e-10 against the tight solve, well inside the bar
+    eps = 5e-7        # mixed: |dP| ~ 1e-8 against the tight solve, 100x inside the bar
     eps_base = 5e-6   # the reference's default precision 1e-5 (nsig 5) for the FP64 baselines
     mixed = BT.SolveOptions(method="jacobi", precision="mixed", eps=eps, diag=args.diag)
 
