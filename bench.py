@@ -111,7 +111,7 @@ def main() -> int:
     nkx, nky, nz = w.potential.shape
     tab = BT.Table(w.potential, w.kinetic)
     bt = BT.Batch(tab, w.conditions)
-    eps = 5e-7        # mixed: |dP| ~ 1e-8 against the tight solve, 100x inside the bar
+    eps = 5e-7        # mixed: |dP| ~ 3e-8 against the tight solve, 30x inside the bar
     eps_base = 5e-6   # the reference's default precision 1e-5 (nsig 5) for the FP64 baselines
     mixed = BT.SolveOptions(method="jacobi", precision="mixed", eps=eps, diag=args.diag)
 

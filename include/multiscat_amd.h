@@ -170,7 +170,7 @@ typedef struct {
     int restart;      /* Krylov dimension before a restart; 0 = maxiter (unrestarted)       */
     int window;       /* reference: 3                                                       */
     double inner_eps; /* mixed precision: relative tolerance of each FP32 inner solve;
-                         0 = sqrt(eps)/3 clamped to [1e-5, 1e-4] (two sweeps reach eps)   */
+                         0 = sqrt(eps)/3 clamped to [1e-5, 2.5e-4] (two sweeps reach eps) */
     int max_refine;   /* mixed precision: FP64 refinement sweeps allowed                    */
     int coupling;     /* full coupling: 0 = DFT form on the matrix cores when the table fits
                          (<= 32 x 32), 1 = direct sum                                      */

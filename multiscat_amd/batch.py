@@ -28,7 +28,7 @@ class SolveOptions:
     maxiter: int = 2000
     restart: int = 0
     window: int = 3
-    inner_eps: float = 0.0  # 0: sqrt(eps)/3 within [1e-5, 1e-4]
+    inner_eps: float = 0.0  # 0: sqrt(eps)/3 within [1e-5, 2.5e-4]
     max_refine: int = 6
     coupling: str = "auto"  # or "direct"
     diag: str = "auto"  # "auto" (fastest), "vector" or "mfma" (matrix cores, nz <= 320)

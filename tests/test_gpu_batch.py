@@ -88,7 +88,7 @@ def test_neumann_degrees(small_batch, degree):
 
 
 def test_mixed_inner_tolerance(small_batch):
-    """The automatic FP32 inner tolerance (sqrt(eps)/3 within [1e-5, 1e-4]) and a fixed one
+    """The automatic FP32 inner tolerance (sqrt(eps)/3 within [1e-5, 2.5e-4]) and a fixed one
     give the same intensities to the outer tolerance; out-of-range values are rejected."""
     _, _, bt, ref = small_batch
     Ps = {}
