@@ -502,6 +502,116 @@ static void launch_diag_lds(const Layout &L, const EigTables<float> &E, const do
     MSX_LAUNCH_CHECK();
 }
 
+// Unpreconditioned residual of the scattering system for the mixed solver's refinement:
+//   rho_j = s_j - D_j x_j - (V x)_j,   D_j x_j = (H0 + e_j) x_j - c_j x_j(N) e_N,
+// s the scattering source (b at node N of each instance's specular channel), V x given (the
+// FP64 coupling), written in FP32 in the vector order of Lo (the inner solver's). H0 x_j is
+// one FP64 product on the matrix cores (D^-1 takes two): every wavefront owns 16 channels
+// (real and imaginary parts as two column tiles) and computes Y^T = H0 X^T as the first
+// product of diag_gemm_kernel does with Q^T, here without the row permutation, so register v
+// of lane group g of tile mt holds row 16 mt + 4 v + g. Every term is FP64 until the sum.
+template <int NT>
+__global__ __launch_bounds__(64 * DG_WAVES, 1) void split_residual_kernel(
+    Layout L, Layout Lo, const double *__restrict__ H0p, int nzp, const double *__restrict__ e,
+    const cx<double> *__restrict__ c, const int *__restrict__ spec,
+    const cx<double> *__restrict__ bsrc, const cx<double> *__restrict__ x,
+    const cx<double> *__restrict__ vx, cx<float> *__restrict__ rho,
+    const int *__restrict__ active) {
+    using M = Mfma<double>;
+    using acc_t = typename M::acc;
+    constexpr int CC = 2, NH = 2;  // k-steps per 16-byte stage, stages per 16-wide K block
+    const int b = blockIdx.y;
+    if (active && !active[b]) return;
+    const int nc = L.nc[b];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int j0 = (blockIdx.x * DG_WAVES + wave) * 16;
+    if (j0 >= nc) return;
+    const int nz = L.nz, NTL = nzp >> 4;
+    const int l16 = lane & 15, g = lane >> 4;
+    const int jc = j0 + l16;
+    const bool live = jc < nc;
+    const long long off = L.off[b];
+    const long long cidx = off + (live ? jc : 0);
+    const long long row0 = cidx * nz;
+    auto tile = [&](int t) { return t < NTL ? t : NTL - 1; };
+
+    acc_t acc[NT][2];
+#pragma unroll
+    for (int mt = 0; mt < NT; ++mt) acc[mt][0] = acc[mt][1] = acc_t{0, 0, 0, 0};
+    double a1[2][NT][CC], x1[2][2][CC];
+    // stage st: K block st / NH, k-steps (st % NH) * CC .. + CC. H0p is zero past column nz,
+    // so X values loaded (at clamped addresses) for z >= nz add nothing; blocks past nzp / 16
+    // repeat the last block's H0 columns, and their X is zeroed.
+    auto load1 = [&](int st, double (&a)[NT][CC], double (&xb)[2][CC]) {
+        const int s = st / NH, h = st - s * NH;
+        const int zb = 16 * s + 4 * g + h * CC;
+        const int zq = 16 * tile(s) + 4 * g + h * CC;
+#pragma unroll
+        for (int cc = 0; cc < CC; ++cc) {
+            const int z = zb + cc;
+            const bool ok = s < NTL;
+            const cx<double> v = x[row0 + (z < nz ? z : 0)];
+            xb[0][cc] = ok ? v.re : 0.0;
+            xb[1][cc] = ok ? v.im : 0.0;
+        }
+#pragma unroll
+        for (int mt = 0; mt < NT; ++mt) load16<CC>(H0p + (long long)(16 * tile(mt) + l16) * nzp + zq, a[mt]);
+    };
+    load1(0, a1[0], x1[0]);
+#pragma unroll
+    for (int st = 0; st < NT * NH; ++st) {
+        const int cur = st & 1;
+        if (st + 1 < NT * NH) load1(st + 1, a1[cur ^ 1], x1[cur ^ 1]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int cc = 0; cc < CC; ++cc)
+#pragma unroll
+            for (int mt = 0; mt < NT; ++mt)
+#pragma unroll
+                for (int r = 0; r < 2; ++r) acc[mt][r] = M::op(a1[cur][mt][cc], x1[cur][r][cc], acc[mt][r]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+
+    if (!live) return;
+    const double ej = e[cidx];
+    const cx<double> cj = c[cidx], xN = x[row0 + nz - 1];
+    const bool is_spec = jc == spec[b];
+    const cx<double> bs = bsrc[b];
+#pragma unroll
+    for (int mt = 0; mt < NT; ++mt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int z = 16 * mt + M::row(g, v);
+            if (mt < NTL && z < nz) {
+                const long long i = row0 + z;
+                const cx<double> xz = x[i];
+                cx<double> dx = mk(acc[mt][0][v] + ej * xz.re, acc[mt][1][v] + ej * xz.im);
+                if (z == nz - 1) dx = dx - cj * xN;
+                cx<double> r = mk(0.0, 0.0) - dx - vx[i];
+                if (is_spec && z == nz - 1) r = r + bs;
+                rho[vec_index(Lo, off, nc, jc, z)] = mk((float)r.re, (float)r.im);
+            }
+        }
+}
+
+bool split_residual_supported(int nzp) { return nzp % 16 == 0 && nzp <= 160; }
+
+void launch_split_residual(const Layout &L, const Layout &Lo, const double *H0p, int nzp,
+                           const double *e, const cx<double> *c, const int *spec,
+                           const cx<double> *bsrc, const cx<double> *x, const cx<double> *vx,
+                           cx<float> *rho, const int *active, hipStream_t s) {
+    if (!split_residual_supported(nzp) || nzp < L.nz)
+        throw Error(3, "split residual: nz <= 160 required");
+    dim3 grid((L.max_nc + 16 * DG_WAVES - 1) / (16 * DG_WAVES), L.n_inst);
+    if (nzp <= 64)
+        hipLaunchKernelGGL((split_residual_kernel<4>), grid, dim3(64 * DG_WAVES), 0, s, L, Lo, H0p,
+                           nzp, e, c, spec, bsrc, x, vx, rho, active);
+    else
+        hipLaunchKernelGGL((split_residual_kernel<10>), grid, dim3(64 * DG_WAVES), 0, s, L, Lo, H0p,
+                           nzp, e, c, spec, bsrc, x, vx, rho, active);
+    MSX_LAUNCH_CHECK();
+}
+
 // Preconditioner factors fac[j][z] = ((H0 + e_j)^-1 e_N)(z) = sum_m Q[z][m] Q[N-1][m] / (e_j + lam_m)
 // for every channel j (multiscat_gmres.f90:255-271), FP64 on the matrix cores: the first
 // product of the diagonal blocks is trivial here (Q^T e_N is the last row of Q), so each

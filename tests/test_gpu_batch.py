@@ -102,6 +102,27 @@ def test_mixed_inner_tolerance(small_batch):
             bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", inner_eps=bad))
 
 
+@pytest.mark.parametrize("coupling,diag", [("auto", "auto"), ("direct", "auto"), ("auto", "vector")])
+def test_mixed_split_residual(small_batch, coupling, diag):
+    """At tolerances >= 1e-9 the refinement residual is D^-1 (s - D x - V x) with D x from H0
+    in FP64 and D^-1 in FP32; at tighter ones d0 - x - D^-1 V x in FP64. Both reach the
+    oracle: the looser to its tolerance, and the two agree where they overlap in accuracy."""
+    w, _, bt, ref = small_batch
+    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=1e-9, coupling=coupling, diag=diag))
+    P_split = bt.intensities()
+    assert _maxdiff(P_split, ref) < 1e-7
+    for c, P in zip(w.conditions, P_split):
+        open_ = c.e[c.channels[0], c.channels[1]] < 0
+        assert abs(P[open_].sum() - 1.0) < 1e-7
+    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-10, coupling=coupling, diag=diag))
+    assert _maxdiff(bt.intensities(), P_split) < 1e-7
+    # warm start through the split residual: the exact state needs no iteration
+    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=1e-9, coupling=coupling,
+                             diag=diag, warm_start=True))
+    assert (bt.stats()["iterations"] == 0).all()
+    assert _maxdiff(bt.intensities(), P_split) < 1e-9
+
+
 def test_jacobi_large_nz_uses_vector_blocks():
     """nz > 320 takes the vector-ALU diagonal-block kernel instead of the MFMA one."""
     w = W.hex_batch(n_conditions=2, nz=336, nk=7, cond_seed=8, channel_counts=[9, 14])
@@ -247,10 +268,11 @@ def test_rectangular_table_fp64_and_mixed():
     bt = BT.Batch(tab, w.conditions)
     bt.solve(BT.SolveOptions(method="jacobi", eps=5e-13))
     assert _maxdiff(bt.intensities(), ref) < 1e-10
-    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11))
-    assert _maxdiff(bt.intensities(), ref) < 1e-6
-    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11, coupling="direct"))
-    assert _maxdiff(bt.intensities(), ref) < 1e-6
+    for eps in (5e-11, 1e-8):  # FP64 and split refinement residuals
+        bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=eps))
+        assert _maxdiff(bt.intensities(), ref) < 1e-6
+        bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=eps, coupling="direct"))
+        assert _maxdiff(bt.intensities(), ref) < 1e-6
 
 
 def test_upper_and_lower_coupling_modes_match_oracle():
