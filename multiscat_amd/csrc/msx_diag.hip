@@ -254,17 +254,24 @@ static void launch_diag_gemm(const Layout &L, const EigTables<T> &E, const doubl
 // conflict-free. Work balance: every workgroup scans the active instances' tile counts into
 // LDS, and the wavefronts stride over the compact list of active tiles (a binary search maps
 // a tile to its instance), so each gets the same number of tiles to within one.
+// The Sherman-Morrison column D0^-1 e_N = Q (s * q_N) (s_m = 1 / (e_j + lam_m), q_N the last
+// row of Q, in LDS) is not read from memory: the correction is added to W before the second
+// product, Y = Q (s * (Q^T X) + corr s * q_N), with f_N = sum_m s_m q_N[m]^2 summed in
+// registers beside y0(N). The only memory traffic is then x, base and y.
 // NM: node-major vectors (Layout::nm), where the 16 channels of a tile at one node are one
-// 128-byte line; the factors stay channel-major.
+// 128-byte line. INV (D^-1 or D) and BASE (y = base + sign * D^-1 x or y = D^-1 x) are
+// template parameters so that every load and store is straight-line code: under a runtime
+// condition the loads of the epilogue became branches, and the compiler then waited for all
+// outstanding memory operations (the previous stores included) before each store.
 constexpr int DL_WAVES = 8;
 constexpr int DL_STRIDE = 164;
 constexpr int DL_MAX_INST = 4096;  // instances the in-LDS tile scan covers
 
-template <int NT, bool NM>
+template <int NT, bool NM, bool INV, bool BASE>
 __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
     Layout L, EigTables<float> E, const double *__restrict__ e, const cx<float> *__restrict__ c,
-    const float *__restrict__ fac, const cx<float> *x, const cx<float> *base, cx<float> *y,
-    float sign, int inverse, const int *__restrict__ active, int tiles_per_inst) {
+    const cx<float> *x, const cx<float> *base, cx<float> *y, float sign,
+    const int *__restrict__ active) {
     using M = Mfma<float>;
     using acc_t = M::acc;
     constexpr int ZG = NT % 4 == 0 ? 4 : 2;
@@ -272,6 +279,7 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
     double *lam = dl_smem;                                    // [nzp], 0 past nz
     float *qt = reinterpret_cast<float *>(dl_smem + 16 * NT);  // [nzp][DL_STRIDE]
     __shared__ int tile_start[DL_MAX_INST + 1];               // exclusive scan of active tiles
+    __shared__ int inst_nc[DL_MAX_INST], inst_off[DL_MAX_INST];  // channel counts and offsets
     __shared__ int wave_sum[DL_WAVES];
     const int nzp = E.nzp, nz = L.nz, NTL = nzp >> 4;
     const int n_inst = L.n_inst;
@@ -294,8 +302,11 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
         for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) before += wave_sum[w];
         int run = before + incl - local;
         for (int i = i0; i < i1; ++i) {
+            const int nci = L.nc[i];
             tile_start[i] = run;
-            run += (active && !active[i]) ? 0 : (L.nc[i] + 15) / 16;
+            inst_nc[i] = nci;
+            inst_off[i] = (int)L.off[i];
+            run += (active && !active[i]) ? 0 : (nci + 15) / 16;
         }
         if (threadIdx.x == blockDim.x - 1) tile_start[n_inst] = before + incl;
     }
@@ -324,13 +335,13 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
         }
         const int b = lo;
         const int j0 = (t - tile_start[b]) * 16;
-        const int nc = L.nc[b];
+        const int nc = inst_nc[b];
         if (j0 >= nc) continue;  // never taken; also keeps the register allocation spill-free
         const int jc = j0 + l16;
         const bool live = jc < nc;
-        const long long cidx = L.off[b] + (live ? jc : 0);
-        const long long row0 = cidx * nz;  // channel-major row (factors, and vectors unless NM)
-        const long long vb = L.off[b] * nz + (live ? jc : 0);
+        const long long cidx = inst_off[b] + (live ? jc : 0);
+        const long long row0 = cidx * nz;  // channel-major row (vectors unless NM)
+        const long long vb = (long long)inst_off[b] * nz + (live ? jc : 0);
         // vector element of this lane's channel at node z
         auto vi = [&](int z) -> long long {
             if constexpr (NM) return vb + (long long)z * nc;
@@ -340,8 +351,7 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
         // covers their latency
         const double ej = e[cidx];
         const cx<float> cj = c[cidx];
-        const float facN = inverse ? fac[row0 + nz - 1] : 0.0f;
-        const cx<float> xN = inverse ? mk(0.0f, 0.0f) : x[vi(nz - 1)];
+        const cx<float> xN = INV ? mk(0.0f, 0.0f) : x[vi(nz - 1)];
 
         // ---- W^T = Q^T X^T, stages in pairs with the X data of each pair's second stage in
         // flight during the first: the two register buffers keep fixed names, so each stage
@@ -375,23 +385,34 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
                 }
         };
         // an unconditional pair body over an even stage count (Q^T is zero in the extra
-        // block): each stage then waits for the four loads of its own buffer only
+        // block): each stage then waits for the four loads of its own buffer only. NM: the
+        // last pair is peeled, so that no loads are in flight when the scaling waits for e_j
+        // and c_j (the channel-major forms keep the loop whole: peeled, they spill).
         cx<float> xa[4], xb[4];
         load_x(0, xa);
         load_x(1, xb);
+        int s = 0;
 #pragma unroll 1
-        for (int s = 0; s < NTL; s += 2) {
+        for (; s + (NM ? 2 : 0) < NTL; s += 2) {
             stage(s, xa);
             __builtin_amdgcn_sched_barrier(0);
-            load_x(s + 2, xa);  // past the last block: clamped, unused
+            load_x(s + 2, xa);  // past the last block (channel-major, odd NTL): clamped, unused
             stage(s + 1, xb);
             __builtin_amdgcn_sched_barrier(0);
             load_x(s + 3, xb);
         }
+        if constexpr (NM) {
+            stage(s, xa);
+            stage(s + 1, xb);
+        }
 
         // ---- scaling and y0(N) = Q[N-1][:] W (Q[N-1][m] = Q^T[m][N-1]); 1 / (e_j + lam_m)
         // from the FP64 sum by the hardware reciprocal (1 ulp), 0 past nz ----
-        float yn0 = 0.f, yn1 = 0.f;
+        auto scale = [&](int m, double lv) {
+            const float df = (float)(ej + lv);
+            return m < nz ? (INV ? __builtin_amdgcn_rcpf(df) : df) : 0.f;
+        };
+        float yn0 = 0.f, yn1 = 0.f, fn = 0.f;
 #pragma unroll
         for (int mt = 0; mt < NT; ++mt) {
             double lv[4];
@@ -403,48 +424,69 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
             }
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int m = 16 * mt + 4 * g + v;
-                const float df = (float)(ej + lv[v]);
-                const float sc = m < nz ? (inverse ? __builtin_amdgcn_rcpf(df) : df) : 0.f;
+                const float sc = scale(16 * mt + 4 * g + v, lv[v]);
                 acc1[mt][0][v] *= sc;
                 acc1[mt][1][v] *= sc;
                 yn0 += qv[v] * acc1[mt][0][v];
                 yn1 += qv[v] * acc1[mt][1][v];
+                if (INV) fn += qv[v] * qv[v] * sc;
             }
         }
         yn0 += __shfl_xor(yn0, 16);
         yn0 += __shfl_xor(yn0, 32);
         yn1 += __shfl_xor(yn1, 16);
         yn1 += __shfl_xor(yn1, 32);
-        cx<float> corr = mk(0.f, 0.f);
-        if (live) {
-            if (inverse) {
-                const cx<float> den = mk(1.0f, 0.0f) - cj * facN;
-                corr = cdiv(mk(yn0, yn1) * cj, den);
-            } else {
-                corr = cj * xN;
+        cx<float> corr;
+        if constexpr (INV) {
+            fn += __shfl_xor(fn, 16);
+            fn += __shfl_xor(fn, 32);
+            // D^-1 x = D0^-1 x + corr D0^-1 e_N, corr = c y0(N) / (1 - c f_N): folded into W
+            const cx<float> den = mk(1.0f - cj.re * fn, -cj.im * fn);
+            corr = cdiv(mk(yn0, yn1) * cj, den);
+            if (!live) corr = mk(0.f, 0.f);
+#pragma unroll
+            for (int mt = 0; mt < NT; ++mt) {
+                double lv[4];
+                float qv[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    lv[v] = lam[16 * mt + 4 * g + v];
+                    qv[v] = qt[(16 * tile(mt) + 4 * g + v) * DL_STRIDE + nz - 1];
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const float w = scale(16 * mt + 4 * g + v, lv[v]) * qv[v];
+                    acc1[mt][0][v] += corr.re * w;
+                    acc1[mt][1][v] += corr.im * w;
+                }
             }
+        } else {
+            corr = cj * xN;  // D x = D0 x - c x(N) e_N
         }
 
         // ---- Y^T = Q W^T: A[i][k] = Q[16 zt + i][m] = Q^T[m][16 zt + i] ----
+        // base values one z-group ahead, at clamped addresses (unused values are never
+        // stored), issued after the previous group's stores: a whole group of products then
+        // covers both (issued before the stores, they could not reuse the registers of the
+        // values being stored, and the copy at the loop's end waited for them at once)
+        cx<float> bz[ZG][4];
+        auto load_base = [&](int zt) {
+            if constexpr (BASE) {
+#pragma unroll
+                for (int zg = 0; zg < ZG; ++zg)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int z = 16 * (zt + zg) + 4 * g + v;
+                        bz[zg][v] = base[vi(zt + zg < NTL && z < nz ? z : 0)];
+                    }
+            }
+        };
+        load_base(0);
 #pragma unroll 1
         for (int zt0 = 0; zt0 < NTL; zt0 += ZG) {
             acc_t acc2[ZG][2];
 #pragma unroll
             for (int zg = 0; zg < ZG; ++zg) acc2[zg][0] = acc2[zg][1] = acc_t{0, 0, 0, 0};
-            // this group's factors and base values, loaded ahead of its products
-            float fz[ZG][4];
-            cx<float> bz[ZG][4];
-#pragma unroll
-            for (int zg = 0; zg < ZG; ++zg)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    // unpredicated loads at clamped addresses; unused values are never stored
-                    const int z = 16 * (zt0 + zg) + 4 * g + v;
-                    const int zc = zt0 + zg < NTL && z < nz ? z : 0;
-                    fz[zg][v] = inverse ? fac[row0 + zc] : 0.0f;
-                    bz[zg][v] = base ? base[vi(zc)] : mk(0.0f, 0.0f);
-                }
 #pragma unroll
             for (int mt = 0; mt < NT; ++mt) {
                 float a[ZG][4];
@@ -462,32 +504,52 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
                         acc2[zg][1] = M::op(a[zg][v], acc1[mt][1][v], acc2[zg][1]);
                     }
             }
+            // all of the group's results first, then its stores back to back
+            cx<float> out[ZG][4];
 #pragma unroll
-            for (int zg = 0; zg < ZG; ++zg) {
-                if (!live || zt0 + zg >= NTL) continue;
+            for (int zg = 0; zg < ZG; ++zg)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int z = 16 * (zt0 + zg) + 4 * g + v;
-                    if (z >= nz) continue;
                     cx<float> val = mk(acc2[zg][0][v], acc2[zg][1][v]);
-                    if (inverse) val = val + corr * fz[zg][v];
-                    else if (z == nz - 1) val = val - corr;
-                    y[vi(z)] = base ? bz[zg][v] + val * sign : val;
+                    if (!INV && z == nz - 1) val = val - corr;
+                    out[zg][v] = BASE ? bz[zg][v] + val * sign : val;
                 }
+            // a group inside [0, nz) stores under the lane mask alone (one block of stores:
+            // conditions per element split them into blocks, and the waits the compiler
+            // places at the joins held later stores back for earlier ones)
+            if (16 * (zt0 + ZG) <= nz) {
+                if (live) {
+#pragma unroll
+                    for (int zg = 0; zg < ZG; ++zg)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) y[vi(16 * (zt0 + zg) + 4 * g + v)] = out[zg][v];
+                }
+            } else {
+#pragma unroll
+                for (int zg = 0; zg < ZG; ++zg)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int z = 16 * (zt0 + zg) + 4 * g + v;
+                        if (live && zt0 + zg < NTL && z < nz) y[vi(z)] = out[zg][v];
+                    }
             }
+            load_base(zt0 + ZG);  // past the last group: clamped, unused
+            // keeps the loads here: without a possible write after them, the compiler merges
+            // them with the loads before the loop into one load at the point of use
+            asm volatile("" ::: "memory");
         }
     }
 }
 
-template <int NT, bool NM>
+template <int NT, bool NM, bool INV, bool BASE>
 static void launch_diag_lds(const Layout &L, const EigTables<float> &E, const double *e,
-                            const cx<float> *c, const float *fac, const cx<float> *x,
-                            const cx<float> *base, cx<float> *y, float sign, int inverse,
-                            const int *active, hipStream_t s) {
+                            const cx<float> *c, const cx<float> *x, const cx<float> *base,
+                            cx<float> *y, float sign, const int *active, hipStream_t s) {
     if (E.nzp % 16 != 0 || E.nzp > 16 * NT || E.nzp < L.nz)
         throw Error(3, "diagonal blocks: padded size does not fit the kernel");
     const size_t shm = sizeof(double) * 16 * NT + sizeof(float) * (size_t)E.nzp * DL_STRIDE;
-    set_dynamic_lds((const void *)diag_lds_kernel<NT, NM>, shm);
+    set_dynamic_lds((const void *)diag_lds_kernel<NT, NM, INV, BASE>, shm);
     static int n_cu = 0;
     if (n_cu == 0) {
         int dev = 0;
@@ -495,11 +557,33 @@ static void launch_diag_lds(const Layout &L, const EigTables<float> &E, const do
         MSX_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
     }
     if (L.n_inst > DL_MAX_INST) throw Error(3, "diagonal blocks: too many instances for one launch");
+    if ((L.total > 0 ? L.total : (long long)L.n_inst * L.max_nc) >= (1LL << 31))
+        throw Error(3, "diagonal blocks: channel offsets past 32 bits");
     const long long tiles_max = (long long)L.n_inst * ((L.max_nc + 15) / 16);
     const int grid = (int)std::min<long long>(n_cu, (tiles_max + DL_WAVES - 1) / DL_WAVES);
-    hipLaunchKernelGGL((diag_lds_kernel<NT, NM>), dim3(std::max(grid, 1)), dim3(64 * DL_WAVES), shm, s,
-                       L, E, e, c, fac, x, base, y, sign, inverse, active, 0);
+    hipLaunchKernelGGL((diag_lds_kernel<NT, NM, INV, BASE>), dim3(std::max(grid, 1)),
+                       dim3(64 * DL_WAVES), shm, s, L, E, e, c, x, base, y, sign, active);
     MSX_LAUNCH_CHECK();
+}
+
+// The runtime choices of the LDS form as template parameters: node-major vectors come from
+// the mixed solver's inner iteration, which applies D^-1 only.
+template <int NT>
+static void launch_diag_lds_any(const Layout &L, const EigTables<float> &E, const double *e,
+                                const cx<float> *c, const cx<float> *x, const cx<float> *base,
+                                cx<float> *y, float sign, int inverse, const int *active,
+                                hipStream_t s) {
+    if (L.nm) {
+        if (!inverse) throw Error(3, "diagonal blocks: node-major vectors with D^-1 only");
+        if (base) return launch_diag_lds<NT, true, true, true>(L, E, e, c, x, base, y, sign, active, s);
+        return launch_diag_lds<NT, true, true, false>(L, E, e, c, x, base, y, sign, active, s);
+    }
+    if (inverse) {
+        if (base) return launch_diag_lds<NT, false, true, true>(L, E, e, c, x, base, y, sign, active, s);
+        return launch_diag_lds<NT, false, true, false>(L, E, e, c, x, base, y, sign, active, s);
+    }
+    if (base) return launch_diag_lds<NT, false, false, true>(L, E, e, c, x, base, y, sign, active, s);
+    return launch_diag_lds<NT, false, false, false>(L, E, e, c, x, base, y, sign, active, s);
 }
 
 // Unpreconditioned residual of the scattering system for the mixed solver's refinement:
@@ -692,14 +776,9 @@ void launch_diag_blocks(const Layout &L, const EigTables<T> &E, const double *e,
         return;                                                                             \
     }
     if constexpr (sizeof(T) == 4) {
-        // Q^T resident in LDS up to nzp = 160
-#define MSX_DIAG_LDS(NT_)                                                                         \
-    if (nt <= NT_) {                                                                              \
-        if (L.nm) return launch_diag_lds<NT_, true>(L, E, e, c, fac, x, base, y, sign, inverse, active, s); \
-        return launch_diag_lds<NT_, false>(L, E, e, c, fac, x, base, y, sign, inverse, active, s); \
-    }
-        MSX_DIAG_LDS(7) MSX_DIAG_LDS(10)
-#undef MSX_DIAG_LDS
+        // Q^T in LDS up to nzp = 160 (the Sherman-Morrison column formed in the kernel)
+        if (nt <= 7) return launch_diag_lds_any<7>(L, E, e, c, x, base, y, sign, inverse, active, s);
+        if (nt <= 10) return launch_diag_lds_any<10>(L, E, e, c, x, base, y, sign, inverse, active, s);
     } else {
         // 16 channels per wavefront up to nzp = 160: each Q fragment feeds two products
         MSX_DIAG(2, 4) MSX_DIAG(2, 7) MSX_DIAG(2, 8) MSX_DIAG(2, 10)
