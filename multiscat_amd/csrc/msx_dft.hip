@@ -373,6 +373,9 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
     const int z_first = blockIdx.y * zpb + wave;
     const int z_end = min(nz, (blockIdx.y + 1) * zpb);
     if (z_first < z_end) load_x(z_first);
+    // the first node's data waited for here: with loads pending on entry, the loop head
+    // waited for every outstanding operation, the previous node's stores included
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     for (int z = z_first; z < z_end; z += WPB) {
         {
             acc_t *p4 = reinterpret_cast<acc_t *>(sr);  // both planes, 16-byte stores
@@ -402,8 +405,10 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
         __builtin_amdgcn_wave_barrier();
         store_tiles<T, NP>(sr, si, SA, cr, ci);
         __builtin_amdgcn_wave_barrier();
-        // (2) Z = Y F_y, then Z .* Vhat(z) (the slab is loaded just before this product)
+        // (2) Z = Y F_y, then Z .* Vhat(z) (the slab is loaded just before this product; the
+        // barrier keeps the scheduler from moving the loads down to their use after it)
         load_v(z);
+        __builtin_amdgcn_sched_barrier(0);
         right(plain_t{});
 #pragma unroll
         for (int ti = 0; ti < NT; ++ti)

@@ -254,12 +254,12 @@ static void launch_diag_gemm(const Layout &L, const EigTables<T> &E, const doubl
 // conflict-free. Work balance: every workgroup scans the active instances' tile counts into
 // LDS, and the wavefronts stride over the compact list of active tiles (a binary search maps
 // a tile to its instance), so each gets the same number of tiles to within one.
-// The Sherman-Morrison column D0^-1 e_N = Q (s * q_N) (s_m = 1 / (e_j + lam_m), q_N the last
-// row of Q, in LDS) is not read from memory: the correction is added to W before the second
-// product, Y = Q (s * (Q^T X) + corr s * q_N), with f_N = sum_m s_m q_N[m]^2 summed in
-// registers beside y0(N). The only memory traffic is then x, base and y.
-// NM: node-major vectors (Layout::nm), where the 16 channels of a tile at one node are one
-// 128-byte line. INV (D^-1 or D) and BASE (y = base + sign * D^-1 x or y = D^-1 x) are
+// The Sherman-Morrison column f_j = D0^-1 e_N is read from its table (summed in FP64 when
+// the table was built: forming it here from Q and 1 / (e_j + lam_m) in FP32 loses up to
+// 1e-3 of f_j(N) to cancellation when e_j + lam_m nearly vanishes, which cost the mixed
+// solver an extra refinement sweep).
+// NM: node-major vectors and factors (Layout::nm), where the 16 channels of a tile at one
+// node are one 128-byte line (64 bytes of factors). INV (D^-1 or D) and BASE (y = base + sign * D^-1 x or y = D^-1 x) are
 // template parameters so that every load and store is straight-line code: under a runtime
 // condition the loads of the epilogue became branches, and the compiler then waited for all
 // outstanding memory operations (the previous stores included) before each store.
@@ -270,8 +270,8 @@ constexpr int DL_MAX_INST = 4096;  // instances the in-LDS tile scan covers
 template <int NT, bool NM, bool INV, bool BASE>
 __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
     Layout L, EigTables<float> E, const double *__restrict__ e, const cx<float> *__restrict__ c,
-    const cx<float> *x, const cx<float> *base, cx<float> *y, float sign,
-    const int *__restrict__ active) {
+    const float *__restrict__ fac, const cx<float> *x, const cx<float> *base, cx<float> *y,
+    float sign, const int *__restrict__ active) {
     using M = Mfma<float>;
     using acc_t = M::acc;
     constexpr int ZG = NT % 4 == 0 ? 4 : 2;
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
         const int jc = j0 + l16;
         const bool live = jc < nc;
         const long long cidx = inst_off[b] + (live ? jc : 0);
-        const long long row0 = cidx * nz;  // channel-major row (vectors unless NM)
+        const long long row0 = cidx * nz;  // channel-major row (vectors and factors unless NM)
         const long long vb = (long long)inst_off[b] * nz + (live ? jc : 0);
         // vector element of this lane's channel at node z
         auto vi = [&](int z) -> long long {
@@ -351,6 +351,7 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
         // covers their latency
         const double ej = e[cidx];
         const cx<float> cj = c[cidx];
+        const float facN = INV ? fac[vi(nz - 1)] : 0.0f;
         const cx<float> xN = INV ? mk(0.0f, 0.0f) : x[vi(nz - 1)];
 
         // ---- W^T = Q^T X^T, stages in pairs with the X data of each pair's second stage in
@@ -385,34 +386,24 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
                 }
         };
         // an unconditional pair body over an even stage count (Q^T is zero in the extra
-        // block): each stage then waits for the four loads of its own buffer only. NM: the
-        // last pair is peeled, so that no loads are in flight when the scaling waits for e_j
-        // and c_j (the channel-major forms keep the loop whole: peeled, they spill).
+        // block): each stage then waits for the four loads of its own buffer only (peeling
+        // the last pair, whose loads are unused, measured no faster and spills)
         cx<float> xa[4], xb[4];
         load_x(0, xa);
         load_x(1, xb);
-        int s = 0;
 #pragma unroll 1
-        for (; s + (NM ? 2 : 0) < NTL; s += 2) {
+        for (int s = 0; s < NTL; s += 2) {
             stage(s, xa);
             __builtin_amdgcn_sched_barrier(0);
-            load_x(s + 2, xa);  // past the last block (channel-major, odd NTL): clamped, unused
+            load_x(s + 2, xa);  // past the last block: clamped, unused
             stage(s + 1, xb);
             __builtin_amdgcn_sched_barrier(0);
             load_x(s + 3, xb);
         }
-        if constexpr (NM) {
-            stage(s, xa);
-            stage(s + 1, xb);
-        }
 
         // ---- scaling and y0(N) = Q[N-1][:] W (Q[N-1][m] = Q^T[m][N-1]); 1 / (e_j + lam_m)
         // from the FP64 sum by the hardware reciprocal (1 ulp), 0 past nz ----
-        auto scale = [&](int m, double lv) {
-            const float df = (float)(ej + lv);
-            return m < nz ? (INV ? __builtin_amdgcn_rcpf(df) : df) : 0.f;
-        };
-        float yn0 = 0.f, yn1 = 0.f, fn = 0.f;
+        float yn0 = 0.f, yn1 = 0.f;
 #pragma unroll
         for (int mt = 0; mt < NT; ++mt) {
             double lv[4];
@@ -424,64 +415,42 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
             }
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const float sc = scale(16 * mt + 4 * g + v, lv[v]);
+                const int m = 16 * mt + 4 * g + v;
+                const float df = (float)(ej + lv[v]);
+                const float sc = m < nz ? (INV ? __builtin_amdgcn_rcpf(df) : df) : 0.f;
                 acc1[mt][0][v] *= sc;
                 acc1[mt][1][v] *= sc;
                 yn0 += qv[v] * acc1[mt][0][v];
                 yn1 += qv[v] * acc1[mt][1][v];
-                if (INV) fn += qv[v] * qv[v] * sc;
             }
         }
         yn0 += __shfl_xor(yn0, 16);
         yn0 += __shfl_xor(yn0, 32);
         yn1 += __shfl_xor(yn1, 16);
         yn1 += __shfl_xor(yn1, 32);
-        cx<float> corr;
-        if constexpr (INV) {
-            fn += __shfl_xor(fn, 16);
-            fn += __shfl_xor(fn, 32);
-            // D^-1 x = D0^-1 x + corr D0^-1 e_N, corr = c y0(N) / (1 - c f_N): folded into W
-            const cx<float> den = mk(1.0f - cj.re * fn, -cj.im * fn);
-            corr = cdiv(mk(yn0, yn1) * cj, den);
-            if (!live) corr = mk(0.f, 0.f);
-#pragma unroll
-            for (int mt = 0; mt < NT; ++mt) {
-                double lv[4];
-                float qv[4];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    lv[v] = lam[16 * mt + 4 * g + v];
-                    qv[v] = qt[(16 * tile(mt) + 4 * g + v) * DL_STRIDE + nz - 1];
-                }
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const float w = scale(16 * mt + 4 * g + v, lv[v]) * qv[v];
-                    acc1[mt][0][v] += corr.re * w;
-                    acc1[mt][1][v] += corr.im * w;
-                }
-            }
-        } else {
-            corr = cj * xN;  // D x = D0 x - c x(N) e_N
-        }
+        // D^-1 x = D0^-1 x + corr f_j, corr = c y0(N) / (1 - c f_j(N)); D x = D0 x - c x(N) e_N
+        cx<float> corr = mk(0.f, 0.f);
+        if (live) corr = INV ? cdiv(mk(yn0, yn1) * cj, mk(1.0f, 0.0f) - cj * facN) : cj * xN;
 
         // ---- Y^T = Q W^T: A[i][k] = Q[16 zt + i][m] = Q^T[m][16 zt + i] ----
-        // base values one z-group ahead, at clamped addresses (unused values are never
-        // stored), issued after the previous group's stores: a whole group of products then
-        // covers both (issued before the stores, they could not reuse the registers of the
-        // values being stored, and the copy at the loop's end waited for them at once)
+        // factors and base values one z-group ahead, at clamped addresses (unused values are
+        // never stored), issued after the previous group's stores: a whole group of products
+        // then covers both (issued before the stores, they could not reuse the registers of
+        // the values being stored, and the copy at the loop's end waited for them at once)
+        float fz[ZG][4];
         cx<float> bz[ZG][4];
-        auto load_base = [&](int zt) {
-            if constexpr (BASE) {
+        auto load_group = [&](int zt) {
 #pragma unroll
-                for (int zg = 0; zg < ZG; ++zg)
+            for (int zg = 0; zg < ZG; ++zg)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int z = 16 * (zt + zg) + 4 * g + v;
-                        bz[zg][v] = base[vi(zt + zg < NTL && z < nz ? z : 0)];
-                    }
-            }
+                for (int v = 0; v < 4; ++v) {
+                    const int z = 16 * (zt + zg) + 4 * g + v;
+                    const int zc = zt + zg < NTL && z < nz ? z : 0;
+                    if constexpr (INV) fz[zg][v] = fac[vi(zc)];
+                    if constexpr (BASE) bz[zg][v] = base[vi(zc)];
+                }
         };
-        load_base(0);
+        load_group(0);
 #pragma unroll 1
         for (int zt0 = 0; zt0 < NTL; zt0 += ZG) {
             acc_t acc2[ZG][2];
@@ -512,7 +481,8 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
                 for (int v = 0; v < 4; ++v) {
                     const int z = 16 * (zt0 + zg) + 4 * g + v;
                     cx<float> val = mk(acc2[zg][0][v], acc2[zg][1][v]);
-                    if (!INV && z == nz - 1) val = val - corr;
+                    if (INV) val = val + corr * fz[zg][v];
+                    else if (z == nz - 1) val = val - corr;
                     out[zg][v] = BASE ? bz[zg][v] + val * sign : val;
                 }
             // a group inside [0, nz) stores under the lane mask alone (one block of stores:
@@ -534,7 +504,7 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
                         if (live && zt0 + zg < NTL && z < nz) y[vi(z)] = out[zg][v];
                     }
             }
-            load_base(zt0 + ZG);  // past the last group: clamped, unused
+            load_group(zt0 + ZG);  // past the last group: clamped, unused
             // keeps the loads here: without a possible write after them, the compiler merges
             // them with the loads before the loop into one load at the point of use
             asm volatile("" ::: "memory");
@@ -544,8 +514,9 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
 
 template <int NT, bool NM, bool INV, bool BASE>
 static void launch_diag_lds(const Layout &L, const EigTables<float> &E, const double *e,
-                            const cx<float> *c, const cx<float> *x, const cx<float> *base,
-                            cx<float> *y, float sign, const int *active, hipStream_t s) {
+                            const cx<float> *c, const float *fac, const cx<float> *x,
+                            const cx<float> *base, cx<float> *y, float sign, const int *active,
+                            hipStream_t s) {
     if (E.nzp % 16 != 0 || E.nzp > 16 * NT || E.nzp < L.nz)
         throw Error(3, "diagonal blocks: padded size does not fit the kernel");
     const size_t shm = sizeof(double) * 16 * NT + sizeof(float) * (size_t)E.nzp * DL_STRIDE;
@@ -562,7 +533,7 @@ static void launch_diag_lds(const Layout &L, const EigTables<float> &E, const do
     const long long tiles_max = (long long)L.n_inst * ((L.max_nc + 15) / 16);
     const int grid = (int)std::min<long long>(n_cu, (tiles_max + DL_WAVES - 1) / DL_WAVES);
     hipLaunchKernelGGL((diag_lds_kernel<NT, NM, INV, BASE>), dim3(std::max(grid, 1)),
-                       dim3(64 * DL_WAVES), shm, s, L, E, e, c, x, base, y, sign, active);
+                       dim3(64 * DL_WAVES), shm, s, L, E, e, c, fac, x, base, y, sign, active);
     MSX_LAUNCH_CHECK();
 }
 
@@ -570,20 +541,23 @@ static void launch_diag_lds(const Layout &L, const EigTables<float> &E, const do
 // the mixed solver's inner iteration, which applies D^-1 only.
 template <int NT>
 static void launch_diag_lds_any(const Layout &L, const EigTables<float> &E, const double *e,
-                                const cx<float> *c, const cx<float> *x, const cx<float> *base,
-                                cx<float> *y, float sign, int inverse, const int *active,
-                                hipStream_t s) {
+                                const cx<float> *c, const float *fac, const cx<float> *x,
+                                const cx<float> *base, cx<float> *y, float sign, int inverse,
+                                const int *active, hipStream_t s) {
+#define MSX_LDS(NM_, INV_, BASE_) \
+    return launch_diag_lds<NT, NM_, INV_, BASE_>(L, E, e, c, fac, x, base, y, sign, active, s)
     if (L.nm) {
         if (!inverse) throw Error(3, "diagonal blocks: node-major vectors with D^-1 only");
-        if (base) return launch_diag_lds<NT, true, true, true>(L, E, e, c, x, base, y, sign, active, s);
-        return launch_diag_lds<NT, true, true, false>(L, E, e, c, x, base, y, sign, active, s);
+        if (base) MSX_LDS(true, true, true);
+        MSX_LDS(true, true, false);
     }
     if (inverse) {
-        if (base) return launch_diag_lds<NT, false, true, true>(L, E, e, c, x, base, y, sign, active, s);
-        return launch_diag_lds<NT, false, true, false>(L, E, e, c, x, base, y, sign, active, s);
+        if (base) MSX_LDS(false, true, true);
+        MSX_LDS(false, true, false);
     }
-    if (base) return launch_diag_lds<NT, false, false, true>(L, E, e, c, x, base, y, sign, active, s);
-    return launch_diag_lds<NT, false, false, false>(L, E, e, c, x, base, y, sign, active, s);
+    if (base) MSX_LDS(false, false, true);
+    MSX_LDS(false, false, false);
+#undef MSX_LDS
 }
 
 // Unpreconditioned residual of the scattering system for the mixed solver's refinement:
@@ -776,9 +750,9 @@ void launch_diag_blocks(const Layout &L, const EigTables<T> &E, const double *e,
         return;                                                                             \
     }
     if constexpr (sizeof(T) == 4) {
-        // Q^T in LDS up to nzp = 160 (the Sherman-Morrison column formed in the kernel)
-        if (nt <= 7) return launch_diag_lds_any<7>(L, E, e, c, x, base, y, sign, inverse, active, s);
-        if (nt <= 10) return launch_diag_lds_any<10>(L, E, e, c, x, base, y, sign, inverse, active, s);
+        // Q^T in LDS up to nzp = 160
+        if (nt <= 7) return launch_diag_lds_any<7>(L, E, e, c, fac, x, base, y, sign, inverse, active, s);
+        if (nt <= 10) return launch_diag_lds_any<10>(L, E, e, c, fac, x, base, y, sign, inverse, active, s);
     } else {
         // 16 channels per wavefront up to nzp = 160: each Q fragment feeds two products
         MSX_DIAG(2, 4) MSX_DIAG(2, 7) MSX_DIAG(2, 8) MSX_DIAG(2, 10)

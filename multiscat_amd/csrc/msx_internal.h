@@ -279,6 +279,8 @@ void launch_split_residual(const Layout &L, const Layout &Lo, const double *H0p,
 // element count of the batch.
 void launch_to_float_vec(const Layout &L, long long n, const cx<double> *src, cx<float> *dst,
                          hipStream_t s);
+// FP64 channel-major real table (one value per channel and node) -> FP32 node-major.
+void launch_to_float_real_nm(const Layout &L, const double *src, float *dst, hipStream_t s);
 void launch_to_float_real(long long n, const double *src, float *dst, hipStream_t s);
 
 // P_j = |2i b_j psi_j(N) + a0 delta_spec|^2

@@ -509,10 +509,14 @@ void launch_to_float_real(long long n, const double *src, float *dst, hipStream_
 
 // FP64 channel-major -> FP32 node-major, one 32-channel x 32-node tile per workgroup
 // (blockIdx.x: tile, blockIdx.y: instance): reads run along z, writes along j, through LDS.
+// Complex vectors and the real factor table alike.
 constexpr int TT = 32;
-__global__ __launch_bounds__(256) void to_float_nm_kernel(Layout L, const cx<double> *__restrict__ src,
-                                                          cx<float> *__restrict__ dst) {
-    __shared__ cx<float> tile[TT][TT + 1];
+__device__ __forceinline__ cx<float> narrow(const cx<double> &v) { return mk((float)v.re, (float)v.im); }
+__device__ __forceinline__ float narrow(double v) { return (float)v; }
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void to_float_nm_kernel(Layout L, const S *__restrict__ src,
+                                                          D *__restrict__ dst) {
+    __shared__ D tile[TT][TT + 1];
     const int b = blockIdx.y;
     const int nc = L.nc[b], nz = L.nz;
     const int tz = (nz + TT - 1) / TT;
@@ -522,10 +526,7 @@ __global__ __launch_bounds__(256) void to_float_nm_kernel(Layout L, const cx<dou
     const int tx = threadIdx.x & (TT - 1), ty = threadIdx.x / TT;
     for (int jj = ty; jj < TT; jj += 256 / TT) {
         const int j = j0 + jj, z = z0 + tx;
-        if (j < nc && z < nz) {
-            const cx<double> v = src[(off + j) * nz + z];
-            tile[jj][tx] = mk((float)v.re, (float)v.im);
-        }
+        if (j < nc && z < nz) tile[jj][tx] = narrow(src[(off + j) * nz + z]);
     }
     __syncthreads();
     for (int zz = ty; zz < TT; zz += 256 / TT) {
@@ -534,13 +535,22 @@ __global__ __launch_bounds__(256) void to_float_nm_kernel(Layout L, const cx<dou
     }
 }
 
+template <typename S, typename D>
+static void launch_nm(const Layout &L, const S *src, D *dst, hipStream_t s) {
+    const unsigned tiles = (unsigned)(((L.max_nc + TT - 1) / TT) * ((L.nz + TT - 1) / TT));
+    hipLaunchKernelGGL((to_float_nm_kernel<S, D>), dim3(std::max(tiles, 1u), L.n_inst), dim3(256), 0,
+                       s, L, src, dst);
+    MSX_LAUNCH_CHECK();
+}
+
 void launch_to_float_vec(const Layout &L, long long n, const cx<double> *src, cx<float> *dst,
                          hipStream_t s) {
     if (!L.nm) return launch_to_float(n, src, dst, s);
-    const unsigned tiles = (unsigned)(((L.max_nc + TT - 1) / TT) * ((L.nz + TT - 1) / TT));
-    hipLaunchKernelGGL(to_float_nm_kernel, dim3(std::max(tiles, 1u), L.n_inst), dim3(256), 0, s, L,
-                       src, dst);
-    MSX_LAUNCH_CHECK();
+    launch_nm(L, src, dst, s);
+}
+
+void launch_to_float_real_nm(const Layout &L, const double *src, float *dst, hipStream_t s) {
+    launch_nm(L, src, dst, s);
 }
 
 __global__ void intensities_kernel(Layout L, const cx<double> *__restrict__ psi,

@@ -444,7 +444,7 @@ struct msx_batch {
     DBuf<double> eps_in_d;  // per-instance tolerances of the mixed solver's inner solves
     DBuf<long long> off_d;
     DBuf<double> e_d, fac_d, P_d, dLdP_d;
-    DBuf<float> facf_d;
+    DBuf<float> facf_d, facf_nm_d;  // FP32 factors, channel-major and node-major
     DBuf<cx<double>> c_d, b_d, a0_d, bsrc_d, psi_d, lam_d;
     DBuf<cx<float>> cf_d;
     // work vectors and Krylov workspaces (allocated on first use, reused by later solves)
@@ -454,7 +454,7 @@ struct msx_batch {
     std::unique_ptr<Krylov<float>> kr32;
     std::vector<int> iters, info;
     std::vector<double> resid;
-    bool factors_ready = false, solved = false;
+    bool factors_ready = false, facf_nm_ready = false, solved = false;
 
     long long n_elem() const { return total_ch * t->nz; }
     Layout layout() const {
@@ -483,6 +483,7 @@ struct msx_batch {
                                              fac_d.p, t->stream);
         launch_to_float_real(total_ch * t->nz, fac_d.p, facf_d.p, t->stream);
         factors_ready = true;
+        facf_nm_ready = false;
     }
 };
 
@@ -525,6 +526,11 @@ struct Ops {
     void set_fp32_order() {
         L32 = L;
         L32.nm = (!direct && !vector_diag && t.nzp > 0 && t.nzp <= 160) ? 1 : 0;
+        if (L32.nm && !B.facf_nm_ready) {  // the FP32 factors in the same order as the vectors
+            msx_batch::need(B.facf_nm_d, (size_t)B.n_elem());
+            launch_to_float_real_nm(L32, B.fac_d.p, B.facf_nm_d.p, s);
+            B.facf_nm_ready = true;
+        }
     }
     template <typename T>
     const Layout &lay() const {
@@ -723,7 +729,7 @@ template <> EigTables<float> Ops::eig<float>() const { return t.eig32(); }
 template <> const cx<double> *Ops::c<double>() const { return B.c_d.p; }
 template <> const cx<float> *Ops::c<float>() const { return B.cf_d.p; }
 template <> const double *Ops::fac<double>() const { return B.fac_d.p; }
-template <> const float *Ops::fac<float>() const { return B.facf_d.p; }
+template <> const float *Ops::fac<float>() const { return L32.nm ? B.facf_nm_d.p : B.facf_d.p; }
 
 int krylov_dim(const msx_solve_options &o, int n_inst) {
     int m = o.restart > 0 ? o.restart : o.maxiter;
