@@ -524,6 +524,183 @@ __global__ __launch_bounds__(1024) void eig_tred_kernel(int n, double *A, double
     }
 }
 
+// Packed form, for n <= 173: the lower triangle of A packed by rows in LDS for the whole
+// reduction (element (r, c), c <= r, at r (r + 1) / 2 + c: 90 KB at n = 150, where the full
+// matrix would not fit and the first steps ran from global memory). Every wavefront forms the
+// reflector itself from column k (u in registers, lane j holding coordinates j, j + 64,
+// j + 128, and in a private LDS copy that the product reads as broadcasts), and after the
+// partial sums of p = A22 u every wavefront completes p, K and q for all coordinates the same
+// way, so a step has two workgroup barriers. q[r] for a wave-uniform row r comes from the
+// owning lane by readlane. Same formulas as tred_step.
+constexpr int TP_CH = 3;  // coordinate chunks of 64 per lane (n <= 192)
+
+__device__ __forceinline__ double readlane_d(double x, int l) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// Sum over the wavefront by DPP within rows of 16 lanes (quad_perm [1,0,3,2] and [2,3,0,1],
+// half-row mirror, row mirror: every lane of a row then holds the row's sum, each step adding
+// the same two operands in both lanes) and the four row sums read out in a fixed order: the
+// same bits in every lane and every wavefront. A few cycles per step where the shuffle form
+// waits on an LDS permute twice a step.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    v += dpp_d<0x140>(v);
+    return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+// v[r] for a wave-uniform coordinate r of a lane-distributed vector: all three chunks are read
+// (scalar results) and one is selected, since a select among the registers before the
+// readlane becomes an indexed array in scratch memory
+__device__ __forceinline__ double coord(const double (&v)[TP_CH], int r) {
+    const int s = r >> 6, l = r & 63;
+    const double a = readlane_d(v[0], l), b = readlane_d(v[1], l), c = readlane_d(v[2], l);
+    return s == 0 ? a : (s == 1 ? b : c);
+}
+
+__global__ __launch_bounds__(1024) void eig_tred_packed_kernel(int n, double *A, double *h_out,
+                                                               double *d_out, double *e_out,
+                                                               long long *ticks) {
+    extern __shared__ double tl[];
+    // partial sums [TRED_WAVES][64], per-wavefront copies of u [TRED_WAVES][64 TP_CH], triangle
+    double *pp = tl, *Lp = pp + TRED_WAVES * 64 * (1 + TP_CH);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    double *uw = pp + TRED_WAVES * 64 + wave * 64 * TP_CH;
+    if (ticks && tid == 0) ticks[0] = wall_clock64();
+    auto tri = [](int r) { return (r * (r + 1)) >> 1; };
+    for (int r = wave; r < n; r += TRED_WAVES)
+        for (int c = lane; c <= r; c += 64) Lp[tri(r) + c] = A[(long long)r * n + c];
+    __syncthreads();
+    for (int k = 0; k + 2 < n; ++k) {
+        const int m = n - k - 1, b = k + 1;  // block coordinates t = 0 .. m-1: rows b + t
+        // x[t] = A[b + t][k] (column k below the diagonal), u = x - g e_0
+        double u[TP_CH];
+        double loc = 0.0;
+#pragma unroll
+        for (int s = 0; s < TP_CH; ++s) {
+            const int t = lane + 64 * s;
+            u[s] = t < m ? Lp[tri(b + t) + k] : 0.0;
+            if (t >= 1) loc = fma(u[s], u[s], loc);
+        }
+        const double sigma = wave_sum_dpp(loc);
+        const double f = readlane_d(u[0], 0);
+        if (!(sigma > 0.0)) {  // column k is already reduced: no reflector, no update
+            if (tid == 0) {
+                e_out[k] = f;
+                h_out[k] = 0.0;
+            }
+            continue;
+        }
+        const double g = f > 0.0 ? -sqrt(sigma + f * f) : sqrt(sigma + f * f);
+        const double h = sigma + f * f - f * g;  // ||u||^2 / 2
+        const double hinv = 1.0 / h;
+        if (lane == 0) u[0] = f - g;
+#pragma unroll
+        for (int s = 0; s < TP_CH; ++s) uw[lane + 64 * s] = u[s];  // read by this wavefront only
+        if (wave == 0) {  // the reflector into row k of A for the back-transformation
+#pragma unroll
+            for (int s = 0; s < TP_CH; ++s) {
+                const int t = lane + 64 * s;
+                if (t < m) A[(long long)k * n + b + t] = u[s];
+            }
+        }
+        if (tid == 0) {
+            e_out[k] = g;
+            h_out[k] = h;
+        }
+        // p = A22 u: wavefront (cs, rgp) sums the rows of group rgp for the 64 columns of
+        // chunk cs, eight independent row loads at a time. A22[r][c] is L[c][r] for rows above
+        // the chunk (down the packed row of c), L[r][c] below it (along the packed row of r),
+        // and either within it
+        const int CS = (m + 63) >> 6, RGP = TRED_WAVES / CS;
+        const int cs = wave % CS, rgp = wave / CS;
+        if (rgp < RGP) {
+            const int c = 64 * cs + lane, cc = c < m ? c : m - 1;  // past m: clamped, unused
+            const double *colp = Lp + tri(b + cc) + b;  // colp[r] = A22[r][cc] for r <= cc
+            const int rb = (m * rgp) / RGP, re = (m * (rgp + 1)) / RGP;
+            const int c0 = 64 * cs, c1 = c0 + 64;  // rows [c0, c1): mixed
+            double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            auto run = [&](int r0, int r1, auto elem) {
+                int r = r0;
+                for (; r + 8 <= r1; r += 8) {
+                    double v[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) v[j] = elem(r + j);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) acc[j] = fma(v[j], uw[r + j], acc[j]);
+                }
+                for (; r < r1; ++r) acc[0] = fma(elem(r), uw[r], acc[0]);
+            };
+            run(rb, min(re, c0), [&](int r) { return colp[r]; });
+            run(max(rb, c0), min(re, c1),
+                [&](int r) { return r >= cc ? Lp[tri(b + r) + b + cc] : colp[r]; });
+            run(max(rb, c1), re, [&](int r) { return Lp[tri(b + r) + b + cc]; });
+            pp[wave * 64 + lane] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) +
+                                   ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        }
+        __syncthreads();
+        // every wavefront: p, K = u^T p / (2 h^2) and q = p / h - K u for its lanes
+        double q[TP_CH];
+        double kl = 0.0;
+#pragma unroll
+        for (int s = 0; s < TP_CH; ++s) {
+            double pc = 0.0;
+            if (s < CS)
+                for (int g2 = 0; g2 < RGP; ++g2) pc += pp[(g2 * CS + s) * 64 + lane];
+            q[s] = lane + 64 * s < m ? pc : 0.0;
+            kl = fma(u[s], q[s], kl);
+        }
+        const double K = 0.5 * wave_sum_dpp(kl) * hinv * hinv;
+#pragma unroll
+        for (int s = 0; s < TP_CH; ++s) q[s] = q[s] * hinv - K * u[s];
+        // A22 -= u q^T + q u^T on the packed rows (columns 0 .. r of block row r; one stored
+        // copy, so fused), two rows per wavefront at a time, both loaded before either is
+        // stored (four at a time measured slower)
+        for (int r = wave; r < m; r += 2 * TRED_WAVES) {
+            const int r2 = r + TRED_WAVES;
+            double *row = Lp + tri(b + r) + b;
+            double *row2 = Lp + tri(b + (r2 < m ? r2 : r)) + b;
+            double a1[TP_CH], a2[TP_CH];
+#pragma unroll
+            for (int s = 0; s < TP_CH; ++s) {
+                const int c = lane + 64 * s;
+                a1[s] = c <= r ? row[c] : 0.0;
+                a2[s] = (r2 < m && c <= r2) ? row2[c] : 0.0;
+            }
+            const double ur = uw[r], qr = coord(q, r);
+            const double ur2 = r2 < m ? uw[r2] : 0.0, qr2 = r2 < m ? coord(q, r2) : 0.0;
+#pragma unroll
+            for (int s = 0; s < TP_CH; ++s) {
+                const int c = lane + 64 * s;
+                if (c <= r) row[c] = fma(-ur, q[s], fma(-qr, u[s], a1[s]));
+                if (r2 < m && c <= r2) row2[c] = fma(-ur2, q[s], fma(-qr2, u[s], a2[s]));
+            }
+        }
+        __syncthreads();
+    }
+    for (int r = tid; r < n; r += blockDim.x) d_out[r] = Lp[tri(r) + r];
+    if (tid == 0) {
+        if (n >= 2) {
+            e_out[n - 2] = Lp[tri(n - 1) + n - 2];
+            h_out[n - 2] = 0.0;
+        }
+        e_out[n - 1] = 0.0;
+        h_out[n - 1] = 0.0;
+        if (ticks) ticks[1] = ticks[2] = wall_clock64();
+    }
+}
+
 // Q = H_0 H_1 ... H_{n-3} S: G lanes own one column of S (R rows each, in registers) and apply
 // the reflectors from the last to the first, with no workgroup barrier. Also copies the
 // eigenvalues, sets info = 1 if any is not finite, and Q^T.
@@ -606,9 +783,17 @@ void launch_eigensystem_dc(int n, double *A, double *work, double *lam_sorted,
     double *S = work, *Wq = S + n2, *Wu = Wq + n2, *d = Wu + n2, *e = d + n, *lam = e + n;
     double *h = lam + n;
     if (ticks) MSX_HIP(hipMemsetAsync(ticks, 0, 8 * sizeof(long long), s));
-    // Householder stage: rows b0 .. n-1 of stride lds_ld (odd) fit 150 KiB of LDS beside the
-    // partial sums and q; the steps before that work in place in A
-    {
+    // Householder stage: the packed lower triangle in LDS throughout when it fits 150 KiB
+    // (n <= 173); otherwise rows b0 .. n-1 of stride lds_ld (odd) fit 150 KiB of LDS beside the
+    // partial sums and q, and the steps before that work in place in A
+    const size_t packed =
+        sizeof(double) * ((size_t)TRED_WAVES * 64 * (1 + TP_CH) + (size_t)n * (n + 1) / 2);
+    if (n <= 64 * TP_CH && packed <= 150 * 1024) {
+        set_dynamic_lds((const void *)eig_tred_packed_kernel, packed);
+        hipLaunchKernelGGL(eig_tred_packed_kernel, dim3(1), dim3(64 * TRED_WAVES), packed, s, n, A,
+                           h, d, e, ticks);
+        MSX_LAUNCH_CHECK();
+    } else {
         const size_t fixed = sizeof(double) * ((size_t)TRED_WAVES * 64 + n);
         const size_t budget = 150 * 1024 - fixed;
         int rows = std::min(n, 1);
