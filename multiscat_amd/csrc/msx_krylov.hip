@@ -70,6 +70,85 @@ __global__ __launch_bounds__(KR_THREADS) void dots_kernel(KLayout K, const cx<T>
     }
 }
 
+// Block sums of the pairs j < np of KM register pairs (the pair at index npos, if >= 0, is
+// (nr, 0) instead), written as dots_kernel writes them: the same wave sums and the same in-order
+// sum over the wavefronts as block_sum2, with one barrier for all pairs.
+template <int KM>
+__device__ inline void write_pair_sums(const double (&sr)[KM], const double (&si)[KM], int kcount,
+                                       bool with_norm, double nr, double *red, double *partial,
+                                       long long slot, int nchunk, bool live) {
+    constexpr int NW = KR_THREADS / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < KM; ++j) {
+        if (j >= kcount) break;
+        const double a = wave_sum(sr[j]), c = wave_sum(si[j]);
+        if (lane == 0) {
+            red[(wave * (KM + 1) + j) * 2] = a;
+            red[(wave * (KM + 1) + j) * 2 + 1] = c;
+        }
+    }
+    if (with_norm) {
+        const double a = wave_sum(nr), c = wave_sum(0.0);
+        if (lane == 0) {
+            red[(wave * (KM + 1) + kcount) * 2] = a;
+            red[(wave * (KM + 1) + kcount) * 2 + 1] = c;
+        }
+    }
+    __syncthreads();
+    const int j = threadIdx.x;
+    if (j < kcount + (with_norm ? 1 : 0)) {
+        double ro = 0, io = 0;
+        for (int w = 0; w < NW; ++w) {
+            ro += red[(w * (KM + 1) + j) * 2];
+            io += red[(w * (KM + 1) + j) * 2 + 1];
+        }
+        double *p = partial + ((slot + j) * nchunk + blockIdx.x) * 2;
+        p[0] = live ? ro : 0.0;
+        p[1] = live ? io : 0.0;
+    }
+}
+
+// dots_kernel for kcount (+ the norm) <= KM with the sums in registers: w is read once per
+// element, the kcount basis values of an element are loaded together, and one barrier serves
+// all the block sums (dots_kernel walks the basis with a block reduction, two barriers, per
+// vector). Same per-thread element order and the same reductions as dots_kernel.
+template <typename T, int KM>
+__global__ __launch_bounds__(KR_THREADS) void dots_reg_kernel(KLayout K, const cx<T> *const *vptr,
+                                                              int kcount,
+                                                              const cx<T> *__restrict__ w,
+                                                              int with_norm,
+                                                              double *__restrict__ partial) {
+    __shared__ double red[(KR_THREADS / 64) * (KM + 1) * 2];
+    const int b = blockIdx.y;
+    if (K.active && !K.active[b]) return;
+    const long long n = K.len[b], base = K.start[b];
+    const long long cbeg = (long long)blockIdx.x * K.chunk;
+    const long long cend = cbeg + K.chunk < n ? cbeg + K.chunk : n;
+    const long long slot = (long long)b * (K.kmax + 1);
+    const cx<T> *vb[KM];
+#pragma unroll
+    for (int j = 0; j < KM; ++j) vb[j] = j < kcount ? vptr[j] + base : nullptr;
+    double sr[KM], si[KM], nr = 0.0;
+#pragma unroll
+    for (int j = 0; j < KM; ++j) sr[j] = si[j] = 0.0;
+    for (long long i = cbeg + threadIdx.x; i < cend; i += KR_THREADS) {
+        const cx<T> c = w[base + i];
+        cx<T> v[KM];
+#pragma unroll
+        for (int j = 0; j < KM; ++j)
+            if (j < kcount) v[j] = vb[j][i];
+#pragma unroll
+        for (int j = 0; j < KM; ++j)
+            if (j < kcount) {
+                sr[j] += (double)v[j].re * c.re + (double)v[j].im * c.im;
+                si[j] += (double)v[j].re * c.im - (double)v[j].im * c.re;
+            }
+        if (with_norm) nr += (double)c.re * c.re + (double)c.im * c.im;
+    }
+    write_pair_sums<KM>(sr, si, kcount, with_norm != 0, nr, red, partial, slot, K.nchunk, cbeg < n);
+}
+
 // h[b*(kmax+1) + j] (+)= sum_chunk partial   for first <= j < count
 __global__ void reduce_partials_kernel(KLayout K, int first, int count,
                                        const double *__restrict__ partial,
@@ -143,7 +222,7 @@ __global__ __launch_bounds__(KR_THREADS) void axpy_dots_kernel(KLayout K,
                                                                cx<T> *__restrict__ w,
                                                                double *__restrict__ partial) {
     __shared__ cx<T> hs[KM];
-    __shared__ double red[2 * (KR_THREADS / 64)];
+    __shared__ double red[(KR_THREADS / 64) * (KM + 1) * 2];
     const int b = blockIdx.y;
     if (K.active && !K.active[b]) return;
     const long long n = K.len[b], base = K.start[b];
@@ -182,17 +261,7 @@ __global__ __launch_bounds__(KR_THREADS) void axpy_dots_kernel(KLayout K,
                 si[j] += (double)v[j].re * acc.im - (double)v[j].im * acc.re;
             }
     }
-#pragma unroll
-    for (int j = 0; j < KM; ++j) {
-        if (j >= kcount) break;
-        double a = sr[j], c = si[j];
-        block_sum2(a, c, red);
-        if (threadIdx.x == 0) {
-            double *p = partial + ((slot + j) * K.nchunk + blockIdx.x) * 2;
-            p[0] = cbeg < n ? a : 0.0;
-            p[1] = cbeg < n ? c : 0.0;
-        }
-    }
+    write_pair_sums<KM>(sr, si, kcount, false, 0.0, red, partial, slot, K.nchunk, cbeg < n);
 }
 
 // x += sum_{j<kk_b} v_j y_j  (kk_b per instance)
@@ -490,8 +559,16 @@ static dim3 kgrid(const KLayout &K) { return dim3(K.nchunk, K.n_inst); }
 template <typename T>
 void k_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<T> *w,
             int with_norm, double *partial, cx<double> *h, int accumulate, hipStream_t s) {
-    hipLaunchKernelGGL((dots_kernel<T>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr, kcount, w,
-                       with_norm, partial);
+    const int np = kcount + (with_norm ? 1 : 0);
+    if (np <= 8)
+        hipLaunchKernelGGL((dots_reg_kernel<T, 8>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr,
+                           kcount, w, with_norm, partial);
+    else if (np <= 16)
+        hipLaunchKernelGGL((dots_reg_kernel<T, 16>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr,
+                           kcount, w, with_norm, partial);
+    else
+        hipLaunchKernelGGL((dots_kernel<T>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr, kcount, w,
+                           with_norm, partial);
     MSX_LAUNCH_CHECK();
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(K.n_inst), dim3(64), 0, s, K, 0,
                        kcount + (with_norm ? 1 : 0), partial, h, accumulate);
