@@ -33,6 +33,9 @@ struct KState {
     const double *eps_inst = nullptr;  // [n_inst] per-instance tolerance, overrides eps if set
 };
 
+// dst[b] = src[b * stride].re for b < n (dst may be pinned host memory mapped for the device)
+void k_gather_re(int n, int stride, const cx<double> *src, double *dst, hipStream_t s);
+
 template <typename T>
 void k_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<T> *w, int with_norm,
             double *partial, cx<double> *h, int accumulate, hipStream_t s);

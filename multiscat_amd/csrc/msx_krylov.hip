@@ -553,8 +553,20 @@ void k_reactivate(const KLayout &K, const KState &S, const int *enabled, hipStre
     MSX_LAUNCH_CHECK();
 }
 
+// dst[b] = src[b * stride].re (dst may be pinned host memory mapped for the device)
+__global__ void gather_re_kernel(int n, int stride, const cx<double> *__restrict__ src,
+                                 double *__restrict__ dst) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n) dst[b] = src[(long long)b * stride].re;
+}
+
 // ---- launchers ----------------------------------------------------------------------------
 static dim3 kgrid(const KLayout &K) { return dim3(K.nchunk, K.n_inst); }
+
+void k_gather_re(int n, int stride, const cx<double> *src, double *dst, hipStream_t s) {
+    hipLaunchKernelGGL(gather_re_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, stride, src, dst);
+    MSX_LAUNCH_CHECK();
+}
 
 template <typename T>
 void k_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<T> *w,

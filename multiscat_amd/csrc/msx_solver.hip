@@ -66,6 +66,7 @@ struct Krylov {
     hipEvent_t ev[2] = {nullptr, nullptr};
     int next_slot = 0;
     int *mask_h = nullptr;  // pinned, two slots of n_inst flags staged for upload
+    double *dbl_h = nullptr, *dbl_dev = nullptr;  // pinned n_inst doubles, and its device view
     hipEvent_t mask_ev[2] = {nullptr, nullptr};
     int mask_slot = 0;
     std::vector<long long> len_h;
@@ -108,6 +109,8 @@ struct Krylov {
         MSX_HIP(hipHostMalloc((void **)&flags_h, 2 * sizeof(int) * (n_inst + 1), hipHostMallocDefault));
         MSX_HIP(hipHostGetDevicePointer((void **)&flags_dev, flags_h, 0));
         MSX_HIP(hipHostMalloc((void **)&mask_h, 2 * sizeof(int) * n_inst, hipHostMallocDefault));
+        MSX_HIP(hipHostMalloc((void **)&dbl_h, sizeof(double) * n_inst, hipHostMallocDefault));
+        MSX_HIP(hipHostGetDevicePointer((void **)&dbl_dev, dbl_h, 0));
         for (auto &e : ev) MSX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         for (auto &e : mask_ev) MSX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         set_all_active(nullptr);
@@ -115,6 +118,7 @@ struct Krylov {
     ~Krylov() {
         if (flags_h) (void)hipHostFree(flags_h);
         if (mask_h) (void)hipHostFree(mask_h);
+        if (dbl_h) (void)hipHostFree(dbl_h);
         for (auto *arr : {ev, mask_ev})
             for (int i = 0; i < 2; ++i)
                 if (arr[i]) (void)hipEventDestroy(arr[i]);
@@ -197,15 +201,14 @@ struct Krylov {
         MSX_HIP(hipStreamSynchronize(s));
         return h;
     }
-    // Per-instance squared norms of an arbitrary vector with this layout (host result).
+    // Per-instance squared norms of an arbitrary vector with this layout (host result), written
+    // by the device straight into pinned host memory (no staged copy of the whole column
+    // array, n_inst (kmax + 1) values, to read one per instance)
     std::vector<double> norms2(const cx<T> *v) {
         k_dots<T>(K, vp_d.p, 0, v, 1, partial.p, hcol.p, 0, s);
-        std::vector<cx<double>> h((size_t)n_inst * (kmax + 1));
-        hcol.download(h.data(), h.size(), s);
+        k_gather_re(n_inst, kmax + 1, hcol.p, dbl_dev, s);
         MSX_HIP(hipStreamSynchronize(s));
-        std::vector<double> out(n_inst);
-        for (int b = 0; b < n_inst; ++b) out[b] = h[(size_t)b * (kmax + 1)].re;
-        return out;
+        return std::vector<double>(dbl_h, dbl_h + n_inst);
     }
 };
 
