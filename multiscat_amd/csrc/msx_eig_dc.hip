@@ -176,9 +176,11 @@ constexpr int DC_LDS_DOUBLES = 10, DC_LDS_INTS = 7;  // arrays of length m in th
 // Merge of the two children of node blockIdx.x at depth t. S (n x n, row-major) holds in its
 // diagonal block [a, b) x [a, b) the eigenvectors of the node (columns, ascending eigenvalues
 // in lam[a .. b)); Wq and Wu are n x n workspaces used in the same block.
+// S, Wq and Wu are disjoint (restrict), so the copies below keep their loads in flight.
 __global__ __launch_bounds__(1024) void dc_merge_kernel(int n, int t, const double *__restrict__ e,
-                                                        double *lam, double *S, double *Wq,
-                                                        double *Wu) {
+                                                        double *lam, double *__restrict__ S,
+                                                        double *__restrict__ Wq,
+                                                        double *__restrict__ Wu) {
     const int node = blockIdx.x;
     const int a = dc_row(n, t, node), b = dc_row(n, t, node + 1);
     const int m = b - a;
@@ -225,42 +227,72 @@ __global__ __launch_bounds__(1024) void dc_merge_kernel(int n, int t, const doub
         z[pos] = c < m1 ? r2 * at(S, m1 - 1, c) : sg * r2 * at(S, m1, c);
     }
     __syncthreads();
-    // (2) Q-hat = diag(Q1, Q2) with its columns in merged order
-    for (int r = wave; r < m; r += NW)
-        for (int q = lane; q < m; q += 64) {
-            const int c = perm[q];
-            at(Wq, r, q) = ((r < m1) == (c < m1)) ? at(S, r, c) : 0.0;
+    // (2) Q-hat = diag(Q1, Q2) with its columns in merged order, a row's loads (up to 16
+    // columns per lane) issued before its stores, by the wavefronts other than the first
+    // (whose first thread runs the deflation scan (3) meanwhile) when there are several
+    const int cw0 = NW > 1 ? 1 : 0;
+    for (int r = wave - cw0; wave >= cw0 && r < m; r += NW - cw0) {
+        constexpr int QC = 16;  // m <= 1024
+        double v[QC];
+#pragma unroll
+        for (int k = 0; k < QC; ++k) {
+            const int q = lane + 64 * k;
+            const int c = q < m ? perm[q] : 0;
+            v[k] = (q < m && (r < m1) == (c < m1)) ? at(S, r, c) : 0.0;
         }
-    // (3) deflation (one thread: the pair test chains along the non-deflated entries)
+#pragma unroll
+        for (int k = 0; k < QC; ++k) {
+            const int q = lane + 64 * k;
+            if (q < m) at(Wq, r, q) = v[k];
+        }
+    }
+    // (3) deflation (one thread: the pair test chains along the non-deflated entries). The
+    // close-pole test |u w (d_j - d_prev)| <= tol, u = z_prev / t, w = z_j / t, t^2 = z_prev^2 +
+    // z_j^2, is evaluated as |z_prev z_j (d_j - d_prev)| <= tol t^2 (no square root or division
+    // unless the pair rotates), the current pair's values stay in registers and the next
+    // entry is loaded ahead: the loop is this kernel's serial part
     if (tid == 0) {
         const double tol = 8.0 * DBL_EPSILON * fmax(fmax(fabs(D[0]), fabs(D[m - 1])), rho);
         int K = 0, nq = 0, R = 0, prev = -1;
+        double zp = 0.0, Dp = 0.0;  // z and D of prev, as updated
+        double zn = z[0], Dn = D[0];
         for (int j = 0; j < m; ++j) {
-            if (rho * fabs(z[j]) <= tol) {
+            const double zj = zn, Dj = Dn;
+            if (j + 1 < m) {
+                zn = z[j + 1];
+                Dn = D[j + 1];
+            }
+            if (rho * fabs(zj) <= tol) {
                 dfl[nq++] = j;
                 continue;
             }
             if (prev < 0) {
                 prev = j;
+                zp = zj;
+                Dp = Dj;
                 continue;
             }
-            const double tt = hypot(z[prev], z[j]);
-            const double u = z[prev] / tt, w = z[j] / tt;
-            if (fabs(u * w * (D[j] - D[prev])) <= tol) {
+            if (fabs(zp * zj * (Dj - Dp)) <= tol * (zp * zp + zj * zj)) {
                 // rotate columns (prev, j) so that z_prev = 0: prev deflates
+                const double tt = hypot(zp, zj);
+                const double u = zp / tt, w = zj / tt;
                 rp[R] = prev;
                 rj[R] = j;
                 ru[R] = u;
                 rw[R] = w;
                 ++R;
-                const double dp = D[prev], dj = D[j];
-                D[prev] = w * w * dp + u * u * dj;
-                D[j] = u * u * dp + w * w * dj;
+                D[prev] = w * w * Dp + u * u * Dj;
+                const double dj_new = u * u * Dp + w * w * Dj;
+                D[j] = dj_new;
                 z[j] = tt;
                 z[prev] = 0.0;
                 dfl[nq++] = prev;
+                zp = tt;
+                Dp = dj_new;
             } else {
                 nd[K++] = prev;
+                zp = zj;
+                Dp = Dj;
             }
             prev = j;
         }
