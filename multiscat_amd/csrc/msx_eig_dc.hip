@@ -591,6 +591,14 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
     v += dpp_d<0x140>(v);
     return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
+// Sum over a DPP row of 16 lanes, the same bits in all of them.
+__device__ __forceinline__ double row16_sum(double v) {
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    v += dpp_d<0x140>(v);
+    return v;
+}
 // v[r] for a wave-uniform coordinate r of a lane-distributed vector: all three chunks are read
 // (scalar results) and one is selected, since a select among the registers before the
 // readlane becomes an indexed array in scratch memory
@@ -734,7 +742,8 @@ __global__ __launch_bounds__(1024) void eig_tred_packed_kernel(int n, double *A,
 }
 
 // Q = H_0 H_1 ... H_{n-3} S: G lanes own one column of S (R rows each, in registers) and apply
-// the reflectors from the last to the first, with no workgroup barrier. Also copies the
+// the reflectors from the last to the first, with no workgroup barrier, each reflector's
+// loads and reciprocal issued while the previous one is applied. Also copies the
 // eigenvalues, sets info = 1 if any is not finite, and Q^T.
 template <int G, int R>
 __global__ __launch_bounds__(256) void eig_backtransform_kernel(
@@ -751,21 +760,36 @@ __global__ __launch_bounds__(256) void eig_backtransform_kernel(
             const int r = gl + G * t;
             y[t] = r < n ? S[(long long)r * n + col] : 0.0;
         }
-        for (int k = n - 3; k >= 0; --k) {
-            const double hk = h[k];
-            if (hk == 0.0) continue;
-            const double *u = U + (long long)k * n;  // u[r] for r = k+1 .. n-1
-            double ur[R];
-            double s = 0.0;
+        // reflector k + 1 is loaded while reflector k is applied (u[r] for r = k+1 .. n-1 in
+        // row k of U; h = 0: no reflector, w = 0)
+        double un[R], hin = 0.0;
+        auto load_refl = [&](int k) {
+            const double *u = U + (long long)k * n;
 #pragma unroll
             for (int t = 0; t < R; ++t) {
                 const int r = gl + G * t;
-                ur[t] = (r > k && r < n) ? u[r] : 0.0;
-                s = fma(ur[t], y[t], s);
+                un[t] = (r > k && r < n) ? u[r] : 0.0;
             }
+            const double hk = h[k];
+            hin = hk != 0.0 ? 1.0 / hk : 0.0;
+        };
+        if (n >= 3) load_refl(n - 3);
+        for (int k = n - 3; k >= 0; --k) {
+            double ur[R];
 #pragma unroll
-            for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
-            const double w = s / hk;
+            for (int t = 0; t < R; ++t) ur[t] = un[t];
+            const double hinv = hin;
+            if (k > 0) load_refl(k - 1);
+            double s = 0.0;
+#pragma unroll
+            for (int t = 0; t < R; ++t) s = fma(ur[t], y[t], s);
+            if constexpr (G == 16) {
+                s = row16_sum(s);
+            } else {
+#pragma unroll
+                for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+            }
+            const double w = s * hinv;
 #pragma unroll
             for (int t = 0; t < R; ++t) y[t] = fma(-ur[t], w, y[t]);
         }
