@@ -635,21 +635,45 @@ __global__ __launch_bounds__(64 * DG_WAVES, 1) void split_residual_kernel(
     const cx<double> cj = c[cidx], xN = x[row0 + nz - 1];
     const bool is_spec = jc == spec[b];
     const cx<double> bs = bsrc[b];
-#pragma unroll
-    for (int mt = 0; mt < NT; ++mt)
+    // per row tile: its x and (V x) values loaded together at clamped addresses (the next
+    // tile's before this tile's stores), and a tile inside [0, nz) stored with no condition.
+    // With loads and stores under per-element conditions the compiler waited for every
+    // outstanding memory operation before each store: 40 round trips per tile.
+    cx<double> xz[4], vz[4];
+    auto load_t = [&](int mt) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int z = 16 * mt + M::row(g, v);
-            if (mt < NTL && z < nz) {
-                const long long i = row0 + z;
-                const cx<double> xz = x[i];
-                cx<double> dx = mk(acc[mt][0][v] + ej * xz.re, acc[mt][1][v] + ej * xz.im);
-                if (z == nz - 1) dx = dx - cj * xN;
-                cx<double> r = mk(0.0, 0.0) - dx - vx[i];
-                if (is_spec && z == nz - 1) r = r + bs;
-                rho[vec_index(Lo, off, nc, jc, z)] = mk((float)r.re, (float)r.im);
+            const long long i = row0 + (mt < NTL && z < nz ? z : 0);
+            xz[v] = x[i];
+            vz[v] = vx[i];
+        }
+    };
+    load_t(0);
+#pragma unroll
+    for (int mt = 0; mt < NT; ++mt) {
+        cx<float> out[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int z = 16 * mt + M::row(g, v);
+            cx<double> dx = mk(acc[mt][0][v] + ej * xz[v].re, acc[mt][1][v] + ej * xz[v].im);
+            if (z == nz - 1) dx = dx - cj * xN;
+            cx<double> r = mk(0.0, 0.0) - dx - vz[v];
+            if (is_spec && z == nz - 1) r = r + bs;
+            out[v] = mk((float)r.re, (float)r.im);
+        }
+        if (mt + 1 < NT) load_t(mt + 1);
+        if (mt < NTL && 16 * mt + 16 <= nz) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) rho[vec_index(Lo, off, nc, jc, 16 * mt + M::row(g, v))] = out[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int z = 16 * mt + M::row(g, v);
+                if (mt < NTL && z < nz) rho[vec_index(Lo, off, nc, jc, z)] = out[v];
             }
         }
+    }
 }
 
 bool split_residual_supported(int nzp) { return nzp % 16 == 0 && nzp <= 160; }
