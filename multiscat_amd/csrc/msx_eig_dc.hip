@@ -382,12 +382,22 @@ __global__ __launch_bounds__(1024) void dc_merge_kernel(int n, int t, const doub
         const int ra = r0 + l16, qb = q0 + l16;
         const double *arow = &at(Wq, ra < m ? ra : m - 1, 0);
         M::acc acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-        for (int i0 = 0; i0 < K; i0 += 4) {
-            const int i = i0 + g;
-            const double av = (ra < m && i < K) ? arow[nd[i]] : 0.0;
-            const double bv = (i < K && qb < K) ? at(Wu, i, qb) : 0.0;
-            acc = M::op(av, bv, acc);
+        // eight k-steps at a time, their loads unconditional at clamped indices (the masks are
+        // applied after the loads) and all issued before the products: loads under
+        // per-element conditions became branches that waited for each load in turn
+        const bool rok = ra < m, qok = qb < K;
+        const int qbc = qok ? qb : 0;
+        for (int i0 = 0; i0 < K; i0 += 32) {
+            double av[8], bv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = i0 + 4 * u + g, ic = i < K ? i : 0;
+                const double a_raw = arow[nd[ic]], b_raw = at(Wu, ic, qbc);
+                av[u] = (rok && i < K) ? a_raw : 0.0;
+                bv[u] = (qok && i < K) ? b_raw : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = M::op(av[u], bv[u], acc);
         }
         if (qb < K) {
             const int col = rank[qb];
