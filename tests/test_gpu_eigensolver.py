@@ -28,7 +28,8 @@ def _orthogonal(n, rng):
 def _matrices():
     rng = np.random.default_rng(7)
     out = {}
-    for n in (2, 3, 5, 17, 64, 150, 300):
+    # 173 and 174: the largest size whose packed triangle fits LDS, and the first that does not
+    for n in (2, 3, 5, 17, 64, 150, 173, 174, 300):
         a = rng.standard_normal((n, n))
         out[f"random{n}"] = a + a.T
     m = 10
