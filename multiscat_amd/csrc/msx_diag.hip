@@ -715,31 +715,47 @@ __global__ __launch_bounds__(256) void factors_mfma_kernel(long long total, int 
     const int nzp = E.nzp, NTL = nzp >> 4;
     const int pa = M::perm(l16);
     auto tile = [&](int t) { return t < NTL ? t : NTL - 1; };
+    // W with every load unconditional at clamped indices: Q is zero past column nz (and the
+    // extra tiles repeat the last one), so the quotient needs no select. Under a condition the
+    // 40 loads and divisions became branches waited for one at a time.
     double w[NT][4];
 #pragma unroll
     for (int mt = 0; mt < NT; ++mt)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int m = 16 * mt + 4 * g + v;
-            w[mt][v] = m < nz ? E.Qp[(long long)(nz - 1) * nzp + m] / (ej + E.lam[m]) : 0.0;
+            const int m = 16 * tile(mt) + 4 * g + v;
+            const bool ok = 16 * mt + 4 * g + v < nz;
+            const double q = ok ? E.Qp[(long long)(nz - 1) * nzp + m] : 0.0;
+            const double lv = E.lam[m < nz ? m : nz - 1];
+            w[mt][v] = q / (ok ? ej + lv : 1.0);
         }
     for (int zt = 0; zt < NTL; ++zt) {
         acc_t acc = acc_t{0, 0, 0, 0};
+        // all of the row tile's A fragments loaded before its products
+        double2 a01[NT], a23[NT];
 #pragma unroll
         for (int mt = 0; mt < NT; ++mt) {
             const double *qa = E.Qp + (long long)(16 * zt + pa) * nzp + 16 * tile(mt) + 4 * g;
-            const double2 a01 = reinterpret_cast<const double2 *>(qa)[0];
-            const double2 a23 = reinterpret_cast<const double2 *>(qa)[1];
-            acc = M::op(a01.x, w[mt][0], acc);
-            acc = M::op(a01.y, w[mt][1], acc);
-            acc = M::op(a23.x, w[mt][2], acc);
-            acc = M::op(a23.y, w[mt][3], acc);
+            a01[mt] = reinterpret_cast<const double2 *>(qa)[0];
+            a23[mt] = reinterpret_cast<const double2 *>(qa)[1];
+        }
+#pragma unroll
+        for (int mt = 0; mt < NT; ++mt) {
+            acc = M::op(a01[mt].x, w[mt][0], acc);
+            acc = M::op(a01[mt].y, w[mt][1], acc);
+            acc = M::op(a23[mt].x, w[mt][2], acc);
+            acc = M::op(a23[mt].y, w[mt][3], acc);
         }
         if (!live) continue;
+        if (16 * zt + 16 <= nz) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int z = 16 * zt + 4 * g + v;
-            if (z < nz) fac[j * nz + z] = acc[v];
+            for (int v = 0; v < 4; ++v) fac[j * nz + 16 * zt + 4 * g + v] = acc[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int z = 16 * zt + 4 * g + v;
+                if (z < nz) fac[j * nz + z] = acc[v];
+            }
         }
     }
 }
