@@ -209,6 +209,59 @@ __global__ __launch_bounds__(KR_THREADS) void axpy_kernel(KLayout K, const cx<T>
     }
 }
 
+// axpy_kernel for kcount <= KM: the basis pointers held in registers and the kcount basis values
+// of an element loaded together before its updates (the runtime-length loop over the basis
+// issued them one at a time). Same per-thread order and reduction as axpy_kernel.
+template <typename T, int KM>
+__global__ __launch_bounds__(KR_THREADS) void axpy_reg_kernel(KLayout K, const cx<T> *const *vptr,
+                                                              int kcount,
+                                                              const cx<double> *__restrict__ h,
+                                                              cx<T> *__restrict__ w, int with_norm,
+                                                              double *__restrict__ partial) {
+    __shared__ cx<T> hs[KM];
+    __shared__ double red[2 * (KR_THREADS / 64)];
+    const int b = blockIdx.y;
+    if (K.active && !K.active[b]) return;
+    const long long n = K.len[b], base = K.start[b];
+    const long long slot = (long long)b * (K.kmax + 1);
+    for (int j = threadIdx.x; j < kcount; j += KR_THREADS) {
+        const cx<double> hv = h[slot + j];
+        hs[j] = mk((T)hv.re, (T)hv.im);
+    }
+    __syncthreads();
+    const cx<T> *vb[KM];
+#pragma unroll
+    for (int j = 0; j < KM; ++j) vb[j] = j < kcount ? vptr[j] + base : nullptr;
+    const long long cbeg = (long long)blockIdx.x * K.chunk;
+    const long long cend = cbeg + K.chunk < n ? cbeg + K.chunk : n;
+    double nrm = 0;
+    for (long long i = cbeg + threadIdx.x; i < cend; i += KR_THREADS) {
+        cx<T> v[KM];
+#pragma unroll
+        for (int j = 0; j < KM; ++j)
+            if (j < kcount) v[j] = vb[j][i];
+        cx<T> acc = w[base + i];
+#pragma unroll
+        for (int j = 0; j < KM; ++j)
+            if (j < kcount) {
+                const cx<T> hj = hs[j];
+                acc.re -= v[j].re * hj.re - v[j].im * hj.im;
+                acc.im -= v[j].re * hj.im + v[j].im * hj.re;
+            }
+        w[base + i] = acc;
+        nrm += (double)acc.re * acc.re + (double)acc.im * acc.im;
+    }
+    if (with_norm) {
+        double dummy = 0;
+        block_sum2(nrm, dummy, red);
+        if (threadIdx.x == 0) {
+            double *p = partial + ((slot + kcount) * K.nchunk + blockIdx.x) * 2;
+            p[0] = cbeg < n ? nrm : 0.0;
+            p[1] = 0.0;
+        }
+    }
+}
+
 // The first update of CGS2 and the second projection in one pass over the basis:
 //   w <- w - sum_{j<kcount} v_j h_j,  then partial sums of conj(v_j) w (as dots_kernel writes
 // them), with the kcount <= KM basis values of each element held in registers between the two.
@@ -591,8 +644,15 @@ template <typename T>
 void k_axpy(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<double> *h,
             cx<T> *w, int with_norm, double *partial, cx<double> *hnorm_dst, hipStream_t s) {
     if (kcount > KR_MAX_BASIS) throw Error(3, "Krylov dimension exceeds the kernel limit");
-    hipLaunchKernelGGL((axpy_kernel<T>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr, kcount, h, w,
-                       with_norm, partial);
+    if (kcount <= 8)
+        hipLaunchKernelGGL((axpy_reg_kernel<T, 8>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr,
+                           kcount, h, w, with_norm, partial);
+    else if (kcount <= 16)
+        hipLaunchKernelGGL((axpy_reg_kernel<T, 16>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr,
+                           kcount, h, w, with_norm, partial);
+    else
+        hipLaunchKernelGGL((axpy_kernel<T>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr, kcount, h,
+                           w, with_norm, partial);
     MSX_LAUNCH_CHECK();
     if (with_norm && hnorm_dst) {
         // the norm partials sit in slot kcount; reduce only that slot into hnorm_dst (without
