@@ -163,11 +163,16 @@ enum { MSX_PRECISION_FP64 = 0, MSX_PRECISION_MIXED = 1 };
 typedef struct {
     int method;       /* MSX_METHOD_*                                                       */
     int precision;    /* MSX_PRECISION_*                                                    */
-    int neumann;      /* 1: left preconditioner (I - M^-1 N), the reference's flag; 2 .. 4 (block
-                         Jacobi only): the Neumann series to that degree, sum_i (-D^-1 V)^i  */
+    int neumann;      /* 0: none; 1: left preconditioner (I - M^-1 N), the reference's flag;
+                         p >= 2 (block Jacobi only): the von Neumann series to degree p,
+                         sum_{i<=p} (-D^-1 V)^i, the reference's `order`
+                         (multiscat/multiscat/_scipy_von_neumann.py:40-58)                   */
     double eps;       /* stop when |g_{k+1}| < eps |g_1| for `window` consecutive iterations */
     int maxiter;      /* iteration budget per condition (reference: 2000)                   */
-    int restart;      /* Krylov dimension before a restart; 0 = maxiter (unrestarted)       */
+    int restart;      /* Krylov dimension before a restart; 0 = unrestarted up to maxiter while
+                         the basis and Hessenberg matrices fit in a quarter of the free device
+                         memory, else restarted at the largest dimension that fits (fixed at a
+                         batch's first solve with these options)                            */
     int window;       /* reference: 3                                                       */
     double inner_eps; /* mixed precision: relative tolerance of each FP32 inner solve;
                          0 = sqrt(eps)/3 clamped to [1e-5, 2.5e-4] (two sweeps reach eps) */
@@ -233,6 +238,17 @@ int msx_batch_state(const msx_batch *batch, double *psi);
 int msx_batch_set_state(msx_batch *batch, const double *psi);
 int msx_batch_intensities(const msx_batch *batch, double *P);
 int msx_batch_stats(const msx_batch *batch, int *iterations, int *info, double *residual);
+
+/* The lower block operator of the Gauss-Seidel split in each condition's channel order,
+ * L = D + V_{i<j} (D_j = T + diag(Re V_00) + e_j - c_j e_N e_N^T): state_out = L state_in
+ * (inverse = 0) or L^-1 state_in (inverse = 1), host arrays in the msx_batch_state layout.
+ * Replaces the `lower` and `inverse_lower` operators of the reference's SciPy path
+ * (multiscat/multiscat/_scipy.py:499-527, applied per channel at :263-296 and :366-388): the
+ * reference's get_scattering_state returns L psi (multiscat/multiscat/_gmres.py:127), its
+ * get_scattering_matrix_from_state applies L^-1 to a state (_multiscat.py:81-88) and an
+ * initial state x enters GMRES as L^-1 x (_gmres.py:106-108). */
+int msx_batch_apply_lower(msx_batch *batch, int inverse, const double *state_in,
+                          double *state_out);
 
 /* Device pointers for callers that keep data on the GPU (e.g. a torch autograd bridge):
  * the state psi (complex FP64, same layout as msx_batch_state) and the intensities. */

@@ -138,11 +138,13 @@ class ScatteringCondition:
     incident_k: tuple[float, float, float]
     potential: ScatteringPotential
     units: UnitSystem = field(default_factory=UnitSystem.si)
-    # Initial guess psi (nkx, nky, nz) for the GMRES solve, as the reference's
-    # ScatteringCondition.initial_state (multiscat/config.py:115-121; zero when None). It is
-    # the state of the DVR basis that get_scattering_state returns, so the solution of a
-    # nearby condition (energy continuation) can be passed on. Used by the "gpu" and "mixed"
-    # backends and by get_scattering_matrices.
+    # Initial guess (nkx, nky, nz) for the GMRES solve, the reference's
+    # ScatteringCondition.initial_state (multiscat/config.py:115-121; zero when None), in the
+    # form get_scattering_state returns (L psi): the "gpu" and "mixed" backends and
+    # get_scattering_matrices start GMRES from x0 = L^-1 initial_state as
+    # multiscat/multiscat/_gmres.py:106-108 does, so the state of a nearby condition (energy
+    # continuation) can be passed on; get_scattering_matrix_von_neumann takes it as x0 itself
+    # (multiscat/multiscat/_scipy_von_neumann.py:161-170).
     initial_state: np.ndarray | None = field(default=None, compare=False)
 
     @staticmethod
@@ -207,32 +209,70 @@ def _intensities_from_state(psi: np.ndarray, inputs: B.SolverInputs) -> np.ndarr
     return (np.abs(S) ** 2).astype(np.complex128)
 
 
+class _OneCondition:
+    """A table and a one-condition batch on the GPU for `inputs` over the channel set
+    `channels` (solve order), closed on exit."""
+
+    def __init__(self, inputs: B.SolverInputs, channels):
+        from multiscat_amd import batch as BT
+
+        chx, chy = channels
+        if not np.any((chx == 0) & (chy == 0)):
+            raise ValueError("n_channels too small: the specular channel is not included")
+        self.channels = channels
+        self.shape = inputs.potential_values.shape
+        self.table = BT.Table(inputs.potential_values, inputs.parallel_kinetic_energy)
+        self.batch = BT.Batch(self.table, [dict(
+            e=inputs.perpendicular_kinetic_difference, wave_a=inputs.wave_a,
+            wave_b=inputs.wave_b, wave_c=inputs.wave_c, channels=channels)])
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.batch.close()
+        self.table.close()
+
+    def restrict(self, full: np.ndarray) -> np.ndarray:
+        return np.asarray(full, dtype=np.complex128).reshape(self.shape)[self.channels]
+
+    def expand(self, rows: np.ndarray, fill=None) -> np.ndarray:
+        out = (np.zeros(self.shape, dtype=np.complex128) if fill is None
+               else np.array(fill, dtype=np.complex128).reshape(self.shape))
+        out[self.channels] = rows
+        return out
+
+
+def _scipy_channels(inputs: B.SolverInputs, n_channels):
+    """The channel set and order of the reference's SciPy path: argsort of e_G, cut at
+    n_channels (multiscat/multiscat/_scipy.py:414-416)."""
+    return B.lowest_channels(inputs.perpendicular_kinetic_difference, n_channels)
+
+
 def _batch_solve(inputs: B.SolverInputs, config: OptimizationConfig, *, method: str,
-                 precision: str, neumann: bool, initial_state=None):
+                 precision: str, neumann: int, initial_state=None,
+                 initial_is_lower_state: bool = True):
+    """psi (nkx, nky, nz) from the batched GPU solver on the SciPy path's channel set.
+
+    initial_state: a state in the reference's convention, L psi (what get_scattering_state
+    returns), entering GMRES as x0 = L^-1 initial_state as multiscat/multiscat/_gmres.py:106-108
+    does; with initial_is_lower_state=False it is x0 itself (the von Neumann path,
+    multiscat/multiscat/_scipy_von_neumann.py:161-170)."""
     from multiscat_amd import batch as BT
 
     flag, nsig = config.fortran_parameters()
     eps = 0.5 * 10.0 ** (-nsig)
-    e = inputs.perpendicular_kinetic_difference
-    chx, chy = B.lowest_channels(e, config.n_channels)
-    if not np.any((chx == 0) & (chy == 0)):
-        raise ValueError("n_channels too small: the specular channel is not included")
-    cond = dict(e=e, wave_a=inputs.wave_a, wave_b=inputs.wave_b, wave_c=inputs.wave_c,
-                channels=(chx, chy))
-    tab = BT.Table(inputs.potential_values, inputs.parallel_kinetic_energy)
-    bt = BT.Batch(tab, [cond])
-    try:
+    with _OneCondition(inputs, _scipy_channels(inputs, config.n_channels)) as one:
         warm = initial_state is not None
         if warm:
-            bt.set_state([np.asarray(initial_state, dtype=np.complex128)[chx, chy]])
-        bt.solve(BT.SolveOptions(method=method, precision=precision, neumann=neumann, eps=eps,
-                                 maxiter=max(config.max_iterations, 1), warm_start=warm))
-        psi = np.zeros(inputs.potential_values.shape, dtype=np.complex128)
-        psi[chx, chy] = bt.state()[0]
-        return psi
-    finally:
-        bt.close()
-        tab.close()
+            x0 = one.restrict(initial_state)
+            if initial_is_lower_state:
+                x0 = one.batch.apply_lower([x0], inverse=True)[0]
+            one.batch.set_state([x0])
+        one.batch.solve(BT.SolveOptions(method=method, precision=precision, neumann=int(neumann),
+                                        eps=eps, maxiter=max(config.max_iterations, 1),
+                                        warm_start=warm))
+        return one.expand(one.batch.state()[0])
 
 
 def _solve_state(condition: ScatteringCondition, config: OptimizationConfig,
@@ -255,7 +295,7 @@ def _solve_state(condition: ScatteringCondition, config: OptimizationConfig,
                            neumann=config.use_neumann_preconditioner,
                            initial_state=condition.initial_state)
     elif backend == "mixed":
-        psi = _batch_solve(inputs, config, method="jacobi", precision="mixed", neumann=True,
+        psi = _batch_solve(inputs, config, method="jacobi", precision="mixed", neumann=1,
                            initial_state=condition.initial_state)
     else:
         raise ValueError(f"Unknown backend '{backend}'. Expected 'fortran', 'gpu' or 'mixed'.")
@@ -273,27 +313,42 @@ def get_scattering_matrix_von_neumann(condition: ScatteringCondition,
                                       config: OptimizationConfig, *,
                                       order: int = 1) -> np.ndarray:
     """Intensities with the block-Jacobi preconditioner and its von Neumann series
-    M = sum_{k <= order} (-D^-1 V)^k D^-1 (multiscat/multiscat/_multiscat.py:178-224), FP64
-    on the GPU. Orders 0 and 1 are supported."""
-    if order not in (0, 1):
-        raise NotImplementedError("von Neumann orders 0 and 1 are supported")
+    M = sum_{k <= order} (-D^-1 V)^k D^-1 (multiscat/multiscat/_multiscat.py:178-224,
+    _scipy_von_neumann.py:40-58,131-184), FP64 on the GPU, for any order >= 0; a negative
+    order raises ValueError as the reference does (_scipy_von_neumann.py:145-147). The
+    condition's initial_state is GMRES's x0 as given (_scipy_von_neumann.py:161-170)."""
+    order = int(order)
+    if order < 0:
+        raise ValueError(f"Invalid von Neumann order {order}. Expected a non-negative integer.")
     inputs = solver_inputs(condition)
-    psi = _batch_solve(inputs, config, method="jacobi", precision="fp64", neumann=order == 1)
+    psi = _batch_solve(inputs, config, method="jacobi", precision="fp64", neumann=order,
+                       initial_state=condition.initial_state, initial_is_lower_state=False)
     return _intensities_from_state(psi, inputs)
 
 
 def get_scattering_state(condition: ScatteringCondition, config: OptimizationConfig, *,
                          backend: Literal["fortran", "gpu", "mixed"] = "gpu") -> np.ndarray:
-    """The full scattered state psi (nkx, nky, nz), interior included
-    (multiscat/multiscat/_multiscat.py:227-246)."""
-    return _solve_state(condition, config, backend)[0]
+    """The full scattering state (nkx, nky, nz), interior included, in the reference's
+    convention: L psi, with L = D + V_{i<j} the lower block operator of the Gauss-Seidel split
+    in the SciPy path's channel order (argsort of e_G, cut at config.n_channels), as
+    multiscat/multiscat/_multiscat.py:227-246 -> _scipy.py:580-613 -> _gmres.py:127 return it.
+    psi itself comes from `backend`; L is applied on the GPU. Channels outside the set are 0.
+    get_scattering_matrix_from_state and ScatteringCondition.initial_state take this form."""
+    psi, inputs = _solve_state(condition, config, backend)
+    with _OneCondition(inputs, _scipy_channels(inputs, config.n_channels)) as one:
+        return one.expand(one.batch.apply_lower([one.restrict(psi)])[0], fill=psi)
 
 
-def get_scattering_matrix_from_state(state: np.ndarray,
-                                     condition: ScatteringCondition) -> np.ndarray:
-    """Intensities from a state returned by get_scattering_state
-    (multiscat/multiscat/_multiscat.py:63-99)."""
-    return _intensities_from_state(np.asarray(state), solver_inputs(condition))
+def get_scattering_matrix_from_state(state: np.ndarray, condition: ScatteringCondition, *,
+                                     n_channels: int | None = None) -> np.ndarray:
+    """Intensities from a state in the form get_scattering_state returns (L psi): psi =
+    L^-1 state over the channel set of `n_channels`, then P (multiscat/multiscat/
+    _multiscat.py:63-99); L^-1 runs on the GPU."""
+    inputs = solver_inputs(condition)
+    with _OneCondition(inputs, _scipy_channels(inputs, n_channels)) as one:
+        rows = one.batch.apply_lower([one.restrict(state)], inverse=True)[0]
+        psi = one.expand(rows, fill=state)
+    return _intensities_from_state(psi, inputs)
 
 
 def get_scattering_matrices(conditions: Sequence[ScatteringCondition],
@@ -327,11 +382,14 @@ def get_scattering_matrices(conditions: Sequence[ScatteringCondition],
     try:
         warm = any(c.initial_state is not None for c in conditions)
         if warm:
-            bt.set_state([np.zeros((len(d["channels"][0]), V0.shape[2]), complex)
-                          if c.initial_state is None else
-                          np.asarray(c.initial_state, dtype=np.complex128)[d["channels"][0],
-                                                                           d["channels"][1]]
-                          for c, d in zip(conditions, conds)])
+            # states in the get_scattering_state form (L psi) enter as x0 = L^-1 state
+            # (multiscat/multiscat/_gmres.py:106-108); L^-1 0 = 0 for conditions without one
+            x0 = [np.zeros((len(d["channels"][0]), V0.shape[2]), complex)
+                  if c.initial_state is None else
+                  np.asarray(c.initial_state, dtype=np.complex128)[d["channels"][0],
+                                                                   d["channels"][1]]
+                  for c, d in zip(conditions, conds)]
+            bt.set_state(bt.apply_lower(x0, inverse=True))
         bt.solve(BT.SolveOptions(method=method, precision=precision,
                                  neumann=config.use_neumann_preconditioner,
                                  eps=0.5 * 10.0 ** (-nsig), maxiter=max(config.max_iterations, 1),

@@ -17,6 +17,10 @@ import torch
 
 from multiscat_amd import batch as BT
 
+# tolerance of the default adjoint solve (FP64 block Jacobi): far below the 1e-5 relative
+# accuracy asked of gradients, so the adjoint's own error does not show in them
+FP64_ADJOINT_EPS = 5e-13
+
 
 class _Intensities(torch.autograd.Function):
     @staticmethod
@@ -25,16 +29,17 @@ class _Intensities(torch.autograd.Function):
         batch = BT.Batch(table, conditions)
         batch.solve(options)
         P = torch.from_numpy(batch.intensities_flat()).to(V.device)
+        # the handles live as long as the graph node (ctx): a backward with retain_graph=True
+        # can run again; they are released when ctx is freed
         ctx.batch, ctx.table = batch, table
-        ctx.adjoint_options = adjoint_options or options
+        ctx.adjoint_options = adjoint_options or BT.SolveOptions(method="jacobi", precision="fp64",
+                                                                 eps=FP64_ADJOINT_EPS)
         return P
 
     @staticmethod
     def backward(ctx, grad_P):
         g = grad_P.detach().to(torch.float64).cpu().numpy()
         grad = ctx.batch.adjoint(g, ctx.adjoint_options)
-        ctx.batch.close()
-        ctx.table.close()
         return torch.from_numpy(grad).to(grad_P.device), None, None, None, None
 
 
@@ -43,8 +48,9 @@ def scattering_intensities(V: torch.Tensor, conditions, kinetic: np.ndarray,
                            adjoint_options: BT.SolveOptions | None = None) -> torch.Tensor:
     """P for every channel of every condition (concatenated in batch order) as a function of
     the natural-unit table V (nkx, nky, nz) complex128. `conditions` are what
-    multiscat_amd.batch.Batch takes; the adjoint uses FP64 (`adjoint_options`, default the
-    forward options)."""
+    multiscat_amd.batch.Batch takes; the adjoint solve uses `adjoint_options`, by default FP64
+    block Jacobi to eps 5e-13 whatever the forward precision (a mixed-precision adjoint is
+    available by passing mixed options)."""
     if V.dtype != torch.complex128:
         raise TypeError("V must be complex128")
     return _Intensities.apply(V, conditions, np.ascontiguousarray(kinetic, dtype=np.float64),

@@ -165,9 +165,11 @@ def momentum_from_angles(theta: float, phi: float, energy_natural: float):
 
 
 def lowest_channels(e: np.ndarray, n_channels: int | None):
-    """The n lowest-e channels, argsort order (multiscat/multiscat/_scipy.py:414-416)."""
+    """The n lowest-e channels in argsort order, with numpy's default (not stable) sort exactly
+    as multiscat/multiscat/_scipy.py:414-416 calls it: where e_G ties (normal incidence, mirror
+    symmetry) and the cut falls inside the tie, the set kept is the one the reference keeps."""
     flat = np.asarray(e).ravel()
-    order = np.argsort(flat, kind="stable")
+    order = np.argsort(flat)
     if n_channels is not None:
         order = order[:n_channels]
     ix, iy = np.divmod(order, e.shape[1])

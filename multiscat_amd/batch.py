@@ -23,7 +23,7 @@ class SolveOptions:
 
     method: str = "gauss_seidel"  # or "jacobi"
     precision: str = "fp64"  # or "mixed"
-    neumann: int = 1  # 0/False none, 1/True the reference's factor, 2..4 higher-degree (Jacobi)
+    neumann: int = 1  # 0/False none, 1/True the reference's factor, p >= 2 degree p (Jacobi)
     eps: float = 0.5e-5
     maxiter: int = 2000
     restart: int = 0
@@ -201,6 +201,26 @@ class Batch:
             raise ValueError("state must have one row of nz values per channel of the batch")
         flat = np.ascontiguousarray(flat)
         _lib.check(_lib.lib().msx_batch_set_state(self._h, _lib.ptr(flat)), "msx_batch_set_state")
+
+    def apply_lower(self, psi, *, inverse: bool = False):
+        """L psi (or L^-1 psi with inverse) per condition, L = D + V_{i<j} the lower block
+        operator of the Gauss-Seidel split in each condition's channel order, on the GPU
+        (msx_batch_apply_lower; the reference's `lower` / `inverse_lower` operators,
+        multiscat/multiscat/_scipy.py:499-527). psi as Batch.set_state takes it; returns a
+        list of (n_channels, nz) arrays."""
+        if isinstance(psi, (list, tuple)):
+            flat = np.concatenate([np.asarray(p, dtype=np.complex128).reshape(-1, self.nz)
+                                   for p in psi])
+        else:
+            flat = np.asarray(psi, dtype=np.complex128).reshape(-1, self.nz)
+        if flat.shape[0] != self.total_channels:
+            raise ValueError("state must have one row of nz values per channel of the batch")
+        flat = np.ascontiguousarray(flat)
+        out = np.empty_like(flat)
+        _lib.check(_lib.lib().msx_batch_apply_lower(self._h, 1 if inverse else 0,
+                                                    _lib.ptr(flat), _lib.ptr(out)),
+                   "msx_batch_apply_lower")
+        return [out[o:o + n] for o, n in zip(self.offsets, self.counts)]
 
     def intensities(self):
         """P per condition: list of (n_channels,) arrays, channel order as given."""

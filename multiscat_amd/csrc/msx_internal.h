@@ -204,10 +204,12 @@ void launch_diag_blocks(const Layout &L, const EigTables<T> &E, const double *e,
 void launch_pad_square(int n, int np, const double *src, double *dst, hipStream_t s);
 void launch_pad_square(int n, int np, const double *src, float *dst, hipStream_t s);
 
-// y = D x (the diagonal blocks themselves: (H0 + e_j) x_j - c_j x_j(N) e_N).
+// y = D x (the diagonal blocks themselves: (H0 + e_j) x_j - c_j x_j(N) e_N), or
+// y = base + D x when base is given (vector ALU, channel-major vectors).
 void launch_apply_diag(const Layout &L, const double *Q, const double *QT, const double *lam,
                        const double *e, const cx<double> *c, const cx<double> *x,
-                       cx<double> *y, const int *active, hipStream_t s);
+                       cx<double> *y, const int *active, hipStream_t s,
+                       const cx<double> *base = nullptr);
 
 // Eigensystem of a symmetric n x n matrix (one workgroup, Householder + implicit QL).
 // ticks (nullable, 8 entries): wall-clock stamps at the start, after the tridiagonalisation,

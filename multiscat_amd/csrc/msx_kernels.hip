@@ -442,8 +442,9 @@ template void launch_dinv<float>(const Layout &, const float *, const float *, c
 
 void launch_apply_diag(const Layout &L, const double *Q, const double *QT, const double *lam,
                        const double *e, const cx<double> *c, const cx<double> *x,
-                       cx<double> *y, const int *active, hipStream_t s) {
-    launch_dinv_impl<double>(L, Q, QT, lam, e, c, nullptr, x, nullptr, y, 1.0, 0, active, s);
+                       cx<double> *y, const int *active, hipStream_t s, const cx<double> *base) {
+    if (L.nm) throw Error(3, "vector D: channel-major vectors only");
+    launch_dinv_impl<double>(L, Q, QT, lam, e, c, nullptr, x, base, y, 1.0, 0, active, s);
 }
 
 // ---------------------------------------------------------------------------------------

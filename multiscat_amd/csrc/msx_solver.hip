@@ -458,6 +458,9 @@ struct msx_batch {
     std::vector<int> iters, info;
     std::vector<double> resid;
     bool factors_ready = false, facf_nm_ready = false, solved = false;
+    struct KdimMemo {
+        int restart = -1, maxiter = -1, dim = 0;
+    } kdim_memo[2];  // Krylov dimension per precision (FP64, mixed) for the last options seen
 
     long long n_elem() const { return total_ch * t->nz; }
     Layout layout() const {
@@ -734,10 +737,29 @@ template <> const cx<float> *Ops::c<float>() const { return B.cf_d.p; }
 template <> const double *Ops::fac<double>() const { return B.fac_d.p; }
 template <> const float *Ops::fac<float>() const { return L32.nm ? B.facf_nm_d.p : B.facf_d.p; }
 
-int krylov_dim(const msx_solve_options &o, int n_inst) {
+// Krylov dimension of a solve. restart > 0: that dimension. restart = 0 (unrestarted, the
+// reference's setting): maxiter, as long as the basis (n_elem complex values of elem_bytes
+// per vector, allocated in blocks as the iterations need them) and the per-instance
+// Hessenberg matrices ((k+1)^2 complex FP64 each, allocated up front) fit in a quarter of the
+// device memory free now; past that the largest dimension that fits, after which GMRES
+// restarts (at the benchmark's 256 conditions, 110,498 channels and nz = 150 this is ~260).
+int krylov_dim(const msx_solve_options &o, int n_inst, long long n_elem, size_t elem_bytes) {
     int m = o.restart > 0 ? o.restart : o.maxiter;
-    if (o.restart <= 0 && n_inst > 1) m = std::min(m, 256);
-    return std::max(1, std::min(m, KR_MAX_BASIS - 1));
+    m = std::max(1, std::min(m, KR_MAX_BASIS - 1));
+    if (o.restart <= 0) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+            (void)hipGetLastError();
+            free_b = (size_t)64 << 30;
+        }
+        const double budget = 0.25 * (double)free_b;
+        auto need = [&](double k) {
+            return (k + 1.0) * (double)n_elem * (double)elem_bytes +
+                   (double)n_inst * (k + 1.0) * (k + 1.0) * 16.0;
+        };
+        while (m > 16 && need(m) > budget) m = std::max(16, (int)(m * 0.9));
+    }
+    return m;
 }
 
 void check_options(const msx_solve_options *o) {
@@ -754,7 +776,7 @@ void check_options(const msx_solve_options *o) {
     if (o->maxiter <= 0) throw Error(MSX_ERR_INVALID, "maxiter must be positive");
     if (o->window <= 0) throw Error(MSX_ERR_INVALID, "window must be positive");
     if (o->warm_start != 0 && o->warm_start != 1) throw Error(MSX_ERR_INVALID, "warm_start is 0 or 1");
-    if (o->neumann < 0 || o->neumann > 4) throw Error(MSX_ERR_INVALID, "neumann is 0 .. 4");
+    if (o->neumann < 0) throw Error(MSX_ERR_INVALID, "neumann must be a non-negative degree");
     if (o->neumann > 1 && o->method != MSX_METHOD_JACOBI)
         throw Error(MSX_ERR_INVALID, "a Neumann factor of degree > 1 needs the block-Jacobi method");
     if (o->inner_orth < 0 || o->inner_orth > 2) throw Error(MSX_ERR_INVALID, "inner_orth is 0, 1 or 2");
@@ -788,7 +810,17 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     // A warm start (forward solves only) keeps x as the initial guess.
     const bool warm = o.warm_start == 1 && !transposed;
     if (!warm) MSX_HIP(hipMemsetAsync(x, 0, sizeof(cx<double>) * n, s));
-    const int kdim = krylov_dim(o, B.n_inst);
+    // sized once per batch and option pair (free memory shrinks once the basis exists, and a
+    // dimension that moved with it would reallocate the workspace on every solve)
+    auto &memo = B.kdim_memo[o.precision == MSX_PRECISION_FP64 ? 0 : 1];
+    if (memo.restart != o.restart || memo.maxiter != o.maxiter) {
+        memo.dim = krylov_dim(o, B.n_inst, B.n_elem(),
+                              o.precision == MSX_PRECISION_FP64 ? sizeof(cx<double>)
+                                                                : sizeof(cx<float>));
+        memo.restart = o.restart;
+        memo.maxiter = o.maxiter;
+    }
+    const int kdim = memo.dim;
     iters.assign(B.n_inst, 0);
     info.assign(B.n_inst, 0);
     resid.assign(B.n_inst, 0.0);
@@ -1291,8 +1323,14 @@ int msx_batch_solve(msx_batch *B, const msx_solve_options *opt) {
         launch_intensities(L, B->psi_d.p, B->b_d.p, B->a0_d.p, B->spec_d.p, B->P_d.p, t.stream);
         MSX_HIP(hipStreamSynchronize(t.stream));
         B->solved = true;
-        for (int b = 0; b < B->n_inst; ++b)
-            if (B->info[b]) return MSX_ERR_NOT_CONVERGED;
+        int missed = 0;
+        for (int b = 0; b < B->n_inst; ++b) missed += B->info[b] ? 1 : 0;
+        if (missed) {
+            set_last_error(std::to_string(missed) + " of " + std::to_string(B->n_inst) +
+                           " conditions did not reach the tolerance within maxiter");
+            return MSX_ERR_NOT_CONVERGED;
+        }
+        set_last_error("");
         return MSX_OK;
     } catch (const Error &e) {
         return fail(e);
@@ -1315,6 +1353,40 @@ int msx_batch_state(const msx_batch *B, double *psi) {
         if (!B || !psi) throw Error(MSX_ERR_INVALID, "null argument");
         B->psi_d.download(reinterpret_cast<cx<double> *>(psi), (size_t)B->n_elem(), B->t->stream);
         MSX_HIP(hipStreamSynchronize(B->t->stream));
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
+int msx_batch_apply_lower(msx_batch *B, int inverse, const double *state_in,
+                          double *state_out) {
+    try {
+        if (!B || !state_in || !state_out) throw Error(MSX_ERR_INVALID, "null argument");
+        if (inverse != 0 && inverse != 1) throw Error(MSX_ERR_INVALID, "inverse is 0 or 1");
+        msx_table &t = *B->t;
+        hipStream_t s = t.stream;
+        const size_t n = (size_t)B->n_elem();
+        const Layout L = B->layout();
+        B->build_factors();  // the table's eigensystem may have been re-prepared
+        msx_batch::need(B->tmp1, n);
+        msx_batch::need(B->tmp2, n);
+        msx_batch::need(B->tmp3, n);
+        B->tmp1.upload(reinterpret_cast<const cx<double> *>(state_in), n, s);
+        cx<double> *res = B->tmp1.p;
+        if (inverse) {
+            // L^-1: the block forward substitution of the Gauss-Seidel split, in place
+            launch_lower_solve(L, t.V.p, t.Q.p, t.QT.p, t.lam.p, B->e_d.p, B->c_d.p, B->fac_d.p,
+                               B->tmp1.p, nullptr, nullptr, 0.0, nullptr, s);
+        } else {
+            // L x = D x + sum_{i<j} V[G_j - G_i] x_i
+            launch_coupling<double>(L, t.V.p, B->tmp1.p, B->tmp2.p, COUPLE_LOWER, nullptr, s);
+            launch_apply_diag(L, t.Q.p, t.QT.p, t.lam.p, B->e_d.p, B->c_d.p, B->tmp1.p,
+                              B->tmp3.p, nullptr, s, B->tmp2.p);
+            res = B->tmp3.p;
+        }
+        MSX_HIP(hipMemcpyAsync(state_out, res, sizeof(cx<double>) * n, hipMemcpyDeviceToHost, s));
+        MSX_HIP(hipStreamSynchronize(s));
         return MSX_OK;
     } catch (const Error &e) {
         return fail(e);

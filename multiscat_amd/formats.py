@@ -39,13 +39,19 @@ def _crop_modes(a: np.ndarray, nkx: int, nky: int) -> np.ndarray:
     return out
 
 
-def interpolate_potential(values_xyz, z_values, *, shape, z_range):
+def interpolate_potential(values_xyz, z_values, *, shape, z_range, weighted: bool = False):
     """V_G(z_k) table from real-space samples.
 
     values_xyz[ix, iy, iz] = V(ix/Nx a_1 + iy/Ny a_2, z_values[iz]) over one unit cell,
     z_values ascending and evenly spaced (any spacing works for the linear interpolation);
     shape = (nkx, nky, nz) of the result, z_range = (z_min, z_max) of its Lobatto nodes.
-    V_G = (1/(Nx Ny)) sum_r V(r) exp(-i G.r), so V(r) = sum_G V_G exp(i G.r)."""
+    V_G = (1/(Nx Ny)) sum_r V(r) exp(-i G.r), so V(r) = sum_G V_G exp(i G.r).
+
+    The result is the solver's table: the values at the nodes. weighted=True multiplies
+    node k by the Lobatto basis weight 1/sqrt(w_k), as multiscat/interpolate.py:84-92 does
+    when it stores the interpolated values as the raw data of a Lobatto-basis operator; that
+    is the reference's operator storage, which its own test reads back by multiplying with
+    the same weights (tests/test_fixed_example.py:115-132), not the table the solve takes."""
     v = np.asarray(values_xyz)
     z_values = np.asarray(z_values, dtype=np.float64)
     if v.ndim != 3 or v.shape[2] != z_values.size:
@@ -61,6 +67,9 @@ def interpolate_potential(values_xyz, z_values, *, shape, z_range):
         for j in range(nky):
             out[i, j] = (np.interp(nodes, z_values, spec[i, j].real)
                          + 1j * np.interp(nodes, z_values, spec[i, j].imag))
+    if weighted:
+        _, w = B.lobatto_rule(nz, *z_range)
+        out *= (1.0 / np.sqrt(w))[None, None, :]
     return out
 
 

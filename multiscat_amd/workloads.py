@@ -21,7 +21,8 @@ Named workloads:
 * ``hex_batch``     -- hex_table (sigma 0.4 A, 150 nodes) at 256 seeded
   incidence conditions E ~ U(10, 100) meV, polar ~ U(0, 60) deg,
   azimuth ~ U(0, 60) deg, each with its own channel count: the channels with
-  |K+G|^2 <= 6 k^2, clipped to [200, 961].
+  |K+G|^2 <= 10 k^2 (the rule that gives hex_single's point, 60 meV at 30 deg,
+  706 channels), clipped to [200, 1200]: 200..876, mean 618 at the defaults.
 * ``hex_large``     -- 65 x 65 table, sigma 0.25 A, 300 nodes, 32 energies
   E ~ U(20, 150) meV at normal incidence, the 4000 lowest channels.
 """
@@ -89,7 +90,7 @@ def _condition(V, a1, a2, z_min, z_max, energy_mev, theta, phi, n_channels, fact
     else:
         # the lowest-n set must reach the specular channel (e_00 = -kz^2 is not always the
         # lowest e_G at oblique incidence)
-        rank_spec = int(np.nonzero(np.argsort(e.ravel(), kind="stable") == 0)[0][0])
+        rank_spec = int(np.nonzero(np.argsort(e.ravel()) == 0)[0][0])
         channels = B.lowest_channels(e, max(int(n_channels), rank_spec + 1))
     return Condition(energy_mev, theta, phi, k, e, a.ravel(), b.ravel(), c.ravel(), channels)
 
@@ -116,14 +117,18 @@ def square_morse(nz: int = 100, full_grid: bool = False) -> Workload:
     return w
 
 
-def hex_table(nk: int, nz: int, sigma: float, seed: int = 1234, rms_mev: float = 0.3):
-    """Corrugated hexagonal table in meV: (V (nk, nk, nz), a1, a2, z)."""
+def hex_table(nk: int, nz: int, sigma: float, seed: int = 1234, rms_mev: float = 0.3,
+              cg_noise: float = 0.0, noise_seed: int = 4321):
+    """Corrugated hexagonal table in meV: (V (nk, nk, nz), a1, a2, z). cg_noise > 0 multiplies
+    every c_G by (1 + cg_noise xi), xi standard normal from noise_seed (the perturbed potential
+    of a fitting target); the c_G themselves are the same draws as without noise."""
     a = 3.16
     a1, a2 = (a, 0.0), (0.5 * a, 0.5 * math.sqrt(3.0) * a)
     z, _ = B.lobatto_rule(nz, Z_MIN, Z_MAX)
     ga, gb = B.reciprocal_vectors(a1, a2)
     modes = B.fft_modes(nk)
     rng = np.random.default_rng(seed)
+    noise = np.random.default_rng(noise_seed)
     V = np.zeros((nk, nk, nz), dtype=np.complex128)
     D, alpha = 8.0, 1.2
     V[0, 0] = morse(z, D, alpha, 0.0)
@@ -136,6 +141,8 @@ def hex_table(nk: int, nz: int, sigma: float, seed: int = 1234, rms_mev: float =
                 continue
             g = mx * ga + my * gb
             cg = complex(rng.normal(0.0, half), rng.normal(0.0, half))
+            if cg_noise:
+                cg *= 1.0 + cg_noise * float(noise.standard_normal())
             val = cg * math.exp(-float(g @ g) * sigma * sigma) * decay
             V[ix, iy] = val
             V[(-ix) % nk, (-iy) % nk] = np.conj(val)
@@ -153,17 +160,23 @@ def hex_single(nz: int = 200, n_channels: int = 700, seed: int = 1234) -> Worklo
     return w
 
 
-def batch_channel_count(e: np.ndarray, k2: float, lo: int, hi: int, ratio: float = 6.0) -> int:
+BATCH_CUTOFF_RATIO = 10.0
+
+
+def batch_channel_count(e: np.ndarray, k2: float, lo: int, hi: int,
+                        ratio: float = BATCH_CUTOFF_RATIO) -> int:
     """Channels with |K+G|^2 <= ratio k^2 (e_G <= (ratio-1) k^2), clipped to [lo, hi]."""
     n = int(np.count_nonzero(e <= (ratio - 1.0) * k2))
     return int(min(max(n, lo), hi, e.size))
 
 
 def hex_batch(n_conditions: int = 256, nz: int = 150, seed: int = 1234, cond_seed: int = 99,
-              nk: int = 31, lo: int = 200, hi: int = 1200, channel_counts=None) -> Workload:
-    """``channel_counts`` (one per condition) overrides the |K+G|^2 <= 6 k^2 rule."""
+              nk: int = 31, lo: int = 200, hi: int = 1200, channel_counts=None,
+              cg_noise: float = 0.0) -> Workload:
+    """``channel_counts`` (one per condition) overrides the |K+G|^2 <= 10 k^2 rule;
+    ``cg_noise`` perturbs the table's c_G (hex_table)."""
     factor = B.mev_to_natural(HE4_MASS_KG)
-    Vm, a1, a2, _ = hex_table(nk, nz, 0.4, seed)
+    Vm, a1, a2, _ = hex_table(nk, nz, 0.4, seed, cg_noise=cg_noise)
     V = Vm * factor
     T = B.kinetic_matrix(nz, Z_MIN, Z_MAX)
     w = Workload("hex_batch", V, T, a1, a2, Z_MIN, Z_MAX)

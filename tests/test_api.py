@@ -93,10 +93,13 @@ def test_unknown_backend():
         api.get_scattering_matrix(wolken_condition(), api.OptimizationConfig(), backend="x")
 
 
-def test_von_neumann_orders():
-    with pytest.raises(NotImplementedError):
-        api.get_scattering_matrix_von_neumann(wolken_condition(), api.OptimizationConfig(),
-                                              order=2)
+def test_von_neumann_negative_order_raises_value_error():
+    """The reference rejects order < 0 with ValueError before any solve
+    (multiscat/multiscat/_scipy_von_neumann.py:145-147; test_fixed_example.py:411-414)."""
+    for order in (-1, -5):
+        with pytest.raises(ValueError):
+            api.get_scattering_matrix_von_neumann(wolken_condition(), api.OptimizationConfig(),
+                                                  order=order)
 
 
 def test_torch_complex_gradient_convention():

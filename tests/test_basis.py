@@ -112,7 +112,19 @@ def test_ab_waves_production_convention():
     assert a[0, 1] == 0 and b[1, 1] == 0
 
 
-def test_lowest_channels_is_stable_argsort():
-    e = np.array([[3.0, -1.0, 2.0], [-1.0, 0.5, 7.0]])
-    cx, cy = B.lowest_channels(e, 4)
-    assert list(zip(cx, cy)) == [(0, 1), (1, 0), (1, 1), (0, 2)]
+def test_lowest_channels_cut_inside_a_tie_keeps_the_reference_set():
+    """multiscat/multiscat/_scipy.py:414-416 cuts `np.argsort(e)` (numpy's default sort, not
+    stable). At normal incidence on a square grid e_G ties in groups of 4 and 8 (+-G_x, +-G_y
+    mirror images); a cut of 7 falls inside the second shell, where the default sort and a
+    stable one keep different channels. The set kept must be the default sort's."""
+    k = B.momentum_from_angles(0.0, 0.0, 20.0 * 1.913803)
+    e = B.perpendicular_kinetic_difference(k, (2.5, 0.0), (0.0, 2.5), 11, 11)
+    for n in (3, 7, 10, 11, 14, 20):
+        cx, cy = B.lowest_channels(e, n)
+        order = np.argsort(e.ravel())[:n]
+        assert list(zip(cx, cy)) == list(zip(*np.divmod(order, 11)))
+    stable = set(zip(*np.divmod(np.argsort(e.ravel(), kind="stable")[:7], 11)))
+    assert set(zip(*B.lowest_channels(e, 7))) != stable  # the cut really is inside a tie
+    # ascending energies in either case
+    cx, cy = B.lowest_channels(e, 30)
+    assert np.all(np.diff(e[cx, cy]) >= 0)

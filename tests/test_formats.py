@@ -38,6 +38,26 @@ def test_interpolate_known_modes_exactly():
     np.testing.assert_allclose(out, exp, atol=1e-12)
 
 
+def test_interpolate_weighted_form_carries_the_lobatto_basis_weights():
+    """weighted=True is the reference's storage (multiscat/interpolate.py:84-92): node k times
+    the Lobatto basis weight 1/sqrt(w_k), with sqrt(w_k) the reference Fortran helper's
+    get_lobatto_weights output (golden fixture helpers.npz) on the same rule."""
+    zmin, zmax, n = golden("helpers.npz")["lobatto0"]
+    sqrt_w = golden("helpers.npz")["lobatto0_w"]
+    nz = int(n)
+    x, y, z = _grid(6, 6, 17, zmin, zmax)
+    V = np.exp(-z)[None, None, :] * (1.0 + 0.2 * np.cos(2 * np.pi * x) + 0 * y)
+    plain = FM.interpolate_potential(V, z, shape=(3, 3, nz), z_range=(zmin, zmax))
+    weighted = FM.interpolate_potential(V, z, shape=(3, 3, nz), z_range=(zmin, zmax),
+                                        weighted=True)
+    np.testing.assert_allclose(weighted, plain / sqrt_w[None, None, :], rtol=1e-11)
+    # a constant potential's weighted form is 1/sqrt(w_k): largest at the two end nodes
+    const = FM.interpolate_potential(np.ones((2, 2, 5)), np.linspace(zmin, zmax, 5),
+                                     shape=(1, 1, nz), z_range=(zmin, zmax),
+                                     weighted=True)[0, 0].real
+    assert np.argmax(const) in (0, nz - 1) and np.all(const > 0)
+
+
 def test_interpolate_crops_and_pads_the_g_grid():
     nx, ny = 8, 8
     x, y, z = _grid(nx, ny, 5, 0.0, 4.0)

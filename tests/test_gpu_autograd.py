@@ -20,7 +20,7 @@ def test_autograd_gradient_matches_adjoint_and_fd():
     w = W.hex_batch(n_conditions=3, nz=24, nk=7, cond_seed=11, channel_counts=[12, 20, 30])
     opts = BT.SolveOptions(method="jacobi", eps=5e-14)
     V = torch.tensor(w.potential, dtype=torch.complex128, requires_grad=True)
-    P = AG.scattering_intensities(V, w.conditions, w.kinetic, opts)
+    P = AG.scattering_intensities(V, w.conditions, w.kinetic, opts, adjoint_options=opts)
     rng = np.random.default_rng(9)
     target = P.detach() * (1 + 0.05 * torch.from_numpy(rng.standard_normal(P.numel())))
     loss = ((P - target) ** 2).sum()
@@ -50,3 +50,27 @@ def test_autograd_gradient_matches_adjoint_and_fd():
         fd = fd[0] + 1j * fd[1]
         scale = max(abs(fd), 1e-3 * np.abs(g).max())
         assert abs(g[gx, gy, z] - fd) <= 1e-5 * scale, ((gx, gy, z), g[gx, gy, z], fd)
+
+
+def test_autograd_mixed_forward_default_adjoint_and_retained_graph():
+    """A mixed-precision forward solve: the backward runs the FP64 adjoint by default (the
+    gradient equals the FP64 adjoint of the same intensities to the mixed solve's accuracy),
+    and a retained graph can run backward twice (the gradients accumulate)."""
+    w = W.hex_batch(n_conditions=3, nz=24, nk=7, cond_seed=11, channel_counts=[12, 20, 30])
+    V = torch.tensor(w.potential, dtype=torch.complex128, requires_grad=True)
+    P = AG.scattering_intensities(V, w.conditions, w.kinetic,
+                                  BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11))
+    rng = np.random.default_rng(3)
+    target = P.detach() * (1 + 0.05 * torch.from_numpy(rng.standard_normal(P.numel())))
+    loss = ((P - target) ** 2).sum()
+    loss.backward(retain_graph=True)
+    g1 = V.grad.numpy().copy()
+    loss.backward()
+    np.testing.assert_allclose(V.grad.numpy(), 2.0 * g1, rtol=1e-12, atol=0)
+
+    fp64 = BT.SolveOptions(method="jacobi", eps=5e-14)
+    tab = BT.Table(w.potential, w.kinetic)
+    bt = BT.Batch(tab, w.conditions)
+    bt.solve(fp64)
+    direct = bt.adjoint(2.0 * (bt.intensities_flat() - target.numpy()), fp64)
+    assert np.abs(g1 - direct).max() <= 1e-6 * np.abs(direct).max()

@@ -12,6 +12,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+from multiscat_amd import basis as B
 from multiscat_amd import batch as BT
 from multiscat_amd import fortran_compat as F
 from multiscat_amd import workloads as W
@@ -62,7 +63,8 @@ def test_c0_square_morse_97_channels():
 
 def test_c1_hex_700_channels_nz200():
     """C1: ~700 channels of the 31 x 31 hexagonal table, nz = 200: FP64 (both
-    preconditioners) vs the oracle at full size, mixed vs FP64."""
+    preconditioners) vs the oracle at full size, and the mixed-precision result's own error
+    against the oracle (bar 1e-6)."""
     w = W.hex_single()
     c = w.conditions[0]
     ref = _oracle_P(w, c, eps=5e-12)
@@ -73,78 +75,119 @@ def test_c1_hex_700_channels_nz200():
     np.testing.assert_allclose(P64, ref, atol=1e-10)
     bt.solve(BT.SolveOptions(method="gauss_seidel", **TIGHT))
     np.testing.assert_allclose(bt.intensities()[0], P64, atol=1e-10)
-    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11))
-    assert np.abs(bt.intensities()[0] - P64).max() <= 1e-6
+    for eps in (5e-11, 5e-7):  # tight, and the benchmark's tolerance
+        bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=eps))
+        err = np.abs(bt.intensities()[0] - ref).max()
+        print(f"C1 mixed eps {eps:g}: max|dP| vs oracle {err:.2e}")
+        assert err <= 1e-6
     assert abs(_open_sums(w.conditions, [P64])[0] - 1.0) < 1e-9
 
 
 def test_c2_batch_256_conditions():
-    """C2: 256 uneven conditions (200..738 channels), nz = 150: mixed vs tight FP64 for every
-    condition, unitarity of every condition, three conditions against the oracle."""
+    """C2: 256 uneven conditions (200..876 channels), nz = 150: mixed at the benchmark's
+    tolerance vs tight FP64 for every condition, unitarity of every condition, three
+    conditions (FP64 and mixed) against the oracle."""
     w = W.hex_batch(n_conditions=256)
+    counts = np.array(w.channel_counts())
+    assert counts.min() >= 200 and counts.max() <= 1200 and 550 < counts.mean() < 700
     tab = BT.Table(w.potential, w.kinetic)
     bt = BT.Batch(tab, w.conditions)
     bt.solve(BT.SolveOptions(method="jacobi", **TIGHT))
     P64 = bt.intensities()
-    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11))
+    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-7))
     Pm = bt.intensities()
     assert max(np.abs(a - b).max() for a, b in zip(Pm, P64)) <= 1e-6
     assert np.abs(_open_sums(w.conditions, P64) - 1.0).max() < 1e-9
     for i in (0, 97, 255):
-        np.testing.assert_allclose(P64[i], _oracle_P(w, w.conditions[i]), atol=1e-10)
+        ref = _oracle_P(w, w.conditions[i])
+        np.testing.assert_allclose(P64[i], ref, atol=1e-10)
+        assert np.abs(Pm[i] - ref).max() <= 1e-6
 
 
-def test_c3_gradient_64_conditions_nz200():
-    """C3: gradient of sum_open (P - P_tgt)^2 over 64 conditions w.r.t. the full 31 x 31 x 200
-    table; the directional derivative along random directions against central differences
-    of the loss (relative 1e-5)."""
-    w = W.hex_batch(n_conditions=64, nz=200, cond_seed=31)
-    opts = BT.SolveOptions(method="jacobi", eps=5e-14)
-    rng = np.random.default_rng(2024)
-    open_masks = [c.e[c.channels[0], c.channels[1]] < 0 for c in w.conditions]
-    mask = np.concatenate(open_masks)
-
+def _loss_factory(w, target, mask, opts):
     def solve(V):
         tab = BT.Table(V, w.kinetic)
         bt = BT.Batch(tab, w.conditions)
         bt.solve(opts)
         return bt, tab, bt.intensities_flat()
 
+    def loss(V):
+        return float((((solve(V)[2] - target) ** 2) * mask).sum())
+
+    return solve, loss
+
+
+def test_c3_gradient_64_conditions_nz200():
+    """C3: loss = sum_open (P - P_tgt)^2 over 64 conditions, P_tgt from the same table with
+    every c_G perturbed by 5% seeded noise; the gradient w.r.t. the full 31 x 31 x 200 complex
+    table against central FP64 finite differences with h = 1e-6 meV on 20 seeded components
+    (real and imaginary parts), relative 1e-5. The components are drawn among those whose
+    gradient is at least 1e-3 of the largest: the rest are below what differences of a loss
+    computed to ~1e-15 can resolve at that step."""
+    w = W.hex_batch(n_conditions=64, nz=200, cond_seed=31)
+    w_tgt = W.hex_batch(n_conditions=64, nz=200, cond_seed=31, cg_noise=0.05)
+    opts = BT.SolveOptions(method="jacobi", eps=5e-14)
+    mask = np.concatenate([c.e[c.channels[0], c.channels[1]] < 0 for c in w.conditions])
+    solve, _ = _loss_factory(w, None, mask, opts)
+    target = solve(w_tgt.potential)[2]
+    solve, loss = _loss_factory(w, target, mask, opts)
     bt, tab, P = solve(w.potential)
-    target = P * (1.0 + 0.05 * rng.standard_normal(P.size))
-
-    def loss(Pv):
-        return float((((Pv - target) ** 2) * mask).sum())
-
+    assert np.abs(P - target).max() > 1e-4  # the target differs
     grad = bt.adjoint(2.0 * (P - target) * mask, opts)
-    scale = np.abs(w.potential).max()
-    for _ in range(3):
-        dV = rng.standard_normal(w.potential.shape) + 1j * rng.standard_normal(w.potential.shape)
-        dV *= np.abs(w.potential) / scale + 1e-3  # follow the table's own structure
-        dV[0, 0] = dV[0, 0].real  # Im V_00 does not enter the problem
-        # central differences at h and h/2, Richardson-extrapolated: the O(h^2) truncation of
-        # a step that moves every table entry at once is ~1e-5 here, the extrapolated O(h^4)
-        # error ~1e-10; the solver noise of the loss divided by h stays ~1e-10 relative
-        h = 2e-5 * scale / np.abs(dV).max()
-        d = []
-        for hh in (h, h / 2):
-            Lp = loss(solve(w.potential + hh * dV)[2])
-            Lm = loss(solve(w.potential - hh * dV)[2])
-            d.append((Lp - Lm) / (2 * hh))
-        fd = (4.0 * d[1] - d[0]) / 3.0
-        ad = float(np.sum(grad.real * dV.real + grad.imag * dV.imag))
-        assert abs(ad - fd) <= 1e-5 * abs(fd), (ad, fd)
+    h = 1e-6 * B.mev_to_natural(W.HE4_MASS_KG)
+    rng = np.random.default_rng(2024)
+    big = np.argwhere(np.abs(grad) >= 1e-3 * np.abs(grad).max())
+    picks = big[rng.choice(len(big), size=20, replace=False)]
+    worst = 0.0
+    for gx, gy, z in picks:
+        fd = []
+        for unit in (1.0, 1.0j):
+            Vp = w.potential.copy()
+            Vp[gx, gy, z] += h * unit
+            Vm = w.potential.copy()
+            Vm[gx, gy, z] -= h * unit
+            fd.append((loss(Vp) - loss(Vm)) / (2 * h))
+        fd = fd[0] + 1j * fd[1]
+        rel = abs(grad[gx, gy, z] - fd) / abs(fd)
+        worst = max(worst, rel)
+        assert rel <= 1e-5, ((gx, gy, z), grad[gx, gy, z], fd)
+    print(f"C3: worst relative gradient error over 20 components {worst:.2e}")
 
 
-def test_c4_large_table_4000_channels_nz300():
-    """C4: 65 x 65 table, nz = 300, ~4000 channels, normal incidence (4 of the 32 energies;
-    each at full size): unitarity of FP64 and mixed vs FP64."""
-    w = W.hex_large(n_conditions=4)
-    assert min(len(c.channels[0]) for c in w.conditions) >= 3900
+def test_c4_large_table_32_energies_forward_and_adjoint():
+    """C4: 65 x 65 table (sigma 0.25 A), nz = 300, 4000 channels, 32 energies at normal
+    incidence, state kept in HBM: unitarity of the FP64 solve, mixed vs FP64 (|dP| <= 1e-6);
+    the FP64 and the mixed-precision adjoint of sum_open (P - P_tgt)^2 (P_tgt = P (1 + 5%
+    seeded noise)), the mixed adjoint within 1e-6 of the FP64 one, and the directional
+    derivative of the loss along a random table direction against Richardson-extrapolated
+    central differences (relative 1e-5)."""
+    w = W.hex_large()
+    assert len(w.conditions) == 32
+    assert min(len(c.channels[0]) for c in w.conditions) >= 4000
+    fp64 = BT.SolveOptions(method="jacobi", eps=5e-13)
     tab = BT.Table(w.potential, w.kinetic)
     bt = BT.Batch(tab, w.conditions)
-    bt.solve(BT.SolveOptions(method="jacobi", eps=5e-12))
-    P64 = bt.intensities()
-    assert np.abs(_open_sums(w.conditions, P64) - 1.0).max() < 1e-8
-    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11))
-    assert max(np.abs(a - b).max() for a, b in zip(bt.intensities(), P64)) <= 1e-6
+    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-9))
+    P_mixed = bt.intensities_flat()
+    bt.solve(fp64)  # the adjoint below linearises around this FP64 state
+    P64 = bt.intensities_flat()
+    assert np.abs(_open_sums(w.conditions, bt.intensities()) - 1.0).max() < 1e-8
+    assert np.abs(P_mixed - P64).max() <= 1e-6
+    rng = np.random.default_rng(77)
+    target = P64 * (1.0 + 0.05 * rng.standard_normal(P64.size))
+    mask = np.concatenate([c.e[c.channels[0], c.channels[1]] < 0 for c in w.conditions])
+    dLdP = 2.0 * (P64 - target) * mask
+    g64 = bt.adjoint(dLdP, fp64)
+    gm = bt.adjoint(dLdP, BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11))
+    assert np.abs(gm - g64).max() <= 1e-6 * np.abs(g64).max()
+    _, loss = _loss_factory(w, target, mask, fp64)
+    dV = rng.standard_normal(w.potential.shape) + 1j * rng.standard_normal(w.potential.shape)
+    dV *= np.abs(w.potential) / np.abs(w.potential).max() + 1e-3
+    dV[0, 0] = dV[0, 0].real
+    h = 2e-5 * np.abs(w.potential).max() / np.abs(dV).max()
+    d = []
+    for hh in (h, h / 2):
+        d.append((loss(w.potential + hh * dV) - loss(w.potential - hh * dV)) / (2 * hh))
+    fd = (4.0 * d[1] - d[0]) / 3.0
+    ad = float(np.sum(g64.real * dV.real + g64.imag * dV.imag))
+    assert abs(ad - fd) <= 1e-5 * abs(fd), (ad, fd)

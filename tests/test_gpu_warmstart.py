@@ -15,6 +15,7 @@ from multiscat_amd import api
 from multiscat_amd import batch as BT
 from multiscat_amd import basis as B
 from multiscat_amd import workloads as W
+from oracle import oracle
 from tests.test_api import MEV, wolken_condition
 
 pytestmark = pytest.mark.gpu
@@ -48,13 +49,41 @@ def test_exact_and_zero_initial_guess(method, precision):
         np.testing.assert_allclose(bt.intensities_flat(), P0, atol=1e-10)
 
 
+@pytest.mark.parametrize("method,precision", [("gauss_seidel", "fp64"), ("jacobi", "fp64"),
+                                              ("jacobi", "mixed")])
+def test_warm_started_solves_match_oracle(method, precision):
+    """A warm start from an approximate solution (a solve to eps 5e-7, perturbed by 1e-6
+    relative noise) converges to the oracle's answer (C restatement of the reference GMRES,
+    eps 5e-13): P within 1e-10 for FP64 and 1e-6 for mixed precision, in fewer iterations than
+    from zero."""
+    w = W.hex_batch(n_conditions=4, nz=60, nk=11, cond_seed=4, channel_counts=[30, 55, 80, 41])
+    tab = BT.Table(w.potential, w.kinetic)
+    bt = BT.Batch(tab, w.conditions)
+    bt.solve(BT.SolveOptions(method="jacobi", eps=5e-7))
+    rng = np.random.default_rng(17)
+    guess = [p * (1.0 + 1e-6 * (rng.standard_normal(p.shape) + 1j * rng.standard_normal(p.shape)))
+             for p in bt.state()]
+    opts = dict(method=method, precision=precision, eps=5e-12)
+    bt.solve(BT.SolveOptions(**opts))
+    cold_it = bt.stats()["iterations"].copy()
+    bt.set_state(guess)
+    bt.solve(BT.SolveOptions(warm_start=True, **opts))
+    assert (bt.stats()["iterations"] < cold_it).all(), (bt.stats()["iterations"], cold_it)
+    tol = 1e-10 if precision == "fp64" else 1e-6
+    for c, P in zip(w.conditions, bt.intensities()):
+        psi = oracle.solve(1, 0, w.potential, c.e, c.wave_a, c.wave_b, c.wave_c, w.kinetic,
+                           channels=c.channels, eps=5e-13)
+        ref = oracle.intensities(psi, c.wave_a, c.wave_b)[c.channels[0], c.channels[1]]
+        np.testing.assert_allclose(P, ref, atol=tol)
+
+
 def test_energy_continuation_saves_iterations():
-    """Wolken problem: the solution at 20 meV as the initial guess at 20.5 meV."""
+    """Wolken problem: the solution at 20 meV as the initial guess at 20.5 meV, on the batch
+    (plain psi) and through the API (the state in the reference's L psi form)."""
     cfg = api.OptimizationConfig(precision=1e-10, max_iterations=2000)
     c20 = wolken_condition()
     c205 = api.ScatteringCondition.from_angles(mass=c20.mass, energy=20.5 * MEV,
                                                theta=np.deg2rad(30), potential=c20.potential)
-    psi20 = api.get_scattering_state(c20, cfg)
 
     def batch_for(cond):
         inp = api.solver_inputs(cond)
@@ -64,8 +93,13 @@ def test_energy_continuation_saves_iterations():
         return tab, BT.Batch(tab, [dict(e=e, wave_a=inp.wave_a, wave_b=inp.wave_b,
                                         wave_c=inp.wave_c, channels=ch)]), ch
 
-    tab, bt, ch = batch_for(c205)
     opts = dict(method="jacobi", eps=5e-11)
+    tab20, bt20, ch20 = batch_for(c20)
+    bt20.solve(BT.SolveOptions(**opts))
+    psi20 = np.zeros((10, 10, 550), complex)
+    psi20[ch20] = bt20.state()[0]
+
+    tab, bt, ch = batch_for(c205)
     bt.solve(BT.SolveOptions(**opts))
     cold_it, P_cold = int(bt.stats()["iterations"][0]), bt.intensities_flat()
     bt.set_state([psi20[ch[0], ch[1]]])
@@ -73,7 +107,8 @@ def test_energy_continuation_saves_iterations():
     warm_it = int(bt.stats()["iterations"][0])
     assert warm_it < cold_it, (warm_it, cold_it)
     np.testing.assert_allclose(bt.intensities_flat(), P_cold, atol=1e-9)
-    # through the API: the same answer with the guess attached to the condition
-    P_api = api.get_scattering_matrix(dataclasses.replace(c205, initial_state=psi20), cfg,
+    # through the API: the state of 20 meV (L psi) attached to the 20.5 meV condition
+    state20 = api.get_scattering_state(c20, cfg)
+    P_api = api.get_scattering_matrix(dataclasses.replace(c205, initial_state=state20), cfg,
                                       backend="gpu").real
     np.testing.assert_allclose(P_api[ch[0], ch[1]], P_cold, atol=1e-9)
