@@ -213,6 +213,22 @@ int msx_table_eig_phases(const msx_table *table, double *ms);
  * for callers that need it outside a table. */
 int msx_symmetric_eigensystem(int n, const double *A, int method, double *lam, double *Q);
 
+/* Largest n msx_symmetric_eigensystem_noalloc takes. */
+#define MSX_NOALLOC_EIG_MAX_N 512
+
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the eigensystem of
+ * msx_symmetric_eigensystem (Householder tridiagonalisation + divide and conquer, FP64) on
+ * device memory the caller owns, with no allocation of any kind. All pointers are device
+ * pointers: A (n x n row-major, symmetric, read only), lam[n] (ascending), Q (n x n row-major,
+ * A = Q diag(lam) Q^T), info (one int: 0 on success, nonzero if an eigenvalue is not finite).
+ * stream: a hipStream_t, or null for the null stream. The work is only queued: the results
+ * and info are ready once the stream has reached this point. 2 <= n <= MSX_NOALLOC_EIG_MAX_N
+ * (512), else MSX_ERR_DIMS. Intermediates live in one static device array of about 10 MiB
+ * per GPU (held until the process ends); calls that overlap on one GPU race on it. */
+int msx_symmetric_eigensystem_noalloc(int n, const double *A, double *lam, double *Q, int *info,
+                                      void *stream);
+
 /* Builds a batch of n_cond conditions on `table`. Channel lists are concatenated:
  * condition b owns channels [off_b, off_b + n_channels[b]) with off_b the prefix sum.
  *   chx, chy  storage indices (0 <= chx < nkx, 0 <= chy < nky) in solve order

@@ -35,7 +35,7 @@ EXPORTED = (
     "msx_debug_solve_lower_block", "msx_debug_apply_coupling", "msx_debug_apply_diagonal",
     "msx_default_options", "msx_table_create",
     "msx_table_prepare", "msx_table_destroy", "msx_table_eigensystem", "msx_table_eig_phases",
-    "msx_symmetric_eigensystem", "msx_batch_create",
+    "msx_symmetric_eigensystem", "msx_symmetric_eigensystem_noalloc", "msx_batch_create",
     "msx_batch_destroy", "msx_batch_solve", "msx_batch_state", "msx_batch_set_state",
     "msx_batch_intensities",
     "msx_batch_stats", "msx_batch_apply_lower", "msx_batch_device_state",
@@ -103,6 +103,7 @@ def _declare(lib):
         "msx_table_eigensystem": (i32, [vp, vp, vp]),
         "msx_table_eig_phases": (i32, [vp, vp]),
         "msx_symmetric_eigensystem": (i32, [i32, vp, i32, vp, vp]),
+        "msx_symmetric_eigensystem_noalloc": (i32, [i32, vp, vp, vp, vp, vp]),
         "msx_batch_create": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, pp]),
         "msx_batch_destroy": (i32, [vp]),
         "msx_batch_solve": (i32, [vp, C.POINTER(SolveOptions)]),
