@@ -37,4 +37,12 @@ void launch_dft_coupling(const Layout &L, const DftTables<T> &D, const cx<T> *x,
                          const int *active, hipStream_t s, int active_hint = -1,
                          const double *x_norm = nullptr, cx<T> *x_copy = nullptr);
 
+// Adds dL/dV from the coupling to grad[g][z] (complex FP64, storage order): conj(-C[g](z)),
+// Re(-C[0](z)) at g = 0, with C[g](z) = sum_b sum_j lam_j(z) psi_i(z) over the channel pairs
+// with G_i = G_j - g (mod grid), as a cross-correlation through the DFT tables (np 16 or 32).
+// part: workspace of per-chunk partial sums, grown as needed. Channel-major states.
+void launch_dft_gradient(const Layout &L, const DftTables<double> &D, const cx<double> *lam,
+                         const cx<double> *psi, cx<double> *grad, DBuf<cx<double>> &part,
+                         hipStream_t s);
+
 }  // namespace msx

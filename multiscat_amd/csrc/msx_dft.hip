@@ -27,6 +27,7 @@
 #include "msx_dft.h"
 #include "msx_mfma.h"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace msx {
@@ -505,6 +506,242 @@ template void launch_dft_coupling<float>(const Layout &, const DftTables<float> 
 template void launch_dft_coupling<double>(const Layout &, const DftTables<double> &,
                                           const cx<double> *, cx<double> *, const int *,
                                           hipStream_t, int, const double *, cx<double> *);
+
+// ---------------------------------------------------------------------------------------
+// Gradient of the coupling with respect to the table: the adjoint of the convolution above,
+// a circular cross-correlation on the G grid,
+//     C[g](z) = sum_b sum_j lam_j(z) psi_i(z),   G_i = G_j - g (mod grid),
+// which per condition and node is F^-1( F(Lam) .* F(Psi') ): Lam the adjoint state on the
+// grid at G_j, Psi' the forward state at the reflected points -G_i. The products are summed
+// over conditions in the transformed domain and inverted once per node, so a (condition,
+// node) pair costs two forward transforms (four small complex GEMMs on the matrix cores)
+// instead of nc^2 gathered products. FP64 (the adjoint's precision), channel-major states.
+// dft_gradient_kernel: one wavefront per node, conditions [y * per_chunk, ...) of chunk y,
+// the summed product written to part[y][z][kx][ky] (the accumulator tiles' element order).
+// dft_gradient_finish_kernel: one wavefront per node sums the chunks, applies
+// conj(F_x) . conj(F_y) / (nkx nky) and adds conj(-C) (Re(-C) at g = 0, where only Re V_00
+// enters the problem) to grad[g][z].
+// ---------------------------------------------------------------------------------------
+template <int NP, bool SQ>
+struct DftRegs {
+    static constexpr int NT = DftGeom<NP>::NT, KK = NP / 4;
+    double fxr[NT][KK], fxi[NT][KK], fyr[NT][KK], fyi[NT][KK];
+    __device__ __forceinline__ void load(const cx<double> *Fx, const cx<double> *Fy) {
+        const int lane = threadIdx.x & 63, l16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int kk = 0; kk < KK; ++kk) {
+                const cx<double> f = Fx[(t * 16 + l16) * NP + kk * 4 + k4];
+                fxr[t][kk] = f.re;
+                fxi[t][kk] = f.im;
+                if constexpr (!SQ) {
+                    const cx<double> g = Fy[(kk * 4 + k4) * NP + t * 16 + l16];
+                    fyr[t][kk] = g.re;
+                    fyi[t][kk] = g.im;
+                }
+            }
+    }
+};
+
+// The 2-D transform of the grid in (sr, si) (row stride SB): (cr, ci) = F_x X F_y, or with
+// CONJ = conj(F_x) X conj(F_y); the slab is overwritten by the intermediate.
+template <int NP, bool SQ, bool CONJ>
+__device__ __forceinline__ void dft2d(DftRegs<NP, SQ> &F, double *sr, double *si,
+                                      typename Mfma<double>::acc (&cr)[DftGeom<NP>::NT][DftGeom<NP>::NT],
+                                      typename Mfma<double>::acc (&ci)[DftGeom<NP>::NT][DftGeom<NP>::NT]) {
+    using G = DftGeom<NP>;
+    if constexpr (SQ) {
+        typename Mfma<double>::acc c3[G::NT][G::NT];
+        cgemm3_reg_a<double, NP, CONJ>(F.fxr, F.fxi, sr, si, G::SB, cr, ci, c3);
+        combine3<double, NP, CONJ>(cr, ci, c3);
+        __builtin_amdgcn_wave_barrier();
+        store_tiles<double, NP>(sr, si, G::SA, cr, ci);
+        __builtin_amdgcn_wave_barrier();
+        cgemm3_reg_b<double, NP, CONJ>(sr, si, G::SA, F.fxr, F.fxi, cr, ci, c3);
+        combine3<double, NP, CONJ>(cr, ci, c3);
+    } else {
+        cgemm_reg_a<double, NP>(F.fxr, F.fxi, CONJ, sr, si, G::SB, cr, ci);
+        __builtin_amdgcn_wave_barrier();
+        store_tiles<double, NP>(sr, si, G::SA, cr, ci);
+        __builtin_amdgcn_wave_barrier();
+        cgemm_reg_b<double, NP>(sr, si, G::SA, F.fyr, F.fyi, CONJ, cr, ci);
+    }
+}
+
+template <int NP>
+__device__ __forceinline__ void zero_slab(double *sr) {
+    using acc_t = typename Mfma<double>::acc;
+    acc_t *p4 = reinterpret_cast<acc_t *>(sr);  // both planes (contiguous)
+    constexpr int N4 = 2 * DftGeom<NP>::PLANE * (int)sizeof(double) / (int)sizeof(acc_t);
+    for (int idx = threadIdx.x & 63; idx < N4; idx += 64) p4[idx] = acc_t{0, 0, 0, 0};
+}
+
+template <int NP, int WPB, bool SQ>
+__global__ __launch_bounds__(64 * WPB, 1) void dft_gradient_kernel(
+    Layout L, const cx<double> *__restrict__ Fx, const cx<double> *__restrict__ Fy,
+    const cx<double> *__restrict__ lam, const cx<double> *__restrict__ psi,
+    cx<double> *__restrict__ part, int per_chunk) {
+    using G = DftGeom<NP>;
+    using acc_t = typename Mfma<double>::acc;
+    constexpr int NT = G::NT, SA = G::SA, SB = G::SB;
+    const int nz = L.nz, nkx = L.nkx, nky = L.nky;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 15, g = lane >> 4;
+    const int z = blockIdx.x * WPB + wave;
+    if (z >= nz) return;  // no workgroup barrier below: a wavefront may leave on its own
+    extern __shared__ double dft_smem[];
+    double *sr = dft_smem + (size_t)wave * 4 * G::PLANE;
+    double *si = sr + G::PLANE;
+    double *hr = si + G::PLANE;  // F(Lam) of the current condition, accumulator element order
+    double *hi = hr + G::PLANE;
+    DftRegs<NP, SQ> F;
+    F.load(Fx, Fy);
+    acc_t ar[NT][NT], ai[NT][NT], cr[NT][NT], ci[NT][NT];
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj) ar[ti][tj] = ai[ti][tj] = acc_t{0, 0, 0, 0};
+    const int b0 = blockIdx.y * per_chunk;
+    const int b1 = min(L.n_inst, b0 + per_chunk);
+    using plain_t = std::integral_constant<bool, false>;
+    for (int b = b0; b < b1; ++b) {
+        const int nc = L.nc[b];
+        const long long off = L.off[b];
+        // F(Lam): the adjoint state at the channels' own grid points
+        zero_slab<NP>(sr);
+        __builtin_amdgcn_wave_barrier();
+        for (int j = lane; j < nc; j += 64) {
+            const int p = L.chx[off + j] * SB + L.chy[off + j];
+            const cx<double> v = lam[(off + j) * nz + z];
+            sr[p] = v.re;
+            si[p] = v.im;
+        }
+        __builtin_amdgcn_wave_barrier();
+        dft2d<NP, SQ, plain_t::value>(F, sr, si, cr, ci);
+        __builtin_amdgcn_wave_barrier();
+        store_tiles<double, NP>(hr, hi, SA, cr, ci);
+        // F(Psi'): the forward state at the reflected grid points -G_i
+        zero_slab<NP>(sr);
+        __builtin_amdgcn_wave_barrier();
+        for (int i = lane; i < nc; i += 64) {
+            const int rx = L.chx[off + i], ry = L.chy[off + i];
+            const int p = (rx == 0 ? 0 : nkx - rx) * SB + (ry == 0 ? 0 : nky - ry);
+            const cx<double> v = psi[(off + i) * nz + z];
+            sr[p] = v.re;
+            si[p] = v.im;
+        }
+        __builtin_amdgcn_wave_barrier();
+        dft2d<NP, SQ, plain_t::value>(F, sr, si, cr, ci);
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = ti * 16 + Mfma<double>::row(g, v), c = tj * 16 + col;
+                    const double lr = hr[r * SA + c], li = hi[r * SA + c];
+                    const double pr = cr[ti][tj][v], pi = ci[ti][tj][v];
+                    ar[ti][tj][v] = fma(lr, pr, fma(-li, pi, ar[ti][tj][v]));
+                    ai[ti][tj][v] = fma(lr, pi, fma(li, pr, ai[ti][tj][v]));
+                }
+        __builtin_amdgcn_wave_barrier();
+    }
+    cx<double> *out = part + ((size_t)blockIdx.y * nz + z) * NP * NP;
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = ti * 16 + Mfma<double>::row(g, v), c = tj * 16 + col;
+                out[r * NP + c] = mk(ar[ti][tj][v], ai[ti][tj][v]);
+            }
+}
+
+template <int NP, int WPB, bool SQ>
+__global__ __launch_bounds__(64 * WPB, 1) void dft_gradient_finish_kernel(
+    Layout L, const cx<double> *__restrict__ Fx, const cx<double> *__restrict__ Fy,
+    const cx<double> *__restrict__ part, int n_chunks, cx<double> *__restrict__ grad) {
+    using G = DftGeom<NP>;
+    using acc_t = typename Mfma<double>::acc;
+    constexpr int NT = G::NT, SB = G::SB;
+    const int nz = L.nz, nky = L.nky, nG = L.nkx * L.nky;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int z = blockIdx.x * WPB + wave;
+    if (z >= nz) return;
+    extern __shared__ double dft_smem[];
+    double *sr = dft_smem + (size_t)wave * 2 * G::PLANE;
+    double *si = sr + G::PLANE;
+    DftRegs<NP, SQ> F;
+    F.load(Fx, Fy);
+    for (int idx = lane; idx < NP * NP; idx += 64) {
+        cx<double> acc = mk(0.0, 0.0);
+        for (int c = 0; c < n_chunks; ++c) acc = acc + part[((size_t)c * nz + z) * NP * NP + idx];
+        const int r = idx / NP, cc = idx - r * NP;
+        sr[r * SB + cc] = acc.re;
+        si[r * SB + cc] = acc.im;
+    }
+    __builtin_amdgcn_wave_barrier();
+    acc_t cr[NT][NT], ci[NT][NT];
+    using conj_t = std::integral_constant<bool, true>;
+    dft2d<NP, SQ, conj_t::value>(F, sr, si, cr, ci);
+    __builtin_amdgcn_wave_barrier();
+    store_tiles<double, NP>(sr, si, SB, cr, ci);
+    __builtin_amdgcn_wave_barrier();
+    const double scale = 1.0 / (double)nG;
+    for (int gidx = lane; gidx < nG; gidx += 64) {
+        const int gx = gidx / nky, gy = gidx - gx * nky;
+        const double re = sr[gx * SB + gy] * scale, im = si[gx * SB + gy] * scale;
+        const cx<double> add = gidx == 0 ? mk(-re, 0.0) : mk(-re, im);  // conj(-C)
+        cx<double> &dst = grad[(size_t)gidx * nz + z];
+        dst = dst + add;
+    }
+}
+
+template <int NP, bool SQ>
+static void launch_dft_gradient_np(const Layout &L, const DftTables<double> &D,
+                                   const cx<double> *lam, const cx<double> *psi, cx<double> *grad,
+                                   DBuf<cx<double>> &part, hipStream_t s) {
+    using G = DftGeom<NP>;
+    constexpr int WPB = 2;
+    const int zblocks = (L.nz + WPB - 1) / WPB;
+    // enough workgroups to fill the device (about 1,024 of two wavefronts), at most one
+    // chunk per condition
+    const int want = std::max(1, (1024 + zblocks - 1) / zblocks);
+    const int n_chunks = std::min(L.n_inst, want);
+    const int per_chunk = (L.n_inst + n_chunks - 1) / n_chunks;
+    const int used = (L.n_inst + per_chunk - 1) / per_chunk;
+    const size_t need = (size_t)used * L.nz * NP * NP;
+    if (part.n < need) part.alloc(need);
+    const size_t shm = sizeof(double) * (size_t)WPB * 4 * G::PLANE;
+    set_dynamic_lds((const void *)dft_gradient_kernel<NP, WPB, SQ>, shm);
+    hipLaunchKernelGGL((dft_gradient_kernel<NP, WPB, SQ>), dim3(zblocks, used), dim3(64 * WPB), shm,
+                       s, L, D.Fx, D.Fy, lam, psi, part.p, per_chunk);
+    MSX_LAUNCH_CHECK();
+    const size_t shm2 = sizeof(double) * (size_t)WPB * 2 * G::PLANE;
+    set_dynamic_lds((const void *)dft_gradient_finish_kernel<NP, WPB, SQ>, shm2);
+    hipLaunchKernelGGL((dft_gradient_finish_kernel<NP, WPB, SQ>), dim3(zblocks), dim3(64 * WPB),
+                       shm2, s, L, D.Fx, D.Fy, part.p, used, grad);
+    MSX_LAUNCH_CHECK();
+}
+
+void launch_dft_gradient(const Layout &L, const DftTables<double> &D, const cx<double> *lam,
+                         const cx<double> *psi, cx<double> *grad, DBuf<cx<double>> &part,
+                         hipStream_t s) {
+    if (L.nm) throw Error(3, "DFT gradient: channel-major states only");
+    const bool sq = L.nkx == L.nky;
+    if (D.np == 16) {
+        if (sq) launch_dft_gradient_np<16, true>(L, D, lam, psi, grad, part, s);
+        else launch_dft_gradient_np<16, false>(L, D, lam, psi, grad, part, s);
+    } else if (D.np == 32) {
+        if (sq) launch_dft_gradient_np<32, true>(L, D, lam, psi, grad, part, s);
+        else launch_dft_gradient_np<32, false>(L, D, lam, psi, grad, part, s);
+    } else {
+        throw Error(3, "DFT gradient supports tables up to 32 x 32");
+    }
+}
 
 // ---------------------------------------------------------------------------------------
 // Set-up: DFT matrices (FP64 twiddles, k g reduced mod n) and Vhat(z) = F_x V'(z) F_y / nG.

@@ -452,6 +452,7 @@ struct msx_batch {
     DBuf<cx<float>> cf_d;
     // work vectors and Krylov workspaces (allocated on first use, reused by later solves)
     DBuf<cx<double>> source, src, rhs, tmp1, tmp2, tmp3;
+    DBuf<cx<double>> adj_src, adj_state, grad_d, grad_part;  // adjoint solve and gradient
     DBuf<cx<float>> f_rhs, f_tmp1, f_tmp2;
     std::unique_ptr<Krylov<double>> kr64, kr_outer;
     std::unique_ptr<Krylov<float>> kr32;
@@ -1447,16 +1448,42 @@ int msx_batch_adjoint(msx_batch *B, const msx_solve_options *opt, const double *
         const size_t n = (size_t)B->n_elem();
         msx_batch::need(B->dLdP_d, (size_t)B->total_ch);
         B->dLdP_d.upload(dL_dP, (size_t)B->total_ch, s);
-        DBuf<cx<double>> g(n), lam(n);
-        launch_adjoint_source(L, B->spec_d.p, B->psi_d.p, B->b_d.p, B->a0_d.p, B->dLdP_d.p, g.p, s);
+        // buffers owned by the batch, reused by later adjoints
+        msx_batch::need(B->adj_src, n);
+        msx_batch::need(B->adj_state, n);
+        launch_adjoint_source(L, B->spec_d.p, B->psi_d.p, B->b_d.p, B->a0_d.p, B->dLdP_d.p,
+                              B->adj_src.p, s);
         std::vector<int> it, inf;
         std::vector<double> res;
-        solve_system(*B, *opt, g.p, lam.p, true, it, inf, res, false);
+        solve_system(*B, *opt, B->adj_src.p, B->adj_state.p, true, it, inf, res, false);
         const size_t nG = t.nG();
-        DBuf<cx<double>> grad(nG * t.nz);
-        MSX_HIP(hipMemsetAsync(grad.p, 0, sizeof(cx<double>) * nG * t.nz, s));
-        launch_potential_gradient_map(L, B->chmap_d.p, lam.p, B->psi_d.p, grad.p, s);
-        grad.download(reinterpret_cast<cx<double> *>(grad_potential), nG * t.nz, s);
+        msx_batch::need(B->grad_d, nG * t.nz);
+        MSX_HIP(hipMemsetAsync(B->grad_d.p, 0, sizeof(cx<double>) * nG * t.nz, s));
+        const bool dft_form = opt->coupling == 0 && t.dft.np != 0;
+        {
+            // correlation on the grid: two forward 2-D transforms per condition and node (four
+            // complex GEMMs), one inverse per node; the direct form gathers nc^2 products
+            Work fl, by;
+            if (profiling()) {
+                const double nz = t.nz, nkx = t.nkx, nky = t.nky, ng = nkx * nky;
+                double flops = 0, bytes = 0;
+                for (int b = 0; b < B->n_inst; ++b) {
+                    flops += dft_form ? nz * (16.0 * ng * (nkx + nky) + 8.0 * ng)
+                                      : 8.0 * B->nc[b] * (double)B->nc[b] * nz;
+                    bytes += 2.0 * B->nc[b] * nz * sizeof(cx<double>);
+                }
+                fl = Work(flops);
+                by = Work(bytes + ng * nz * sizeof(cx<double>));
+            }
+            ProfScope ps(dft_form ? "gradient_dft_f64" : "gradient_direct_f64", s, fl, by);
+            if (dft_form)
+                launch_dft_gradient(L, t.dft.d64(), B->adj_state.p, B->psi_d.p, B->grad_d.p,
+                                    B->grad_part, s);
+            else
+                launch_potential_gradient_map(L, B->chmap_d.p, B->adj_state.p, B->psi_d.p,
+                                              B->grad_d.p, s);
+        }
+        B->grad_d.download(reinterpret_cast<cx<double> *>(grad_potential), nG * t.nz, s);
         MSX_HIP(hipStreamSynchronize(s));
         for (int b = 0; b < B->n_inst; ++b)
             if (inf[b]) return MSX_ERR_NOT_CONVERGED;
