@@ -94,6 +94,9 @@ def main() -> int:
     ap.add_argument("--skip-baselines", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--skip-gradient", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every kernel one by one instead of replaying captured iterations")
+    ap.add_argument("--no-autotune", action="store_true", help="built-in launch shapes")
     ap.add_argument("--grad-steps", type=int, default=3)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 20 plain device copies of MIB MiB (calibrates the HBM byte "
@@ -119,7 +122,8 @@ def main() -> int:
     bt = BT.Batch(tab, w.conditions)
     eps = 5e-7        # mixed: |dP| ~ 3e-8 against the tight solve, 30x inside the bar
     eps_base = 5e-6   # the reference's default precision 1e-5 (nsig 5) for the FP64 baselines
-    mixed = BT.SolveOptions(method="jacobi", precision="mixed", eps=eps, diag=args.diag)
+    mixed = BT.SolveOptions(method="jacobi", precision="mixed", eps=eps, diag=args.diag,
+                            graphs=not args.no_graphs, autotune=not args.no_autotune)
 
     def step(opts):
         tab.prepare()
@@ -131,11 +135,14 @@ def main() -> int:
 
     for _ in range(args.warmup):
         step(mixed)
+    # the timed steps, each kernel launch bracketed by HIP events on the library's stream (the
+    # roofline below comes from them); launched one by one, as the events need
+    eager = BT.SolveOptions(**{**mixed.__dict__, "graphs": False})
     _lib.profile_enable(True)
     _sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step(mixed)
+        step(eager)
     _sync()
     elapsed = time.perf_counter() - t0
     prof = _lib.profile_report()
@@ -143,6 +150,21 @@ def main() -> int:
     P_mixed = bt.intensities_flat()
     st = bt.stats()
     value = args.conditions * args.steps / elapsed
+    prof_steps, prof_elapsed = args.steps, elapsed
+    # the same steps without instrumentation: launched one by one, and with each Arnoldi
+    # iteration replayed from a captured HIP graph
+    uninstrumented = {}
+    for label, opts in (("eager", eager), ("graph_replay", BT.SolveOptions(**{**mixed.__dict__, "graphs": True}))):
+        step(opts)
+        _sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(opts)
+        _sync()
+        dt = time.perf_counter() - t0
+        uninstrumented[label] = {"solves_per_s": args.conditions * args.steps / dt,
+                                 "ms_per_step": dt / args.steps * 1e3,
+                                 "max_abs_dP": float(np.abs(bt.intensities_flat() - P_ref).max())}
 
     # roofline of the dominant instrumented kernel
     kern = max(prof.items(), key=lambda kv: kv[1]["ms"]) if prof else None
@@ -166,7 +188,7 @@ def main() -> int:
         roofline.update({"launches": d["count"], "avg_launch_us": avg_s * 1e6,
                          "algorithmic_flops_per_launch": per_launch_flops,
                          "algorithmic_bytes_per_launch": per_launch_bytes,
-                         "share_of_step": d["ms"] / 1e3 / elapsed})
+                         "share_of_step": d["ms"] / 1e3 / prof_elapsed})
     def _rate(name, v):
         secs = v["ms"] / 1e3
         if name.startswith("arnoldi"):
@@ -176,7 +198,7 @@ def main() -> int:
         return {"achieved_TFLOP_s": tf,
                 "frac_of_peak": tf / PEAK_TFLOPS[prec] if tf is not None else None}
 
-    kernels = {k: {"ms_per_step": v["ms"] / args.steps, "launches_per_step": v["count"] / args.steps,
+    kernels = {k: {"ms_per_step": v["ms"] / prof_steps, "launches_per_step": v["count"] / prof_steps,
                    **_rate(k, v)}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
 
@@ -195,6 +217,8 @@ def main() -> int:
                      "meets_bar": bool(np.abs(P_mixed - P_ref).max() <= 1e-6)},
         "roofline": roofline,
         "kernels": kernels,
+        "tuning": bt.tuning(),
+        "uninstrumented": uninstrumented,
     }
 
     if not args.skip_baselines:

@@ -135,8 +135,9 @@ int msx_debug_solve_lower_block(int nkx, int nky, int nz, const double *potentia
 /* Debug face of the channel coupling in the style of the debug group above (extension):
  * y_j = sum_i V[G_j - G_i] x_i over i > j (mode 0), i != j (mode 1) or i < j (mode 2) for an
  * explicit channel list; precision 0 = FP64, 1 = FP32 (x, y still passed as FP64);
- * algorithm 0 = direct sum, 1 = DFT form (mode 1 only). V is C order (nkx, nky, nz) here,
- * states are (nc, nz) row-major. */
+ * algorithm 0 = direct sum, 1 = DFT form (mode 1 only; 2, 3, 4: the DFT form with 1, 2 or 8
+ * nodes per wavefront instead of the built-in 4). V is C order (nkx, nky, nz) here, states
+ * are (nc, nz) row-major. */
 int msx_debug_apply_coupling(int nkx, int nky, int nz, const double *potential, int nc,
                              const int *chx, const int *chy, const double *state_in,
                              double *state_out, int mode, int precision, int algorithm);
@@ -145,7 +146,8 @@ int msx_debug_apply_coupling(int nkx, int nky, int nz, const double *potential, 
  * (inverse = 0) with D_j = T + diag(Re V_00) + e_j - c_j e_N e_N^T, for nc channels
  * (multiscat/multiscat/_scipy.py:249-354). potential00: complex[nz] (V_00(z)); kinetic:
  * nz x nz row-major; states (nc, nz) row-major complex. variant 0 = matrix-core GEMMs
- * (nz <= 320), 1 = vector ALU; precision 0 = FP64, 1 = FP32. */
+ * (nz <= 320), 1 = vector ALU, 2 = matrix-core GEMMs with 8 channels per wavefront (FP64,
+ * nz <= 160; variant 0 takes 16 there); precision 0 = FP64, 1 = FP32. */
 int msx_debug_apply_diagonal(int nz, const double *potential00, const double *kinetic, int nc,
                              const double *e, const double *wave_c, const double *state_in,
                              double *state_out, int inverse, int precision, int variant);
@@ -185,6 +187,15 @@ typedef struct {
                          initial guess does in the reference's SciPy path (_gmres.py:106-114) */
     int inner_orth;   /* Gram-Schmidt passes of the FP32 inner Arnoldi (mixed precision): 0 = default,
                          1 = classical single pass, 2 = classical with one reorthogonalisation */
+    int autotune;     /* 1 (block Jacobi): at the first solve of a batch shape, time the launch
+                         shapes of the DFT coupling (nodes per wavefront), the Arnoldi vector
+                         kernels (block length) and the FP64 diagonal blocks (channels per
+                         wavefront) on the batch and keep the fastest (cached per shape; the file
+                         named by MSX_TUNE_FILE keeps them across processes); 0: built-in shapes */
+    int graphs;       /* 1: each Arnoldi iteration's fixed kernel sequence (operator, Gram-Schmidt
+                         passes, Givens update) is captured once as a HIP graph per iteration index
+                         and launch shape, and replayed by later iterations and solves of the
+                         batch; 0: every kernel launched one by one. Off while profiling.      */
 } msx_solve_options;
 
 /* Fills `opt` with the reference's settings for a given nsig (GS, FP64, window 3, 2000). */
@@ -253,6 +264,9 @@ int msx_batch_state(const msx_batch *batch, double *psi);
  * (the reference's ScatteringCondition.initial_state, multiscat/config.py:115-121). */
 int msx_batch_set_state(msx_batch *batch, const double *psi);
 int msx_batch_intensities(const msx_batch *batch, double *P);
+/* The autotuner's choices for this batch and the times they were chosen from, as a JSON
+ * object ("{}" before a solve with autotune = 1). MSX_ERR_INVALID if buflen is too short. */
+int msx_batch_tuning(const msx_batch *batch, char *buf, int buflen);
 int msx_batch_stats(const msx_batch *batch, int *iterations, int *info, double *residual);
 
 /* The lower block operator of the Gauss-Seidel split in each condition's channel order,

@@ -38,7 +38,7 @@ EXPORTED = (
     "msx_symmetric_eigensystem", "msx_symmetric_eigensystem_noalloc", "msx_batch_create",
     "msx_batch_destroy", "msx_batch_solve", "msx_batch_state", "msx_batch_set_state",
     "msx_batch_intensities",
-    "msx_batch_stats", "msx_batch_apply_lower", "msx_batch_device_state",
+    "msx_batch_stats", "msx_batch_apply_lower", "msx_batch_tuning", "msx_batch_device_state",
     "msx_batch_total_channels",
     "msx_batch_adjoint", "msx_profile_enable", "msx_profile_reset", "msx_profile_read",
     "msx_profile_names",
@@ -62,6 +62,8 @@ class SolveOptions(C.Structure):
         ("diag", C.c_int),
         ("warm_start", C.c_int),
         ("inner_orth", C.c_int),
+        ("autotune", C.c_int),
+        ("graphs", C.c_int),
     ]
 
 
@@ -112,6 +114,7 @@ def _declare(lib):
         "msx_batch_intensities": (i32, [vp, vp]),
         "msx_batch_stats": (i32, [vp, vp, vp, vp]),
         "msx_batch_apply_lower": (i32, [vp, i32, vp, vp]),
+        "msx_batch_tuning": (i32, [vp, C.c_char_p, i32]),
         "msx_batch_device_state": (i32, [vp, pp]),
         "msx_batch_total_channels": (i32, [vp, C.POINTER(ll)]),
         "msx_batch_adjoint": (i32, [vp, C.POINTER(SolveOptions), vp, vp]),

@@ -34,6 +34,8 @@ class SolveOptions:
     diag: str = "auto"  # "auto" (fastest), "vector" or "mfma" (matrix cores, nz <= 320)
     warm_start: bool = False  # start from the batch's current state (Batch.set_state)
     inner_orth: int = 0  # FP32 inner Arnoldi: 0 default, 1 single classical pass, 2 CGS2
+    autotune: bool = True  # time the main kernels' launch shapes at a batch shape's first solve
+    graphs: bool = True  # replay each Arnoldi iteration from a captured HIP graph
 
     @classmethod
     def from_nsig(cls, nsig: int, **kw) -> "SolveOptions":
@@ -66,6 +68,8 @@ class SolveOptions:
         if self.inner_orth not in (0, 1, 2):
             raise ValueError(f"inner_orth must be 0, 1 or 2, not {self.inner_orth!r}")
         o.inner_orth = self.inner_orth
+        o.autotune = 1 if self.autotune else 0
+        o.graphs = 1 if self.graphs else 0
         return o
 
 
@@ -240,6 +244,14 @@ class Batch:
         _lib.check(_lib.lib().msx_batch_stats(self._h, _lib.ptr(it), _lib.ptr(info),
                                               _lib.ptr(res)), "msx_batch_stats")
         return {"iterations": it, "info": info, "residual": res}
+
+    def tuning(self) -> dict:
+        """The autotuner's choices for this batch (empty before a tuned solve)."""
+        import json
+
+        buf = C.create_string_buffer(8192)
+        _lib.check(_lib.lib().msx_batch_tuning(self._h, buf, 8192), "msx_batch_tuning")
+        return json.loads(buf.value.decode() or "{}")
 
     def device_state_ptr(self) -> int:
         p = C.c_void_p()

@@ -31,11 +31,17 @@ void build_dft_tables(int nkx, int nky, int nz, const cx<double> *V, DftStore &s
 // active_hint: an upper bound of the number of active instances (-1: unknown), used to
 // spread few instances over more workgroups. With x_norm and x_copy: the coupling of
 // x / x_norm[b] (per instance), whose elements are also stored to x_copy as they are scaled
-// (a Krylov vector normalised on its way into the operator).
+// (a Krylov vector normalised on its way into the operator). nodes_per_wave: nodes each
+// wavefront of a workgroup handles while every instance iterates (0: the built-in 4).
 template <typename T>
 void launch_dft_coupling(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
                          const int *active, hipStream_t s, int active_hint = -1,
-                         const double *x_norm = nullptr, cx<T> *x_copy = nullptr);
+                         const double *x_norm = nullptr, cx<T> *x_copy = nullptr,
+                         int nodes_per_wave = 0);
+
+// Nodes per workgroup of the DFT coupling's launch for a given active-instance hint and node
+// grouping (the only launch parameter that depends on host-side state).
+int dft_block_nodes(const Layout &L, bool fp64, int active_hint, int nodes_per_wave);
 
 // Adds dL/dV from the coupling to grad[g][z] (complex FP64, storage order): conj(-C[g](z)),
 // Re(-C[0](z)) at g = 0, with C[g](z) = sum_b sum_j lam_j(z) psi_i(z) over the channel pairs

@@ -447,10 +447,20 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
     }
 }
 
+int dft_block_nodes(const Layout &L, bool fp64, int active_hint, int nodes_per_wave) {
+    const int WPB = fp64 ? 2 : 4;
+    int zpb = (nodes_per_wave > 0 ? nodes_per_wave : 4) * WPB;
+    if (active_hint >= 0) {
+        const long long waves = (long long)active_hint * ((L.nz + zpb - 1) / zpb) * WPB;
+        if (waves < 8192) zpb = WPB;
+    }
+    return zpb;
+}
+
 template <typename T, int NP, int WPB, bool SQ, bool NM>
 static void launch_dft_impl(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
                             const int *active, hipStream_t s, int active_hint,
-                            const double *x_norm, cx<T> *x_copy) {
+                            const double *x_norm, cx<T> *x_copy, int nodes_per_wave) {
     using G = DftGeom<NP>;
     const size_t shm = sizeof(T) * (size_t)WPB * 2 * G::PLANE;
     set_dynamic_lds((const void *)dft_coupling_kernel<T, NP, WPB, SQ, NM>, shm);
@@ -460,11 +470,7 @@ static void launch_dft_impl(const Layout &L, const DftTables<T> &D, const cx<T> 
     // four nodes per wavefront (the next node's data is prefetched during the current one);
     // one node per wavefront when so few instances iterate that four-node groups would leave
     // the device short of wavefronts (fewer than about eight per SIMD)
-    int zpb = 4 * WPB;
-    if (active_hint >= 0) {
-        const long long waves = (long long)active_hint * ((L.nz + zpb - 1) / zpb) * WPB;
-        if (waves < 8192) zpb = WPB;
-    }
+    const int zpb = dft_block_nodes(L, sizeof(T) == 8, active_hint, nodes_per_wave);
     dim3 grid(L.n_inst, (L.nz + zpb - 1) / zpb);
     hipLaunchKernelGGL((dft_coupling_kernel<T, NP, WPB, SQ, NM>), grid, dim3(64 * WPB), shm, s, L,
                        D.Fx, D.Fy, D.Vhat, x, y, active, zpb, x_norm, x_copy);
@@ -474,38 +480,39 @@ static void launch_dft_impl(const Layout &L, const DftTables<T> &D, const cx<T> 
 template <typename T, int NP, int WPB>
 static void launch_dft_np(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
                           const int *active, hipStream_t s, int active_hint, const double *x_norm,
-                          cx<T> *x_copy) {
+                          cx<T> *x_copy, int npw) {
     const bool sq = L.nkx == L.nky;
     if constexpr (sizeof(T) == 4) {
         if (L.nm) {
-            if (sq) launch_dft_impl<T, NP, WPB, true, true>(L, D, x, y, active, s, active_hint, x_norm, x_copy);
-            else launch_dft_impl<T, NP, WPB, false, true>(L, D, x, y, active, s, active_hint, x_norm, x_copy);
+            if (sq) launch_dft_impl<T, NP, WPB, true, true>(L, D, x, y, active, s, active_hint, x_norm, x_copy, npw);
+            else launch_dft_impl<T, NP, WPB, false, true>(L, D, x, y, active, s, active_hint, x_norm, x_copy, npw);
             return;
         }
     } else {
         if (L.nm) throw Error(3, "DFT coupling: node-major vectors are FP32 only");
     }
-    if (sq) launch_dft_impl<T, NP, WPB, true, false>(L, D, x, y, active, s, active_hint, x_norm, x_copy);
-    else launch_dft_impl<T, NP, WPB, false, false>(L, D, x, y, active, s, active_hint, x_norm, x_copy);
+    if (sq) launch_dft_impl<T, NP, WPB, true, false>(L, D, x, y, active, s, active_hint, x_norm, x_copy, npw);
+    else launch_dft_impl<T, NP, WPB, false, false>(L, D, x, y, active, s, active_hint, x_norm, x_copy, npw);
 }
 
 template <typename T>
 void launch_dft_coupling(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
                          const int *active, hipStream_t s, int active_hint, const double *x_norm,
-                         cx<T> *x_copy) {
+                         cx<T> *x_copy, int nodes_per_wave) {
     constexpr int WPB = sizeof(T) == 4 ? 4 : 2;  // LDS per wavefront: 9 KB FP32, 18 KB FP64
     if ((x_norm == nullptr) != (x_copy == nullptr))
         throw Error(3, "DFT coupling: x_norm and x_copy go together");
-    if (D.np == 16) launch_dft_np<T, 16, WPB>(L, D, x, y, active, s, active_hint, x_norm, x_copy);
-    else if (D.np == 32) launch_dft_np<T, 32, WPB>(L, D, x, y, active, s, active_hint, x_norm, x_copy);
+    if (nodes_per_wave < 0 || nodes_per_wave > 16) throw Error(3, "DFT coupling: 0..16 nodes per wave");
+    if (D.np == 16) launch_dft_np<T, 16, WPB>(L, D, x, y, active, s, active_hint, x_norm, x_copy, nodes_per_wave);
+    else if (D.np == 32) launch_dft_np<T, 32, WPB>(L, D, x, y, active, s, active_hint, x_norm, x_copy, nodes_per_wave);
     else throw Error(3, "DFT coupling supports tables up to 32 x 32");
 }
 template void launch_dft_coupling<float>(const Layout &, const DftTables<float> &,
                                          const cx<float> *, cx<float> *, const int *, hipStream_t,
-                                         int, const double *, cx<float> *);
+                                         int, const double *, cx<float> *, int);
 template void launch_dft_coupling<double>(const Layout &, const DftTables<double> &,
                                           const cx<double> *, cx<double> *, const int *,
-                                          hipStream_t, int, const double *, cx<double> *);
+                                          hipStream_t, int, const double *, cx<double> *, int);
 
 // ---------------------------------------------------------------------------------------
 // Gradient of the coupling with respect to the table: the adjoint of the convolution above,

@@ -780,8 +780,10 @@ bool diag_mfma_supported(int nz) { return nz <= 320; }
 template <typename T>
 void launch_diag_blocks(const Layout &L, const EigTables<T> &E, const double *e, const cx<T> *c,
                         const T *fac, const cx<T> *x, const cx<T> *base, cx<T> *y, T sign,
-                        int inverse, const int *active, hipStream_t s) {
+                        int inverse, const int *active, hipStream_t s, int channels_per_wave) {
     if (!diag_mfma_supported(L.nz)) throw Error(3, "diagonal blocks on the matrix cores: nz <= 320");
+    if (channels_per_wave != 0 && channels_per_wave != 8 && channels_per_wave != 16)
+        throw Error(3, "diagonal blocks: 0, 8 or 16 channels per wavefront");
     // the smallest instantiated tile count that covers nzp / 16
     const int nt = E.nzp / 16;
 #define MSX_DIAG(NR_, NT_)                                                                  \
@@ -794,8 +796,13 @@ void launch_diag_blocks(const Layout &L, const EigTables<T> &E, const double *e,
         if (nt <= 7) return launch_diag_lds_any<7>(L, E, e, c, fac, x, base, y, sign, inverse, active, s);
         if (nt <= 10) return launch_diag_lds_any<10>(L, E, e, c, fac, x, base, y, sign, inverse, active, s);
     } else {
-        // 16 channels per wavefront up to nzp = 160: each Q fragment feeds two products
-        MSX_DIAG(2, 4) MSX_DIAG(2, 7) MSX_DIAG(2, 8) MSX_DIAG(2, 10)
+        // 16 channels per wavefront up to nzp = 160 unless 8 are asked for: each Q fragment
+        // then feeds two products
+        if (channels_per_wave != 8) {
+            MSX_DIAG(2, 4) MSX_DIAG(2, 7) MSX_DIAG(2, 8) MSX_DIAG(2, 10)
+        } else {
+            MSX_DIAG(1, 4) MSX_DIAG(1, 7) MSX_DIAG(1, 8) MSX_DIAG(1, 10)
+        }
     }
     MSX_DIAG(1, 13) MSX_DIAG(1, 16) MSX_DIAG(1, 19) MSX_DIAG(1, 20)
 #undef MSX_DIAG
@@ -804,11 +811,11 @@ void launch_diag_blocks(const Layout &L, const EigTables<T> &E, const double *e,
 template void launch_diag_blocks<float>(const Layout &, const EigTables<float> &, const double *,
                                         const cx<float> *, const float *, const cx<float> *,
                                         const cx<float> *, cx<float> *, float, int, const int *,
-                                        hipStream_t);
+                                        hipStream_t, int);
 template void launch_diag_blocks<double>(const Layout &, const EigTables<double> &,
                                          const double *, const cx<double> *, const double *,
                                          const cx<double> *, const cx<double> *, cx<double> *,
-                                         double, int, const int *, hipStream_t);
+                                         double, int, const int *, hipStream_t, int);
 
 // Zero-padded copies: dst[i][j] = src[i][j] for i, j < n, 0 up to np.
 template <typename T>

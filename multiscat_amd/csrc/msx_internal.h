@@ -196,11 +196,12 @@ struct EigTables {
 };
 
 // MFMA form of the diagonal blocks (inverse = 1: D^-1, inverse = 0: D), nz <= 320.
+// channels_per_wave (FP64, nzp <= 160): 8 or 16 channels per wavefront, 0 = the built-in 16.
 bool diag_mfma_supported(int nz);
 template <typename T>
 void launch_diag_blocks(const Layout &L, const EigTables<T> &E, const double *e, const cx<T> *c,
                         const T *fac, const cx<T> *x, const cx<T> *base, cx<T> *y, T sign,
-                        int inverse, const int *active, hipStream_t s);
+                        int inverse, const int *active, hipStream_t s, int channels_per_wave = 0);
 void launch_pad_square(int n, int np, const double *src, double *dst, hipStream_t s);
 void launch_pad_square(int n, int np, const double *src, float *dst, hipStream_t s);
 

@@ -15,6 +15,10 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <mutex>
 #include <set>
 
 #include "../../include/multiscat_amd.h"
@@ -72,9 +76,17 @@ struct Krylov {
     std::vector<long long> len_h;
     std::shared_ptr<ActiveSnap> snap;  // host view of the mask the launches issued now see
     int last_count = -1;  // iterating instances at the last count read (an upper bound later)
+    // Arnoldi iterations captured as HIP graphs, keyed by iteration, operator form, flag slot
+    // and launch shape; valid while the operator's buffers and options match graph_sig
+    std::map<long long, hipGraphExec_t> graphs;
+    std::string graph_sig;
+    void clear_graphs() {
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
 
     Krylov(const std::vector<long long> &start, const std::vector<long long> &len, int kmax_,
-           hipStream_t stream)
+           hipStream_t stream, long long chunk = 2048)
         : n_inst((int)start.size()), kmax(kmax_), s(stream), len_h(len) {
         if (kmax > KR_MAX_BASIS - 1) kmax = KR_MAX_BASIS - 1;
         long long maxlen = 1;
@@ -84,7 +96,7 @@ struct Krylov {
         }
         K.n_inst = n_inst;
         K.kmax = kmax;
-        K.chunk = 2048;
+        K.chunk = chunk > 0 ? chunk : 2048;
         K.nchunk = (int)((maxlen + K.chunk - 1) / K.chunk);
         start_d.alloc(n_inst);
         len_d.alloc(n_inst);
@@ -116,6 +128,7 @@ struct Krylov {
         set_all_active(nullptr);
     }
     ~Krylov() {
+        clear_graphs();
         if (flags_h) (void)hipHostFree(flags_h);
         if (mask_h) (void)hipHostFree(mask_h);
         if (dbl_h) (void)hipHostFree(dbl_h);
@@ -220,12 +233,34 @@ struct Krylov {
 // (device, optional): a tolerance per instance instead of eps. op_scaled (optional): the
 // operator with the normalisation of its input folded in; each new basis vector is then
 // normalised by the next operator call instead of a separate pass.
+// graph_sig (optional): replay each Arnoldi iteration from a HIP graph captured the first
+// time that (iteration, operator form, flag slot, launch shape) comes up; graph_sig names the
+// operator's buffers and options, and a different one drops the captured graphs. shape_key
+// gives the host-side launch choices that depend on the active count (part of the key).
+// Profiling needs the eager launches (its events bracket single kernels), so it turns this off.
 template <typename T, typename TX>
 void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_zero,
            double eps, int window, int maxiter, bool allow_restart, int orth_passes = 2,
            const Layout *basis_order = nullptr, const double *eps_inst = nullptr,
-           const OpScaled<T> *op_scaled = nullptr) {
+           const OpScaled<T> *op_scaled = nullptr, const std::string *graph_sig = nullptr,
+           const std::function<int()> *shape_key = nullptr) {
     hipStream_t s = kr.s;
+    const bool use_graphs = graph_sig != nullptr && !profiling();
+    if (use_graphs) {
+        // the recurrence state goes to the Givens kernel by value: its tolerance, window and
+        // per-instance tolerances are part of what a captured iteration replays
+        char extra[160];
+        std::snprintf(extra, sizeof extra, "|eps%.17g|win%d|einst%p|orth%d", eps, window,
+                      (const void *)eps_inst, orth_passes);
+        const std::string sig = *graph_sig + extra;
+        if (sig != kr.graph_sig) {
+            kr.clear_graphs();
+            kr.graph_sig = sig;
+        }
+        // every flag slot posted by an earlier solve has been waited for: start at slot 0, so
+        // iteration k of a cycle always writes the same slot
+        kr.next_slot = 0;
+    }
     kr.S.eps = eps;
     kr.S.eps_inst = eps_inst;
     kr.S.window = window;
@@ -269,9 +304,12 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
                 if (alive == 0) break;
             }
             kr.ensure(k + 2);
-            if (pending_norm) {
+            const bool scaled_in = pending_norm;
+            // the fixed kernel sequence of iteration k: operator, Gram-Schmidt passes, Givens
+            // update (writing the flags to slot next_slot), normalisation
+            auto iteration = [&]() {
+            if (scaled_in) {
                 (*op_scaled)(kr.w.p, kr.hnorm.p, kr.vp[k], kr.w.p, kr.active.p);
-                pending_norm = false;
             } else {
                 op(kr.vp[k], kr.w.p, kr.active.p);
             }
@@ -310,8 +348,38 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
                      kr.next_flags(), kr.partial.p);
             // v_{k+1} = w / h_{k+1,k}: by the next operator call when it can, else here (a
             // vector the loop leaves unnormalised is never read: the update uses v_0 .. v_k)
-            if (op_scaled) pending_norm = true;
-            else k_scale<T>(kr.K, kr.w.p, kr.vp[k + 1], kr.hnorm.p, kr.active.p, s);
+            if (!op_scaled) k_scale<T>(kr.K, kr.w.p, kr.vp[k + 1], kr.hnorm.p, kr.active.p, s);
+            };
+            if (use_graphs) {
+                const long long shape = shape_key ? (long long)(*shape_key)() : 0;
+                const long long key =
+                    ((((long long)k * 2 + (scaled_in ? 1 : 0)) * 2 + kr.next_slot) << 20) + shape;
+                auto it = kr.graphs.find(key);
+                hipGraphExec_t exec = nullptr;
+                if (it == kr.graphs.end()) {
+                    hipGraph_t graph = nullptr;
+                    MSX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    try {
+                        iteration();
+                    } catch (...) {
+                        hipGraph_t partial_graph = nullptr;
+                        (void)hipStreamEndCapture(s, &partial_graph);
+                        if (partial_graph) (void)hipGraphDestroy(partial_graph);
+                        throw;
+                    }
+                    MSX_HIP(hipStreamEndCapture(s, &graph));
+                    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(graph);
+                    MSX_HIP(e);
+                    kr.graphs[key] = exec;
+                } else {
+                    exec = it->second;
+                }
+                MSX_HIP(hipGraphLaunch(exec, s));
+            } else {
+                iteration();
+            }
+            pending_norm = op_scaled != nullptr;
             ++done;
             const typename Krylov<T>::Pending cur = kr.post(true);
             if (prev.slot >= 0) {
@@ -462,6 +530,14 @@ struct msx_batch {
     struct KdimMemo {
         int restart = -1, maxiter = -1, dim = 0;
     } kdim_memo[2];  // Krylov dimension per precision (FP64, mixed) for the last options seen
+    // Launch shapes chosen by the autotuner (built-in values until it runs)
+    struct Tuning {
+        int dft_npw32 = 0, dft_npw64 = 0;  // DFT coupling: nodes per wavefront (0: built-in 4)
+        long long chunk32 = 2048, chunk64 = 2048;  // Arnoldi vector kernels: elements per block
+        int diag64_cpw = 0;                // FP64 diagonal blocks: channels per wavefront
+        bool done = false;
+        std::string key, report;
+    } tuning;
 
     long long n_elem() const { return total_ch * t->nz; }
     Layout layout() const {
@@ -549,6 +625,27 @@ struct Ops {
     const std::shared_ptr<ActiveSnap> *snap_src = nullptr;
     const int *count_src = nullptr;
     int active_hint(const int *act) const { return act && count_src ? *count_src : -1; }
+    // The launch choices that depend on the host's count of iterating instances (the DFT
+    // coupling's nodes per workgroup, FP32 and FP64): part of a captured iteration's key.
+    int launch_shape_key() const {
+        if (direct || !count_src) return 0;
+        const int hint = *count_src;
+        return dft_block_nodes(L32, false, hint, B.tuning.dft_npw32) |
+               (dft_block_nodes(L, true, hint, B.tuning.dft_npw64) << 8);
+    }
+    // Everything a captured operator launch depends on besides the iteration: buffers, tables,
+    // options and launch shapes.
+    std::string graph_signature(const char *what, const void *a, const void *b,
+                                const void *c) const {
+        char buf[512];
+        std::snprintf(buf, sizeof buf,
+                      "%s|%p|%p|%p|V%p|Vf%p|D%p|nm%d|neu%d|ord%d|dir%d|vd%d|npw%d,%d|cpw%d|fac%p",
+                      what, a, b, c, (const void *)V, (const void *)Vf, (const void *)dft,
+                      L32.nm, neumann ? 1 : 0, order, direct ? 1 : 0, vector_diag ? 1 : 0,
+                      B.tuning.dft_npw32, B.tuning.dft_npw64, B.tuning.diag64_cpw,
+                      (const void *)(L32.nm ? B.facf_nm_d.p : B.facf_d.p));
+        return buf;
+    }
     template <typename F>
     Work over_active(const int *act, F f) const {
         Work w;
@@ -577,7 +674,8 @@ struct Ops {
         if (profiling()) couple_full_work(act, sizeof(cx<double>), !direct, fl, by, in_copy != nullptr);
         ProfScope ps(direct ? "coupling_direct_f64" : "coupling_dft_f64", s, fl, by);
         if (direct) launch_coupling<double>(L, V, in, out, COUPLE_FULL, act, s);
-        else launch_dft_coupling<double>(L, dft->d64(), in, out, act, s, active_hint(act), in_norm, in_copy);
+        else launch_dft_coupling<double>(L, dft->d64(), in, out, act, s, active_hint(act), in_norm,
+                                         in_copy, B.tuning.dft_npw64);
     }
     void couple_full(const cx<float> *in, cx<float> *out, const int *act,
                      const double *in_norm = nullptr, cx<float> *in_copy = nullptr) {
@@ -585,7 +683,8 @@ struct Ops {
         if (profiling()) couple_full_work(act, sizeof(cx<float>), !direct, fl, by, in_copy != nullptr);
         ProfScope ps(direct ? "coupling_direct_f32" : "coupling_dft_f32", s, fl, by);
         if (direct) launch_coupling<float>(L32, Vf, in, out, COUPLE_FULL, act, s);
-        else launch_dft_coupling<float>(L32, dft->d32(), in, out, act, s, active_hint(act), in_norm, in_copy);
+        else launch_dft_coupling<float>(L32, dft->d32(), in, out, act, s, active_hint(act), in_norm,
+                                        in_copy, B.tuning.dft_npw32);
     }
     void upper(const cx<double> *in, cx<double> *out, const int *act) {
         Work fl, by;
@@ -651,7 +750,7 @@ struct Ops {
         ProfScope ps(name, s, fl, by);
         if (mfma)
             launch_diag_blocks<T>(lay<T>(), eig<T>(), B.e_d.p, c<T>(), fac<T>(), in, base, out,
-                                  sign, 1, act, s);
+                                  sign, 1, act, s, sizeof(T) == 8 ? B.tuning.diag64_cpw : 0);
         else
             launch_dinv<T>(lay<T>(), q<T>(), qt<T>(), t.lam.p, B.e_d.p, c<T>(), fac<T>(), in, base,
                            out, sign, act, s);
@@ -781,8 +880,231 @@ void check_options(const msx_solve_options *o) {
     if (o->neumann > 1 && o->method != MSX_METHOD_JACOBI)
         throw Error(MSX_ERR_INVALID, "a Neumann factor of degree > 1 needs the block-Jacobi method");
     if (o->inner_orth < 0 || o->inner_orth > 2) throw Error(MSX_ERR_INVALID, "inner_orth is 0, 1 or 2");
+    if (o->autotune != 0 && o->autotune != 1) throw Error(MSX_ERR_INVALID, "autotune is 0 or 1");
+    if (o->graphs != 0 && o->graphs != 1) throw Error(MSX_ERR_INVALID, "graphs is 0 or 1");
     if (!(o->inner_eps >= 0 && o->inner_eps < 1))
         throw Error(MSX_ERR_INVALID, "inner_eps is 0 (automatic) or in (0, 1)");
+}
+
+// ---- autotuner ----------------------------------------------------------------------------
+// At a batch's first solve with autotune set, times the launch-shape choices of its main
+// kernels on its own sizes and keeps the fastest: the DFT coupling's nodes per wavefront (FP32
+// for the mixed solver, FP64), the Arnoldi vector kernels' block length (the precision the
+// Krylov basis runs in) and the FP64 diagonal blocks' channels per wavefront. The node and
+// channel groupings do not change the arithmetic; the block length changes only the order in
+// which the dot products' partial sums add up. Choices are cached per batch shape (table,
+// nz, instances, channel counts, vector order) for the process, and in the text file named by
+// MSX_TUNE_FILE when that is set (one line per shape: key and the five values).
+std::mutex g_tune_mu;
+std::map<std::string, msx_batch::Tuning> g_tune_cache;
+bool g_tune_file_read = false;
+
+std::string tune_key(const msx_batch &B, const Ops &ops, const msx_solve_options &o) {
+    char buf[320];
+    std::snprintf(buf, sizeof buf,
+                  "nkx=%d,nky=%d,nz=%d,inst=%d,channels=%lld,max_nc=%d,nm=%d,direct=%d,vdiag=%d,prec=%d",
+                  B.t->nkx, B.t->nky, B.t->nz, B.n_inst, B.total_ch, B.max_nc, ops.L32.nm,
+                  ops.direct ? 1 : 0, ops.vector_diag ? 1 : 0, o.precision);
+    return buf;
+}
+
+void read_tune_file() {
+    if (g_tune_file_read) return;
+    g_tune_file_read = true;
+    const char *path = std::getenv("MSX_TUNE_FILE");
+    if (!path || !*path) return;
+    std::ifstream in(path);
+    std::string key;
+    msx_batch::Tuning t;
+    while (in >> key >> t.dft_npw32 >> t.dft_npw64 >> t.chunk32 >> t.chunk64 >> t.diag64_cpw) {
+        t.done = true;
+        t.key = key;
+        g_tune_cache[key] = t;
+    }
+}
+
+void append_tune_file(const msx_batch::Tuning &t) {
+    const char *path = std::getenv("MSX_TUNE_FILE");
+    if (!path || !*path) return;
+    std::ofstream out(path, std::ios::app);
+    out << t.key << ' ' << t.dft_npw32 << ' ' << t.dft_npw64 << ' ' << t.chunk32 << ' '
+        << t.chunk64 << ' ' << t.diag64_cpw << '\n';
+}
+
+// Index of the shape to keep among measured times ts: the built-in one (index def) unless
+// another is faster by more than 3%. Launch-time differences below that are within what the
+// same kernel varies between runs, and the built-in shapes were chosen on whole solves, where
+// the active set shrinks as conditions converge (a shape timed with every instance active can
+// lose there: eight nodes per wavefront measured 0.2% faster alone and 8% slower in a solve).
+int pick_with_margin(const std::vector<double> &ts, int def) {
+    int best = def;
+    for (int i = 0; i < (int)ts.size(); ++i)
+        if (ts[i] < ts[best] && ts[i] < 0.97 * ts[def]) best = i;
+    return best;
+}
+
+// Shortest of `reps` timed runs of f (after one untimed run), in microseconds.
+struct LaunchTimer {
+    hipStream_t s;
+    hipEvent_t a = nullptr, b = nullptr;
+    explicit LaunchTimer(hipStream_t st) : s(st) {
+        MSX_HIP(hipEventCreate(&a));
+        MSX_HIP(hipEventCreate(&b));
+    }
+    ~LaunchTimer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    template <typename F>
+    double best_us(F f, int reps = 3) {
+        f();
+        double best = 1e300;
+        for (int r = 0; r < reps; ++r) {
+            MSX_HIP(hipEventRecord(a, s));
+            f();
+            MSX_HIP(hipEventRecord(b, s));
+            MSX_HIP(hipEventSynchronize(b));
+            float ms = 0;
+            MSX_HIP(hipEventElapsedTime(&ms, a, b));
+            best = std::min(best, 1e3 * (double)ms);
+        }
+        return best;
+    }
+};
+
+// Arnoldi block length: the two main passes of an iteration (6 basis vectors: dot products
+// with their reduction, then the update with the norm partials) at each candidate.
+template <typename T>
+long long tune_chunk(msx_batch &B, LaunchTimer &tm, std::string &rep) {
+    const int kc = 6;
+    const size_t n = (size_t)B.n_elem();
+    const std::vector<long long> start = B.kstart(), len = B.klen();
+    long long maxlen = 1;
+    for (long long l : len) maxlen = std::max(maxlen, l);
+    DBuf<long long> start_d(B.n_inst), len_d(B.n_inst);
+    start_d.upload(start.data(), B.n_inst, tm.s);
+    len_d.upload(len.data(), B.n_inst, tm.s);
+    std::vector<int> ones(B.n_inst, 1);
+    DBuf<int> act(B.n_inst);
+    act.upload(ones.data(), B.n_inst, tm.s);
+    DBuf<cx<T>> vecs((size_t)(kc + 1) * n);
+    MSX_HIP(hipMemsetAsync(vecs.p, 0, sizeof(cx<T>) * (size_t)(kc + 1) * n, tm.s));
+    std::vector<cx<T> *> vp(kc);
+    for (int j = 0; j < kc; ++j) vp[j] = vecs.p + (size_t)j * n;
+    DBuf<cx<T> *> vp_d(kc);
+    vp_d.upload(vp.data(), kc, tm.s);
+    cx<T> *w = vecs.p + (size_t)kc * n;
+    const long long cands[] = {1024, 2048, 4096, 8192};
+    const size_t slots = (size_t)B.n_inst * (kc + 1);
+    DBuf<double> partial(slots * (size_t)((maxlen + 1023) / 1024) * 2);
+    DBuf<cx<double>> h(slots);
+    std::vector<double> ts;
+    rep += "\"arnoldi_block_us_" + std::string(sizeof(T) == 4 ? "f32" : "f64") + "\": {";
+    for (size_t i = 0; i < sizeof(cands) / sizeof(cands[0]); ++i) {
+        KLayout K{};
+        K.n_inst = B.n_inst;
+        K.kmax = kc;
+        K.chunk = cands[i];
+        K.nchunk = (int)((maxlen + K.chunk - 1) / K.chunk);
+        K.start = start_d.p;
+        K.len = len_d.p;
+        K.active = act.p;
+        const double t = tm.best_us([&] {
+            k_dots<T>(K, vp_d.p, kc, w, 0, partial.p, h.p, 0, tm.s);
+            k_axpy<T>(K, vp_d.p, kc, h.p, w, 1, partial.p, h.p, tm.s);
+        });
+        rep += (i ? ", \"" : "\"") + std::to_string(cands[i]) + "\": " + std::to_string(t);
+        ts.push_back(t);
+    }
+    rep += "}, ";
+    MSX_HIP(hipStreamSynchronize(tm.s));
+    return cands[pick_with_margin(ts, 1)];
+}
+
+void autotune(msx_batch &B, Ops &ops, const msx_solve_options &o) {
+    const std::string key = tune_key(B, ops, o);
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        read_tune_file();
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            B.tuning = it->second;
+            B.tuning.report = "{\"source\": \"cache\", \"key\": \"" + key + "\", \"dft_nodes_per_wave_f32\": " +
+                              std::to_string(B.tuning.dft_npw32) + ", \"dft_nodes_per_wave_f64\": " +
+                              std::to_string(B.tuning.dft_npw64) + ", \"arnoldi_block_f32\": " +
+                              std::to_string(B.tuning.chunk32) + ", \"arnoldi_block_f64\": " +
+                              std::to_string(B.tuning.chunk64) + ", \"diag_f64_channels_per_wave\": " +
+                              std::to_string(B.tuning.diag64_cpw) + "}";
+            return;
+        }
+    }
+    hipStream_t s = B.t->stream;
+    const size_t n = (size_t)B.n_elem();
+    LaunchTimer tm(s);
+    msx_batch::Tuning T;
+    T.key = key;
+    std::string rep = "{\"source\": \"measured\", \"key\": \"" + key + "\", ";
+    const bool mixed = o.precision == MSX_PRECISION_MIXED;
+    const int npw_c[] = {1, 2, 4, 8};
+    if (!ops.direct) {
+        msx_batch::need(B.tmp1, n);
+        msx_batch::need(B.tmp2, n);
+        std::vector<double> ts;
+        rep += "\"dft_us_f64\": {";
+        for (int i = 0; i < 4; ++i) {
+            ts.push_back(tm.best_us([&] {
+                launch_dft_coupling<double>(ops.L, ops.dft->d64(), B.tmp1.p, B.tmp2.p, nullptr, s,
+                                            -1, nullptr, nullptr, npw_c[i]);
+            }));
+            rep += (i ? ", \"" : "\"") + std::to_string(npw_c[i]) + "\": " + std::to_string(ts[i]);
+        }
+        T.dft_npw64 = npw_c[pick_with_margin(ts, 2)];
+        rep += "}, ";
+        if (mixed) {
+            msx_batch::need(B.f_tmp1, n);
+            msx_batch::need(B.f_tmp2, n);
+            std::vector<double> ts;
+            rep += "\"dft_us_f32\": {";
+            for (int i = 0; i < 4; ++i) {
+                ts.push_back(tm.best_us([&] {
+                    launch_dft_coupling<float>(ops.L32, ops.dft->d32(), B.f_tmp1.p, B.f_tmp2.p,
+                                               nullptr, s, -1, nullptr, nullptr, npw_c[i]);
+                }));
+                rep += (i ? ", \"" : "\"") + std::to_string(npw_c[i]) + "\": " + std::to_string(ts[i]);
+            }
+            T.dft_npw32 = npw_c[pick_with_margin(ts, 2)];
+            rep += "}, ";
+        }
+    }
+    if (!ops.vector_diag && diag_mfma_supported(B.t->nz) && B.t->nzp <= 160) {
+        msx_batch::need(B.tmp1, n);
+        msx_batch::need(B.tmp2, n);
+        const int cpw_c[] = {8, 16};
+        std::vector<double> ts;
+        rep += "\"diag_us_f64\": {";
+        for (int i = 0; i < 2; ++i) {
+            ts.push_back(tm.best_us([&] {
+                launch_diag_blocks<double>(ops.L, B.t->eig64(), B.e_d.p, B.c_d.p, B.fac_d.p, B.tmp1.p,
+                                           nullptr, B.tmp2.p, 1.0, 1, nullptr, s, cpw_c[i]);
+            }));
+            rep += (i ? ", \"" : "\"") + std::to_string(cpw_c[i]) + "\": " + std::to_string(ts[i]);
+        }
+        T.diag64_cpw = cpw_c[pick_with_margin(ts, 1)];
+        rep += "}, ";
+    }
+    if (mixed) T.chunk32 = tune_chunk<float>(B, tm, rep);
+    else T.chunk64 = tune_chunk<double>(B, tm, rep);
+    rep += "\"dft_nodes_per_wave_f32\": " + std::to_string(T.dft_npw32) +
+           ", \"dft_nodes_per_wave_f64\": " + std::to_string(T.dft_npw64) +
+           ", \"arnoldi_block_f32\": " + std::to_string(T.chunk32) +
+           ", \"arnoldi_block_f64\": " + std::to_string(T.chunk64) +
+           ", \"diag_f64_channels_per_wave\": " + std::to_string(T.diag64_cpw) + "}";
+    T.done = true;
+    T.report = rep;
+    B.tuning = T;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune_cache[key] = T;
+    append_tune_file(T);
 }
 
 // Solve A psi = src for all instances into `x` (FP64). transposed: A^T with the flipped table.
@@ -805,6 +1127,9 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     ops.order = std::max(1, o.neumann);
     ops.vector_diag = o.diag == 1;  // auto: matrix cores when nz <= 320
     ops.set_fp32_order();
+    if (o.autotune == 1 && o.method == MSX_METHOD_JACOBI &&
+        tune_key(B, ops, o) != B.tuning.key)
+        autotune(B, ops, o);
     const cx<double> *src = src_in;
     if (!src && !(scatter_source && o.method == MSX_METHOD_JACOBI))
         throw Error(MSX_ERR_INVALID, "solve: no source vector");
@@ -827,8 +1152,9 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     resid.assign(B.n_inst, 0.0);
 
     if (o.precision == MSX_PRECISION_FP64) {
-        if (!B.kr64 || B.kr64->kmax != std::min(kdim, KR_MAX_BASIS - 1))
-            B.kr64.reset(new Krylov<double>(B.kstart(), B.klen(), kdim, s));
+        if (!B.kr64 || B.kr64->kmax != std::min(kdim, KR_MAX_BASIS - 1) ||
+            B.kr64->K.chunk != B.tuning.chunk64)
+            B.kr64.reset(new Krylov<double>(B.kstart(), B.klen(), kdim, s, B.tuning.chunk64));
         Krylov<double> &kr = *B.kr64;
         kr.set_all_active(nullptr);
         Op<double> op;
@@ -850,8 +1176,12 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
             ops.jac<double>(in, out, act, B.tmp1.p, B.tmp2.p, raw, nrm);
         };
         const bool fused = o.method == MSX_METHOD_JACOBI && !ops.direct;
+        const std::string sig64 = ops.graph_signature(
+            o.method == MSX_METHOD_JACOBI ? "jacobi64" : "gs64", B.tmp1.p, B.tmp2.p, B.rhs.p);
+        const std::function<int()> shape64 = [&] { return ops.launch_shape_key(); };
         gmres<double, double>(kr, op, B.rhs.p, x, !warm, o.eps, o.window, o.maxiter, true, 2,
-                              nullptr, nullptr, fused ? &op_s : nullptr);
+                              nullptr, nullptr, fused ? &op_s : nullptr,
+                              o.graphs == 1 ? &sig64 : nullptr, &shape64);
         ops.snap_src = nullptr; ops.count_src = nullptr;
         iters = kr.host_ints(kr.iters);
         auto conv = kr.host_ints(kr.converged);
@@ -871,12 +1201,13 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     msx_batch::need(B.f_tmp2, n);
     cx<double> *d0 = B.rhs.p;  // D^-1 s
     ops.dinv_source(src, d0, scatter_source);
-    if (!B.kr_outer) B.kr_outer.reset(new Krylov<double>(B.kstart(), B.klen(), 1, s));
+    if (!B.kr_outer || B.kr_outer->K.chunk != B.tuning.chunk64)
+        B.kr_outer.reset(new Krylov<double>(B.kstart(), B.klen(), 1, s, B.tuning.chunk64));
     Krylov<double> &outer = *B.kr_outer;
     outer.set_all_active(nullptr);
     const int inner_dim = std::max(2, std::min(kdim, 200));
-    if (!B.kr32 || B.kr32->kmax != inner_dim)
-        B.kr32.reset(new Krylov<float>(B.kstart(), B.klen(), inner_dim, s));
+    if (!B.kr32 || B.kr32->kmax != inner_dim || B.kr32->K.chunk != B.tuning.chunk32)
+        B.kr32.reset(new Krylov<float>(B.kstart(), B.klen(), inner_dim, s, B.tuning.chunk32));
     Krylov<float> &inner = *B.kr32;
     Op<float> op32 = [&](const cx<float> *in, cx<float> *out, const int *act) {
         ops.jac<float>(in, out, act, B.f_tmp1.p, B.f_tmp2.p);
@@ -981,9 +1312,13 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
                                ops.neumann ? B.f_tmp1.p : nullptr);
         inner.set_all_active(act.data());
         ops.snap_src = &inner.snap; ops.count_src = &inner.last_count;
+        const std::string sig32 = ops.graph_signature(
+            o.inner_orth == 1 ? "mixed32-cgs1" : "mixed32", B.f_tmp1.p, B.f_tmp2.p, B.eps_in_d.p);
+        const std::function<int()> shape32 = [&] { return ops.launch_shape_key(); };
         gmres<float, double>(inner, op32, B.f_rhs.p, x, true, inner_eps, 1,
                              std::min(o.maxiter, inner_dim), false, o.inner_orth == 1 ? 1 : 2,
-                             &ops.L32, B.eps_in_d.p, ops.direct ? nullptr : &op32s);
+                             &ops.L32, B.eps_in_d.p, ops.direct ? nullptr : &op32s,
+                             o.graphs == 1 ? &sig32 : nullptr, &shape32);
         launch_add_ints(B.n_inst, inner.iters.p, B.iters_acc.p, s);
     }
     {
@@ -1101,6 +1436,8 @@ void msx_default_options(msx_solve_options *opt, int nsig) {
     opt->diag = 0;
     opt->warm_start = 0;
     opt->inner_orth = 0;
+    opt->autotune = 0;
+    opt->graphs = 0;
 }
 
 int msx_table_create(int nkx, int nky, int nz, const double *potential, const double *kinetic,
@@ -1394,6 +1731,17 @@ int msx_batch_apply_lower(msx_batch *B, int inverse, const double *state_in,
     }
 }
 
+int msx_batch_tuning(const msx_batch *B, char *buf, int buflen) {
+    if (!B || !buf || buflen <= 0) {
+        set_last_error("null argument");
+        return MSX_ERR_INVALID;
+    }
+    const std::string &r = B->tuning.done ? B->tuning.report : std::string("{}");
+    std::strncpy(buf, r.c_str(), (size_t)buflen - 1);
+    buf[buflen - 1] = 0;
+    return (int)r.size() < buflen ? MSX_OK : MSX_ERR_INVALID;
+}
+
 int msx_batch_intensities(const msx_batch *B, double *P) {
     try {
         if (!B || !P) throw Error(MSX_ERR_INVALID, "null argument");
@@ -1611,8 +1959,11 @@ int msx_debug_apply_coupling(int nkx, int nky, int nz, const double *V, int nc, 
     try {
         if (nkx <= 0 || nky <= 0 || nz <= 0 || nc <= 0) return MSX_ERR_DIMS;
         if (!V || !chx || !chy || !x || !y) throw Error(MSX_ERR_INVALID, "null argument");
-        if (mode < 0 || mode > 2 || precision < 0 || precision > 1 || algorithm < 0 || algorithm > 1)
+        if (mode < 0 || mode > 2 || precision < 0 || precision > 1 || algorithm < 0 || algorithm > 4)
             throw Error(MSX_ERR_INVALID, "bad mode/precision/algorithm");
+        // DFT form with the built-in node grouping (1), or 1, 2 or 8 nodes per wavefront
+        const int npw = algorithm == 2 ? 1 : algorithm == 3 ? 2 : algorithm == 4 ? 8 : 0;
+        if (algorithm > 1) algorithm = 1;
         std::vector<double> Tr((size_t)nz * nz, 0.0);
         for (int i = 0; i < nz; ++i) Tr[(size_t)i * nz + i] = 1.0;
         TableGuard tg;
@@ -1637,12 +1988,14 @@ int msx_debug_apply_coupling(int nkx, int nky, int nz, const double *V, int nc, 
         if (algorithm == 1 && t.dft.np == 0) throw Error(MSX_ERR_INVALID, "table too large for the DFT form");
         if (precision == 0) {
             if (algorithm == 0) launch_coupling<double>(L, t.V.p, xin.p, yout.p, mode, nullptr, s);
-            else launch_dft_coupling<double>(L, t.dft.d64(), xin.p, yout.p, nullptr, s);
+            else launch_dft_coupling<double>(L, t.dft.d64(), xin.p, yout.p, nullptr, s, -1, nullptr,
+                                             nullptr, npw);
         } else {
             DBuf<cx<float>> xf(n), yf(n);
             launch_to_float((long long)n, xin.p, xf.p, s);
             if (algorithm == 0) launch_coupling<float>(L, t.Vf.p, xf.p, yf.p, mode, nullptr, s);
-            else launch_dft_coupling<float>(L, t.dft.d32(), xf.p, yf.p, nullptr, s);
+            else launch_dft_coupling<float>(L, t.dft.d32(), xf.p, yf.p, nullptr, s, -1, nullptr,
+                                            nullptr, npw);
             std::vector<cx<float>> hf(n);
             yf.download(hf.data(), n, s);
             MSX_HIP(hipStreamSynchronize(s));
@@ -1667,9 +2020,12 @@ int msx_debug_apply_diagonal(int nz, const double *potential00, const double *ki
         if (nz <= 0 || nc <= 0) return MSX_ERR_DIMS;
         if (!potential00 || !kinetic || !e || !wave_c || !state_in || !state_out)
             throw Error(MSX_ERR_INVALID, "null argument");
-        if (precision < 0 || precision > 1 || variant < 0 || variant > 1 || inverse < 0 ||
+        if (precision < 0 || precision > 1 || variant < 0 || variant > 2 || inverse < 0 ||
             inverse > 1)
             throw Error(MSX_ERR_INVALID, "bad inverse/precision/variant");
+        // variant 2: the matrix-core form with 8 channels per wavefront (FP64)
+        const int cpw = variant == 2 ? 8 : 0;
+        if (variant == 2) variant = 0;
         if (variant == 0 && !diag_mfma_supported(nz))
             throw Error(MSX_ERR_INVALID, "matrix-core diagonal blocks need nz <= 320");
         TableGuard tg;
@@ -1697,7 +2053,7 @@ int msx_debug_apply_diagonal(int nz, const double *potential00, const double *ki
         if (precision == 0) {
             if (variant == 0)
                 launch_diag_blocks<double>(L, t.eig64(), e_d.p, c_d.p, fac.p, xin.p, nullptr,
-                                           yout.p, 1.0, inverse, nullptr, s);
+                                           yout.p, 1.0, inverse, nullptr, s, cpw);
             else if (inverse)
                 launch_dinv<double>(L, t.Q.p, t.QT.p, t.lam.p, e_d.p, c_d.p, fac.p, xin.p,
                                     nullptr, yout.p, 1.0, nullptr, s);

@@ -237,7 +237,7 @@ def test_dft_coupling_matches_direct(nkx, nky, nc):
     ref = oracle.apply_coupling(V, x, channels=(chx, chy))
     out = {}
     for prec in (0, 1):
-        for alg in (0, 1):
+        for alg in (0, 1, 2, 3, 4):  # direct; DFT with 4 (built-in), 1, 2 and 8 nodes per wave
             y = np.zeros_like(x)
             rc = _lib.lib().msx_debug_apply_coupling(nkx, nky, nz, _lib.ptr(V), nc, _lib.ptr(chx),
                                                      _lib.ptr(chy), _lib.ptr(x), _lib.ptr(y), 1,
@@ -249,6 +249,9 @@ def test_dft_coupling_matches_direct(nkx, nky, nc):
     np.testing.assert_allclose(out[0, 1], ref, rtol=0, atol=1e-12 * scale)
     np.testing.assert_allclose(out[1, 0], ref, rtol=0, atol=1e-5 * scale)
     np.testing.assert_allclose(out[1, 1], ref, rtol=0, atol=1e-5 * scale)
+    for alg in (2, 3, 4):  # the node grouping the autotuner may pick changes no arithmetic
+        np.testing.assert_array_equal(out[0, alg], out[0, 1])
+        np.testing.assert_array_equal(out[1, alg], out[1, 1])
     del C
 
 
@@ -346,3 +349,66 @@ def test_upper_and_lower_coupling_modes_match_oracle():
                                                        _lib.ptr(chy), _lib.ptr(x), _lib.ptr(y),
                                                        mode, 0, 0), "coupling")
         np.testing.assert_allclose(y, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def test_autotuner_choices_are_reported_cached_and_change_no_answer():
+    """The autotuner (first solve of a batch shape) times the DFT coupling's node grouping,
+    the Arnoldi block length and the FP64 diagonal blocks' channels per wavefront, keeps the
+    fastest and reports them; a second batch of the same shape takes them from the cache; and
+    the tuned solve gives the built-in shapes' intensities (FP64 to 1e-11, the solve
+    tolerance: only the order of dot-product partial sums can move; mixed likewise)."""
+    w = W.hex_batch(n_conditions=5, nz=48, nk=15, cond_seed=9, channel_counts=[40, 90, 60, 120, 75])
+    tab = BT.Table(w.potential, w.kinetic)
+    res = {}
+    for prec, eps in (("fp64", 5e-13), ("mixed", 5e-10)):
+        for tune in (False, True):
+            bt = BT.Batch(tab, w.conditions)
+            bt.solve(BT.SolveOptions(method="jacobi", precision=prec, eps=eps, autotune=tune))
+            res[prec, tune] = bt.intensities_flat()
+            rep = bt.tuning()
+            if not tune:
+                assert rep == {}
+                continue
+            assert rep["source"] in ("measured", "cache")
+            assert rep["dft_nodes_per_wave_f64"] in (1, 2, 4, 8)
+            assert rep["arnoldi_block_f64"] in (1024, 2048, 4096, 8192)
+            assert rep["arnoldi_block_f32"] in (1024, 2048, 4096, 8192)
+            assert rep["diag_f64_channels_per_wave"] in (8, 16)
+            if prec == "mixed":
+                assert rep["dft_nodes_per_wave_f32"] in (1, 2, 4, 8)
+            again = BT.Batch(tab, w.conditions)
+            again.solve(BT.SolveOptions(method="jacobi", precision=prec, eps=eps))
+            assert again.tuning()["source"] == "cache"
+            for k in ("dft_nodes_per_wave_f64", "arnoldi_block_f64", "arnoldi_block_f32"):
+                assert again.tuning()[k] == rep[k]
+    assert np.abs(res["fp64", True] - res["fp64", False]).max() < 1e-11
+    assert np.abs(res["mixed", True] - res["mixed", False]).max() < 1e-9
+
+
+@pytest.mark.parametrize("method,precision", [("jacobi", "fp64"), ("gauss_seidel", "fp64"),
+                                              ("jacobi", "mixed")])
+def test_graph_replay_matches_eager_launches(method, precision):
+    """Arnoldi iterations replayed from captured HIP graphs run the same kernels with the same
+    arguments as the eager launches: every replay (three solves, with eager solves of another
+    tolerance and host synchronisations between them) gives bit-identical intensities and
+    iteration counts."""
+    w = W.hex_batch(n_conditions=4, nz=40, nk=15, cond_seed=12, channel_counts=[40, 85, 60, 110])
+    tab = BT.Table(w.potential, w.kinetic)
+    bt = BT.Batch(tab, w.conditions)
+    opts = dict(method=method, precision=precision, eps=5e-11 if precision == "mixed" else 5e-12)
+    bt.solve(BT.SolveOptions(graphs=False, **opts))
+    P_e, it_e = bt.intensities_flat(), bt.stats()["iterations"].copy()
+    loose = {**opts, "eps": 5e-6}
+    bt.solve(BT.SolveOptions(graphs=False, **loose))
+    P_loose, it_loose = bt.intensities_flat(), bt.stats()["iterations"].copy()
+    for _ in range(3):
+        bt.solve(BT.SolveOptions(graphs=True, **opts))
+        np.testing.assert_array_equal(bt.intensities_flat(), P_e)
+        np.testing.assert_array_equal(bt.stats()["iterations"], it_e)
+        bt.solve(BT.SolveOptions(graphs=False, **loose))  # eager work between the replays
+        bt.intensities_flat()
+        # graphs at another tolerance: the tolerance reaches the captured Givens launch, so
+        # these must not replay the graphs of the tighter solve (nor those these)
+        bt.solve(BT.SolveOptions(graphs=True, **loose))
+        np.testing.assert_array_equal(bt.intensities_flat(), P_loose)
+        np.testing.assert_array_equal(bt.stats()["iterations"], it_loose)
