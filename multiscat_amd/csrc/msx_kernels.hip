@@ -148,9 +148,14 @@ template void launch_specular_coupling<float>(const Layout &, const cx<float> *,
                                               hipStream_t);
 
 // ---------------------------------------------------------------------------------------
-// Block forward substitution x <- L^-1 x, one workgroup (8 waves) per instance, sequential
-// over channels. Per channel j:
-//   (1) r(z) = x_j(z) - sum_{i<j} V[G_j-G_i](z) psi_i(z)   waves split i, lanes run over z
+// Block forward substitution x <- L^-1 x in channel blocks [j0, j1) of LS_BLOCK channels:
+// lower_solve_kernel solves one block, one workgroup (8 waves) per instance, sequential over
+// its channels, with the coupling to the earlier channels of the same block; before it,
+// lower_update_kernel has subtracted the coupling to every earlier block from the block's
+// right-hand side (all later channels at once, many workgroups). The chain of dependent
+// per-channel steps stays the reference's, but the sum over earlier channels, most of each
+// step's work, leaves the sequential kernel. Per channel j of the block:
+//   (1) r(z) = x_j(z) - sum_{j0<=i<j} V[G_j-G_i](z) psi_i(z)   waves split i, lanes run over z
 //   (2) t_m = sum_z r(z) Q[z][m] / (e_j + lam_m)           waves split z, lanes run over m
 //   (3) g(z) = sum_m QT[m][z] t_m                          waves split m, lanes run over z
 //   (4) psi_j = g + f_j g(N) c_j / (1 - f_j(N) c_j)
@@ -160,11 +165,14 @@ template void launch_specular_coupling<float>(const Layout &, const cx<float> *,
 constexpr int LS_WAVES = 8;
 constexpr int LS_MAXMZ = 10;  // nz <= 640
 
+constexpr int LS_BLOCK = 32;
+
 __global__ __launch_bounds__(512) void lower_solve_kernel(
     Layout L, const cx<double> *__restrict__ V, const double *__restrict__ Q,
     const double *__restrict__ QT, const double *__restrict__ lam, const double *__restrict__ e,
     const cx<double> *__restrict__ c, const double *__restrict__ fac, cx<double> *x,
-    const cx<double> *base, cx<double> *out, double sign, const int *__restrict__ active) {
+    const cx<double> *base, cx<double> *out, double sign, const int *__restrict__ active,
+    int j0, int j1) {
     const int b = blockIdx.x;
     if (active && !active[b]) return;
     extern __shared__ double smem[];
@@ -182,13 +190,14 @@ __global__ __launch_bounds__(512) void lower_solve_kernel(
     const int mz = (nz + 63) >> 6;
     const int zchunk = (nz + LS_WAVES - 1) / LS_WAVES;
 
-    for (int j = 0; j < nc; ++j) {
-        // (1) coupling to the already solved channels
+    const int jend = min(j1, nc);
+    for (int j = j0; j < jend; ++j) {
+        // (1) coupling to the already solved channels of this block
         cx<double> acc[LS_MAXMZ];
 #pragma unroll
         for (int m = 0; m < LS_MAXMZ; ++m) acc[m] = mk(0.0, 0.0);
         const int gxj = chx[j], gyj = chy[j];
-        for (int i = wave; i < j; i += LS_WAVES) {
+        for (int i = j0 + wave; i < j; i += LS_WAVES) {
             const int dx = wrap_mod(gxj - chx[i], nkx);
             const int dy = wrap_mod(gyj - chy[i], nky);
             const cx<double> *vrow = V + ((long long)dx * nky + dy) * nz;
@@ -296,6 +305,46 @@ __global__ __launch_bounds__(512) void lower_solve_kernel(
     }
 }
 
+// x_k(z) -= sum_{j0<=i<j1} V[G_k - G_i](z) psi_i(z) for every channel k >= j1 of each active
+// instance (psi_i = x_i, already solved): one wavefront per target channel, lanes over z.
+constexpr int LU_WAVES = 4;
+__global__ __launch_bounds__(64 * LU_WAVES) void lower_update_kernel(
+    Layout L, const cx<double> *__restrict__ V, cx<double> *x, int j0, int j1,
+    const int *__restrict__ active) {
+    const int b = blockIdx.y;
+    if (active && !active[b]) return;
+    const int nc = L.nc[b];
+    const int k = j1 + blockIdx.x * LU_WAVES + (threadIdx.x >> 6);
+    if (k >= nc) return;
+    const int nz = L.nz, nkx = L.nkx, nky = L.nky, lane = threadIdx.x & 63;
+    const long long off = L.off[b];
+    const int *chx = L.chx + off, *chy = L.chy + off;
+    cx<double> *xb = x + off * nz;
+    const int gxk = chx[k], gyk = chy[k];
+    const int iend = min(j1, nc);
+    for (int z0 = 0; z0 < nz; z0 += 64 * 4) {
+        cx<double> acc[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) acc[m] = mk(0.0, 0.0);
+        for (int i = j0; i < iend; ++i) {
+            const int dx = wrap_mod(gxk - chx[i], nkx);
+            const int dy = wrap_mod(gyk - chy[i], nky);
+            const cx<double> *vrow = V + ((long long)dx * nky + dy) * nz;
+            const cx<double> *xi = xb + (long long)i * nz;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int z = z0 + lane + 64 * m;
+                if (z < nz) cfma(acc[m], vrow[z], xi[z]);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int z = z0 + lane + 64 * m;
+            if (z < nz) xb[(long long)k * nz + z] = xb[(long long)k * nz + z] - acc[m];
+        }
+    }
+}
+
 void launch_lower_solve(const Layout &L, const cx<double> *V, const double *Q, const double *QT,
                         const double *lam, const double *e, const cx<double> *c,
                         const double *fac, cx<double> *x, const cx<double> *base,
@@ -303,9 +352,18 @@ void launch_lower_solve(const Layout &L, const cx<double> *V, const double *Q, c
     if (L.nz > 64 * LS_MAXMZ) throw Error(3, "lower solve supports nz <= 640");
     const size_t shm = sizeof(cx<double>) * (size_t)L.nz * (LS_WAVES + 2);
     set_dynamic_lds((const void *)lower_solve_kernel, shm);
-    hipLaunchKernelGGL(lower_solve_kernel, dim3(L.n_inst), dim3(64 * LS_WAVES), shm, s, L, V, Q,
-                       QT, lam, e, c, fac, x, base, out, sign, active);
-    MSX_LAUNCH_CHECK();
+    for (int j0 = 0; j0 < L.max_nc; j0 += LS_BLOCK) {
+        const int j1 = std::min(L.max_nc, j0 + LS_BLOCK);
+        hipLaunchKernelGGL(lower_solve_kernel, dim3(L.n_inst), dim3(64 * LS_WAVES), shm, s, L, V,
+                           Q, QT, lam, e, c, fac, x, base, out, sign, active, j0, j1);
+        MSX_LAUNCH_CHECK();
+        if (j1 < L.max_nc) {
+            const unsigned gx = (unsigned)((L.max_nc - j1 + LU_WAVES - 1) / LU_WAVES);
+            hipLaunchKernelGGL(lower_update_kernel, dim3(gx, L.n_inst), dim3(64 * LU_WAVES), 0, s,
+                               L, V, x, j0, j1, active);
+            MSX_LAUNCH_CHECK();
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------
