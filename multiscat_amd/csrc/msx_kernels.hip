@@ -163,39 +163,42 @@ template void launch_specular_coupling<float>(const Layout &, const cx<float> *,
 // were written by this workgroup, and __syncthreads orders those writes before the reads.
 // ---------------------------------------------------------------------------------------
 constexpr int LS_WAVES = 8;
-constexpr int LS_MAXMZ = 10;  // nz <= 640
-
+constexpr int LS_MAXMZ = 9;  // nz <= 576 (LDS: 17 vectors of nz complex values)
 constexpr int LS_BLOCK = 32;
 
+// MZ = ceil(nz / 64) values per lane; ZB rows of Q / Q^T loaded together. Two reduction
+// buffers alternate so that a step reads the previous step's partial sums while writing its
+// own, and each wavefront sums the partials of its own rows into a shared vector (vs) that
+// it alone reads next: four workgroup barriers per channel.
+template <int MZ>
 __global__ __launch_bounds__(512) void lower_solve_kernel(
     Layout L, const cx<double> *__restrict__ V, const double *__restrict__ Q,
     const double *__restrict__ QT, const double *__restrict__ lam, const double *__restrict__ e,
     const cx<double> *__restrict__ c, const double *__restrict__ fac, cx<double> *x,
     const cx<double> *base, cx<double> *out, double sign, const int *__restrict__ active,
     int j0, int j1) {
+    constexpr int ZB = MZ <= 4 ? 8 : 4;
     const int b = blockIdx.x;
     if (active && !active[b]) return;
     extern __shared__ double smem[];
     const int nz = L.nz, nkx = L.nkx, nky = L.nky;
-    cx<double> *red = reinterpret_cast<cx<double> *>(smem);  // [LS_WAVES][nz]
-    cx<double> *rs = red + LS_WAVES * nz;                    // [nz]
-    cx<double> *ts = rs + nz;                                // [nz]
-    __shared__ cx<double> s_fac;
+    cx<double> *redA = reinterpret_cast<cx<double> *>(smem);  // [LS_WAVES][nz]
+    cx<double> *redB = redA + LS_WAVES * nz;                  // [LS_WAVES][nz]
+    cx<double> *vs = redB + LS_WAVES * nz;                    // [nz]
 
     const int nc = L.nc[b];
     const long long off = L.off[b];
     const int *chx = L.chx + off, *chy = L.chy + off;
     cx<double> *xb = x + off * nz;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int mz = (nz + 63) >> 6;
     const int zchunk = (nz + LS_WAVES - 1) / LS_WAVES;
-
+    const int rb = wave * zchunk, re = min(nz, rb + zchunk);  // this wavefront's rows
     const int jend = min(j1, nc);
     for (int j = j0; j < jend; ++j) {
-        // (1) coupling to the already solved channels of this block
-        cx<double> acc[LS_MAXMZ];
+        // (1) coupling to the already solved channels of this block -> redA
+        cx<double> acc[MZ];
 #pragma unroll
-        for (int m = 0; m < LS_MAXMZ; ++m) acc[m] = mk(0.0, 0.0);
+        for (int m = 0; m < MZ; ++m) acc[m] = mk(0.0, 0.0);
         const int gxj = chx[j], gyj = chy[j];
         for (int i = j0 + wave; i < j; i += LS_WAVES) {
             const int dx = wrap_mod(gxj - chx[i], nkx);
@@ -203,100 +206,118 @@ __global__ __launch_bounds__(512) void lower_solve_kernel(
             const cx<double> *vrow = V + ((long long)dx * nky + dy) * nz;
             const cx<double> *xi = xb + (long long)i * nz;
 #pragma unroll
-            for (int m = 0; m < LS_MAXMZ; ++m) {
+            for (int m = 0; m < MZ; ++m) {
                 const int z = lane + 64 * m;
-                if (m < mz && z < nz) cfma(acc[m], vrow[z], xi[z]);
+                if (z < nz) cfma(acc[m], vrow[z], xi[z]);
             }
         }
 #pragma unroll
-        for (int m = 0; m < LS_MAXMZ; ++m) {
+        for (int m = 0; m < MZ; ++m) {
             const int z = lane + 64 * m;
-            if (m < mz && z < nz) red[wave * nz + z] = acc[m];
+            if (z < nz) redA[wave * nz + z] = acc[m];
         }
         __syncthreads();
-        for (int z = threadIdx.x; z < nz; z += blockDim.x) {
-            cx<double> r = xb[(long long)j * nz + z];
+        // (2) r = x_j - sum_w redA over this wavefront's rows -> vs; t = Q^T r (rows z of this
+        // wavefront, lanes over m) -> redB
+        const cx<double> *xj = xb + (long long)j * nz;
+        for (int z = rb + lane; z < re; z += 64) {
+            cx<double> r = xj[z];
 #pragma unroll
-            for (int w = 0; w < LS_WAVES; ++w) r = r - red[w * nz + z];
-            rs[z] = r;
+            for (int w = 0; w < LS_WAVES; ++w) r = r - redA[w * nz + z];
+            vs[z] = r;
         }
-        __syncthreads();
-        // (2) transform to the eigenbasis of H0 and divide by (e_j + lam)
-        {
-            const int zb = wave * zchunk, ze = min(nz, zb + zchunk);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wavefront's vs stores landed
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int m = 0; m < LS_MAXMZ; ++m) acc[m] = mk(0.0, 0.0);
-            for (int z = zb; z < ze; ++z) {
-                const cx<double> r = rs[z];
-                const double *qrow = Q + (long long)z * nz;
+        for (int m = 0; m < MZ; ++m) acc[m] = mk(0.0, 0.0);
+        for (int z0 = rb; z0 < re; z0 += ZB) {
+            double q[ZB][MZ];
+            cx<double> r[ZB];
 #pragma unroll
-                for (int m = 0; m < LS_MAXMZ; ++m) {
+            for (int u = 0; u < ZB; ++u) {
+                const int z = z0 + u;
+                const bool ok = z < re;
+                const int zc = ok ? z : rb;
+                r[u] = ok ? vs[zc] : mk(0.0, 0.0);
+                const double *qrow = Q + (long long)zc * nz;
+#pragma unroll
+                for (int m = 0; m < MZ; ++m) {
                     const int mm = lane + 64 * m;
-                    if (m < mz && mm < nz) {
-                        const double q = qrow[mm];
-                        acc[m].re = fma(r.re, q, acc[m].re);
-                        acc[m].im = fma(r.im, q, acc[m].im);
-                    }
+                    q[u][m] = mm < nz ? qrow[mm] : 0.0;
                 }
             }
 #pragma unroll
-            for (int m = 0; m < LS_MAXMZ; ++m) {
-                const int mm = lane + 64 * m;
-                if (m < mz && mm < nz) red[wave * nz + mm] = acc[m];
-            }
+            for (int u = 0; u < ZB; ++u)
+#pragma unroll
+                for (int m = 0; m < MZ; ++m) {
+                    acc[m].re = fma(r[u].re, q[u][m], acc[m].re);
+                    acc[m].im = fma(r[u].im, q[u][m], acc[m].im);
+                }
+        }
+#pragma unroll
+        for (int m = 0; m < MZ; ++m) {
+            const int mm = lane + 64 * m;
+            if (mm < nz) redB[wave * nz + mm] = acc[m];
         }
         __syncthreads();
+        // (3) t_m = sum_w redB / (e_j + lam_m) over this wavefront's eigenvectors -> vs;
+        // g = Q t (eigenvectors of this wavefront, lanes over z) -> redA
         const double ej = e[off + j];
-        for (int mm = threadIdx.x; mm < nz; mm += blockDim.x) {
+        for (int mm = rb + lane; mm < re; mm += 64) {
             cx<double> t = mk(0.0, 0.0);
 #pragma unroll
-            for (int w = 0; w < LS_WAVES; ++w) t = t + red[w * nz + mm];
-            ts[mm] = t * (1.0 / (ej + lam[mm]));
+            for (int w = 0; w < LS_WAVES; ++w) t = t + redB[w * nz + mm];
+            vs[mm] = t * (1.0 / (ej + lam[mm]));
         }
-        __syncthreads();
-        // (3) back to the node basis
-        {
-            const int mb = wave * zchunk, me = min(nz, mb + zchunk);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int m = 0; m < LS_MAXMZ; ++m) acc[m] = mk(0.0, 0.0);
-            for (int mm = mb; mm < me; ++mm) {
-                const cx<double> t = ts[mm];
-                const double *qtrow = QT + (long long)mm * nz;
+        for (int m = 0; m < MZ; ++m) acc[m] = mk(0.0, 0.0);
+        for (int m0 = rb; m0 < re; m0 += ZB) {
+            double q[ZB][MZ];
+            cx<double> t[ZB];
 #pragma unroll
-                for (int m = 0; m < LS_MAXMZ; ++m) {
+            for (int u = 0; u < ZB; ++u) {
+                const int mm = m0 + u;
+                const bool ok = mm < re;
+                const int mc = ok ? mm : rb;
+                t[u] = ok ? vs[mc] : mk(0.0, 0.0);
+                const double *qtrow = QT + (long long)mc * nz;
+#pragma unroll
+                for (int m = 0; m < MZ; ++m) {
                     const int z = lane + 64 * m;
-                    if (m < mz && z < nz) {
-                        const double q = qtrow[z];
-                        acc[m].re = fma(t.re, q, acc[m].re);
-                        acc[m].im = fma(t.im, q, acc[m].im);
-                    }
+                    q[u][m] = z < nz ? qtrow[z] : 0.0;
                 }
             }
 #pragma unroll
-            for (int m = 0; m < LS_MAXMZ; ++m) {
-                const int z = lane + 64 * m;
-                if (m < mz && z < nz) red[wave * nz + z] = acc[m];
-            }
+            for (int u = 0; u < ZB; ++u)
+#pragma unroll
+                for (int m = 0; m < MZ; ++m) {
+                    acc[m].re = fma(t[u].re, q[u][m], acc[m].re);
+                    acc[m].im = fma(t[u].im, q[u][m], acc[m].im);
+                }
+        }
+#pragma unroll
+        for (int m = 0; m < MZ; ++m) {
+            const int z = lane + 64 * m;
+            if (z < nz) redA[wave * nz + z] = acc[m];
         }
         __syncthreads();
+        // (4) g = sum_w redA; the Sherman-Morrison boundary correction (every thread forms the
+        // factor from g(N) itself: eight broadcast reads, no extra barrier)
+        const double *fj = fac + (off + j) * nz;
+        cx<double> gN = mk(0.0, 0.0);
+#pragma unroll
+        for (int w = 0; w < LS_WAVES; ++w) gN = gN + redA[w * nz + nz - 1];
+        const cx<double> cj = c[off + j];
+        const cx<double> f = cdiv(gN * cj, mk(1.0, 0.0) - cj * fj[nz - 1]);
         for (int z = threadIdx.x; z < nz; z += blockDim.x) {
             cx<double> g = mk(0.0, 0.0);
 #pragma unroll
-            for (int w = 0; w < LS_WAVES; ++w) g = g + red[w * nz + z];
-            rs[z] = g;
-        }
-        __syncthreads();
-        // (4) Sherman-Morrison boundary correction
-        const double *fj = fac + (off + j) * nz;
-        if (threadIdx.x == 0) {
-            const cx<double> cj = c[off + j];
-            const cx<double> den = mk(1.0, 0.0) - cj * fj[nz - 1];
-            s_fac = cdiv(rs[nz - 1] * cj, den);
-        }
-        __syncthreads();
-        const cx<double> f = s_fac;
-        for (int z = threadIdx.x; z < nz; z += blockDim.x) {
-            const cx<double> psi = rs[z] + f * fj[z];
+            for (int w = 0; w < LS_WAVES; ++w) g = g + redA[w * nz + z];
+            const cx<double> psi = g + f * fj[z];
             const long long idx = (off + j) * nz + z;
             xb[(long long)j * nz + z] = psi;
             if (out) out[idx] = base[idx] + psi * sign;
@@ -345,17 +366,36 @@ __global__ __launch_bounds__(64 * LU_WAVES) void lower_update_kernel(
     }
 }
 
+template <int MZ>
+static void launch_lower_block(const Layout &L, const cx<double> *V, const double *Q,
+                               const double *QT, const double *lam, const double *e,
+                               const cx<double> *c, const double *fac, cx<double> *x,
+                               const cx<double> *base, cx<double> *out, double sign,
+                               const int *active, hipStream_t s, int j0, int j1) {
+    const size_t shm = sizeof(cx<double>) * (size_t)L.nz * (2 * LS_WAVES + 1);
+    set_dynamic_lds((const void *)lower_solve_kernel<MZ>, shm);
+    hipLaunchKernelGGL(lower_solve_kernel<MZ>, dim3(L.n_inst), dim3(64 * LS_WAVES), shm, s, L, V,
+                       Q, QT, lam, e, c, fac, x, base, out, sign, active, j0, j1);
+}
+
 void launch_lower_solve(const Layout &L, const cx<double> *V, const double *Q, const double *QT,
                         const double *lam, const double *e, const cx<double> *c,
                         const double *fac, cx<double> *x, const cx<double> *base,
                         cx<double> *out, double sign, const int *active, hipStream_t s) {
-    if (L.nz > 64 * LS_MAXMZ) throw Error(3, "lower solve supports nz <= 640");
-    const size_t shm = sizeof(cx<double>) * (size_t)L.nz * (LS_WAVES + 2);
-    set_dynamic_lds((const void *)lower_solve_kernel, shm);
+    if (L.nz > 576) throw Error(3, "lower solve supports nz <= 576");
+    const int mz = (L.nz + 63) / 64;
     for (int j0 = 0; j0 < L.max_nc; j0 += LS_BLOCK) {
         const int j1 = std::min(L.max_nc, j0 + LS_BLOCK);
-        hipLaunchKernelGGL(lower_solve_kernel, dim3(L.n_inst), dim3(64 * LS_WAVES), shm, s, L, V,
-                           Q, QT, lam, e, c, fac, x, base, out, sign, active, j0, j1);
+        switch (mz) {
+#define MSX_LS(M)                                                                             \
+    case M:                                                                                   \
+        launch_lower_block<M>(L, V, Q, QT, lam, e, c, fac, x, base, out, sign, active, s, j0, j1); \
+        break;
+            MSX_LS(1) MSX_LS(2) MSX_LS(3) MSX_LS(4) MSX_LS(5)
+            MSX_LS(6) MSX_LS(7) MSX_LS(8) MSX_LS(9)
+#undef MSX_LS
+            default: throw Error(3, "lower solve supports nz <= 576");
+        }
         MSX_LAUNCH_CHECK();
         if (j1 < L.max_nc) {
             const unsigned gx = (unsigned)((L.max_nc - j1 + LU_WAVES - 1) / LU_WAVES);
