@@ -180,7 +180,8 @@ typedef struct {
                          0 = sqrt(eps)/3 clamped to [1e-5, 2.5e-4] (two sweeps reach eps) */
     int max_refine;   /* mixed precision: FP64 refinement sweeps allowed                    */
     int coupling;     /* full coupling: 0 = DFT form on the matrix cores when the table fits
-                         (<= 32 x 32), 1 = direct sum                                      */
+                         (<= 80 x 80; the gradient map of the adjoint: <= 32 x 32), 1 = direct
+                         sum                                                                 */
     int diag;         /* all-channel D^-1: 0 = auto (matrix cores when nz <= 320), 1 = vector, 2 = matrix cores */
     int warm_start;   /* 1: GMRES starts from the batch's current state (msx_batch_set_state, or the
                          last solve) instead of 0; tolerance still relative to the source, as an

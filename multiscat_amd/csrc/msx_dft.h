@@ -21,7 +21,7 @@ struct DftStore {
     DftTables<float> d32() const { return {np, Fxf.p, Fyf.p, Vhatf.p}; }
 };
 
-// Padded transform size (16 or 32) for an nkx x nky table, 0 if larger than 32.
+// Padded transform size (16, 32, 48, 64 or 80) for an nkx x nky table, 0 if larger than 80.
 int dft_padded_size(int nkx, int nky);
 
 // Builds the DFT matrices and Vhat from the FP64 table V[g][z] (both precisions).

@@ -32,6 +32,15 @@
 
 namespace msx {
 
+// Workgroups go to the device's 8 XCDs (each with its own L2) round-robin by linear index.
+// xcd_order maps a 1-D launch of 8 ceil(total / 8) workgroups onto work items 0 .. total-1 so
+// that consecutive items land on one XCD in dispatch order (false: an item past the end).
+__device__ __forceinline__ bool xcd_order(int total, int &item) {
+    const int L = blockIdx.x, S = (total + 7) >> 3;
+    item = (L & 7) * S + (L >> 3);
+    return item < total;
+}
+
 template <int NP>
 struct DftGeom {
     static constexpr int NT = NP / 16;             // 16 x 16 output tiles per side
@@ -289,7 +298,20 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
     // three real products per complex one when one F in registers serves both transforms
     // (register budget)
     constexpr bool M3 = SQ;
-    const int b = blockIdx.x;
+    // NM: instances fastest (workgroups resident together share the Vhat slabs of their
+    // nodes); channel-major: the node groups of one instance consecutive on one XCD (128-byte
+    // lines of the state hold several nodes of a channel, so neighbouring groups share lines)
+    int b, zg;
+    if constexpr (NM) {
+        b = blockIdx.x;
+        zg = blockIdx.y;
+    } else {
+        const int nzb = (L.nz + zpb - 1) / zpb;
+        int item;
+        if (!xcd_order(L.n_inst * nzb, item)) return;
+        b = item / nzb;
+        zg = item - b * nzb;
+    }
     if (active && !active[b]) return;
     const int nz = L.nz;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -371,8 +393,8 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
     };
     using plain_t = std::integral_constant<bool, false>;
     using conj_t = std::integral_constant<bool, true>;
-    const int z_first = blockIdx.y * zpb + wave;
-    const int z_end = min(nz, (blockIdx.y + 1) * zpb);
+    const int z_first = zg * zpb + wave;
+    const int z_end = min(nz, (zg + 1) * zpb);
     if (z_first < z_end) load_x(z_first);
     // the first node's data waited for here: with loads pending on entry, the loop head
     // waited for every outstanding operation, the previous node's stores included
@@ -447,6 +469,246 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Tables beyond 32 x 32 (padded to NP = 48, 64 or 80): one workgroup of NT = NP / 16
+// wavefronts per (instance, node); wavefront w owns rows [16 w, 16 w + 16) of the NP x NP grid
+// in one LDS slab. NP^2 DFT coefficients are too many registers to hold and, at NP = 80, too
+// much LDS beside the slab, so the matrices are read from L2 as the MFMA operands. A left
+// product F Y needs every row of Y: each wavefront forms its row strip in registers, the
+// workgroup waits, then the strips are stored; a right product Y F reads only the wavefront's
+// own strip. Square tables use the three-product complex form with one matrix (F_y = F_x).
+// Channel-major vectors; the node of a workgroup runs fastest in an XCD's dispatch order, so
+// the nodes of one channel that share a 128-byte line meet in that XCD's L2.
+// ---------------------------------------------------------------------------------------
+template <int NT>
+struct BigGeom {
+    static constexpr int NP = 16 * NT;
+    static constexpr int S = NP + 4;  // row stride of the slab
+    static constexpr int PLANE = NP * S;
+};
+
+// (cr, ci) = F Y (CONJ: conj(F) Y) for this wavefront's row strip ti: F[ti*16 + i][k] from
+// global memory (row-major NP x NP), Y[k][j] from the slab.
+template <typename T, int NT, bool CONJ, bool M3>
+__device__ __forceinline__ void big_left(const cx<T> *__restrict__ F, const T *br, const T *bi,
+                                         typename Mfma<T>::acc (&cr)[NT],
+                                         typename Mfma<T>::acc (&ci)[NT]) {
+    using G = BigGeom<NT>;
+    using acc_t = typename Mfma<T>::acc;
+    const int lane = threadIdx.x & 63, l16 = lane & 15, k4 = lane >> 4, ti = threadIdx.x >> 6;
+    acc_t c3[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) cr[t] = ci[t] = c3[t] = acc_t{0, 0, 0, 0};
+    const cx<T> *frow = F + (size_t)(ti * 16 + l16) * G::NP;
+#pragma unroll 4
+    for (int kk = 0; kk < G::NP / 4; ++kk) {
+        const int k = kk * 4 + k4;
+        const cx<T> f = frow[k];
+        T yr[NT], yi[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            yr[t] = br[k * G::S + t * 16 + l16];
+            yi[t] = bi[k * G::S + t * 16 + l16];
+        }
+        if constexpr (M3) {
+            const T fs = f.re + f.im;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                cr[t] = Mfma<T>::op(f.re, yr[t], cr[t]);
+                ci[t] = Mfma<T>::op(f.im, yi[t], ci[t]);
+                c3[t] = Mfma<T>::op(fs, CONJ ? yr[t] - yi[t] : yr[t] + yi[t], c3[t]);
+            }
+        } else {
+            // F y: re += fr yr - fi yi, im += fr yi + fi yr; conj(F) y: re += fr yr + fi yi,
+            // im += fr yi - fi yr
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                cr[t] = Mfma<T>::op(f.re, yr[t], cr[t]);
+                cr[t] = Mfma<T>::op(f.im, CONJ ? yi[t] : -yi[t], cr[t]);
+                ci[t] = Mfma<T>::op(f.re, yi[t], ci[t]);
+                ci[t] = Mfma<T>::op(f.im, CONJ ? -yr[t] : yr[t], ci[t]);
+            }
+        }
+    }
+    if constexpr (M3) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const T a = cr[t][v], b = ci[t][v], c = c3[t][v];
+                cr[t][v] = CONJ ? a + b : a - b;
+                ci[t][v] = CONJ ? a - b - c : c - a - b;
+            }
+    }
+}
+
+// (cr, ci) = Y F (CONJ: Y conj(F)) for this wavefront's row strip: Y[ti*16 + i][k] from the
+// slab, F[k][j] from global memory.
+template <typename T, int NT, bool CONJ, bool M3>
+__device__ __forceinline__ void big_right(const T *ar, const T *ai, const cx<T> *__restrict__ F,
+                                          typename Mfma<T>::acc (&cr)[NT],
+                                          typename Mfma<T>::acc (&ci)[NT]) {
+    using G = BigGeom<NT>;
+    using acc_t = typename Mfma<T>::acc;
+    const int lane = threadIdx.x & 63, l16 = lane & 15, k4 = lane >> 4, ti = threadIdx.x >> 6;
+    acc_t c3[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) cr[t] = ci[t] = c3[t] = acc_t{0, 0, 0, 0};
+    const int arow = (ti * 16 + l16) * G::S;
+#pragma unroll 4
+    for (int kk = 0; kk < G::NP / 4; ++kk) {
+        const int k = kk * 4 + k4;
+        const T xr = ar[arow + k], xi = ai[arow + k];
+        cx<T> f[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) f[t] = F[(size_t)k * G::NP + t * 16 + l16];
+        if constexpr (M3) {
+            const T xs = CONJ ? xr - xi : xr + xi;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                cr[t] = Mfma<T>::op(xr, f[t].re, cr[t]);
+                ci[t] = Mfma<T>::op(xi, f[t].im, ci[t]);
+                c3[t] = Mfma<T>::op(xs, f[t].re + f[t].im, c3[t]);
+            }
+        } else {
+            // x F: re += xr fr - xi fi, im += xr fi + xi fr; x conj(F): re += xr fr + xi fi,
+            // im += xi fr - xr fi
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                cr[t] = Mfma<T>::op(xr, f[t].re, cr[t]);
+                cr[t] = Mfma<T>::op(CONJ ? xi : -xi, f[t].im, cr[t]);
+                ci[t] = Mfma<T>::op(CONJ ? -xr : xr, f[t].im, ci[t]);
+                ci[t] = Mfma<T>::op(xi, f[t].re, ci[t]);
+            }
+        }
+    }
+    if constexpr (M3) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const T a = cr[t][v], b = ci[t][v], c = c3[t][v];
+                cr[t][v] = CONJ ? a + b : a - b;
+                ci[t][v] = CONJ ? a - b - c : c - a - b;
+            }
+    }
+}
+
+template <typename T, int NT>
+__device__ __forceinline__ void big_store(T *sr, T *si, const typename Mfma<T>::acc (&cr)[NT],
+                                          const typename Mfma<T>::acc (&ci)[NT]) {
+    using G = BigGeom<NT>;
+    const int lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4, ti = threadIdx.x >> 6;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = ti * 16 + Mfma<T>::row(g, v), c = t * 16 + col;
+            sr[r * G::S + c] = cr[t][v];
+            si[r * G::S + c] = ci[t][v];
+        }
+}
+
+template <typename T, int NT, bool SQ>
+__global__ __launch_bounds__(64 * NT) void dft_coupling_big_kernel(
+    Layout L, const cx<T> *__restrict__ Fx, const cx<T> *__restrict__ Fy,
+    const cx<T> *__restrict__ Vhat, const cx<T> *__restrict__ x, cx<T> *__restrict__ y,
+    const int *__restrict__ active, const double *__restrict__ x_norm, cx<T> *__restrict__ x_copy) {
+    using G = BigGeom<NT>;
+    using acc_t = typename Mfma<T>::acc;
+    constexpr bool M3 = SQ;
+    const int nz = L.nz;
+    int item;
+    if (!xcd_order(L.n_inst * nz, item)) return;
+    const int b = item / nz, z = item - b * nz;
+    if (active && !active[b]) return;
+    extern __shared__ double dft_smem[];
+    T *sr = reinterpret_cast<T *>(dft_smem);
+    T *si = sr + G::PLANE;
+    const int nc = L.nc[b];
+    const long long off = L.off[b];
+    const T sc = x_copy ? (T)(1.0 / x_norm[b]) : T(1);
+    const int lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4, ti = threadIdx.x >> 6;
+    {
+        acc_t *p4 = reinterpret_cast<acc_t *>(sr);  // both planes
+        constexpr int N4 = 2 * G::PLANE * (int)sizeof(T) / (int)sizeof(acc_t);
+        for (int idx = threadIdx.x; idx < N4; idx += blockDim.x) p4[idx] = acc_t{0, 0, 0, 0};
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < nc; j += blockDim.x) {
+        const int p = L.chx[off + j] * G::S + L.chy[off + j];
+        const long long idx = (off + j) * nz + z;
+        cx<T> v = x[idx];
+        if (x_copy) {
+            v = mk(v.re * sc, v.im * sc);
+            x_copy[idx] = v;
+        }
+        sr[p] = v.re;
+        si[p] = v.im;
+    }
+    __syncthreads();
+    acc_t cr[NT], ci[NT];
+    const cx<T> *FyR = SQ ? Fx : Fy;
+    // (1) Y = F_x X
+    big_left<T, NT, false, M3>(Fx, sr, si, cr, ci);
+    __syncthreads();
+    big_store<T, NT>(sr, si, cr, ci);
+    __builtin_amdgcn_wave_barrier();
+    // (2) Z = Y F_y, then Z .* Vhat(z), back into this wavefront's strip
+    big_right<T, NT, false, M3>(sr, si, FyR, cr, ci);
+    {
+        const cx<T> *vz = Vhat + (size_t)z * G::NP * G::NP;
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const cx<T> h = vz[(ti * 16 + Mfma<T>::row(g, v)) * G::NP + t * 16 + col];
+                const T a = cr[t][v], c = ci[t][v];
+                cr[t][v] = a * h.re - c * h.im;
+                ci[t][v] = a * h.im + c * h.re;
+            }
+    }
+    __builtin_amdgcn_wave_barrier();
+    big_store<T, NT>(sr, si, cr, ci);
+    __syncthreads();
+    // (3) W = conj(F_x) Zhat
+    big_left<T, NT, true, M3>(Fx, sr, si, cr, ci);
+    __syncthreads();
+    big_store<T, NT>(sr, si, cr, ci);
+    __builtin_amdgcn_wave_barrier();
+    // (4) out = W conj(F_y)
+    big_right<T, NT, true, M3>(sr, si, FyR, cr, ci);
+    __builtin_amdgcn_wave_barrier();
+    big_store<T, NT>(sr, si, cr, ci);
+    __syncthreads();
+    for (int j = threadIdx.x; j < nc; j += blockDim.x) {
+        const int p = L.chx[off + j] * G::S + L.chy[off + j];
+        y[(off + j) * nz + z] = mk(sr[p], si[p]);
+    }
+}
+
+template <typename T, int NT>
+static void launch_dft_big(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
+                           const int *active, hipStream_t s, const double *x_norm, cx<T> *x_copy) {
+    using G = BigGeom<NT>;
+    if (L.nm) throw Error(3, "DFT coupling beyond 32 x 32: channel-major vectors only");
+    const size_t shm = sizeof(T) * 2 * (size_t)G::PLANE;
+    const long long items = (long long)L.n_inst * L.nz;
+    if (items * 8 >= (1LL << 31)) throw Error(3, "DFT coupling: too many (instance, node) pairs");
+    const bool sq = L.nkx == L.nky;
+    const dim3 grid((unsigned)(8 * ((items + 7) / 8)));
+    if (sq) {
+        set_dynamic_lds((const void *)dft_coupling_big_kernel<T, NT, true>, shm);
+        hipLaunchKernelGGL((dft_coupling_big_kernel<T, NT, true>), grid, dim3(64 * NT), shm, s, L,
+                           D.Fx, D.Fy, D.Vhat, x, y, active, x_norm, x_copy);
+    } else {
+        set_dynamic_lds((const void *)dft_coupling_big_kernel<T, NT, false>, shm);
+        hipLaunchKernelGGL((dft_coupling_big_kernel<T, NT, false>), grid, dim3(64 * NT), shm, s, L,
+                           D.Fx, D.Fy, D.Vhat, x, y, active, x_norm, x_copy);
+    }
+    MSX_LAUNCH_CHECK();
+}
+
 int dft_block_nodes(const Layout &L, bool fp64, int active_hint, int nodes_per_wave) {
     const int WPB = fp64 ? 2 : 4;
     int zpb = (nodes_per_wave > 0 ? nodes_per_wave : 4) * WPB;
@@ -471,7 +733,8 @@ static void launch_dft_impl(const Layout &L, const DftTables<T> &D, const cx<T> 
     // one node per wavefront when so few instances iterate that four-node groups would leave
     // the device short of wavefronts (fewer than about eight per SIMD)
     const int zpb = dft_block_nodes(L, sizeof(T) == 8, active_hint, nodes_per_wave);
-    dim3 grid(L.n_inst, (L.nz + zpb - 1) / zpb);
+    const int nzb = (L.nz + zpb - 1) / zpb;
+    const dim3 grid = NM ? dim3(L.n_inst, nzb) : dim3(8 * ((L.n_inst * nzb + 7) / 8));
     hipLaunchKernelGGL((dft_coupling_kernel<T, NP, WPB, SQ, NM>), grid, dim3(64 * WPB), shm, s, L,
                        D.Fx, D.Fy, D.Vhat, x, y, active, zpb, x_norm, x_copy);
     MSX_LAUNCH_CHECK();
@@ -505,7 +768,10 @@ void launch_dft_coupling(const Layout &L, const DftTables<T> &D, const cx<T> *x,
     if (nodes_per_wave < 0 || nodes_per_wave > 16) throw Error(3, "DFT coupling: 0..16 nodes per wave");
     if (D.np == 16) launch_dft_np<T, 16, WPB>(L, D, x, y, active, s, active_hint, x_norm, x_copy, nodes_per_wave);
     else if (D.np == 32) launch_dft_np<T, 32, WPB>(L, D, x, y, active, s, active_hint, x_norm, x_copy, nodes_per_wave);
-    else throw Error(3, "DFT coupling supports tables up to 32 x 32");
+    else if (D.np == 48) launch_dft_big<T, 3>(L, D, x, y, active, s, x_norm, x_copy);
+    else if (D.np == 64) launch_dft_big<T, 4>(L, D, x, y, active, s, x_norm, x_copy);
+    else if (D.np == 80) launch_dft_big<T, 5>(L, D, x, y, active, s, x_norm, x_copy);
+    else throw Error(3, "DFT coupling supports tables up to 80 x 80");
 }
 template void launch_dft_coupling<float>(const Layout &, const DftTables<float> &,
                                          const cx<float> *, cx<float> *, const int *, hipStream_t,
@@ -524,7 +790,8 @@ template void launch_dft_coupling<double>(const Layout &, const DftTables<double
 // node) pair costs two forward transforms (four small complex GEMMs on the matrix cores)
 // instead of nc^2 gathered products. FP64 (the adjoint's precision), channel-major states.
 // dft_gradient_kernel: one wavefront per node, conditions [y * per_chunk, ...) of chunk y,
-// the summed product written to part[y][z][kx][ky] (the accumulator tiles' element order).
+// the summed product written to part[y][z][kx][ky] (the accumulator tiles' element order);
+// a chunk's node blocks run consecutively on one XCD.
 // dft_gradient_finish_kernel: one wavefront per node sums the chunks, applies
 // conj(F_x) . conj(F_y) / (nkx nky) and adds conj(-C) (Re(-C) at g = 0, where only Re V_00
 // enters the problem) to grad[g][z].
@@ -588,14 +855,20 @@ template <int NP, int WPB, bool SQ>
 __global__ __launch_bounds__(64 * WPB, 1) void dft_gradient_kernel(
     Layout L, const cx<double> *__restrict__ Fx, const cx<double> *__restrict__ Fy,
     const cx<double> *__restrict__ lam, const cx<double> *__restrict__ psi,
-    cx<double> *__restrict__ part, int per_chunk) {
+    cx<double> *__restrict__ part, int per_chunk, int n_chunks) {
     using G = DftGeom<NP>;
     using acc_t = typename Mfma<double>::acc;
     constexpr int NT = G::NT, SA = G::SA, SB = G::SB;
     const int nz = L.nz, nkx = L.nkx, nky = L.nky;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 15, g = lane >> 4;
-    const int z = blockIdx.x * WPB + wave;
+    // node blocks of a chunk consecutive on one XCD: neighbouring nodes of a channel share the
+    // 128-byte lines of the channel-major states
+    const int zblocks = (nz + WPB - 1) / WPB;
+    int item;
+    if (!xcd_order(zblocks * n_chunks, item)) return;
+    const int chunk = item / zblocks;
+    const int z = (item - chunk * zblocks) * WPB + wave;
     if (z >= nz) return;  // no workgroup barrier below: a wavefront may leave on its own
     extern __shared__ double dft_smem[];
     double *sr = dft_smem + (size_t)wave * 4 * G::PLANE;
@@ -609,7 +882,7 @@ __global__ __launch_bounds__(64 * WPB, 1) void dft_gradient_kernel(
     for (int ti = 0; ti < NT; ++ti)
 #pragma unroll
         for (int tj = 0; tj < NT; ++tj) ar[ti][tj] = ai[ti][tj] = acc_t{0, 0, 0, 0};
-    const int b0 = blockIdx.y * per_chunk;
+    const int b0 = chunk * per_chunk;
     const int b1 = min(L.n_inst, b0 + per_chunk);
     using plain_t = std::integral_constant<bool, false>;
     for (int b = b0; b < b1; ++b) {
@@ -655,7 +928,7 @@ __global__ __launch_bounds__(64 * WPB, 1) void dft_gradient_kernel(
                 }
         __builtin_amdgcn_wave_barrier();
     }
-    cx<double> *out = part + ((size_t)blockIdx.y * nz + z) * NP * NP;
+    cx<double> *out = part + ((size_t)chunk * nz + z) * NP * NP;
 #pragma unroll
     for (int ti = 0; ti < NT; ++ti)
 #pragma unroll
@@ -724,8 +997,8 @@ static void launch_dft_gradient_np(const Layout &L, const DftTables<double> &D,
     if (part.n < need) part.alloc(need);
     const size_t shm = sizeof(double) * (size_t)WPB * 4 * G::PLANE;
     set_dynamic_lds((const void *)dft_gradient_kernel<NP, WPB, SQ>, shm);
-    hipLaunchKernelGGL((dft_gradient_kernel<NP, WPB, SQ>), dim3(zblocks, used), dim3(64 * WPB), shm,
-                       s, L, D.Fx, D.Fy, lam, psi, part.p, per_chunk);
+    hipLaunchKernelGGL((dft_gradient_kernel<NP, WPB, SQ>), dim3(8 * ((zblocks * used + 7) / 8)),
+                       dim3(64 * WPB), shm, s, L, D.Fx, D.Fy, lam, psi, part.p, per_chunk, used);
     MSX_LAUNCH_CHECK();
     const size_t shm2 = sizeof(double) * (size_t)WPB * 2 * G::PLANE;
     set_dynamic_lds((const void *)dft_gradient_finish_kernel<NP, WPB, SQ>, shm2);
@@ -814,6 +1087,9 @@ int dft_padded_size(int nkx, int nky) {
     const int m = nkx > nky ? nkx : nky;
     if (m <= 16) return 16;
     if (m <= 32) return 32;
+    if (m <= 48) return 48;
+    if (m <= 64) return 64;
+    if (m <= 80) return 80;
     return 0;
 }
 

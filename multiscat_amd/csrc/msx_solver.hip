@@ -605,10 +605,11 @@ struct Ops {
         L32 = L;
     }
     // FP32 vectors node-major when every FP32 kernel of this configuration reads and writes
-    // that order: the DFT coupling and the diagonal blocks with Q^T in LDS (nzp <= 160)
+    // that order: the DFT coupling of tables up to 32 x 32 and the diagonal blocks with Q^T in
+    // LDS (nzp <= 160)
     void set_fp32_order() {
         L32 = L;
-        L32.nm = (!direct && !vector_diag && t.nzp > 0 && t.nzp <= 160) ? 1 : 0;
+        L32.nm = (!direct && !vector_diag && t.nzp > 0 && t.nzp <= 160 && dft->np <= 32) ? 1 : 0;
         if (L32.nm && !B.facf_nm_ready) {  // the FP32 factors in the same order as the vectors
             msx_batch::need(B.facf_nm_d, (size_t)B.n_elem());
             launch_to_float_real_nm(L32, B.fac_d.p, B.facf_nm_d.p, s);
@@ -1807,7 +1808,7 @@ int msx_batch_adjoint(msx_batch *B, const msx_solve_options *opt, const double *
         const size_t nG = t.nG();
         msx_batch::need(B->grad_d, nG * t.nz);
         MSX_HIP(hipMemsetAsync(B->grad_d.p, 0, sizeof(cx<double>) * nG * t.nz, s));
-        const bool dft_form = opt->coupling == 0 && t.dft.np != 0;
+        const bool dft_form = opt->coupling == 0 && (t.dft.np == 16 || t.dft.np == 32);
         {
             // correlation on the grid: two forward 2-D transforms per condition and node (four
             // complex GEMMs), one inverse per node; the direct form gathers nc^2 products

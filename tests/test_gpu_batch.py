@@ -217,12 +217,14 @@ def test_eigensystem_on_gpu_large(nz):
 
 
 @pytest.mark.parametrize("nkx,nky,nc", [(15, 15, 120), (31, 31, 300), (11, 11, 121), (7, 7, 20),
-                                         (15, 9, 100), (31, 20, 400), (9, 13, 60)])
+                                         (15, 9, 100), (31, 20, 400), (9, 13, 60),
+                                         (33, 33, 200), (65, 65, 900), (64, 50, 700)])
 def test_dft_coupling_matches_direct(nkx, nky, nc):
     """Convolution (DFT on the matrix cores) vs direct sum of the full coupling, and the
     direct sum vs the oracle, on a random table and state; square tables take the
     three-product kernel with one DFT matrix in registers, rectangular ones the four-product
-    kernel with two."""
+    kernel with two; tables beyond 32 x 32 (padded to 48, 64, 80) the workgroup kernel with the
+    DFT matrices read from L2."""
     import ctypes as C
 
     from multiscat_amd import _lib
@@ -412,3 +414,21 @@ def test_graph_replay_matches_eager_launches(method, precision):
         bt.solve(BT.SolveOptions(graphs=True, **loose))
         np.testing.assert_array_equal(bt.intensities_flat(), P_loose)
         np.testing.assert_array_equal(bt.stats()["iterations"], it_loose)
+
+
+@pytest.mark.parametrize("precision,eps,tol", [("fp64", 5e-13, 1e-10), ("mixed", 5e-10, 1e-7)])
+def test_large_table_dft_solve_matches_oracle(precision, eps, tol):
+    """A 45 x 45 table (the DFT coupling on a 48-point padded grid, workgroup kernel): block-
+    Jacobi solves of three uneven conditions against the oracle."""
+    w = W.hex_batch(n_conditions=3, nz=36, nk=45, cond_seed=14, channel_counts=[300, 520, 410])
+    tab = BT.Table(w.potential, w.kinetic)
+    bt = BT.Batch(tab, w.conditions)
+    bt.solve(BT.SolveOptions(method="jacobi", precision=precision, eps=eps))
+    bt_d = BT.Batch(tab, w.conditions)
+    bt_d.solve(BT.SolveOptions(method="jacobi", precision=precision, eps=eps, coupling="direct"))
+    for c, P, Pd in zip(w.conditions, bt.intensities(), bt_d.intensities()):
+        psi = oracle.solve(1, 0, w.potential, c.e, c.wave_a, c.wave_b, c.wave_c, w.kinetic,
+                           channels=c.channels, eps=5e-13)
+        ref = oracle.intensities(psi, c.wave_a, c.wave_b)[c.channels[0], c.channels[1]]
+        assert np.abs(P - ref).max() < tol
+        assert np.abs(P - Pd).max() < tol
