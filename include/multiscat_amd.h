@@ -302,6 +302,9 @@ int msx_profile_reset(void);
 int msx_profile_read(const char *kernel, long long *count, double *total_ms, double *flops,
                      double *bytes);
 int msx_profile_names(char *buf, int buflen);
+/* Restricts the instrumentation to the kernel names in a comma-separated list (NULL or ""
+ * instruments every kernel again): the others launch with no events around them. */
+int msx_profile_select(const char *names);
 
 #ifdef __cplusplus
 }

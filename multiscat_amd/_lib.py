@@ -41,7 +41,7 @@ EXPORTED = (
     "msx_batch_stats", "msx_batch_apply_lower", "msx_batch_tuning", "msx_batch_device_state",
     "msx_batch_total_channels",
     "msx_batch_adjoint", "msx_profile_enable", "msx_profile_reset", "msx_profile_read",
-    "msx_profile_names",
+    "msx_profile_names", "msx_profile_select",
 )
 
 
@@ -123,6 +123,7 @@ def _declare(lib):
         "msx_profile_read": (i32, [C.c_char_p, C.POINTER(ll), C.POINTER(f64), C.POINTER(f64),
                                    C.POINTER(f64)]),
         "msx_profile_names": (i32, [C.c_char_p, i32]),
+        "msx_profile_select": (i32, [C.c_char_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -167,6 +168,11 @@ def default_options(nsig: int) -> SolveOptions:
 def profile_enable(on: bool = True) -> None:
     lib().msx_profile_enable(1 if on else 0)
     lib().msx_profile_reset()
+
+
+def profile_select(names=None) -> None:
+    """Instrument only the listed kernel names (None: every kernel)."""
+    lib().msx_profile_select(",".join(names).encode() if names else None)
 
 
 def profile_report() -> dict:

@@ -745,14 +745,10 @@ static void launch_dft_np(const Layout &L, const DftTables<T> &D, const cx<T> *x
                           const int *active, hipStream_t s, int active_hint, const double *x_norm,
                           cx<T> *x_copy, int npw) {
     const bool sq = L.nkx == L.nky;
-    if constexpr (sizeof(T) == 4) {
-        if (L.nm) {
-            if (sq) launch_dft_impl<T, NP, WPB, true, true>(L, D, x, y, active, s, active_hint, x_norm, x_copy, npw);
-            else launch_dft_impl<T, NP, WPB, false, true>(L, D, x, y, active, s, active_hint, x_norm, x_copy, npw);
-            return;
-        }
-    } else {
-        if (L.nm) throw Error(3, "DFT coupling: node-major vectors are FP32 only");
+    if (L.nm) {
+        if (sq) launch_dft_impl<T, NP, WPB, true, true>(L, D, x, y, active, s, active_hint, x_norm, x_copy, npw);
+        else launch_dft_impl<T, NP, WPB, false, true>(L, D, x, y, active, s, active_hint, x_norm, x_copy, npw);
+        return;
     }
     if (sq) launch_dft_impl<T, NP, WPB, true, false>(L, D, x, y, active, s, active_hint, x_norm, x_copy, npw);
     else launch_dft_impl<T, NP, WPB, false, false>(L, D, x, y, active, s, active_hint, x_norm, x_copy, npw);

@@ -563,18 +563,23 @@ static void launch_diag_lds_any(const Layout &L, const EigTables<float> &E, cons
 // Unpreconditioned residual of the scattering system for the mixed solver's refinement:
 //   rho_j = s_j - D_j x_j - (V x)_j,   D_j x_j = (H0 + e_j) x_j - c_j x_j(N) e_N,
 // s the scattering source (b at node N of each instance's specular channel), V x given (the
-// FP64 coupling), written in FP32 in the vector order of Lo (the inner solver's). H0 x_j is
-// one FP64 product on the matrix cores (D^-1 takes two): every wavefront owns 16 channels
-// (real and imaginary parts as two column tiles) and computes Y^T = H0 X^T as the first
-// product of diag_gemm_kernel does with Q^T, here without the row permutation, so register v
+// FP64 coupling), x and V x in the vector order of L, written in FP32 in the vector order of
+// Lo (the inner solver's). H0 x_j is one FP64 product on the matrix cores (D^-1 takes two):
+// 16 channels (real and imaginary parts as two column tiles) share RT row tiles of Y^T = H0 X^T
+// per wavefront, so NT / RT wavefronts cover a channel tile (for NT = 10, two of five tiles:
+// 80 accumulator registers instead of 160, and two or three wavefronts per SIMD hide the
+// latency of the H0 fragments, read from L2, and of the epilogue). The product runs as the
+// first product of diag_gemm_kernel does with Q^T, without the row permutation, so register v
 // of lane group g of tile mt holds row 16 mt + 4 v + g. Every term is FP64 until the sum.
-template <int NT>
-__global__ __launch_bounds__(64 * DG_WAVES, 1) void split_residual_kernel(
+template <int NT, int RT>
+__global__ __launch_bounds__(64 * DG_WAVES, 2) void split_residual_kernel(
     Layout L, Layout Lo, const double *__restrict__ H0p, int nzp, const double *__restrict__ e,
     const cx<double> *__restrict__ c, const int *__restrict__ spec,
     const cx<double> *__restrict__ bsrc, const cx<double> *__restrict__ x,
     const cx<double> *__restrict__ vx, cx<float> *__restrict__ rho,
     const int *__restrict__ active) {
+    static_assert(NT % RT == 0, "row tiles split evenly over the wavefronts of a channel tile");
+    constexpr int NHALF = NT / RT;  // wavefronts per channel tile
     using M = Mfma<double>;
     using acc_t = typename M::acc;
     constexpr int CC = 2, NH = 2;  // k-steps per 16-byte stage, stages per 16-wide K block
@@ -582,25 +587,32 @@ __global__ __launch_bounds__(64 * DG_WAVES, 1) void split_residual_kernel(
     if (active && !active[b]) return;
     const int nc = L.nc[b];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int j0 = (blockIdx.x * DG_WAVES + wave) * 16;
+    const int wid = blockIdx.x * DG_WAVES + wave;
+    const int j0 = (wid / NHALF) * 16;
+    const int mt0 = (wid % NHALF) * RT;  // first row tile of this wavefront
     if (j0 >= nc) return;
     const int nz = L.nz, NTL = nzp >> 4;
+    if (mt0 >= NTL) return;  // every row tile of this wavefront lies past nzp
     const int l16 = lane & 15, g = lane >> 4;
     const int jc = j0 + l16;
     const bool live = jc < nc;
     const long long off = L.off[b];
     const long long cidx = off + (live ? jc : 0);
-    const long long row0 = cidx * nz;
+    // element (this lane's channel, z) of x and V x in the vector order of L: node-major puts
+    // the 16 channels of a wavefront at one node in one 256-byte run
+    const long long vb = L.nm ? off * nz + (live ? jc : 0) : cidx * nz;
+    const long long vstep = L.nm ? nc : 1;
+    auto vi = [&](int z) { return vb + z * vstep; };
     auto tile = [&](int t) { return t < NTL ? t : NTL - 1; };
 
-    acc_t acc[NT][2];
+    acc_t acc[RT][2];
 #pragma unroll
-    for (int mt = 0; mt < NT; ++mt) acc[mt][0] = acc[mt][1] = acc_t{0, 0, 0, 0};
-    double a1[2][NT][CC], x1[2][2][CC];
+    for (int mt = 0; mt < RT; ++mt) acc[mt][0] = acc[mt][1] = acc_t{0, 0, 0, 0};
+    double a1[2][RT][CC], x1[2][2][CC];
     // stage st: K block st / NH, k-steps (st % NH) * CC .. + CC. H0p is zero past column nz,
     // so X values loaded (at clamped addresses) for z >= nz add nothing; blocks past nzp / 16
     // repeat the last block's H0 columns, and their X is zeroed.
-    auto load1 = [&](int st, double (&a)[NT][CC], double (&xb)[2][CC]) {
+    auto load1 = [&](int st, double (&a)[RT][CC], double (&xb)[2][CC]) {
         const int s = st / NH, h = st - s * NH;
         const int zb = 16 * s + 4 * g + h * CC;
         const int zq = 16 * tile(s) + 4 * g + h * CC;
@@ -608,12 +620,13 @@ __global__ __launch_bounds__(64 * DG_WAVES, 1) void split_residual_kernel(
         for (int cc = 0; cc < CC; ++cc) {
             const int z = zb + cc;
             const bool ok = s < NTL;
-            const cx<double> v = x[row0 + (z < nz ? z : 0)];
+            const cx<double> v = x[vi(z < nz ? z : 0)];
             xb[0][cc] = ok ? v.re : 0.0;
             xb[1][cc] = ok ? v.im : 0.0;
         }
 #pragma unroll
-        for (int mt = 0; mt < NT; ++mt) load16<CC>(H0p + (long long)(16 * tile(mt) + l16) * nzp + zq, a[mt]);
+        for (int mt = 0; mt < RT; ++mt)
+            load16<CC>(H0p + (long long)(16 * tile(mt0 + mt) + l16) * nzp + zq, a[mt]);
     };
     load1(0, a1[0], x1[0]);
 #pragma unroll
@@ -624,7 +637,7 @@ __global__ __launch_bounds__(64 * DG_WAVES, 1) void split_residual_kernel(
 #pragma unroll
         for (int cc = 0; cc < CC; ++cc)
 #pragma unroll
-            for (int mt = 0; mt < NT; ++mt)
+            for (int mt = 0; mt < RT; ++mt)
 #pragma unroll
                 for (int r = 0; r < 2; ++r) acc[mt][r] = M::op(a1[cur][mt][cc], x1[cur][r][cc], acc[mt][r]);
         __builtin_amdgcn_sched_barrier(0);
@@ -632,7 +645,7 @@ __global__ __launch_bounds__(64 * DG_WAVES, 1) void split_residual_kernel(
 
     if (!live) return;
     const double ej = e[cidx];
-    const cx<double> cj = c[cidx], xN = x[row0 + nz - 1];
+    const cx<double> cj = c[cidx], xN = x[vi(nz - 1)];
     const bool is_spec = jc == spec[b];
     const cx<double> bs = bsrc[b];
     // per row tile: its x and (V x) values loaded together at clamped addresses (the next
@@ -644,25 +657,26 @@ __global__ __launch_bounds__(64 * DG_WAVES, 1) void split_residual_kernel(
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int z = 16 * mt + M::row(g, v);
-            const long long i = row0 + (mt < NTL && z < nz ? z : 0);
+            const long long i = vi(mt < NTL && z < nz ? z : 0);
             xz[v] = x[i];
             vz[v] = vx[i];
         }
     };
-    load_t(0);
+    load_t(mt0);
 #pragma unroll
-    for (int mt = 0; mt < NT; ++mt) {
+    for (int t = 0; t < RT; ++t) {
+        const int mt = mt0 + t;
         cx<float> out[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int z = 16 * mt + M::row(g, v);
-            cx<double> dx = mk(acc[mt][0][v] + ej * xz[v].re, acc[mt][1][v] + ej * xz[v].im);
+            cx<double> dx = mk(acc[t][0][v] + ej * xz[v].re, acc[t][1][v] + ej * xz[v].im);
             if (z == nz - 1) dx = dx - cj * xN;
             cx<double> r = mk(0.0, 0.0) - dx - vz[v];
             if (is_spec && z == nz - 1) r = r + bs;
             out[v] = mk((float)r.re, (float)r.im);
         }
-        if (mt + 1 < NT) load_t(mt + 1);
+        if (t + 1 < RT) load_t(mt + 1);
         if (mt < NTL && 16 * mt + 16 <= nz) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) rho[vec_index(Lo, off, nc, jc, 16 * mt + M::row(g, v))] = out[v];
@@ -684,13 +698,17 @@ void launch_split_residual(const Layout &L, const Layout &Lo, const double *H0p,
                            cx<float> *rho, const int *active, hipStream_t s) {
     if (!split_residual_supported(nzp) || nzp < L.nz)
         throw Error(3, "split residual: nz <= 160 required");
-    dim3 grid((L.max_nc + 16 * DG_WAVES - 1) / (16 * DG_WAVES), L.n_inst);
+    // wavefronts: (channel tiles) x (NT / RT row groups), DG_WAVES per workgroup
+    auto grid = [&](int halves) {
+        const long long waves = (long long)((L.max_nc + 15) / 16) * halves;
+        return dim3((unsigned)((waves + DG_WAVES - 1) / DG_WAVES), L.n_inst);
+    };
     if (nzp <= 64)
-        hipLaunchKernelGGL((split_residual_kernel<4>), grid, dim3(64 * DG_WAVES), 0, s, L, Lo, H0p,
-                           nzp, e, c, spec, bsrc, x, vx, rho, active);
+        hipLaunchKernelGGL((split_residual_kernel<4, 4>), grid(1), dim3(64 * DG_WAVES), 0, s, L,
+                           Lo, H0p, nzp, e, c, spec, bsrc, x, vx, rho, active);
     else
-        hipLaunchKernelGGL((split_residual_kernel<10>), grid, dim3(64 * DG_WAVES), 0, s, L, Lo, H0p,
-                           nzp, e, c, spec, bsrc, x, vx, rho, active);
+        hipLaunchKernelGGL((split_residual_kernel<10, 5>), grid(2), dim3(64 * DG_WAVES), 0, s, L,
+                           Lo, H0p, nzp, e, c, spec, bsrc, x, vx, rho, active);
     MSX_LAUNCH_CHECK();
 }
 

@@ -249,9 +249,14 @@ void launch_flip_table(int nkx, int nky, int nz, const cx<double> *V, cx<double>
 void launch_set_source(const Layout &L, const int *spec, const cx<double> *bsrc, cx<double> *s,
                        hipStream_t st);
 // d = D^-1 s for that source without touching the other channels' blocks (Sherman-Morrison
-// with the preconditioner factors; d is zero outside the specular channels).
+// with the channel-major preconditioner factors; d is zero outside the specular channels), in
+// the vector order of L; d32 (optional) receives the same vector in FP32.
 void launch_specular_dinv(const Layout &L, const int *spec, const cx<double> *bsrc,
-                          const cx<double> *c, const double *fac, cx<double> *d, hipStream_t st);
+                          const cx<double> *c, const double *fac, cx<double> *d, hipStream_t st,
+                          cx<float> *d32 = nullptr);
+// FP64 state from channel-major to node-major order (to_nm) or back, src != dst.
+void launch_state_order(const Layout &L, const cx<double> *src, cx<double> *dst, bool to_nm,
+                        hipStream_t s);
 
 // Full coupling of a state nonzero only on the specular channels (y_j = V[G_j - G_s] x_s).
 template <typename T>
@@ -286,7 +291,7 @@ void launch_to_float_vec(const Layout &L, long long n, const cx<double> *src, cx
 void launch_to_float_real_nm(const Layout &L, const double *src, float *dst, hipStream_t s);
 void launch_to_float_real(long long n, const double *src, float *dst, hipStream_t s);
 
-// P_j = |2i b_j psi_j(N) + a0 delta_spec|^2
+// P_j = |2i b_j psi_j(N) + a0 delta_spec|^2 (psi in the vector order of L)
 void launch_intensities(const Layout &L, const cx<double> *psi, const cx<double> *b,
                         const cx<double> *a0, const int *spec, double *P, hipStream_t s);
 

@@ -652,6 +652,55 @@ void launch_to_float_real_nm(const Layout &L, const double *src, float *dst, hip
     launch_nm(L, src, dst, s);
 }
 
+// FP64 state between the two vector orders of each instance block, one 32-channel x 32-node
+// tile per workgroup through LDS: reads and writes both run along the contiguous index.
+template <bool TO_NM>
+__global__ __launch_bounds__(256) void state_order_kernel(Layout L, const cx<double> *__restrict__ src,
+                                                          cx<double> *__restrict__ dst) {
+    __shared__ cx<double> tile[TT][TT + 1];  // [j - j0][z - z0]
+    const int b = blockIdx.y;
+    const int nc = L.nc[b], nz = L.nz;
+    const int tz = (nz + TT - 1) / TT;
+    const int j0 = (blockIdx.x / tz) * TT, z0 = (blockIdx.x % tz) * TT;
+    if (j0 >= nc) return;
+    const long long cm = L.off[b] * nz, nmb = L.off[b] * nz;  // block start in both orders
+    const int tx = threadIdx.x & (TT - 1), ty = threadIdx.x / TT;
+    if constexpr (TO_NM) {
+        for (int jj = ty; jj < TT; jj += 256 / TT) {
+            const int j = j0 + jj, z = z0 + tx;
+            if (j < nc && z < nz) tile[jj][tx] = src[cm + (long long)j * nz + z];
+        }
+        __syncthreads();
+        for (int zz = ty; zz < TT; zz += 256 / TT) {
+            const int z = z0 + zz, j = j0 + tx;
+            if (j < nc && z < nz) dst[nmb + (long long)z * nc + j] = tile[tx][zz];
+        }
+    } else {
+        for (int zz = ty; zz < TT; zz += 256 / TT) {
+            const int z = z0 + zz, j = j0 + tx;
+            if (j < nc && z < nz) tile[tx][zz] = src[nmb + (long long)z * nc + j];
+        }
+        __syncthreads();
+        for (int jj = ty; jj < TT; jj += 256 / TT) {
+            const int j = j0 + jj, z = z0 + tx;
+            if (j < nc && z < nz) dst[cm + (long long)j * nz + z] = tile[jj][tx];
+        }
+    }
+}
+
+void launch_state_order(const Layout &L, const cx<double> *src, cx<double> *dst, bool to_nm,
+                        hipStream_t s) {
+    if (src == dst) throw Error(3, "state order change: source and destination must differ");
+    const unsigned tiles = (unsigned)(((L.max_nc + TT - 1) / TT) * ((L.nz + TT - 1) / TT));
+    if (to_nm)
+        hipLaunchKernelGGL(state_order_kernel<true>, dim3(std::max(tiles, 1u), L.n_inst), dim3(256),
+                           0, s, L, src, dst);
+    else
+        hipLaunchKernelGGL(state_order_kernel<false>, dim3(std::max(tiles, 1u), L.n_inst), dim3(256),
+                           0, s, L, src, dst);
+    MSX_LAUNCH_CHECK();
+}
+
 __global__ void intensities_kernel(Layout L, const cx<double> *__restrict__ psi,
                                    const cx<double> *__restrict__ b,
                                    const cx<double> *__restrict__ a0,
@@ -662,7 +711,7 @@ __global__ void intensities_kernel(Layout L, const cx<double> *__restrict__ psi,
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
         const long long g = off + j;
         const cx<double> two_i = mk(0.0, 2.0);
-        cx<double> s = two_i * b[g] * psi[g * L.nz + L.nz - 1];
+        cx<double> s = two_i * b[g] * psi[vec_index(L, off, nc, j, L.nz - 1)];
         if (j == spec[inst]) s = s + a0[inst];
         P[g] = norm2(s);
     }
@@ -784,24 +833,38 @@ void launch_set_source(const Layout &L, const int *spec, const cx<double> *bsrc,
 __global__ void specular_dinv_kernel(Layout L, const int *__restrict__ spec,
                                      const cx<double> *__restrict__ bsrc,
                                      const cx<double> *__restrict__ c,
-                                     const double *__restrict__ fac, cx<double> *__restrict__ d) {
+                                     const double *__restrict__ fac, cx<double> *__restrict__ d,
+                                     cx<float> *__restrict__ d32) {
     const int b = blockIdx.y;
-    const long long nz = L.nz, beg = L.off[b] * nz, len = (long long)L.nc[b] * nz;
-    const long long js = L.off[b] + spec[b], sbeg = js * nz;
+    const int nc = L.nc[b];
+    const long long nz = L.nz, beg = L.off[b] * nz, len = (long long)nc * nz;
+    const int jsp = spec[b];
+    const long long js = L.off[b] + jsp, sbeg = js * nz;  // the factors are channel-major
     const cx<double> coef = cdiv(bsrc[b], mk(1.0, 0.0) - c[js] * fac[sbeg + nz - 1]);
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < len;
          i += (long long)gridDim.x * blockDim.x) {
-        const long long g = beg + i;
-        d[g] = (g >= sbeg && g < sbeg + nz) ? coef * fac[g] : mk(0.0, 0.0);
+        // (channel, node) of position i of the instance block in the vector order of L
+        long long j, z;
+        if (L.nm) {
+            z = i / nc;
+            j = i - z * nc;
+        } else {
+            j = i / nz;
+            z = i - j * nz;
+        }
+        const cx<double> v = j == jsp ? coef * fac[sbeg + z] : mk(0.0, 0.0);
+        d[beg + i] = v;
+        if (d32) d32[beg + i] = mk((float)v.re, (float)v.im);
     }
 }
 
 void launch_specular_dinv(const Layout &L, const int *spec, const cx<double> *bsrc,
-                          const cx<double> *c, const double *fac, cx<double> *d, hipStream_t st) {
+                          const cx<double> *c, const double *fac, cx<double> *d, hipStream_t st,
+                          cx<float> *d32) {
     const long long per = (long long)L.max_nc * L.nz;
     const unsigned bx = (unsigned)std::min<long long>((per + 255) / 256, 64);
     hipLaunchKernelGGL(specular_dinv_kernel, dim3(std::max(bx, 1u), L.n_inst), dim3(256), 0, st, L,
-                       spec, bsrc, c, fac, d);
+                       spec, bsrc, c, fac, d, d32);
     MSX_LAUNCH_CHECK();
 }
 

@@ -31,6 +31,14 @@ struct Sum {
     double ms = 0, flops = 0, bytes = 0;
 };
 std::map<std::string, Sum> g_sums;
+std::vector<std::string> g_select;  // empty: every kernel
+
+bool selected(const char *name) {
+    if (g_select.empty()) return true;
+    for (const auto &n : g_select)
+        if (n == name) return true;
+    return false;
+}
 
 hipEvent_t take_event() {
     if (!g_pool.empty()) {
@@ -79,6 +87,7 @@ bool profiling() { return g_on; }
 ProfScope::ProfScope(const char *name, hipStream_t s, Work flops, Work bytes) : s_(s) {
     if (!g_on) return;
     std::lock_guard<std::mutex> lk(g_mu);
+    if (!selected(name)) return;
     Rec r{name, take_event(), take_event(), std::move(flops), std::move(bytes), g_next_id++, false};
     if (r.a) (void)hipEventRecord(r.a, s);
     id_ = r.id;
@@ -125,6 +134,23 @@ int msx_profile_read(const char *kernel, long long *count, double *total_ms, dou
     if (total_ms) *total_ms = s.ms;
     if (flops) *flops = s.flops;
     if (bytes) *bytes = s.bytes;
+    return MSX_OK;
+}
+
+int msx_profile_select(const char *names) {
+    std::lock_guard<std::mutex> lk(msx::g_mu);
+    msx::g_select.clear();
+    if (!names) return MSX_OK;
+    std::string cur;
+    for (const char *p = names;; ++p) {
+        if (*p == ',' || *p == 0) {
+            if (!cur.empty()) msx::g_select.push_back(cur);
+            cur.clear();
+            if (*p == 0) break;
+        } else {
+            cur += *p;
+        }
+    }
     return MSX_OK;
 }
 

@@ -527,6 +527,9 @@ struct msx_batch {
     std::vector<int> iters, info;
     std::vector<double> resid;
     bool factors_ready = false, facf_nm_ready = false, solved = false;
+    // psi_d holds the node-major order (the mixed solver's refinement runs in that order and
+    // leaves the state so); psi_channel_major() restores the batch layout for consumers
+    bool psi_nm = false;
     struct KdimMemo {
         int restart = -1, maxiter = -1, dim = 0;
     } kdim_memo[2];  // Krylov dimension per precision (FP64, mixed) for the last options seen
@@ -558,6 +561,17 @@ struct msx_batch {
     static void need(DBuf<T> &b, size_t n) {
         if (b.n < n) b.alloc(n);
     }
+    // the state in the given order (through tmp2, which holds nothing between calls)
+    void psi_order(bool nm) {
+        if (psi_nm == nm) return;
+        const size_t n = (size_t)n_elem();
+        need(tmp2, n);
+        launch_state_order(layout(), psi_d.p, tmp2.p, nm, t->stream);
+        MSX_HIP(hipMemcpyAsync(psi_d.p, tmp2.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice,
+                               t->stream));
+        psi_nm = nm;
+    }
+    void psi_channel_major() { psi_order(false); }
     void build_factors() {
         ProfScope ps("factors_f64", t->stream, 2.0 * total_ch * t->nz * t->nz,
                      (double)total_ch * t->nz * 8.0);
@@ -580,6 +594,7 @@ struct Ops {
     msx_table &t;
     Layout L;
     Layout L32;  // order of the FP32 vectors (set_fp32_order)
+    Layout L64;  // order of the FP64 vectors the full coupling takes (L unless node-major)
     bool neumann;
     int order = 1;  // Jacobi: degree of the Neumann factor M = sum_{i<=order} (-D^-1 V)^i
     const cx<double> *V;  // coupling table (flipped for the transposed system)
@@ -603,6 +618,7 @@ struct Ops {
         }
         direct = force_direct || dft->np == 0;
         L32 = L;
+        L64 = L;
     }
     // FP32 vectors node-major when every FP32 kernel of this configuration reads and writes
     // that order: the DFT coupling of tables up to 32 x 32 and the diagonal blocks with Q^T in
@@ -674,8 +690,8 @@ struct Ops {
         Work fl, by;
         if (profiling()) couple_full_work(act, sizeof(cx<double>), !direct, fl, by, in_copy != nullptr);
         ProfScope ps(direct ? "coupling_direct_f64" : "coupling_dft_f64", s, fl, by);
-        if (direct) launch_coupling<double>(L, V, in, out, COUPLE_FULL, act, s);
-        else launch_dft_coupling<double>(L, dft->d64(), in, out, act, s, active_hint(act), in_norm,
+        if (direct) launch_coupling<double>(L64, V, in, out, COUPLE_FULL, act, s);
+        else launch_dft_coupling<double>(L64, dft->d64(), in, out, act, s, active_hint(act), in_norm,
                                          in_copy, B.tuning.dft_npw64);
     }
     void couple_full(const cx<float> *in, cx<float> *out, const int *act,
@@ -903,9 +919,9 @@ bool g_tune_file_read = false;
 std::string tune_key(const msx_batch &B, const Ops &ops, const msx_solve_options &o) {
     char buf[320];
     std::snprintf(buf, sizeof buf,
-                  "nkx=%d,nky=%d,nz=%d,inst=%d,channels=%lld,max_nc=%d,nm=%d,direct=%d,vdiag=%d,prec=%d",
+                  "nkx=%d,nky=%d,nz=%d,inst=%d,channels=%lld,max_nc=%d,nm=%d%d,direct=%d,vdiag=%d,prec=%d",
                   B.t->nkx, B.t->nky, B.t->nz, B.n_inst, B.total_ch, B.max_nc, ops.L32.nm,
-                  ops.direct ? 1 : 0, ops.vector_diag ? 1 : 0, o.precision);
+                  ops.L64.nm, ops.direct ? 1 : 0, ops.vector_diag ? 1 : 0, o.precision);
     return buf;
 }
 
@@ -1054,7 +1070,7 @@ void autotune(msx_batch &B, Ops &ops, const msx_solve_options &o) {
         rep += "\"dft_us_f64\": {";
         for (int i = 0; i < 4; ++i) {
             ts.push_back(tm.best_us([&] {
-                launch_dft_coupling<double>(ops.L, ops.dft->d64(), B.tmp1.p, B.tmp2.p, nullptr, s,
+                launch_dft_coupling<double>(ops.L64, ops.dft->d64(), B.tmp1.p, B.tmp2.p, nullptr, s,
                                             -1, nullptr, nullptr, npw_c[i]);
             }));
             rep += (i ? ", \"" : "\"") + std::to_string(npw_c[i]) + "\": " + std::to_string(ts[i]);
@@ -1128,14 +1144,32 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     ops.order = std::max(1, o.neumann);
     ops.vector_diag = o.diag == 1;  // auto: matrix cores when nz <= 320
     ops.set_fp32_order();
+    // The mixed solver's refinement residual as D^-1 (s - D x - V x) with D^-1 in FP32
+    // (split_residual) instead of d0 - x - D^-1 V x in FP64: for the scattering source and
+    // tolerances down to 1e-9, where the FP64 cancellation in s - D x - V x (about 1e-16 |D| |x|)
+    // stays far below the residuals the test has to resolve
+    const bool mixed = o.precision == MSX_PRECISION_MIXED;
+    const bool split = mixed && scatter_source && !transposed && o.eps >= 1e-9 &&
+                       split_residual_supported(t.nzp) && t.nzp >= t.nz;
+    // With the split residual and node-major FP32 vectors, the FP64 refinement vectors (the
+    // state, D^-1 s, the coupling of the state) are node-major too: the FP64 coupling, the
+    // split residual and the solution update then read and write whole lines. The batch's
+    // state is left in that order (psi_nm).
+    const bool nm64 = split && ops.L32.nm && x == B.psi_d.p;
+    if (nm64) ops.L64.nm = 1;
     if (o.autotune == 1 && o.method == MSX_METHOD_JACOBI &&
         tune_key(B, ops, o) != B.tuning.key)
         autotune(B, ops, o);
     const cx<double> *src = src_in;
     if (!src && !(scatter_source && o.method == MSX_METHOD_JACOBI))
         throw Error(MSX_ERR_INVALID, "solve: no source vector");
-    // A warm start (forward solves only) keeps x as the initial guess.
+    // A warm start (forward solves only) keeps x as the initial guess, in the order this path
+    // runs in.
     const bool warm = o.warm_start == 1 && !transposed;
+    if (x == B.psi_d.p) {
+        if (warm) B.psi_order(nm64);
+        else B.psi_nm = nm64;
+    }
     if (!warm) MSX_HIP(hipMemsetAsync(x, 0, sizeof(cx<double>) * n, s));
     // sized once per batch and option pair (free memory shrinks once the basis exists, and a
     // dimension that moved with it would reallocate the workspace on every solve)
@@ -1201,7 +1235,13 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     msx_batch::need(B.f_tmp1, n);
     msx_batch::need(B.f_tmp2, n);
     cx<double> *d0 = B.rhs.p;  // D^-1 s
-    ops.dinv_source(src, d0, scatter_source);
+    if (nm64) {
+        // node-major, with its FP32 copy (the inner right-hand side of a cold first sweep)
+        ProfScope ps("specular_dinv_f64", s, 0.0, (double)B.n_elem() * (sizeof(cx<double>) + sizeof(cx<float>)));
+        launch_specular_dinv(ops.L64, B.spec_d.p, B.bsrc_d.p, B.c_d.p, B.fac_d.p, d0, s, B.f_tmp2.p);
+    } else {
+        ops.dinv_source(src, d0, scatter_source);
+    }
     if (!B.kr_outer || B.kr_outer->K.chunk != B.tuning.chunk64)
         B.kr_outer.reset(new Krylov<double>(B.kstart(), B.klen(), 1, s, B.tuning.chunk64));
     Krylov<double> &outer = *B.kr_outer;
@@ -1230,12 +1270,6 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         o.inner_eps > 0 ? o.inner_eps : std::min(2.5e-4, std::max(1e-5, std::sqrt(o.eps) / 3.0));
     const int max_refine = o.max_refine > 0 ? o.max_refine : 6;
     cx<double> *r64 = B.tmp3.p;
-    // The refinement residual as D^-1 (s - D x - V x) with D^-1 in FP32 (split_residual)
-    // instead of d0 - x - D^-1 V x in FP64: for the scattering source and tolerances down to
-    // 1e-9, where the FP64 cancellation in s - D x - V x (about 1e-16 |D| |x|) stays far below
-    // the residuals the test has to resolve
-    const bool split = scatter_source && !transposed && o.eps >= 1e-9 &&
-                       split_residual_supported(t.nzp) && t.nzp >= t.nz;
     // per-instance inner tolerances (host copy; uploaded before each inner solve)
     std::vector<double> eps_in(B.n_inst, inner_eps);
     msx_batch::need(B.eps_in_d, (size_t)B.n_inst);
@@ -1248,6 +1282,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         std::vector<double> r_n2;
         if (sweep == 0 && !warm) {
             r_n2 = rhs_n2;
+            r_in_fp32 = nm64;
         } else if (split) {
             // r = D^-1 rho, rho = s - D x - V x in FP64 (one FP64 coupling, one FP64 product
             // with H0), D^-1 rho in FP32: rho is small, so FP32 holds r to ~1e-6 relative
@@ -1261,7 +1296,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
                     by = ops.over_active(outer.active.p, [&](int nc) { return nc * nzd * 40.0; });
                 }
                 ProfScope ps("residual_split_f64", s, fl, by);
-                launch_split_residual(ops.L, ops.L32, t.H0p.p, t.nzp, B.e_d.p, B.c_d.p, B.spec_d.p,
+                launch_split_residual(ops.L64, ops.L32, t.H0p.p, t.nzp, B.e_d.p, B.c_d.p, B.spec_d.p,
                                       B.bsrc_d.p, x, B.tmp1.p, B.f_tmp1.p, outer.active.p, s);
             }
             ops.dinv<float>(B.f_tmp1.p, nullptr, B.f_tmp2.p, 1.0f, outer.active.p);
@@ -1297,7 +1332,10 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         ops.snap_src = &outer.snap; ops.count_src = &outer.last_count;
         // inner right-hand side M r in FP32 (M = I - D^-1 V to first degree; on a cold start the
         // first r = D^-1 s lives on the specular channels only, and V r is a single-source product)
-        if (!r_in_fp32) launch_to_float_vec(ops.L32, (long long)n, res64, B.f_tmp2.p, s);
+        if (!r_in_fp32) {
+            if (nm64) launch_to_float((long long)n, res64, B.f_tmp2.p, s);  // same order
+            else launch_to_float_vec(ops.L32, (long long)n, res64, B.f_tmp2.p, s);
+        }
         if (!ops.neumann) {
             // no Neumann factor: nothing to couple
         } else if (sweep == 0 && !warm && scatter_source && !transposed) {
@@ -1318,7 +1356,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         const std::function<int()> shape32 = [&] { return ops.launch_shape_key(); };
         gmres<float, double>(inner, op32, B.f_rhs.p, x, true, inner_eps, 1,
                              std::min(o.maxiter, inner_dim), false, o.inner_orth == 1 ? 1 : 2,
-                             &ops.L32, B.eps_in_d.p, ops.direct ? nullptr : &op32s,
+                             nm64 ? nullptr : &ops.L32, B.eps_in_d.p, ops.direct ? nullptr : &op32s,
                              o.graphs == 1 ? &sig32 : nullptr, &shape32);
         launch_add_ints(B.n_inst, inner.iters.p, B.iters_acc.p, s);
     }
@@ -1659,7 +1697,9 @@ int msx_batch_solve(msx_batch *B, const msx_solve_options *opt) {
             src = B->source.p;
         }
         solve_system(*B, *opt, src, B->psi_d.p, false, B->iters, B->info, B->resid, true);
-        launch_intensities(L, B->psi_d.p, B->b_d.p, B->a0_d.p, B->spec_d.p, B->P_d.p, t.stream);
+        Layout Lpsi = L;
+        Lpsi.nm = B->psi_nm ? 1 : 0;
+        launch_intensities(Lpsi, B->psi_d.p, B->b_d.p, B->a0_d.p, B->spec_d.p, B->P_d.p, t.stream);
         MSX_HIP(hipStreamSynchronize(t.stream));
         B->solved = true;
         int missed = 0;
@@ -1680,6 +1720,7 @@ int msx_batch_set_state(msx_batch *B, const double *psi) {
     try {
         if (!B || !psi) throw Error(MSX_ERR_INVALID, "null argument");
         B->psi_d.upload(reinterpret_cast<const cx<double> *>(psi), (size_t)B->n_elem(), B->t->stream);
+        B->psi_nm = false;
         MSX_HIP(hipStreamSynchronize(B->t->stream));
         return MSX_OK;
     } catch (const Error &e) {
@@ -1690,6 +1731,8 @@ int msx_batch_set_state(msx_batch *B, const double *psi) {
 int msx_batch_state(const msx_batch *B, double *psi) {
     try {
         if (!B || !psi) throw Error(MSX_ERR_INVALID, "null argument");
+        // the order the state is kept in is internal: callers always see the batch layout
+        const_cast<msx_batch *>(B)->psi_channel_major();
         B->psi_d.download(reinterpret_cast<cx<double> *>(psi), (size_t)B->n_elem(), B->t->stream);
         MSX_HIP(hipStreamSynchronize(B->t->stream));
         return MSX_OK;
@@ -1772,6 +1815,12 @@ int msx_batch_device_state(const msx_batch *B, void **psi_device) {
         set_last_error("null argument");
         return MSX_ERR_INVALID;
     }
+    try {
+        const_cast<msx_batch *>(B)->psi_channel_major();
+        MSX_HIP(hipStreamSynchronize(B->t->stream));
+    } catch (const Error &e) {
+        return fail(e);
+    }
     *psi_device = B->psi_d.p;
     return MSX_OK;
 }
@@ -1795,6 +1844,7 @@ int msx_batch_adjoint(msx_batch *B, const msx_solve_options *opt, const double *
         hipStream_t s = t.stream;
         const Layout L = B->layout();
         const size_t n = (size_t)B->n_elem();
+        B->psi_channel_major();
         msx_batch::need(B->dLdP_d, (size_t)B->total_ch);
         B->dLdP_d.upload(dL_dP, (size_t)B->total_ch, s);
         // buffers owned by the batch, reused by later adjoints
