@@ -909,7 +909,9 @@ void check_options(const msx_solve_options *o) {
 // for the mixed solver, FP64), the Arnoldi vector kernels' block length (the precision the
 // Krylov basis runs in) and the FP64 diagonal blocks' channels per wavefront. The node and
 // channel groupings do not change the arithmetic; the block length changes only the order in
-// which the dot products' partial sums add up. Choices are cached per batch shape (table,
+// which the dot products' partial sums add up. Every shape is timed as a solve launches it with
+// all instances iterating (the coupling's fall-back to one node per wavefront when a grouping
+// leaves the device short of wavefronts included). Choices are cached per batch shape (table,
 // nz, instances, channel counts, vector order) for the process, and in the text file named by
 // MSX_TUNE_FILE when that is set (one line per shape: key and the five values).
 std::mutex g_tune_mu;
@@ -1071,7 +1073,7 @@ void autotune(msx_batch &B, Ops &ops, const msx_solve_options &o) {
         for (int i = 0; i < 4; ++i) {
             ts.push_back(tm.best_us([&] {
                 launch_dft_coupling<double>(ops.L64, ops.dft->d64(), B.tmp1.p, B.tmp2.p, nullptr, s,
-                                            -1, nullptr, nullptr, npw_c[i]);
+                                            B.n_inst, nullptr, nullptr, npw_c[i]);
             }));
             rep += (i ? ", \"" : "\"") + std::to_string(npw_c[i]) + "\": " + std::to_string(ts[i]);
         }
@@ -1085,7 +1087,7 @@ void autotune(msx_batch &B, Ops &ops, const msx_solve_options &o) {
             for (int i = 0; i < 4; ++i) {
                 ts.push_back(tm.best_us([&] {
                     launch_dft_coupling<float>(ops.L32, ops.dft->d32(), B.f_tmp1.p, B.f_tmp2.p,
-                                               nullptr, s, -1, nullptr, nullptr, npw_c[i]);
+                                               nullptr, s, B.n_inst, nullptr, nullptr, npw_c[i]);
                 }));
                 rep += (i ? ", \"" : "\"") + std::to_string(npw_c[i]) + "\": " + std::to_string(ts[i]);
             }
