@@ -98,6 +98,10 @@ def main() -> int:
                     help="launch every kernel one by one instead of replaying captured iterations")
     ap.add_argument("--no-autotune", action="store_true", help="built-in launch shapes")
     ap.add_argument("--grad-steps", type=int, default=3)
+    ap.add_argument("--neumann", type=int, default=1,
+                    help="degree of the Neumann factor of the timed mixed solve")
+    ap.add_argument("--inner-eps", type=float, default=0.0,
+                    help="inner FP32 tolerance of the timed mixed solve (0: the solver's rule)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 20 plain device copies of MIB MiB (calibrates the HBM byte "
                          "counters of a profiler run against a known size) and exit")
@@ -123,7 +127,8 @@ def main() -> int:
     eps = 5e-7        # mixed: |dP| ~ 3e-8 against the tight solve, 30x inside the bar
     eps_base = 5e-6   # the reference's default precision 1e-5 (nsig 5) for the FP64 baselines
     mixed = BT.SolveOptions(method="jacobi", precision="mixed", eps=eps, diag=args.diag,
-                            graphs=not args.no_graphs, autotune=not args.no_autotune)
+                            graphs=not args.no_graphs, autotune=not args.no_autotune,
+                            neumann=args.neumann, inner_eps=args.inner_eps)
 
     def step(opts):
         tab.prepare()
