@@ -237,13 +237,15 @@ struct Krylov {
 // time that (iteration, operator form, flag slot, launch shape) comes up; graph_sig names the
 // operator's buffers and options, and a different one drops the captured graphs. shape_key
 // gives the host-side launch choices that depend on the active count (part of the key).
+// stop_below: once the host sees at most this many instances iterating, the solve stops after
+// the iteration already queued and every instance keeps the solution of its current subspace.
 // Profiling needs the eager launches (its events bracket single kernels), so it turns this off.
 template <typename T, typename TX>
 void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_zero,
            double eps, int window, int maxiter, bool allow_restart, int orth_passes = 2,
            const Layout *basis_order = nullptr, const double *eps_inst = nullptr,
            const OpScaled<T> *op_scaled = nullptr, const std::string *graph_sig = nullptr,
-           const std::function<int()> *shape_key = nullptr) {
+           const std::function<int()> *shape_key = nullptr, int stop_below = 0) {
     hipStream_t s = kr.s;
     const bool use_graphs = graph_sig != nullptr && !profiling();
     if (use_graphs) {
@@ -388,6 +390,10 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
                     --done;
                     alive = kr.wait(cur);
                     prev.slot = -1;
+                    break;
+                }
+                if (alive <= stop_below) {  // the few still iterating stop after iteration k
+                    prev = cur;
                     break;
                 }
             }
@@ -1356,10 +1362,16 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         const std::string sig32 = ops.graph_signature(
             o.inner_orth == 1 ? "mixed32-cgs1" : "mixed32", B.f_tmp1.p, B.f_tmp2.p, B.eps_in_d.p);
         const std::function<int()> shape32 = [&] { return ops.launch_shape_key(); };
+        // The first sweep's inner solve ends for everyone once at most a tenth of a large batch
+        // still iterates: its last iterations carry a few conditions at latency-bound cost, and
+        // the next sweep, which every condition takes after a cold first sweep, continues them
+        // from their current solution (21.6 -> 21.3 ms a step on the benchmark batch, |dP|
+        // 2.6e-8 -> 8e-8 against the tight solve)
+        const int stop_below = sweep == 0 && n_act >= 32 ? n_act / 10 : 0;
         gmres<float, double>(inner, op32, B.f_rhs.p, x, true, inner_eps, 1,
                              std::min(o.maxiter, inner_dim), false, o.inner_orth == 1 ? 1 : 2,
                              nm64 ? nullptr : &ops.L32, B.eps_in_d.p, ops.direct ? nullptr : &op32s,
-                             o.graphs == 1 ? &sig32 : nullptr, &shape32);
+                             o.graphs == 1 ? &sig32 : nullptr, &shape32, stop_below);
         launch_add_ints(B.n_inst, inner.iters.p, B.iters_acc.p, s);
     }
     {
