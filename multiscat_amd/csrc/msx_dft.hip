@@ -13,17 +13,18 @@
 // Each wavefront transforms one node at a time in its own LDS slab with F_x and F_y held in
 // registers, so every MFMA reads one operand from LDS; the grid runs over instances fastest so
 // that workgroups resident together share the Vhat slabs of the same nodes in L2. For a square
-// table F_y = F_x elementwise in the register layout (the DFT matrix is symmetric), so one copy
-// serves both, and the kernel then forms each complex product from three real ones
-// (T1 = A_r B_r, T2 = A_i B_i, T3 = (A_r + A_i)(B_r + B_i); re = T1 - T2, im = T3 - T1 - T2),
-// 384 MFMAs per node instead of 512 (relative difference to the four-product form: 5e-7 in
-// FP32, 9e-16 in FP64). With node-major vectors (Layout::nm) the channels of one
-// node are contiguous, so the loads of X(z) and the stores of Y(z) are whole cache lines.
-// LDS row stride 36 (= 36 mod 64 words) for both planes at NP = 32: the A-fragment reads
-// (16 rows x 4 columns) and the accumulator stores (rows 4g + v, 16 columns) hit 64 distinct
-// banks, the B-fragment reads (4 rows x 16 columns) two-way; a stride of 48 would make the
-// B reads conflict-free but the stores four-way, and the stores are as many (measured: 1.6%
-// faster at 36).
+// table F_y = F_x (the DFT matrix is symmetric), so one copy serves both, and the kernel then
+// forms each complex product from three real ones (T1 = A_r B_r, T2 = A_i B_i,
+// T3 = (A_r + A_i)(B_r + B_i); re = T1 - T2, im = T3 - T1 - T2), 384 MFMAs per node instead of
+// 512 (relative difference to the four-product form: 5e-7 in FP32, 9e-16 in FP64). With
+// node-major vectors (Layout::nm) the channels of one node are contiguous, so the loads of
+// X(z) and the stores of Y(z) are whole cache lines. The coupling kernel runs all four
+// transforms as right products with transposed stores (see the slab products below): every
+// LDS access of a transform is a 16-byte run (8 reads and 8 stores per lane and transform
+// instead of 32 and 32 single words; 2.5% faster). The gradient kernels keep the left/right
+// products with single-word accesses: LDS row stride 36 (= 36 mod 64 words) at NP = 32, so the
+// A-fragment reads (16 rows x 4 columns) and the accumulator stores (rows 4g + v, 16 columns)
+// hit 64 distinct banks and the B-fragment reads (4 rows x 16 columns) two.
 #include "msx_dft.h"
 #include "msx_mfma.h"
 
@@ -277,13 +278,146 @@ __device__ __forceinline__ void store_tiles(T *dr, T *di, int stride,
             }
 }
 
-// One workgroup = one instance (blockIdx.x, so consecutive workgroups share the Vhat slabs of
-// the same nodes in L2) and zpb nodes (blockIdx.y); each wavefront owns an LDS slab and
-// processes its nodes independently: scatter X(z) onto the grid, the four transforms with
-// F_x / F_y held in registers for the whole kernel, the pointwise product with Vhat(z) in
-// registers after the second, gather Y(z) back to the compact layout. No workgroup barrier.
-// The channel values of a wavefront's next node are loaded into registers while the current
-// node's transforms run; the channels' grid positions are computed once.
+// ---- the coupling kernel's slab products ---------------------------------------------------
+// Every transform is a right product C = A F~ (F~ = F or conj(F), in registers as the B
+// operand) with A read from the wavefront's LDS slab by rows, and its result is stored
+// transposed, slab[j][i] = C[i][j]: with F symmetric, F X = (X^T F)^T, so alternating the
+// roles of rows and columns turns the four transforms of Y = F_x X F_y into four such
+// products (the scatter writes X^T, the gather reads Y^T). The k index of MFMA step kk in lane
+// group k4 is k4 KK + kk (the products sum over k in any order), so a lane reads the KK
+// entries of its A row as 16-byte runs; the accumulator of lane (j, g) holds rows 4 g .. 4 g + 3
+// of column j (FP64 through the row permutation of the A operand), one 16-byte run (two in
+// FP64) of the transposed store. Row stride S: 4 (mod 64) words times an odd number for FP32,
+// so the 16 lanes of a group touch distinct 16-byte bank quads in reads and stores alike.
+template <typename T, int NP>
+struct DftSlab {
+    static constexpr int NT = NP / 16;  // 16 x 16 tiles per side
+    static constexpr int KK = NP / 4;   // k-steps per product
+    static constexpr int S = sizeof(T) == 4 ? (NP == 16 ? 20 : 36) : (NP == 16 ? 18 : 34);
+    static constexpr int PLANE = NP * S;
+};
+
+// x[0 .. KK) = p[0 .. KK), p 16-byte aligned
+template <int KK>
+__device__ __forceinline__ void slab_row(const float *p, float (&x)[KK]) {
+#pragma unroll
+    for (int q = 0; q < KK / 4; ++q) {
+        const float4 v = reinterpret_cast<const float4 *>(p)[q];
+        x[4 * q] = v.x; x[4 * q + 1] = v.y; x[4 * q + 2] = v.z; x[4 * q + 3] = v.w;
+    }
+}
+template <int KK>
+__device__ __forceinline__ void slab_row(const double *p, double (&x)[KK]) {
+#pragma unroll
+    for (int q = 0; q < KK / 2; ++q) {
+        const double2 v = reinterpret_cast<const double2 *>(p)[q];
+        x[2 * q] = v.x; x[2 * q + 1] = v.y;
+    }
+}
+
+// C = A F (CONJ: A conj(F)), A from the slab planes (ar, ai), F in registers:
+// fr/fi[tj][kk] = F[k4 KK + kk][16 tj + l16]. M3: three real products (T1 = Ar Fr, T2 = Ai Fi,
+// T3 = (Ar +- Ai)(Fr + Fi)) combined by combine3; else four.
+template <typename T, int NP, bool CONJ, bool M3>
+__device__ __forceinline__ void slab_product(const T *ar, const T *ai,
+                                             const T (&fr)[DftSlab<T, NP>::NT][NP / 4],
+                                             const T (&fi)[DftSlab<T, NP>::NT][NP / 4],
+                                             typename Mfma<T>::acc (&cr)[DftSlab<T, NP>::NT][DftSlab<T, NP>::NT],
+                                             typename Mfma<T>::acc (&ci)[DftSlab<T, NP>::NT][DftSlab<T, NP>::NT],
+                                             typename Mfma<T>::acc (&c3)[DftSlab<T, NP>::NT][DftSlab<T, NP>::NT]) {
+    using D = DftSlab<T, NP>;
+    constexpr int NT = D::NT, KK = D::KK, S = D::S;
+    using acc_t = typename Mfma<T>::acc;
+    const int lane = threadIdx.x & 63;
+    const int l16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj) cr[ti][tj] = ci[ti][tj] = c3[ti][tj] = acc_t{0, 0, 0, 0};
+    T xr[NT][KK], xi[NT][KK];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int row = 16 * t + Mfma<T>::perm(l16);
+        slab_row<KK>(ar + row * S + k4 * KK, xr[t]);
+        slab_row<KK>(ai + row * S + k4 * KK, xi[t]);
+    }
+#pragma unroll
+    for (int kk = 0; kk < KK; ++kk) {
+        if constexpr (M3) {
+            T xs[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) xs[t] = CONJ ? xr[t][kk] - xi[t][kk] : xr[t][kk] + xi[t][kk];
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj) {
+                const T yr = fr[tj][kk], yi = fi[tj][kk], ys = yr + yi;
+#pragma unroll
+                for (int ti = 0; ti < NT; ++ti) {
+                    cr[ti][tj] = Mfma<T>::op(xr[ti][kk], yr, cr[ti][tj]);
+                    ci[ti][tj] = Mfma<T>::op(xi[ti][kk], yi, ci[ti][tj]);
+                    c3[ti][tj] = Mfma<T>::op(xs[ti], ys, c3[ti][tj]);
+                }
+            }
+        } else {
+            // x F:       cr += xr fr - xi fi,  ci += xr fi + xi fr
+            // x conj(F): cr += xr fr + xi fi,  ci += xi fr - xr fi
+            T xn[NT], xm[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                xn[t] = CONJ ? xi[t][kk] : -xi[t][kk];
+                xm[t] = CONJ ? -xr[t][kk] : xr[t][kk];
+            }
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj) {
+                const T yr = fr[tj][kk], yi = fi[tj][kk];
+#pragma unroll
+                for (int ti = 0; ti < NT; ++ti) {
+                    cr[ti][tj] = Mfma<T>::op(xr[ti][kk], yr, cr[ti][tj]);
+                    cr[ti][tj] = Mfma<T>::op(xn[ti], yi, cr[ti][tj]);
+                    ci[ti][tj] = Mfma<T>::op(xm[ti], yi, ci[ti][tj]);
+                    ci[ti][tj] = Mfma<T>::op(xi[ti][kk], yr, ci[ti][tj]);
+                }
+            }
+        }
+    }
+    if constexpr (M3) combine3<T, NP, CONJ>(cr, ci, c3);
+}
+
+// slab[j][i] = C[i][j]: lane (j, g) of tile (ti, tj) holds rows 16 ti + 4 g .. + 3 of column
+// 16 tj + j, contiguous in the transposed slab
+template <typename T, int NP>
+__device__ __forceinline__ void slab_store_t(T *dr, T *di,
+                                             const typename Mfma<T>::acc (&cr)[DftSlab<T, NP>::NT][DftSlab<T, NP>::NT],
+                                             const typename Mfma<T>::acc (&ci)[DftSlab<T, NP>::NT][DftSlab<T, NP>::NT]) {
+    using D = DftSlab<T, NP>;
+    constexpr int NT = D::NT, S = D::S;
+    using acc_t = typename Mfma<T>::acc;
+    const int lane = threadIdx.x & 63;
+    const int col = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj) {
+            const int a = (16 * tj + col) * S + 16 * ti + 4 * g;
+            if constexpr (sizeof(T) == 4) {
+                *reinterpret_cast<acc_t *>(dr + a) = cr[ti][tj];
+                *reinterpret_cast<acc_t *>(di + a) = ci[ti][tj];
+            } else {  // 16-byte halves (a row run starts on a 16-byte boundary only)
+                reinterpret_cast<double2 *>(dr + a)[0] = make_double2(cr[ti][tj][0], cr[ti][tj][1]);
+                reinterpret_cast<double2 *>(dr + a)[1] = make_double2(cr[ti][tj][2], cr[ti][tj][3]);
+                reinterpret_cast<double2 *>(di + a)[0] = make_double2(ci[ti][tj][0], ci[ti][tj][1]);
+                reinterpret_cast<double2 *>(di + a)[1] = make_double2(ci[ti][tj][2], ci[ti][tj][3]);
+            }
+        }
+}
+
+// One workgroup = one instance and zpb nodes (node-major: blockIdx.x the instance, so that
+// consecutive workgroups share the Vhat slabs of the same nodes in L2, blockIdx.y the node
+// group; channel-major: the XCD-aware order below). Each wavefront owns an LDS slab and
+// processes its nodes independently: scatter X(z)^T onto the grid, the four transforms as
+// slab products with F_x / F_y held in registers for the whole kernel, the pointwise product
+// with Vhat(z) in registers after the second, gather Y(z)^T back to the compact layout. No
+// workgroup barrier. The channel values of a wavefront's next node are loaded into registers
+// while the current node's transforms run; the channels' grid positions are computed once.
 // SQ: square table (F_y = F_x); NM: node-major vectors.
 template <typename T, int NP, int WPB, bool SQ, bool NM>
 __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling_kernel(
@@ -291,9 +425,9 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
     const cx<T> *__restrict__ Vhat, const cx<T> *__restrict__ x, cx<T> *__restrict__ y,
     const int *__restrict__ active, int zpb, const double *__restrict__ x_norm,
     cx<T> *__restrict__ x_copy) {
-    using G = DftGeom<NP>;
+    using G = DftSlab<T, NP>;
     using acc_t = typename Mfma<T>::acc;
-    constexpr int NT = G::NT, SA = G::SA, SB = G::SB, KK = NP / 4;
+    constexpr int NT = G::NT, S = G::S, KK = G::KK;
     constexpr int KC = NP * NP / 64;  // channels per lane (nc <= NP * NP)
     // three real products per complex one when one F in registers serves both transforms
     // (register budget)
@@ -320,16 +454,18 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
     T *sr = reinterpret_cast<T *>(dft_smem) + (size_t)wave * 2 * G::PLANE;
     T *si = sr + G::PLANE;
 
+    // F_x and F_y as B operands: f[t][kk] = F[k4 KK + kk][16 t + l16]
     T fxr[NT][KK], fxi[NT][KK], fyr_[NT][KK], fyi_[NT][KK];
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
         for (int kk = 0; kk < KK; ++kk) {
-            const cx<T> f = Fx[(t * 16 + l16) * NP + kk * 4 + k4];
+            const int e = (k4 * KK + kk) * NP + t * 16 + l16;
+            const cx<T> f = Fx[e];
             fxr[t][kk] = f.re;
             fxi[t][kk] = f.im;
             if constexpr (!SQ) {
-                const cx<T> g = Fy[(kk * 4 + k4) * NP + t * 16 + l16];
+                const cx<T> g = Fy[e];
                 fyr_[t][kk] = g.re;
                 fyi_[t][kk] = g.im;
             }
@@ -339,8 +475,8 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
 
     const int nc = L.nc[b];
     const long long off = L.off[b];
-    __shared__ int s_pos[NP * NP];  // slab position of each channel of the instance
-    for (int j = threadIdx.x; j < nc; j += blockDim.x) s_pos[j] = L.chx[off + j] * SB + L.chy[off + j];
+    __shared__ int s_pos[NP * NP];  // slab position of each channel of the instance (X^T)
+    for (int j = threadIdx.x; j < nc; j += blockDim.x) s_pos[j] = L.chy[off + j] * S + L.chx[off + j];
     __syncthreads();
     const int col = l16, g = k4;
     cx<T> xv[KC], vh[NT][NT][4];
@@ -361,6 +497,7 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
 #pragma unroll
         for (int k = 0; k < KC; ++k) xv[k] = x[vidx(k, z)];
     };
+    // Vhat(z)[kx][ky] in the accumulator layout of the second product (rows kx = 16 ti + 4 g + v)
     auto load_v = [&](int z) {
         const cx<T> *vz = Vhat + (size_t)z * NP * NP;
 #pragma unroll
@@ -369,28 +506,9 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
             for (int tj = 0; tj < NT; ++tj)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                    vh[ti][tj][v] = vz[(ti * 16 + Mfma<T>::row(g, v)) * NP + tj * 16 + col];
+                    vh[ti][tj][v] = vz[(ti * 16 + 4 * g + v) * NP + tj * 16 + col];
     };
-    // C = F Y, C = Y F (CONJ: with conj(F)) into (cr, ci)
     acc_t cr[NT][NT], ci[NT][NT], c3[NT][NT];
-    auto left = [&](auto conj_tag) {
-        constexpr bool CONJ = decltype(conj_tag)::value;
-        if constexpr (M3) {
-            cgemm3_reg_a<T, NP, CONJ>(fxr, fxi, sr, si, SB, cr, ci, c3);
-            combine3<T, NP, CONJ>(cr, ci, c3);
-        } else {
-            cgemm_reg_a<T, NP>(fxr, fxi, CONJ, sr, si, SB, cr, ci);
-        }
-    };
-    auto right = [&](auto conj_tag) {
-        constexpr bool CONJ = decltype(conj_tag)::value;
-        if constexpr (M3) {
-            cgemm3_reg_b<T, NP, CONJ>(sr, si, SA, fyr, fyi, cr, ci, c3);
-            combine3<T, NP, CONJ>(cr, ci, c3);
-        } else {
-            cgemm_reg_b<T, NP>(sr, si, SA, fyr, fyi, CONJ, cr, ci);
-        }
-    };
     using plain_t = std::integral_constant<bool, false>;
     using conj_t = std::integral_constant<bool, true>;
     const int z_first = zg * zpb + wave;
@@ -401,9 +519,9 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     for (int z = z_first; z < z_end; z += WPB) {
         {
-            acc_t *p4 = reinterpret_cast<acc_t *>(sr);  // both planes, 16-byte stores
-            constexpr int N4 = 2 * G::PLANE * (int)sizeof(T) / (int)sizeof(acc_t);
-            for (int idx = lane; idx < N4; idx += 64) p4[idx] = acc_t{0, 0, 0, 0};
+            float4 *p4 = reinterpret_cast<float4 *>(sr);  // both planes, 16-byte stores
+            constexpr int N4 = 2 * G::PLANE * (int)sizeof(T) / 16;
+            for (int idx = lane; idx < N4; idx += 64) p4[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
@@ -423,16 +541,17 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
         const bool more = z + WPB < z_end;
         if (more) load_x(z + WPB);  // consumed by the scatter above
         __builtin_amdgcn_wave_barrier();
-        // (1) Y = F_x X
-        left(plain_t{});
+        // (1) X^T F_x = (F_x X)^T, stored transposed: the slab holds Y1 = F_x X
+        slab_product<T, NP, false, M3>(sr, si, fxr, fxi, cr, ci, c3);
         __builtin_amdgcn_wave_barrier();
-        store_tiles<T, NP>(sr, si, SA, cr, ci);
+        slab_store_t<T, NP>(sr, si, cr, ci);
         __builtin_amdgcn_wave_barrier();
-        // (2) Z = Y F_y, then Z .* Vhat(z) (the slab is loaded just before this product; the
-        // barrier keeps the scheduler from moving the loads down to their use after it)
+        // (2) Y1 F_y, then .* Vhat(z) (the slab is loaded just before this product; the
+        // barrier keeps the scheduler from moving the loads down to their use after it); the
+        // transposed store leaves Z^T
         load_v(z);
         __builtin_amdgcn_sched_barrier(0);
-        right(plain_t{});
+        slab_product<T, NP, false, M3>(sr, si, fyr, fyi, cr, ci, c3);
 #pragma unroll
         for (int ti = 0; ti < NT; ++ti)
 #pragma unroll
@@ -445,17 +564,17 @@ __global__ __launch_bounds__(64 * WPB, sizeof(T) == 4 ? 2 : 1) void dft_coupling
                     ci[ti][tj][v] = a * h.im + c * h.re;
                 }
         __builtin_amdgcn_wave_barrier();
-        store_tiles<T, NP>(sr, si, SB, cr, ci);
+        slab_store_t<T, NP>(sr, si, cr, ci);
         __builtin_amdgcn_wave_barrier();
-        // (3) W = conj(F_x) Zhat
-        left(conj_t{});
+        // (3) Z^T conj(F_x) = (conj(F_x) Z)^T, stored transposed: the slab holds W
+        slab_product<T, NP, true, M3>(sr, si, fxr, fxi, cr, ci, c3);
         __builtin_amdgcn_wave_barrier();
-        store_tiles<T, NP>(sr, si, SA, cr, ci);
+        slab_store_t<T, NP>(sr, si, cr, ci);
         __builtin_amdgcn_wave_barrier();
-        // (4) out = W conj(F_y)
-        right(conj_t{});
+        // (4) W conj(F_y) = out, stored transposed: the slab holds out^T
+        slab_product<T, NP, true, M3>(sr, si, fyr, fyi, cr, ci, c3);
         __builtin_amdgcn_wave_barrier();
-        store_tiles<T, NP>(sr, si, SB, cr, ci);
+        slab_store_t<T, NP>(sr, si, cr, ci);
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
@@ -723,7 +842,7 @@ template <typename T, int NP, int WPB, bool SQ, bool NM>
 static void launch_dft_impl(const Layout &L, const DftTables<T> &D, const cx<T> *x, cx<T> *y,
                             const int *active, hipStream_t s, int active_hint,
                             const double *x_norm, cx<T> *x_copy, int nodes_per_wave) {
-    using G = DftGeom<NP>;
+    using G = DftSlab<T, NP>;
     const size_t shm = sizeof(T) * (size_t)WPB * 2 * G::PLANE;
     set_dynamic_lds((const void *)dft_coupling_kernel<T, NP, WPB, SQ, NM>, shm);
     const long long tot = L.total > 0 ? L.total : (long long)L.max_nc * L.n_inst;
