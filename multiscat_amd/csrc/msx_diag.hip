@@ -326,7 +326,11 @@ __global__ __launch_bounds__(64 * DL_WAVES) void diag_lds_kernel(
     auto tile = [&](int t) { return t < NTL ? t : NTL - 1; };
     const int n_tiles = tile_start[n_inst];
 #pragma unroll 1
-    for (int t = blockIdx.x * DL_WAVES + wave; t < n_tiles; t += gridDim.x * DL_WAVES) {
+    // tile t to wavefront slot t: the first gridDim.x tiles to wavefront 0 of every workgroup,
+    // the next to wavefront 1, ...: a launch with few active tiles gives each its own SIMD
+    // (36 instead of 60 us for one tile per wavefront, where two wavefronts of one workgroup
+    // shared a SIMD's matrix core)
+    for (int t = wave * gridDim.x + blockIdx.x; t < n_tiles; t += gridDim.x * DL_WAVES) {
         int lo = 0, hi = n_inst - 1;  // last instance with tile_start <= t
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;

@@ -76,6 +76,11 @@ struct Krylov {
     std::vector<long long> len_h;
     std::shared_ptr<ActiveSnap> snap;  // host view of the mask the launches issued now see
     int last_count = -1;  // iterating instances at the last count read (an upper bound later)
+    // Estimate of the instances the launches issued now will see (launch shapes only): the
+    // last count read is two iterations old when the next iteration is queued, and at the end
+    // of a solve the count falls by a similar factor each iteration, so the estimate carries
+    // the last fall forward once.
+    int hint_count = -1;
     // Arnoldi iterations captured as HIP graphs, keyed by iteration, operator form, flag slot
     // and launch shape; valid while the operator's buffers and options match graph_sig
     std::map<long long, hipGraphExec_t> graphs;
@@ -150,6 +155,7 @@ struct Krylov {
         snap->on.assign(h, h + n_inst);
         last_count = 0;
         for (int b = 0; b < n_inst; ++b) last_count += h[b];
+        hint_count = last_count;
     }
     void ensure(int nvec) {
         if (nvec > kmax + 1) nvec = kmax + 1;
@@ -189,7 +195,11 @@ struct Krylov {
         const int *a = flags_h + (size_t)p.slot * (n_inst + 1);
         p.next->on.assign(a, a + n_inst);
         p.next->ready.store(true, std::memory_order_release);
+        const int before = last_count;
         last_count = a[n_inst];
+        hint_count = before > last_count && last_count > 0
+                         ? std::max(1, (int)((long long)last_count * last_count / before))
+                         : last_count;
         return last_count;
     }
     int read_count() { return wait(post()); }
@@ -644,7 +654,7 @@ struct Ops {
     }
     // full off-diagonal coupling y = V x (DFT form on the matrix cores when it applies)
     // ---- algorithmic work of one launch over the instances it processes (profiling) ----
-    // host view of the Krylov mask the launches see, if any, and its latest count
+    // host view of the Krylov mask the launches see, if any, and its estimate of the count
     const std::shared_ptr<ActiveSnap> *snap_src = nullptr;
     const int *count_src = nullptr;
     int active_hint(const int *act) const { return act && count_src ? *count_src : -1; }
@@ -1212,7 +1222,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
                 ops.jac<double>(in, out, act, B.tmp1.p, B.tmp2.p);
             };
         }
-        ops.snap_src = &kr.snap; ops.count_src = &kr.last_count;
+        ops.snap_src = &kr.snap; ops.count_src = &kr.hint_count;
         // block Jacobi with the DFT coupling: new basis vectors normalised by the operator
         const OpScaled<double> op_s = [&](const cx<double> *raw, const double *nrm, cx<double> *in,
                                           cx<double> *out, const int *act) {
@@ -1294,7 +1304,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         } else if (split) {
             // r = D^-1 rho, rho = s - D x - V x in FP64 (one FP64 coupling, one FP64 product
             // with H0), D^-1 rho in FP32: rho is small, so FP32 holds r to ~1e-6 relative
-            ops.snap_src = &outer.snap; ops.count_src = &outer.last_count;
+            ops.snap_src = &outer.snap; ops.count_src = &outer.hint_count;
             ops.couple_full(x, B.tmp1.p, outer.active.p);
             {
                 Work fl, by;
@@ -1313,7 +1323,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
             r_in_fp32 = true;
         } else {
             // r = d0 - (x + D^-1 V x)   (B.src serves as scratch)
-            ops.snap_src = &outer.snap; ops.count_src = &outer.last_count;
+            ops.snap_src = &outer.snap; ops.count_src = &outer.hint_count;
             ops.couple_full(x, B.tmp1.p, outer.active.p);
             ops.dinv<double>(B.tmp1.p, x, B.src.p, 1.0, outer.active.p);
             k_axpby<double>(outer.K, d0, B.src.p, -1.0, r64, outer.active.p, s);
@@ -1337,7 +1347,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         if (n_act == 0 || sweep >= max_refine) break;
         B.eps_in_d.upload(eps_in.data(), (size_t)B.n_inst, s);
         outer.set_all_active(act.data());
-        ops.snap_src = &outer.snap; ops.count_src = &outer.last_count;
+        ops.snap_src = &outer.snap; ops.count_src = &outer.hint_count;
         // inner right-hand side M r in FP32 (M = I - D^-1 V to first degree; on a cold start the
         // first r = D^-1 s lives on the specular channels only, and V r is a single-source product)
         if (!r_in_fp32) {
@@ -1358,7 +1368,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         ops.neumann_rhs<float>(B.f_tmp2.p, B.f_rhs.p, outer.active.p, B.f_tmp1.p,
                                ops.neumann ? B.f_tmp1.p : nullptr);
         inner.set_all_active(act.data());
-        ops.snap_src = &inner.snap; ops.count_src = &inner.last_count;
+        ops.snap_src = &inner.snap; ops.count_src = &inner.hint_count;
         const std::string sig32 = ops.graph_signature(
             o.inner_orth == 1 ? "mixed32-cgs1" : "mixed32", B.f_tmp1.p, B.f_tmp2.p, B.eps_in_d.p);
         const std::function<int()> shape32 = [&] { return ops.launch_shape_key(); };
