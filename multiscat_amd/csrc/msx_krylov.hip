@@ -150,19 +150,36 @@ __global__ __launch_bounds__(KR_THREADS) void dots_reg_kernel(KLayout K, const c
 }
 
 // h[b*(kmax+1) + j] (+)= sum_chunk partial   for first <= j < count
-__global__ void reduce_partials_kernel(KLayout K, int first, int count,
-                                       const double *__restrict__ partial,
-                                       cx<double> *__restrict__ h, int accumulate) {
+// One wavefront per sum (wavefront w of the block takes j = first + w, first + w + 4, ...):
+// its lanes load the chunk partials side by side and add them in a fixed butterfly, so the
+// loads of a sum are in flight together (one lane summing a chunk list in turn waited on them
+// one after another: 8 to 16 us a launch).
+constexpr int RP_WAVES = 4;
+__global__ __launch_bounds__(64 * RP_WAVES) void reduce_partials_kernel(
+    KLayout K, int first, int count, const double *__restrict__ partial,
+    cx<double> *__restrict__ h, int accumulate) {
     const int b = blockIdx.x;
     if (K.active && !K.active[b]) return;
     const long long slot = (long long)b * (K.kmax + 1);
-    for (int j = first + threadIdx.x; j < count; j += blockDim.x) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int j = first + wave; j < count; j += RP_WAVES) {
+        const double2 *p = reinterpret_cast<const double2 *>(partial + (slot + j) * K.nchunk * 2);
         double sr = 0, si = 0;
-        const double *p = partial + (slot + j) * K.nchunk * 2;
-        for (int c = 0; c < K.nchunk; ++c) { sr += p[2 * c]; si += p[2 * c + 1]; }
-        cx<double> &dst = h[slot + j];
-        if (accumulate) dst = mk(dst.re + sr, dst.im + si);
-        else dst = mk(sr, si);
+        for (int c = lane; c < K.nchunk; c += 64) {
+            const double2 v = p[c];
+            sr += v.x;
+            si += v.y;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            sr += __shfl_xor(sr, o);
+            si += __shfl_xor(si, o);
+        }
+        if (lane == 0) {
+            cx<double> &dst = h[slot + j];
+            if (accumulate) dst = mk(dst.re + sr, dst.im + si);
+            else dst = mk(sr, si);
+        }
     }
 }
 
@@ -635,7 +652,7 @@ void k_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<T> 
         hipLaunchKernelGGL((dots_kernel<T>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr, kcount, w,
                            with_norm, partial);
     MSX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(K.n_inst), dim3(64), 0, s, K, 0,
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(K.n_inst), dim3(64 * RP_WAVES), 0, s, K, 0,
                        kcount + (with_norm ? 1 : 0), partial, h, accumulate);
     MSX_LAUNCH_CHECK();
 }
@@ -657,8 +674,8 @@ void k_axpy(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<dou
     if (with_norm && hnorm_dst) {
         // the norm partials sit in slot kcount; reduce only that slot into hnorm_dst (without
         // hnorm_dst the consumer sums them itself)
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3(K.n_inst), dim3(64), 0, s, K, kcount,
-                           kcount + 1, partial, hnorm_dst, 0);
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3(K.n_inst), dim3(64 * RP_WAVES), 0, s, K,
+                           kcount, kcount + 1, partial, hnorm_dst, 0);
         MSX_LAUNCH_CHECK();
     }
 }
@@ -675,7 +692,7 @@ bool k_axpy_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const c
     else
         return false;
     MSX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(K.n_inst), dim3(64), 0, s, K, 0, kcount,
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(K.n_inst), dim3(64 * RP_WAVES), 0, s, K, 0, kcount,
                        partial, h2, 0);
     MSX_LAUNCH_CHECK();
     return true;
