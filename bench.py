@@ -102,6 +102,8 @@ def main() -> int:
                     help="degree of the Neumann factor of the timed mixed solve")
     ap.add_argument("--inner-eps", type=float, default=0.0,
                     help="inner FP32 tolerance of the timed mixed solve (0: the solver's rule)")
+    ap.add_argument("--inner-products", default="auto", choices=["auto", "fp32", "split_fp16"],
+                    help="operands of the timed mixed solve's D^-1 products on the matrix cores")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 20 plain device copies of MIB MiB (calibrates the HBM byte "
                          "counters of a profiler run against a known size) and exit")
@@ -128,7 +130,8 @@ def main() -> int:
     eps_base = 5e-6   # the reference's default precision 1e-5 (nsig 5) for the FP64 baselines
     mixed = BT.SolveOptions(method="jacobi", precision="mixed", eps=eps, diag=args.diag,
                             graphs=not args.no_graphs, autotune=not args.no_autotune,
-                            neumann=args.neumann, inner_eps=args.inner_eps)
+                            neumann=args.neumann, inner_eps=args.inner_eps,
+                            inner_products=args.inner_products)
 
     def step(opts):
         tab.prepare()
@@ -228,7 +231,10 @@ def main() -> int:
         "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 inner (c64, MFMA) + fp64 refinement (c128)", "data": "synthetic",
+        "dtype": ("fp32 inner (c64 on MFMA: coupling fp32 operands, D^-1 products on split-fp16 "
+                  "operand pairs, fp32 accumulation) + fp64 refinement (c128)"
+                  if args.inner_products != "fp32" else
+                  "fp32 inner (c64, MFMA) + fp64 refinement (c128)"), "data": "synthetic",
         "config": {"workload": "hex31x31_nz150_256cond_uneven", "conditions": args.conditions,
                    "table": [nkx, nky], "nz": nz, "channels_min": int(counts.min()),
                    "channels_mean": float(counts.mean()), "channels_max": int(counts.max()),

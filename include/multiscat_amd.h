@@ -147,7 +147,8 @@ int msx_debug_apply_coupling(int nkx, int nky, int nz, const double *potential, 
  * (multiscat/multiscat/_scipy.py:249-354). potential00: complex[nz] (V_00(z)); kinetic:
  * nz x nz row-major; states (nc, nz) row-major complex. variant 0 = matrix-core GEMMs
  * (nz <= 320), 1 = vector ALU, 2 = matrix-core GEMMs with 8 channels per wavefront (FP64,
- * nz <= 160; variant 0 takes 16 there); precision 0 = FP64, 1 = FP32. */
+ * nz <= 160; variant 0 takes 16 there), 3 = matrix-core GEMMs on split-FP16 operands (FP32
+ * inverse only, nz <= 160); precision 0 = FP64, 1 = FP32. */
 int msx_debug_apply_diagonal(int nz, const double *potential00, const double *kinetic, int nc,
                              const double *e, const double *wave_c, const double *state_in,
                              double *state_out, int inverse, int precision, int variant);
@@ -197,6 +198,13 @@ typedef struct {
                          passes, Givens update) is captured once as a HIP graph per iteration index
                          and launch shape, and replayed by later iterations and solves of the
                          batch; 0: every kernel launched one by one. Off while profiling.      */
+    int inner_products; /* mixed precision: operands of the FP32 inner operator's D^-1 products
+                         on the matrix cores. 0 = auto (split FP16 where the kernel covers the
+                         shape: node-major vectors, nz <= 160, enough LDS; else FP32), 1 = FP32
+                         operands, 2 = split FP16 (an error where the kernel does not cover the
+                         shape). Split FP16: each FP32 operand as a pair hi + lo of FP16 values
+                         under a power-of-two scale (22 significant bits), three FP16 products
+                         accumulated in FP32.                                               */
 } msx_solve_options;
 
 /* Fills `opt` with the reference's settings for a given nsig (GS, FP64, window 3, 2000). */

@@ -64,6 +64,7 @@ class SolveOptions(C.Structure):
         ("inner_orth", C.c_int),
         ("autotune", C.c_int),
         ("graphs", C.c_int),
+        ("inner_products", C.c_int),
     ]
 
 

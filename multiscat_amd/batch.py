@@ -36,6 +36,10 @@ class SolveOptions:
     inner_orth: int = 0  # FP32 inner Arnoldi: 0 default, 1 single classical pass, 2 CGS2
     autotune: bool = True  # time the main kernels' launch shapes at a batch shape's first solve
     graphs: bool = True  # replay each Arnoldi iteration from a captured HIP graph
+    # mixed precision, operands of the inner D^-1 products on the matrix cores: "auto" (split
+    # FP16 pairs where the kernel covers the shape, else FP32), "fp32", or "split_fp16" (an
+    # error where the kernel does not cover the shape)
+    inner_products: str = "auto"
 
     @classmethod
     def from_nsig(cls, nsig: int, **kw) -> "SolveOptions":
@@ -70,6 +74,10 @@ class SolveOptions:
         o.inner_orth = self.inner_orth
         o.autotune = 1 if self.autotune else 0
         o.graphs = 1 if self.graphs else 0
+        products = {"auto": 0, "fp32": 1, "split_fp16": 2}
+        if self.inner_products not in products:
+            raise ValueError(f"unknown inner_products {self.inner_products!r}")
+        o.inner_products = products[self.inner_products]
         return o
 
 

@@ -1,0 +1,468 @@
+// All-channel inverse diagonal blocks of the mixed solver's FP32 inner operator with split-FP16
+// operands on the matrix cores:
+//
+//   y_j = D_j^-1 x_j = Q diag(1/(e_j + lam)) Q^T x_j + f_j g_j(N) c_j / (1 - f_j(N) c_j)
+//
+// (the reference applies D_j^-1 channel by channel inside its forward substitution,
+// lib/multiscat_fortran/fortran/multiscat_gmres.f90:666-685; its SciPy backend applies D^-1
+// to all channels at once, multiscat/multiscat/_scipy.py:249-259,336-354).
+//
+// Every FP32 operand a of the two products is carried as a pair of FP16 values scaled by a
+// power of two s per matrix column: s a = hi + lo, hi = fp16(s a), lo = fp16(s a - hi), so the
+// pair holds s a to 2^-22 relative (FP32: 2^-24). A product then takes three 16x16x32 FP16
+// MFMAs with FP32 accumulation, A_hi B_hi + A_hi B_lo + A_lo B_hi (the dropped A_lo B_lo is
+// below 2^-22 of the product), and the column scale is removed exactly from the FP32
+// accumulators; the MFMA rate is 16 times FP32's. On the 31 x 31, nz = 150 hexagonal table
+// (H0's spectrum spans 2e6, so eigenvector entries span many decades) D^-1 of random vectors
+// differs from an FP64 solve by 4e-7 to 1e-5 normwise, as the FP32-operand kernel does (the
+// FP32 reciprocals and the Sherman-Morrison quotient dominate), and the mixed solver takes
+// the same sweeps and iterations with either kernel.
+//
+// A wavefront owns 16 channels: their real and imaginary parts are two 16-column tiles.
+//   W^T = Q^T X^T   A = Q^T (m x z, from LDS), B = X^T (z x column), K = z
+//   W'  = W^T scaled by 1/(e_j + lam_m) in FP32 (e and lam summed in FP64), 0 for m >= nz
+//   Y^T = Q W'      A = Q (z x m), B = W' (m x column), K = m
+// K runs in blocks of 32; lane group g (lane / 16) supplies the 8 K values of its operands.
+// Product 1 takes z = 32 s + 8 g + u in block s. The B operand of product 2 is formed in place
+// from product 1's accumulators: lane group g of m tile mt holds rows 16 mt + 4 g + v, so in
+// block p it takes m = 32 p + 16 h + 4 g + v (h = 0, 1 for tiles 2p, 2p + 1; v = 0..3) as its
+// 8 K values, and the Q plane of product 2 is stored with its columns in that order (plane
+// column 32 p + 8 g + 4 h + v holds Q[z][32 p + 16 h + 4 g + v]), one 16-byte read per lane.
+//
+// Planes (built with the table from the FP64 eigenvectors): P1 = Q^T rows m, columns z; P2 =
+// Q rows z, columns m in the order above; each of 2^15 Q as hi and lo (|Q| <= 1: the scale
+// keeps hi below FP16's 65504 and moves lo out of FP16's subnormal range for entries down to
+// 2^-18, below which the pair still holds Q to 2^-39 absolute; unscaled, lo of a typical
+// entry ~0.08 was already subnormal), NQ = 32 ceil(nz / 32) columns of
+// stride NQ + 8 halves (16-byte reads of 8 lanes fall in distinct banks). P1 hi, P2 hi and
+// (when it fits) P1 lo sit in LDS, rows below nz only: rows past nz of P1 give W rows that the
+// scaling zeroes, rows past nz of P2 give outputs that are never stored, so reads there are
+// clamped to row nz - 1. P2 lo (and P1 lo when LDS is short) is read from L2, a z group ahead.
+#include "msx_internal.h"
+#include "msx_mfma.h"
+
+namespace msx {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+
+constexpr int H3_WAVES = 8;
+constexpr int H3_ZG = 2;  // z tiles per group of the second product
+constexpr float H3_QSCALE = 32768.0f;  // the planes hold 2^15 Q
+
+__device__ __forceinline__ v4f mfma_h(h8 a, h8 b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// Power of two that brings a column maximum m >= 0 into [2^14, 2^15) (FP16 holds 65504); a
+// zero or tiny maximum gets at most 2^115, which keeps the scaled values finite.
+__device__ __forceinline__ float h3_scale(float m) {
+    int e = 0;
+    (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
+    return ldexpf(1.0f, 15 - max(e, -100));
+}
+
+// hi + lo = v to 2^-22 relative: hi = fp16(v) (round to nearest), lo = fp16(v - hi) (v - hi is
+// exact in FP32).
+__device__ __forceinline__ void split8(const float (&v)[8], h8 &hi, h8 &lo) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const _Float16 h = (_Float16)v[u];
+        hi[u] = h;
+        lo[u] = (_Float16)(v[u] - (float)h);
+    }
+}
+
+__device__ __forceinline__ float wave_max4(float v) {
+    v = fmaxf(v, __shfl_xor(v, 16));
+    return fmaxf(v, __shfl_xor(v, 32));
+}
+__device__ __forceinline__ float wave_sum4(float v) {
+    v += __shfl_xor(v, 16);
+    return v + __shfl_xor(v, 32);
+}
+
+// P1LO: P1 lo in LDS (else read from L2 like P2 lo). BASE: y = base + sign D^-1 x.
+// Node-major vectors and factors only (element (j, z) of an instance at off nz + z nc + j).
+template <int KT, bool P1LO, bool BASE>
+__global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
+    Layout L, H3Planes H, const double *__restrict__ lam_g, const double *__restrict__ e,
+    const cx<float> *__restrict__ c, const float *__restrict__ fac, const cx<float> *x,
+    const cx<float> *base, cx<float> *y, float sign, const int *__restrict__ active) {
+    constexpr int NQ = 32 * KT, MT = 2 * KT, S = NQ + 8;
+    extern __shared__ double h3_smem[];
+    __shared__ int wave_sum[H3_WAVES];
+    const int nz = L.nz, n_inst = L.n_inst;
+    double *lam = h3_smem;                                    // [NQ], 0 past nz
+    float *qn = reinterpret_cast<float *>(lam + NQ);           // [NQ]: Q[nz - 1][m]
+    _Float16 *p1h = reinterpret_cast<_Float16 *>(qn + NQ);    // [nz][S]
+    _Float16 *p2h = p1h + nz * S;                              // [nz][S]
+    _Float16 *p1l = p2h + nz * S;                              // [nz][S] (P1LO)
+    int *tile_start = reinterpret_cast<int *>(p1l + (P1LO ? nz * S : 0));  // [n_inst + 1]
+    int *inst_nc = tile_start + n_inst + 1;                    // [n_inst]
+    int *inst_off = inst_nc + n_inst;                          // [n_inst]
+    const _Float16 *g1l = H.planes + (size_t)NQ * S;           // P1 lo in global memory
+    const _Float16 *g2l = H.planes + (size_t)3 * NQ * S;       // P2 lo in global memory
+    {
+        // exclusive scan of the active instances' 16-channel tiles: per-thread serial sums
+        // over a contiguous chunk, then a scan across the threads
+        const int per = (n_inst + blockDim.x - 1) / blockDim.x;
+        const int i0 = threadIdx.x * per, i1 = min(n_inst, i0 + per);
+        int local = 0;
+        for (int i = i0; i < i1; ++i) local += (active && !active[i]) ? 0 : (L.nc[i] + 15) / 16;
+        int incl = local;
+        const int ln = threadIdx.x & 63;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o);
+            if (ln >= o) incl += t;
+        }
+        if (ln == 63) wave_sum[threadIdx.x >> 6] = incl;
+        __syncthreads();
+        int before = 0;
+        for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) before += wave_sum[w];
+        int run = before + incl - local;
+        for (int i = i0; i < i1; ++i) {
+            const int nci = L.nc[i];
+            tile_start[i] = run;
+            inst_nc[i] = nci;
+            inst_off[i] = (int)L.off[i];
+            run += (active && !active[i]) ? 0 : (nci + 15) / 16;
+        }
+        if (threadIdx.x == blockDim.x - 1) tile_start[n_inst] = before + incl;
+    }
+    for (int m = threadIdx.x; m < NQ; m += blockDim.x) {
+        lam[m] = m < nz ? lam_g[m] : 0.0;
+        qn[m] = m < nz ? H.qlast[m] : 0.0f;
+    }
+    {
+        // rows 0..nz-1 of the LDS planes, 16 bytes at a time (rows are contiguous at stride S
+        // in both copies)
+        const int n16 = nz * S / 8;
+        const uint4 *s1h = reinterpret_cast<const uint4 *>(H.planes);
+        const uint4 *s1l = reinterpret_cast<const uint4 *>(g1l);
+        const uint4 *s2h = reinterpret_cast<const uint4 *>(H.planes + (size_t)2 * NQ * S);
+        for (int i = threadIdx.x; i < n16; i += blockDim.x) {
+            reinterpret_cast<uint4 *>(p1h)[i] = s1h[i];
+            reinterpret_cast<uint4 *>(p2h)[i] = s2h[i];
+            if (P1LO) reinterpret_cast<uint4 *>(p1l)[i] = s1l[i];
+        }
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int l16 = lane & 15, g = lane >> 4;
+    const int n_tiles = tile_start[n_inst];
+    // tile t to wavefront slot t: the first gridDim.x tiles to wavefront 0 of every
+    // workgroup, the next to wavefront 1, ...: a launch with few active tiles gives each its
+    // own SIMD
+#pragma unroll 1
+    for (int t = wave * gridDim.x + blockIdx.x; t < n_tiles; t += gridDim.x * H3_WAVES) {
+        int lo = 0, hi = n_inst - 1;  // last instance with tile_start <= t
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tile_start[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const int b = lo;
+        const int j0 = (t - tile_start[b]) * 16;
+        const int nc = inst_nc[b];
+        if (j0 >= nc) continue;  // never taken
+        const int jc = j0 + l16;
+        const bool live = jc < nc;
+        // element indices in 32 bits (the launcher checks that the batch's vectors fit);
+        // nodes past nz read the channel's last node instead (a clamp of the index, which
+        // depends on the tile, rather than of z, which the compiler would hoist)
+        const int cidx = inst_off[b] + (live ? jc : 0);
+        const int vb = inst_off[b] * nz + (live ? jc : 0);
+        const int vlast = vb + (nz - 1) * nc;
+        auto vi = [&](int z) -> int { return min(vb + z * nc, vlast); };
+        const double ej = e[cidx];
+        const cx<float> cj = c[cidx];
+        const float facN = fac[vlast];
+
+        // ---- the tile's X, z = 32 s + 8 g + u (past nz: the last node, which P1's zero
+        // columns cancel), column maxima and split ----
+        float xr[KT][8], xi[KT][8];
+#pragma unroll
+        for (int s = 0; s < KT; ++s)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int z = 32 * s + 8 * g + u;
+                const cx<float> v = x[vi(z)];
+                xr[s][u] = v.re;
+                xi[s][u] = v.im;
+            }
+        float mr = 0.f, mi = 0.f;
+#pragma unroll
+        for (int s = 0; s < KT; ++s)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                mr = fmaxf(mr, fabsf(xr[s][u]));
+                mi = fmaxf(mi, fabsf(xi[s][u]));
+            }
+        const float sxr = h3_scale(wave_max4(mr)), sxi = h3_scale(wave_max4(mi));
+        h8 xhr[KT], xlr[KT], xhi[KT], xli[KT];
+#pragma unroll
+        for (int s = 0; s < KT; ++s) {
+            float ar[8], ai[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                ar[u] = xr[s][u] * sxr;
+                ai[u] = xi[s][u] * sxi;
+            }
+            split8(ar, xhr[s], xlr[s]);
+            split8(ai, xhi[s], xli[s]);
+        }
+
+        // ---- W^T = Q^T X^T ----
+        v4f acc1[MT][2];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) acc1[mt][0] = acc1[mt][1] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < KT; ++s) {
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int row = min(16 * mt + l16, nz - 1);
+                const h8 ah = *reinterpret_cast<const h8 *>(p1h + row * S + 32 * s + 8 * g);
+                const h8 al = P1LO ? *reinterpret_cast<const h8 *>(p1l + row * S + 32 * s + 8 * g)
+                                   : *reinterpret_cast<const h8 *>(g1l + row * S + 32 * s + 8 * g);
+                acc1[mt][0] = mfma_h(ah, xhr[s], acc1[mt][0]);
+                acc1[mt][1] = mfma_h(ah, xhi[s], acc1[mt][1]);
+                acc1[mt][0] = mfma_h(ah, xlr[s], acc1[mt][0]);
+                acc1[mt][1] = mfma_h(ah, xli[s], acc1[mt][1]);
+                acc1[mt][0] = mfma_h(al, xhr[s], acc1[mt][0]);
+                acc1[mt][1] = mfma_h(al, xhi[s], acc1[mt][1]);
+            }
+        }
+
+        // P2 lo fragments of the first z group, read during the scaling and the split of W
+        h8 a2l[KT][H3_ZG];
+#pragma unroll
+        for (int zg = 0; zg < H3_ZG; ++zg) {
+            const int zr = min(16 * zg + l16, nz - 1);
+#pragma unroll
+            for (int p = 0; p < KT; ++p)
+                a2l[p][zg] = *reinterpret_cast<const h8 *>(g2l + zr * S + 32 * p + 8 * g);
+        }
+
+        // ---- W' = W / (e_j + lam_m) with the column scales removed; y0(N) = Q[N-1][:] W';
+        // 1 / (e_j + lam_m) from the FP64 sum by the hardware reciprocal, 0 past nz ----
+        // powers of two (exact): the column scales and the planes' 2^15
+        const float ixr = 1.0f / (sxr * H3_QSCALE), ixi = 1.0f / (sxi * H3_QSCALE);
+        float yn0 = 0.f, yn1 = 0.f, wr = 0.f, wi = 0.f;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = 16 * mt + 4 * g + v;
+                // unconditional loads (lam is 0 past nz), then a select: a condition around
+                // the load made every element a branch
+                const float r = __builtin_amdgcn_rcpf((float)(ej + lam[m]));
+                const float sc = m < nz ? r : 0.f;
+                const float q = qn[m];
+                acc1[mt][0][v] *= sc * ixr;
+                acc1[mt][1][v] *= sc * ixi;
+                yn0 += q * acc1[mt][0][v];
+                yn1 += q * acc1[mt][1][v];
+                wr = fmaxf(wr, fabsf(acc1[mt][0][v]));
+                wi = fmaxf(wi, fabsf(acc1[mt][1][v]));
+            }
+        yn0 = wave_sum4(yn0);
+        yn1 = wave_sum4(yn1);
+        // D^-1 x = D0^-1 x + corr f_j, corr = c y0(N) / (1 - c f_j(N))
+        cx<float> corr = mk(0.f, 0.f);
+        if (live) corr = cdiv(mk(yn0, yn1) * cj, mk(1.0f, 0.0f) - cj * facN);
+        const float swr = h3_scale(wave_max4(wr)), swi = h3_scale(wave_max4(wi));
+        h8 whr[KT], wlr[KT], whi[KT], wli[KT];
+#pragma unroll
+        for (int p = 0; p < KT; ++p) {
+            float ar[8], ai[8];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                ar[v] = acc1[2 * p][0][v] * swr;
+                ar[4 + v] = acc1[2 * p + 1][0][v] * swr;
+                ai[v] = acc1[2 * p][1][v] * swi;
+                ai[4 + v] = acc1[2 * p + 1][1][v] * swi;
+            }
+            split8(ar, whr[p], wlr[p]);
+            split8(ai, whi[p], wli[p]);
+        }
+        const float iwr = 1.0f / (swr * H3_QSCALE), iwi = 1.0f / (swi * H3_QSCALE);
+
+        // ---- Y^T = Q W' by groups of H3_ZG z tiles; factors and base values one group
+        // ahead at clamped addresses (unused values are never stored) ----
+        float fz[H3_ZG][4];
+        cx<float> bz[H3_ZG][4];
+        auto load_group = [&](int zt) {
+#pragma unroll
+            for (int zg = 0; zg < H3_ZG; ++zg)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int z = 16 * (zt + zg) + 4 * g + v;
+                    fz[zg][v] = fac[vi(z)];
+                    if constexpr (BASE) bz[zg][v] = base[vi(z)];
+                }
+        };
+        load_group(0);
+#pragma unroll 1
+        for (int zt0 = 0; zt0 < MT; zt0 += H3_ZG) {
+            int zrow[H3_ZG];
+#pragma unroll
+            for (int zg = 0; zg < H3_ZG; ++zg) zrow[zg] = min(16 * (zt0 + zg) + l16, nz - 1);
+            v4f acc2[H3_ZG][2];
+#pragma unroll
+            for (int zg = 0; zg < H3_ZG; ++zg) acc2[zg][0] = acc2[zg][1] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int p = 0; p < KT; ++p)
+#pragma unroll
+                for (int zg = 0; zg < H3_ZG; ++zg) {
+                    const h8 ah = *reinterpret_cast<const h8 *>(p2h + zrow[zg] * S + 32 * p + 8 * g);
+                    acc2[zg][0] = mfma_h(ah, whr[p], acc2[zg][0]);
+                    acc2[zg][1] = mfma_h(ah, whi[p], acc2[zg][1]);
+                    acc2[zg][0] = mfma_h(ah, wlr[p], acc2[zg][0]);
+                    acc2[zg][1] = mfma_h(ah, wli[p], acc2[zg][1]);
+                }
+#pragma unroll
+            for (int p = 0; p < KT; ++p)
+#pragma unroll
+                for (int zg = 0; zg < H3_ZG; ++zg) {
+                    acc2[zg][0] = mfma_h(a2l[p][zg], whr[p], acc2[zg][0]);
+                    acc2[zg][1] = mfma_h(a2l[p][zg], whi[p], acc2[zg][1]);
+                }
+            // the next group's P2 lo fragments (past the last group: clamped rows, unused)
+#pragma unroll
+            for (int zg = 0; zg < H3_ZG; ++zg) {
+                const int zr = min(16 * (zt0 + H3_ZG + zg) + l16, nz - 1);
+#pragma unroll
+                for (int p = 0; p < KT; ++p)
+                    a2l[p][zg] = *reinterpret_cast<const h8 *>(g2l + zr * S + 32 * p + 8 * g);
+            }
+            cx<float> out[H3_ZG][4];
+#pragma unroll
+            for (int zg = 0; zg < H3_ZG; ++zg)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    cx<float> val = mk(acc2[zg][0][v] * iwr, acc2[zg][1][v] * iwi);
+                    val = val + corr * fz[zg][v];
+                    out[zg][v] = BASE ? bz[zg][v] + val * sign : val;
+                }
+            if (16 * (zt0 + H3_ZG) <= nz) {
+                if (live) {
+#pragma unroll
+                    for (int zg = 0; zg < H3_ZG; ++zg)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) y[vi(16 * (zt0 + zg) + 4 * g + v)] = out[zg][v];
+                }
+            } else {
+#pragma unroll
+                for (int zg = 0; zg < H3_ZG; ++zg)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int z = 16 * (zt0 + zg) + 4 * g + v;
+                        if (live && zt0 + zg < MT && z < nz) y[vi(z)] = out[zg][v];
+                    }
+            }
+            load_group(zt0 + H3_ZG);  // past the last group: clamped, unused
+            asm volatile("" ::: "memory");
+        }
+    }
+}
+
+// Planes from the FP64 eigenvectors Q[z][m] (nz x nz row-major): 2^15 Q as P1 hi, P1 lo,
+// P2 hi, P2 lo, each nq rows of stride nq + 8 halves, zero outside the nz x nz block.
+__global__ void h3_planes_kernel(int nz, int nq, const double *__restrict__ Q, _Float16 *planes) {
+    const int S = nq + 8;
+    const size_t plane = (size_t)nq * S;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nq * S; idx += gridDim.x * blockDim.x) {
+        const int r = idx / S, col = idx - r * S;
+        // P1[m = r][z = col] = Q[z][m]
+        const double v1 = (r < nz && col < nz) ? (double)H3_QSCALE * Q[(size_t)col * nz + r] : 0.0;
+        // P2[z = r][col] = Q[z][m(col)], m = 32 p + 16 h + 4 g + v for col = 32 p + 8 g + 4 h + v
+        const int p = col >> 5, rr = col & 31, gg = rr >> 3, h = (rr >> 2) & 1, v = rr & 3;
+        const int m = 32 * p + 16 * h + 4 * gg + v;
+        const double v2 = (r < nz && col < nq && m < nz) ? (double)H3_QSCALE * Q[(size_t)r * nz + m] : 0.0;
+        const _Float16 h1 = (_Float16)v1, h2 = (_Float16)v2;
+        planes[idx] = h1;
+        planes[plane + idx] = (_Float16)(v1 - (double)h1);
+        planes[2 * plane + idx] = h2;
+        planes[3 * plane + idx] = (_Float16)(v2 - (double)h2);
+    }
+}
+
+int h3_padded_size(int nz) { return nz <= 160 ? (nz + 31) / 32 * 32 : 0; }
+
+size_t h3_plane_halves(int nq) { return 4 * (size_t)nq * (nq + 8); }
+
+void launch_h3_planes(int nz, int nq, const double *Q, _Float16 *planes, hipStream_t s) {
+    hipLaunchKernelGGL(h3_planes_kernel, dim3(64), dim3(256), 0, s, nz, nq, Q, planes);
+    MSX_LAUNCH_CHECK();
+}
+
+namespace {
+size_t h3_lds_bytes(int nq, int nz, int n_inst, bool p1lo) {
+    const size_t S = (size_t)nq + 8;
+    return 12 * (size_t)nq + (p1lo ? 3 : 2) * (size_t)nz * S * 2 +
+           (3 * (size_t)n_inst + 1) * sizeof(int);
+}
+size_t lds_limit() {
+    static size_t lim = 0;
+    if (lim == 0) {
+        int dev = 0, v = 0;
+        MSX_HIP(hipGetDevice(&dev));
+        MSX_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        lim = (size_t)v;
+    }
+    return lim;
+}
+int cu_count() {
+    static int n = 0;
+    if (n == 0) {
+        int dev = 0;
+        MSX_HIP(hipGetDevice(&dev));
+        MSX_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    return n;
+}
+
+template <int KT, bool P1LO, bool BASE>
+void launch_h3(const Layout &L, const H3Planes &H, const double *lam, const double *e,
+               const cx<float> *c, const float *fac, const cx<float> *x, const cx<float> *base,
+               cx<float> *y, float sign, const int *active, size_t shm, hipStream_t s) {
+    set_dynamic_lds((const void *)diag_h3_kernel<KT, P1LO, BASE>, shm);
+    const long long tiles_max = (long long)L.n_inst * ((L.max_nc + 15) / 16);
+    const int grid = (int)std::min<long long>(cu_count(), (tiles_max + H3_WAVES - 1) / H3_WAVES);
+    hipLaunchKernelGGL((diag_h3_kernel<KT, P1LO, BASE>), dim3(std::max(grid, 1)),
+                       dim3(64 * H3_WAVES), shm, s, L, H, lam, e, c, fac, x, base, y, sign, active);
+    MSX_LAUNCH_CHECK();
+}
+}  // namespace
+
+bool diag_h3_fits(const Layout &L, const H3Planes &H) {
+    if (!H.planes || H.nq == 0 || !L.nm || L.nz > H.nq || H.nq > 160) return false;
+    // 32-bit element indices, including vb + z nc for the padded nodes z < nq before the clamp
+    const long long total = L.total > 0 ? L.total : (long long)L.n_inst * L.max_nc;
+    if (total * L.nz + (long long)H.nq * L.max_nc >= (1LL << 31)) return false;
+    return h3_lds_bytes(H.nq, L.nz, L.n_inst, false) <= lds_limit();
+}
+
+void launch_diag_h3(const Layout &L, const H3Planes &H, const double *lam, const double *e,
+                    const cx<float> *c, const float *fac, const cx<float> *x,
+                    const cx<float> *base, cx<float> *y, float sign, const int *active,
+                    hipStream_t s) {
+    if (!diag_h3_fits(L, H))
+        throw Error(3, "split-FP16 diagonal blocks: shape or LDS size outside the kernel");
+    const bool p1lo = h3_lds_bytes(H.nq, L.nz, L.n_inst, true) <= lds_limit();
+    const size_t shm = h3_lds_bytes(H.nq, L.nz, L.n_inst, p1lo);
+#define MSX_H3(KT_)                                                                            \
+    if (H.nq == 32 * KT_) {                                                                    \
+        if (p1lo) {                                                                            \
+            if (base) return launch_h3<KT_, true, true>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s); \
+            return launch_h3<KT_, true, false>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s);         \
+        }                                                                                      \
+        if (base) return launch_h3<KT_, false, true>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s);    \
+        return launch_h3<KT_, false, false>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s);            \
+    }
+    MSX_H3(1) MSX_H3(2) MSX_H3(3) MSX_H3(4) MSX_H3(5)
+#undef MSX_H3
+    throw Error(3, "split-FP16 diagonal blocks: padded size not instantiated");
+}
+
+}  // namespace msx

@@ -195,6 +195,25 @@ struct EigTables {
     const double *lam;   // [nz]
 };
 
+// Split-FP16 planes of the eigenvectors for the FP32 inner operator's D^-1 (msx_diag_h3.hip):
+// Q^T and Q (columns reordered for the second product), each as FP16 hi and lo, nq x (nq + 8)
+// halves apiece, and Q[nz-1][m] in FP32 (the Sherman-Morrison row).
+struct H3Planes {
+    const _Float16 *planes = nullptr;  // [4][nq][nq + 8]: P1 hi, P1 lo, P2 hi, P2 lo
+    int nq = 0;                        // 32 ceil(nz / 32), 0 when nz > 160
+    const float *qlast = nullptr;      // [nz]
+};
+int h3_padded_size(int nz);
+size_t h3_plane_halves(int nq);
+void launch_h3_planes(int nz, int nq, const double *Q, _Float16 *planes, hipStream_t s);
+// Whether the split-FP16 kernel takes this layout (node-major, nz <= 160, LDS large enough).
+bool diag_h3_fits(const Layout &L, const H3Planes &H);
+// y = D^-1 x, or y = base + sign D^-1 x, node-major FP32 vectors and factors.
+void launch_diag_h3(const Layout &L, const H3Planes &H, const double *lam, const double *e,
+                    const cx<float> *c, const float *fac, const cx<float> *x,
+                    const cx<float> *base, cx<float> *y, float sign, const int *active,
+                    hipStream_t s);
+
 // MFMA form of the diagonal blocks (inverse = 1: D^-1, inverse = 0: D), nz <= 320.
 // channels_per_wave (FP64, nzp <= 160): 8 or 16 channels per wavefront, 0 = the built-in 16.
 bool diag_mfma_supported(int nz);

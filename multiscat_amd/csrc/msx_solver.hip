@@ -462,6 +462,8 @@ struct msx_table {
     DBuf<double> T, A, Vm, lam, Q, QT, Qp, QTp, dcw;
     DBuf<double> H0, H0p;  // symmetrised H0 and its zero-padded copy (nzp x nzp)
     DBuf<float> Qf, QTf, Qpf, QTpf;
+    DBuf<_Float16> H3;  // split-FP16 planes of Q^T and Q (nz <= 160)
+    int nq3 = 0;
     DBuf<int> info;
     DBuf<long long> ticks;  // phase stamps of the last eigensolve
     int nzp = 0;          // nz rounded up to a multiple of 16 (matrix-core tiles)
@@ -473,6 +475,11 @@ struct msx_table {
     size_t nG() const { return (size_t)nkx * nky; }
     EigTables<double> eig64() const { return {nzp, Qp.p, QTp.p, lam.p}; }
     EigTables<float> eig32() const { return {nzp, Qpf.p, QTpf.p, lam.p}; }
+    H3Planes h3() const {
+        H3Planes h;
+        if (nq3 > 0) h = H3Planes{H3.p, nq3, Qpf.p + (size_t)(nz - 1) * nzp};
+        return h;
+    }
     void prepare() {
         const size_t n2 = (size_t)nz * nz;
         launch_build_h0(nz, T.p, V.p, A.p, stream);
@@ -495,6 +502,11 @@ struct msx_table {
         launch_pad_square(nz, nzp, QT.p, QTp.p, stream);
         launch_pad_square(nz, nzp, Q.p, Qpf.p, stream);
         launch_pad_square(nz, nzp, QT.p, QTpf.p, stream);
+        nq3 = h3_padded_size(nz);
+        if (nq3 > 0) {
+            if (H3.n != h3_plane_halves(nq3)) H3.alloc(h3_plane_halves(nq3));
+            launch_h3_planes(nz, nq3, Q.p, H3.p, stream);
+        }
         if (H0p.n != p2) {
             H0.alloc(n2);
             H0p.alloc(p2);
@@ -618,6 +630,8 @@ struct Ops {
     const DftStore *dft;  // its convolution form
     bool direct;          // force the direct sum for the full coupling
     bool vector_diag = false;  // force the vector-ALU D^-1 kernel
+    bool split_f16 = true;            // split-FP16 FP32 D^-1 where the kernel covers the shape
+    bool split_f16_required = false;  // ... and an error where it does not
     hipStream_t s;
 
     Ops(msx_batch &b, bool neu, bool transposed, bool force_direct)
@@ -672,9 +686,10 @@ struct Ops {
                                 const void *c) const {
         char buf[512];
         std::snprintf(buf, sizeof buf,
-                      "%s|%p|%p|%p|V%p|Vf%p|D%p|nm%d|neu%d|ord%d|dir%d|vd%d|npw%d,%d|cpw%d|fac%p",
+                      "%s|%p|%p|%p|V%p|Vf%p|D%p|nm%d|neu%d|ord%d|dir%d|vd%d|h3%d|npw%d,%d|cpw%d|fac%p",
                       what, a, b, c, (const void *)V, (const void *)Vf, (const void *)dft,
                       L32.nm, neumann ? 1 : 0, order, direct ? 1 : 0, vector_diag ? 1 : 0,
+                      split_f16 ? 1 : 0,
                       B.tuning.dft_npw32, B.tuning.dft_npw64, B.tuning.diag64_cpw,
                       (const void *)(L32.nm ? B.facf_nm_d.p : B.facf_d.p));
         return buf;
@@ -777,6 +792,18 @@ struct Ops {
             by = over_active(act, [&](int nc) {
                      return nc * nz * (3.0 * sizeof(cx<T>) + sizeof(T));
                  }) + 2.0 * nz * nz * sizeof(T);
+        }
+        if constexpr (sizeof(T) == 4) {
+            if (mfma && split_f16 && diag_h3_fits(lay<float>(), t.h3())) {
+                ProfScope ps("diag_h3_f32", s, fl, by);
+                launch_diag_h3(lay<float>(), t.h3(), t.lam.p, B.e_d.p, c<float>(), fac<float>(), in,
+                               base, out, sign, act, s);
+                return;
+            }
+            if (split_f16_required)
+                throw Error(MSX_ERR_INVALID,
+                            "inner_products = 2: split-FP16 diagonal blocks need node-major FP32 "
+                            "vectors, nz <= 160 and LDS for the batch's tile scan");
         }
         const char *name = sizeof(T) == 4 ? (mfma ? "diag_mfma_f32" : "diag_vector_f32")
                                           : (mfma ? "diag_mfma_f64" : "diag_vector_f64");
@@ -915,6 +942,8 @@ void check_options(const msx_solve_options *o) {
     if (o->inner_orth < 0 || o->inner_orth > 2) throw Error(MSX_ERR_INVALID, "inner_orth is 0, 1 or 2");
     if (o->autotune != 0 && o->autotune != 1) throw Error(MSX_ERR_INVALID, "autotune is 0 or 1");
     if (o->graphs != 0 && o->graphs != 1) throw Error(MSX_ERR_INVALID, "graphs is 0 or 1");
+    if (o->inner_products < 0 || o->inner_products > 2)
+        throw Error(MSX_ERR_INVALID, "inner_products is 0, 1 or 2");
     if (!(o->inner_eps >= 0 && o->inner_eps < 1))
         throw Error(MSX_ERR_INVALID, "inner_eps is 0 (automatic) or in (0, 1)");
 }
@@ -1161,6 +1190,8 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     Ops ops(B, o.neumann >= 1, transposed, o.coupling == 1);
     ops.order = std::max(1, o.neumann);
     ops.vector_diag = o.diag == 1;  // auto: matrix cores when nz <= 320
+    ops.split_f16 = o.inner_products != 1;
+    ops.split_f16_required = o.inner_products == 2;
     ops.set_fp32_order();
     // The mixed solver's refinement residual as D^-1 (s - D x - V x) with D^-1 in FP32
     // (split_residual) instead of d0 - x - D^-1 V x in FP64: for the scattering source and
@@ -1501,6 +1532,7 @@ void msx_default_options(msx_solve_options *opt, int nsig) {
     opt->inner_orth = 0;
     opt->autotune = 0;
     opt->graphs = 0;
+    opt->inner_products = 0;
 }
 
 int msx_table_create(int nkx, int nky, int nz, const double *potential, const double *kinetic,
@@ -2095,9 +2127,13 @@ int msx_debug_apply_diagonal(int nz, const double *potential00, const double *ki
         if (nz <= 0 || nc <= 0) return MSX_ERR_DIMS;
         if (!potential00 || !kinetic || !e || !wave_c || !state_in || !state_out)
             throw Error(MSX_ERR_INVALID, "null argument");
-        if (precision < 0 || precision > 1 || variant < 0 || variant > 2 || inverse < 0 ||
+        if (precision < 0 || precision > 1 || variant < 0 || variant > 3 || inverse < 0 ||
             inverse > 1)
             throw Error(MSX_ERR_INVALID, "bad inverse/precision/variant");
+        if (variant == 3 && (precision != 1 || inverse != 1))
+            throw Error(MSX_ERR_INVALID, "variant 3 (split FP16) is the FP32 inverse only");
+        if (variant == 3 && h3_padded_size(nz) == 0)
+            throw Error(MSX_ERR_INVALID, "split-FP16 diagonal blocks need nz <= 160");
         // variant 2: the matrix-core form with 8 channels per wavefront (FP64)
         const int cpw = variant == 2 ? 8 : 0;
         if (variant == 2) variant = 0;
@@ -2125,6 +2161,41 @@ int msx_debug_apply_diagonal(int nz, const double *potential00, const double *ki
         c_d.upload(reinterpret_cast<const cx<double> *>(wave_c), nc, s);
         xin.upload(reinterpret_cast<const cx<double> *>(state_in), n, s);
         launch_preconditioner_factors_qt(nc, nz, t.Q.p, t.QT.p, t.lam.p, e_d.p, fac.p, s);
+        if (variant == 3) {
+            // the split-FP16 kernel takes node-major vectors and factors: transposed here
+            std::vector<double> hfac(n);
+            fac.download(hfac.data(), n, s);
+            MSX_HIP(hipStreamSynchronize(s));
+            std::vector<cx<float>> hx(n);
+            std::vector<float> hf(n);
+            for (int j = 0; j < nc; ++j)
+                for (int z = 0; z < nz; ++z) {
+                    const size_t cm = (size_t)j * nz + z, nm = (size_t)z * nc + j;
+                    hx[nm] = mk((float)state_in[2 * cm], (float)state_in[2 * cm + 1]);
+                    hf[nm] = (float)hfac[cm];
+                }
+            DBuf<cx<float>> cf(nc), xf(n), yf(n);
+            DBuf<float> facn(n);
+            launch_to_float(nc, c_d.p, cf.p, s);
+            xf.upload(hx.data(), n, s);
+            facn.upload(hf.data(), n, s);
+            Layout Ln = L;
+            Ln.nm = 1;
+            Ln.total = nc;
+            if (!diag_h3_fits(Ln, t.h3()))
+                throw Error(MSX_ERR_INVALID, "split-FP16 diagonal blocks: shape outside the kernel");
+            launch_diag_h3(Ln, t.h3(), t.lam.p, e_d.p, cf.p, facn.p, xf.p, nullptr, yf.p, 1.0f,
+                           nullptr, s);
+            yf.download(hx.data(), n, s);
+            MSX_HIP(hipStreamSynchronize(s));
+            for (int j = 0; j < nc; ++j)
+                for (int z = 0; z < nz; ++z) {
+                    const size_t cm = (size_t)j * nz + z, nm = (size_t)z * nc + j;
+                    state_out[2 * cm] = hx[nm].re;
+                    state_out[2 * cm + 1] = hx[nm].im;
+                }
+            return MSX_OK;
+        }
         if (precision == 0) {
             if (variant == 0)
                 launch_diag_blocks<double>(L, t.eig64(), e_d.p, c_d.p, fac.p, xin.p, nullptr,

@@ -72,6 +72,45 @@ def test_mixed_batch_within_bar(small_batch, coupling, diag):
         assert abs(P[open_].sum() - 1.0) < 1e-6
 
 
+@pytest.mark.parametrize("products", ["auto", "fp32", "split_fp16"])
+def test_mixed_inner_products(small_batch, products):
+    """The inner operator's D^-1 products with FP32 operands or split-FP16 pairs: both reach
+    the oracle to the outer tolerance; the split form needs the node-major FP32 layout, which
+    the direct coupling does not take."""
+    w, _, bt, ref = small_batch
+    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11,
+                             inner_products=products))
+    Ps = bt.intensities()
+    assert _maxdiff(Ps, ref) < 1e-6
+    for c, P in zip(w.conditions, Ps):
+        open_ = c.e[c.channels[0], c.channels[1]] < 0
+        assert abs(P[open_].sum() - 1.0) < 1e-6
+    if products == "split_fp16":
+        with pytest.raises(RuntimeError, match="inner_products"):
+            bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11,
+                                     inner_products=products, coupling="direct"))
+    with pytest.raises(ValueError, match="inner_products"):
+        BT.SolveOptions(inner_products="fp16").native()
+
+
+def test_split_fp16_with_planes_in_l2():
+    """nz = 160 with 60 conditions: the tile scan of 60 instances leaves LDS for two of the
+    three split-FP16 planes, so the kernel reads Q^T's low plane from L2 as well."""
+    w = W.hex_batch(n_conditions=60, nz=160, nk=11, cond_seed=12,
+                    channel_counts=[12, 20, 31, 17, 26])
+    tab = BT.Table(w.potential, w.kinetic)
+    bt = BT.Batch(tab, w.conditions)
+    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11,
+                             inner_products="split_fp16"))
+    worst = 0.0
+    for c, P in zip(w.conditions[::6], bt.intensities()[::6]):
+        psi = oracle.solve(1, 0, w.potential, c.e, c.wave_a, c.wave_b, c.wave_c, w.kinetic,
+                           channels=c.channels, eps=5e-13)
+        Pr = oracle.intensities(psi, c.wave_a, c.wave_b)[c.channels[0], c.channels[1]]
+        worst = max(worst, float(np.abs(P - Pr).max()))
+    assert worst < 1e-6
+
+
 @pytest.mark.parametrize("degree", [0, 2, 3, 4, 7])
 def test_neumann_degrees(small_batch, degree):
     """Block Jacobi with no Neumann factor and with the series to degree 2..7: the same
