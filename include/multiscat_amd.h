@@ -136,8 +136,9 @@ int msx_debug_solve_lower_block(int nkx, int nky, int nz, const double *potentia
  * y_j = sum_i V[G_j - G_i] x_i over i > j (mode 0), i != j (mode 1) or i < j (mode 2) for an
  * explicit channel list; precision 0 = FP64, 1 = FP32 (x, y still passed as FP64);
  * algorithm 0 = direct sum, 1 = DFT form (mode 1 only; 2, 3, 4: the DFT form with 1, 2 or 8
- * nodes per wavefront instead of the built-in 4). V is C order (nkx, nky, nz) here, states
- * are (nc, nz) row-major. */
+ * nodes per wavefront instead of the built-in 4; 5: the DFT form on split-FP16 operands, FP32,
+ * square tables up to 32 x 32). V is C order (nkx, nky, nz) here, states are (nc, nz)
+ * row-major. */
 int msx_debug_apply_coupling(int nkx, int nky, int nz, const double *potential, int nc,
                              const int *chx, const int *chy, const double *state_in,
                              double *state_out, int mode, int precision, int algorithm);
@@ -198,13 +199,14 @@ typedef struct {
                          passes, Givens update) is captured once as a HIP graph per iteration index
                          and launch shape, and replayed by later iterations and solves of the
                          batch; 0: every kernel launched one by one. Off while profiling.      */
-    int inner_products; /* mixed precision: operands of the FP32 inner operator's D^-1 products
-                         on the matrix cores. 0 = auto (split FP16 where the kernel covers the
-                         shape: node-major vectors, nz <= 160, enough LDS; else FP32), 1 = FP32
-                         operands, 2 = split FP16 (an error where the kernel does not cover the
-                         shape). Split FP16: each FP32 operand as a pair hi + lo of FP16 values
-                         under a power-of-two scale (22 significant bits), three FP16 products
-                         accumulated in FP32.                                               */
+    int inner_products; /* mixed precision: operands of the FP32 inner operator's products on
+                         the matrix cores (D^-1 and the DFT coupling). 0 = auto (split FP16
+                         where the kernels cover the shape: node-major vectors; D^-1 nz <= 160
+                         with enough LDS, the coupling square tables up to 32 x 32; else FP32),
+                         1 = FP32 operands, 2 = split FP16 (an error where the D^-1 kernel does
+                         not cover the shape). Split FP16: each FP32 operand as a pair hi + lo
+                         of FP16 values under a power-of-two scale (22 significant bits), three
+                         FP16 products accumulated in FP32.                                 */
 } msx_solve_options;
 
 /* Fills `opt` with the reference's settings for a given nsig (GS, FP64, window 3, 2000). */

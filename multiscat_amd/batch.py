@@ -36,9 +36,9 @@ class SolveOptions:
     inner_orth: int = 0  # FP32 inner Arnoldi: 0 default, 1 single classical pass, 2 CGS2
     autotune: bool = True  # time the main kernels' launch shapes at a batch shape's first solve
     graphs: bool = True  # replay each Arnoldi iteration from a captured HIP graph
-    # mixed precision, operands of the inner D^-1 products on the matrix cores: "auto" (split
-    # FP16 pairs where the kernel covers the shape, else FP32), "fp32", or "split_fp16" (an
-    # error where the kernel does not cover the shape)
+    # mixed precision, operands of the inner operator's products on the matrix cores (D^-1
+    # and the DFT coupling): "auto" (split FP16 pairs where the kernels cover the shape, else
+    # FP32), "fp32", or "split_fp16" (an error where the D^-1 kernel does not cover the shape)
     inner_products: str = "auto"
 
     @classmethod

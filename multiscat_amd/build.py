@@ -23,7 +23,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "libmultiscat_amd.so"
 ARCH = os.environ.get("MSX_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["msx_kernels.hip", "msx_eig.hip", "msx_eig_dc.hip", "msx_krylov.hip", "msx_dft.hip", "msx_diag_h3.hip",
+HIP_SOURCES = ["msx_kernels.hip", "msx_eig.hip", "msx_eig_dc.hip", "msx_krylov.hip", "msx_dft.hip", "msx_diag_h3.hip", "msx_dft_h3.hip",
                "msx_diag.hip", "msx_profile.hip", "msx_solver.hip", "msx_noalloc.hip"]
 CXX_SOURCES = ["msx_host.cpp"]
 

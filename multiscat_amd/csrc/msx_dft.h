@@ -39,6 +39,14 @@ void launch_dft_coupling(const Layout &L, const DftTables<T> &D, const cx<T> *x,
                          const double *x_norm = nullptr, cx<T> *x_copy = nullptr,
                          int nodes_per_wave = 0);
 
+// The same coupling (FP32 vectors) with the four products on split-FP16 operands
+// (msx_dft_h3.hip): node-major vectors and square tables of at most 32 x 32 only.
+bool dft_h3_fits(const Layout &L, const DftTables<float> &D);
+void launch_dft_coupling_h3(const Layout &L, const DftTables<float> &D, const cx<float> *x,
+                            cx<float> *y, const int *active, hipStream_t s, int active_hint = -1,
+                            const double *x_norm = nullptr, cx<float> *x_copy = nullptr,
+                            int nodes_per_wave = 0);
+
 // Nodes per workgroup of the DFT coupling's launch for a given active-instance hint and node
 // grouping (the only launch parameter that depends on host-side state).
 int dft_block_nodes(const Layout &L, bool fp64, int active_hint, int nodes_per_wave);

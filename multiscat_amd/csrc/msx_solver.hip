@@ -729,6 +729,12 @@ struct Ops {
                      const double *in_norm = nullptr, cx<float> *in_copy = nullptr) {
         Work fl, by;
         if (profiling()) couple_full_work(act, sizeof(cx<float>), !direct, fl, by, in_copy != nullptr);
+        if (!direct && split_f16 && dft_h3_fits(L32, dft->d32())) {
+            ProfScope ps("coupling_dft_h3", s, fl, by);
+            launch_dft_coupling_h3(L32, dft->d32(), in, out, act, s, active_hint(act), in_norm,
+                                   in_copy, B.tuning.dft_npw32);
+            return;
+        }
         ProfScope ps(direct ? "coupling_direct_f32" : "coupling_dft_f32", s, fl, by);
         if (direct) launch_coupling<float>(L32, Vf, in, out, COUPLE_FULL, act, s);
         else launch_dft_coupling<float>(L32, dft->d32(), in, out, act, s, active_hint(act), in_norm,
@@ -2066,8 +2072,11 @@ int msx_debug_apply_coupling(int nkx, int nky, int nz, const double *V, int nc, 
     try {
         if (nkx <= 0 || nky <= 0 || nz <= 0 || nc <= 0) return MSX_ERR_DIMS;
         if (!V || !chx || !chy || !x || !y) throw Error(MSX_ERR_INVALID, "null argument");
-        if (mode < 0 || mode > 2 || precision < 0 || precision > 1 || algorithm < 0 || algorithm > 4)
+        if (mode < 0 || mode > 2 || precision < 0 || precision > 1 || algorithm < 0 || algorithm > 5)
             throw Error(MSX_ERR_INVALID, "bad mode/precision/algorithm");
+        if (algorithm == 5 && (precision != 1 || mode != COUPLE_FULL))
+            throw Error(MSX_ERR_INVALID, "algorithm 5 (split FP16) is the FP32 full coupling only");
+        const bool split = algorithm == 5;
         // DFT form with the built-in node grouping (1), or 1, 2 or 8 nodes per wavefront
         const int npw = algorithm == 2 ? 1 : algorithm == 3 ? 2 : algorithm == 4 ? 8 : 0;
         if (algorithm > 1) algorithm = 1;
@@ -2097,6 +2106,31 @@ int msx_debug_apply_coupling(int nkx, int nky, int nz, const double *V, int nc, 
             if (algorithm == 0) launch_coupling<double>(L, t.V.p, xin.p, yout.p, mode, nullptr, s);
             else launch_dft_coupling<double>(L, t.dft.d64(), xin.p, yout.p, nullptr, s, -1, nullptr,
                                              nullptr, npw);
+        } else if (split) {
+            // the split-FP16 kernel takes node-major vectors: transposed here
+            std::vector<cx<float>> hx(n);
+            for (int j = 0; j < nc; ++j)
+                for (int z = 0; z < nz; ++z) {
+                    const size_t cm = (size_t)j * nz + z;
+                    hx[(size_t)z * nc + j] = mk((float)x[2 * cm], (float)x[2 * cm + 1]);
+                }
+            DBuf<cx<float>> xf(n), yf(n);
+            xf.upload(hx.data(), n, s);
+            Layout Ln = L;
+            Ln.nm = 1;
+            Ln.total = nc;
+            if (!dft_h3_fits(Ln, t.dft.d32()))
+                throw Error(MSX_ERR_INVALID, "split-FP16 DFT coupling: square tables up to 32 x 32");
+            launch_dft_coupling_h3(Ln, t.dft.d32(), xf.p, yf.p, nullptr, s);
+            yf.download(hx.data(), n, s);
+            MSX_HIP(hipStreamSynchronize(s));
+            for (int j = 0; j < nc; ++j)
+                for (int z = 0; z < nz; ++z) {
+                    const size_t cm = (size_t)j * nz + z;
+                    y[2 * cm] = hx[(size_t)z * nc + j].re;
+                    y[2 * cm + 1] = hx[(size_t)z * nc + j].im;
+                }
+            return MSX_OK;
         } else {
             DBuf<cx<float>> xf(n), yf(n);
             launch_to_float((long long)n, xin.p, xf.p, s);

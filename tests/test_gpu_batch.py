@@ -296,6 +296,62 @@ def test_dft_coupling_matches_direct(nkx, nky, nc):
     del C
 
 
+@pytest.mark.parametrize("nkx,nc", [(31, 300), (31, 961), (17, 200), (32, 1024), (25, 1)])
+def test_split_fp16_dft_coupling(nkx, nc):
+    """The DFT coupling with its four products on split-FP16 operands (algorithm 5: 2^15 F as
+    FP16 pairs in registers, the grid scaled per node and product by a power of two) against
+    the oracle's direct sum: per node within a small factor of the FP32 kernel's normwise
+    error, with nodes 40 orders of magnitude apart (the scale follows each node)."""
+    from multiscat_amd import _lib
+
+    rng = np.random.default_rng(nkx * 31 + nc)
+    nz = 23
+    V = rng.standard_normal((nkx, nkx, nz)) + 1j * rng.standard_normal((nkx, nkx, nz))
+    V = np.ascontiguousarray(V)
+    order = rng.permutation(nkx * nkx)[:nc]
+    chx, chy = (np.ascontiguousarray(a, dtype=np.int32) for a in np.divmod(order, nkx))
+    x = rng.standard_normal((nc, nz)) + 1j * rng.standard_normal((nc, nz))
+    x *= 10.0 ** rng.uniform(-20, 20, nz)[None, :]
+    x = np.ascontiguousarray(x)
+    ref = oracle.apply_coupling(V, x, channels=(chx, chy))
+    out = {}
+    for alg in (1, 5):
+        y = np.zeros_like(x)
+        rc = _lib.lib().msx_debug_apply_coupling(nkx, nkx, nz, _lib.ptr(V), nc, _lib.ptr(chx),
+                                                 _lib.ptr(chy), _lib.ptr(x), _lib.ptr(y), 1, 1,
+                                                 alg)
+        _lib.check(rc, "msx_debug_apply_coupling")
+        out[alg] = y
+    if nc == 1:
+        # no other channel to couple to: the transforms leave rounding noise only
+        assert not ref.any()
+        for z in range(nz):
+            bound = 1e-5 * np.abs(V[:, :, z]).max() * np.abs(x[:, z]).max()
+            assert np.abs(out[5][:, z]).max() <= bound and np.abs(out[1][:, z]).max() <= bound
+        return
+    for z in range(nz):
+        scale = np.abs(ref[:, z]).max()
+        e32 = np.abs(out[1][:, z] - ref[:, z]).max() / scale
+        e16 = np.abs(out[5][:, z] - ref[:, z]).max() / scale
+        assert e16 < 1e-5, (z, e16)
+        assert e16 < 4.0 * e32 + 1e-7, (z, e16, e32)
+
+
+def test_split_fp16_dft_coupling_rejects_other_tables():
+    from multiscat_amd import _lib
+
+    for nkx, nky in ((15, 15), (31, 20), (33, 33)):
+        nz, nc = 5, 10
+        V = np.ascontiguousarray(np.ones((nkx, nky, nz), np.complex128))
+        chx = np.ascontiguousarray(np.arange(nc, dtype=np.int32) % nkx)
+        chy = np.ascontiguousarray(np.arange(nc, dtype=np.int32) // nkx)
+        x = np.ascontiguousarray(np.ones((nc, nz), np.complex128))
+        y = np.zeros_like(x)
+        rc = _lib.lib().msx_debug_apply_coupling(nkx, nky, nz, _lib.ptr(V), nc, _lib.ptr(chx),
+                                                 _lib.ptr(chy), _lib.ptr(x), _lib.ptr(y), 1, 1, 5)
+        assert rc != 0
+
+
 def _rect_workload(nkx=13, nky=9, nz=40, counts=(30, 64, 90, 45)):
     """A rectangular cell (a1 = 2.6, a2 = 3.9) with a corrugated table on an nkx x nky grid."""
     from multiscat_amd import basis as B
