@@ -43,20 +43,66 @@ __device__ __forceinline__ v4f mfma16(hh8 a, hh8 b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
-// hi + lo = v to 2^-22 relative (lo exact up to its own rounding)
-__device__ __forceinline__ void split8h(const float (&v)[8], hh8 &hi, hh8 &lo) {
+// hi + lo = s v to 2^-22 relative: hi = fp16(s v), lo = fp16(s v - hi), each one mixed-
+// precision FMA that rounds once to FP16 (s is a power of two, so s v - hi is exact before
+// that rounding); pairs of elements fill the low and high halves of one register. The
+// compiler formed a multiply, a conversion back and a subtraction for some of these.
+__device__ __forceinline__ uint32_t split_hi2(float v0, float v1, float s) {
+    uint32_t r;
+    asm("v_fma_mixlo_f16 %0, %1, %2, 0\n\tv_fma_mixhi_f16 %0, %3, %2, 0"
+        : "=&v"(r) : "v"(v0), "v"(s), "v"(v1));
+    return r;
+}
+__device__ __forceinline__ uint32_t split_lo2(float v0, float v1, float s, uint32_t hi) {
+    uint32_t r;
+    asm("v_fma_mixlo_f16 %0, %1, %2, -%4 op_sel_hi:[0,0,1]\n\t"
+        "v_fma_mixhi_f16 %0, %3, %2, -%4 op_sel:[0,0,1] op_sel_hi:[0,0,1]"
+        : "=&v"(r) : "v"(v0), "v"(s), "v"(v1), "v"(hi));
+    return r;
+}
+__device__ __forceinline__ void split8h(const float (&v)[8], float s, hh8 &hi, hh8 &lo) {
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    u4 h, l;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-        const _Float16 h = (_Float16)v[u];
-        hi[u] = h;
-        lo[u] = (_Float16)(v[u] - (float)h);
+    for (int q = 0; q < 4; ++q) {
+        h[q] = split_hi2(v[2 * q], v[2 * q + 1], s);
+        l[q] = split_lo2(v[2 * q], v[2 * q + 1], s, h[q]);
     }
+    hi = __builtin_bit_cast(hh8, h);
+    lo = __builtin_bit_cast(hh8, l);
 }
 
+// max(|a|, |b|, m) in one instruction (the abs modifiers are free; fmaxf under the IEEE mode
+// of compute kernels quiets every input first, two extra instructions per element)
+__device__ __forceinline__ float max3abs(float a, float b, float m) {
+    float r;
+    asm("v_max3_f32 %0, |%1|, |%2|, %3" : "=v"(r) : "v"(a), "v"(b), "v"(m));
+    return r;
+}
+
+// Maximum of a non-negative value over the wavefront (uniform result): non-negative floats
+// order as their bit patterns, so the 16-lane rows reduce by integer maxima through DPP
+// (quad swaps, half-row and row mirrors; no LDS round trips), then the four rows by readlane.
 __device__ __forceinline__ float wave_max64(float v) {
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) v = fmaxf(v, __shfl_xor(v, o));
-    return v;
+    int x = __builtin_bit_cast(int, v);
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, false));  // row_mirror
+    const int m = max(max(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 16)),
+                      max(__builtin_amdgcn_readlane(x, 32), __builtin_amdgcn_readlane(x, 48)));
+    return __builtin_bit_cast(float, m);
+}
+
+// element at a 32-bit byte offset from a uniform base: one VGPR offset on the scalar base
+// (the launcher keeps the batch's vectors below 4 GiB)
+template <typename T>
+__device__ __forceinline__ const T &at(const T *base, uint32_t byte_off) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ T &at(T *base, uint32_t byte_off) {
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off);
 }
 
 // The split planes of one operand row tile: lane (i, kg) holds A[16 ti + i][8 kg .. 8 kg + 8)
@@ -125,7 +171,7 @@ __device__ __forceinline__ int load_split(const float *sr, const float *si, ATil
         xi[ti][0] = i0.x; xi[ti][1] = i0.y; xi[ti][2] = i0.z; xi[ti][3] = i0.w;
         xi[ti][4] = i1.x; xi[ti][5] = i1.y; xi[ti][6] = i1.z; xi[ti][7] = i1.w;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) mx = fmaxf(mx, fmaxf(fabsf(xr[ti][u]), fabsf(xi[ti][u])));
+        for (int u = 0; u < 8; ++u) mx = max3abs(xr[ti][u], xi[ti][u], mx);
     }
     mx = wave_max64(mx);
     int e2 = 0;
@@ -134,14 +180,8 @@ __device__ __forceinline__ int load_split(const float *sr, const float *si, ATil
     const float sc = ldexpf(1.0f, e);
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti) {
-        float vr[8], vi[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            vr[u] = xr[ti][u] * sc;
-            vi[u] = xi[ti][u] * sc;
-        }
-        split8h(vr, a[ti].rh, a[ti].rl);
-        split8h(vi, a[ti].ih, a[ti].il);
+        split8h(xr[ti], sc, a[ti].rh, a[ti].rl);
+        split8h(xi[ti], sc, a[ti].ih, a[ti].il);
     }
     return e;
 }
@@ -169,7 +209,10 @@ __global__ __launch_bounds__(64 * HD_WPB, 2) void dft_h3_kernel(
     const int b = blockIdx.x, zg = blockIdx.y;
     if (active && !active[b]) return;
     const int nz = L.nz;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wavefront index is uniform (readfirstlane makes that visible to the compiler, so node
+    // indices and the pointers built from them stay in scalar registers)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     const int l16 = lane & 15, kg = lane >> 4;
     extern __shared__ double hd_smem[];
     float *sr = reinterpret_cast<float *>(hd_smem) + (size_t)wave * 2 * HD_PLANE;
@@ -180,17 +223,17 @@ __global__ __launch_bounds__(64 * HD_WPB, 2) void dft_h3_kernel(
     hh8 frh[2], frl[2], fih[2], fil[2], nfih[2], nfil[2];
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj) {
-        float vr[8], vi[8], ni[8];
+        float vr[8], vi[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             const cx<float> f = Fx[(8 * kg + t) * HD_NP + 16 * tj + l16];
-            vr[t] = ldexpf(f.re, HD_FEXP);
-            vi[t] = ldexpf(f.im, HD_FEXP);
-            ni[t] = -vi[t];
+            vr[t] = f.re;
+            vi[t] = f.im;
         }
-        split8h(vr, frh[tj], frl[tj]);
-        split8h(vi, fih[tj], fil[tj]);
-        split8h(ni, nfih[tj], nfil[tj]);
+        const float fs = (float)(1 << HD_FEXP);
+        split8h(vr, fs, frh[tj], frl[tj]);
+        split8h(vi, fs, fih[tj], fil[tj]);
+        split8h(vi, -fs, nfih[tj], nfil[tj]);
     }
 
     const int nc = L.nc[b];
@@ -199,18 +242,24 @@ __global__ __launch_bounds__(64 * HD_WPB, 2) void dft_h3_kernel(
     for (int j = threadIdx.x; j < nc; j += blockDim.x) s_pos[j] = L.chy[off + j] * HD_S + L.chx[off + j];
     __syncthreads();
     const int g = kg, col = l16;
-    // element offsets in 32 bits (the launcher checks); lanes past nc read the instance's
-    // first element and store nothing
-    const int row_0 = (int)off * nz;
-    auto vidx = [&](int k, int z) {
-        const int j = lane + 64 * k;
-        return row_0 + z * nc + (j < nc ? j : 0);
-    };
+    // Addressing: a uniform pointer to the node's channels (node-major: contiguous) and the
+    // lane's byte offset 8 lane + 512 (k mod 8) (32 bits; the pointer advances by 512 channels
+    // every eight blocks, so the rest fits the instructions' offset field) for channel
+    // j = lane + 64 k; blocks of 64
+    // channels inside nc take no lane mask, blocks past nc are skipped (uniform branches)
+    const size_t base0 = (size_t)off * nz;
+    const uint32_t lb = (uint32_t)lane * (uint32_t)sizeof(cx<float>);
+    auto full = [&](int k) { return 64 * k + 64 <= nc; };
+    auto part = [&](int k) { return 64 * k < nc && lane + 64 * k < nc; };
     const float xs = x_copy ? (float)(1.0 / x_norm[b]) : 1.0f;
     cx<float> xv[KC];
     auto load_x = [&](int z) {
+        const cx<float> *xz = x + base0 + (size_t)z * nc;
 #pragma unroll
-        for (int k = 0; k < KC; ++k) xv[k] = x[vidx(k, z)];
+        for (int k = 0; k < KC; ++k) {
+            if (full(k)) xv[k] = at(xz + 512 * (k / 8), lb + 512u * (k % 8));
+            else if (part(k)) xv[k] = at(xz + 512 * (k / 8), lb + 512u * (k % 8));
+        }
     };
     const int z_first = zg * zpb + wave;
     const int z_end = min(nz, (zg + 1) * zpb);
@@ -223,15 +272,15 @@ __global__ __launch_bounds__(64 * HD_WPB, 2) void dft_h3_kernel(
             for (int idx = lane; idx < N4; idx += 64) p4[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         __builtin_amdgcn_wave_barrier();
+        cx<float> *cz = x_copy ? x_copy + base0 + (size_t)z * nc : nullptr;
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
-            const int j = lane + 64 * k;
-            if (j < nc) {
-                const int p = s_pos[j];
+            if (full(k) || part(k)) {
+                const int p = s_pos[lane + 64 * k];
                 cx<float> v = xv[k];
                 if (x_copy) {
                     v = mk(v.re * xs, v.im * xs);
-                    x_copy[vidx(k, z)] = v;
+                    at(cz + 512 * (k / 8), lb + 512u * (k % 8)) = v;
                 }
                 sr[p] = v.re;
                 si[p] = v.im;
@@ -290,12 +339,12 @@ __global__ __launch_bounds__(64 * HD_WPB, 2) void dft_h3_kernel(
         __builtin_amdgcn_wave_barrier();
         store_t(sr, si, cr, ci);
         __builtin_amdgcn_wave_barrier();
+        cx<float> *yz = y + base0 + (size_t)z * nc;
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
-            const int j = lane + 64 * k;
-            if (j < nc) {
-                const int p = s_pos[j];
-                y[vidx(k, z)] = mk(ldexpf(sr[p], -ex), ldexpf(si[p], -ex));
+            if (full(k) || part(k)) {
+                const int p = s_pos[lane + 64 * k];
+                at(yz + 512 * (k / 8), lb + 512u * (k % 8)) = mk(ldexpf(sr[p], -ex), ldexpf(si[p], -ex));
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -306,8 +355,9 @@ __global__ __launch_bounds__(64 * HD_WPB, 2) void dft_h3_kernel(
 
 bool dft_h3_fits(const Layout &L, const DftTables<float> &D) {
     if (D.np != HD_NP || !L.nm || L.nkx != L.nky || L.max_nc > HD_NP * HD_NP) return false;
+    // byte offsets of the batch's FP32 vectors in 32 bits
     const long long tot = L.total > 0 ? L.total : (long long)L.max_nc * L.n_inst;
-    return (long long)L.nz * (tot + 64) < (1LL << 31);
+    return (long long)L.nz * (tot + 64) * (long long)sizeof(cx<float>) < (1LL << 32);
 }
 
 void launch_dft_coupling_h3(const Layout &L, const DftTables<float> &D, const cx<float> *x,
