@@ -26,10 +26,9 @@
 // flops per cycle per MFMA).
 #include "msx_dft.h"
 #include "msx_mfma.h"
+#include "msx_split.h"
 
 namespace msx {
-
-typedef _Float16 hh8 __attribute__((ext_vector_type(8)));
 
 namespace {
 
@@ -39,45 +38,8 @@ constexpr int HD_PLANE = HD_NP * HD_S;
 constexpr int HD_WPB = 4;        // wavefronts per workgroup
 constexpr int HD_FEXP = 15;      // F is held as 2^15 F
 
-__device__ __forceinline__ v4f mfma16(hh8 a, hh8 b, v4f c) {
+__device__ __forceinline__ v4f mfma16(h8 a, h8 b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-}
-
-// hi + lo = s v to 2^-22 relative: hi = fp16(s v), lo = fp16(s v - hi), each one mixed-
-// precision FMA that rounds once to FP16 (s is a power of two, so s v - hi is exact before
-// that rounding); pairs of elements fill the low and high halves of one register. The
-// compiler formed a multiply, a conversion back and a subtraction for some of these.
-__device__ __forceinline__ uint32_t split_hi2(float v0, float v1, float s) {
-    uint32_t r;
-    asm("v_fma_mixlo_f16 %0, %1, %2, 0\n\tv_fma_mixhi_f16 %0, %3, %2, 0"
-        : "=&v"(r) : "v"(v0), "v"(s), "v"(v1));
-    return r;
-}
-__device__ __forceinline__ uint32_t split_lo2(float v0, float v1, float s, uint32_t hi) {
-    uint32_t r;
-    asm("v_fma_mixlo_f16 %0, %1, %2, -%4 op_sel_hi:[0,0,1]\n\t"
-        "v_fma_mixhi_f16 %0, %3, %2, -%4 op_sel:[0,0,1] op_sel_hi:[0,0,1]"
-        : "=&v"(r) : "v"(v0), "v"(s), "v"(v1), "v"(hi));
-    return r;
-}
-__device__ __forceinline__ void split8h(const float (&v)[8], float s, hh8 &hi, hh8 &lo) {
-    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
-    u4 h, l;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        h[q] = split_hi2(v[2 * q], v[2 * q + 1], s);
-        l[q] = split_lo2(v[2 * q], v[2 * q + 1], s, h[q]);
-    }
-    hi = __builtin_bit_cast(hh8, h);
-    lo = __builtin_bit_cast(hh8, l);
-}
-
-// max(|a|, |b|, m) in one instruction (the abs modifiers are free; fmaxf under the IEEE mode
-// of compute kernels quiets every input first, two extra instructions per element)
-__device__ __forceinline__ float max3abs(float a, float b, float m) {
-    float r;
-    asm("v_max3_f32 %0, |%1|, |%2|, %3" : "=v"(r) : "v"(a), "v"(b), "v"(m));
-    return r;
 }
 
 // Maximum of a non-negative value over the wavefront (uniform result): non-negative floats
@@ -94,30 +56,19 @@ __device__ __forceinline__ float wave_max64(float v) {
     return __builtin_bit_cast(float, m);
 }
 
-// element at a 32-bit byte offset from a uniform base: one VGPR offset on the scalar base
-// (the launcher keeps the batch's vectors below 4 GiB)
-template <typename T>
-__device__ __forceinline__ const T &at(const T *base, uint32_t byte_off) {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
-}
-template <typename T>
-__device__ __forceinline__ T &at(T *base, uint32_t byte_off) {
-    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off);
-}
-
 // The split planes of one operand row tile: lane (i, kg) holds A[16 ti + i][8 kg .. 8 kg + 8)
 struct ATile {
-    hh8 rh, rl, ih, il;
+    h8 rh, rl, ih, il;
 };
 
 // F~ = F (CONJ false) or conj(F): C += A F~ for the 2 x 2 output tiles.
 //   plain: Cr = Ar Fr - Ai Fi, Ci = Ar Fi + Ai Fr
 //   conj:  Cr = Ar Fr + Ai Fi, Ci = Ai Fr - Ar Fi
 template <bool CONJ>
-__device__ __forceinline__ void tile_products(const ATile (&a)[2], const hh8 (&frh)[2],
-                                              const hh8 (&frl)[2], const hh8 (&fih)[2],
-                                              const hh8 (&fil)[2], const hh8 (&nfih)[2],
-                                              const hh8 (&nfil)[2], v4f (&cr)[2][2],
+__device__ __forceinline__ void tile_products(const ATile (&a)[2], const h8 (&frh)[2],
+                                              const h8 (&frl)[2], const h8 (&fih)[2],
+                                              const h8 (&fil)[2], const h8 (&nfih)[2],
+                                              const h8 (&nfil)[2], v4f (&cr)[2][2],
                                               v4f (&ci)[2][2]) {
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti)
@@ -180,8 +131,8 @@ __device__ __forceinline__ int load_split(const float *sr, const float *si, ATil
     const float sc = ldexpf(1.0f, e);
 #pragma unroll
     for (int ti = 0; ti < 2; ++ti) {
-        split8h(xr[ti], sc, a[ti].rh, a[ti].rl);
-        split8h(xi[ti], sc, a[ti].ih, a[ti].il);
+        split8s(xr[ti], sc, a[ti].rh, a[ti].rl);
+        split8s(xi[ti], sc, a[ti].ih, a[ti].il);
     }
     return e;
 }
@@ -220,7 +171,7 @@ __global__ __launch_bounds__(64 * HD_WPB, 2) void dft_h3_kernel(
 
     // B operands: lane (j, kg) of column tile tj holds 2^15 F[8 kg + t][16 tj + j], t = 0..7,
     // as hi and lo, and the negated imaginary planes
-    hh8 frh[2], frl[2], fih[2], fil[2], nfih[2], nfil[2];
+    h8 frh[2], frl[2], fih[2], fil[2], nfih[2], nfil[2];
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj) {
         float vr[8], vi[8];
@@ -231,9 +182,9 @@ __global__ __launch_bounds__(64 * HD_WPB, 2) void dft_h3_kernel(
             vi[t] = f.im;
         }
         const float fs = (float)(1 << HD_FEXP);
-        split8h(vr, fs, frh[tj], frl[tj]);
-        split8h(vi, fs, fih[tj], fil[tj]);
-        split8h(vi, -fs, nfih[tj], nfil[tj]);
+        split8s(vr, fs, frh[tj], frl[tj]);
+        split8s(vi, fs, fih[tj], fil[tj]);
+        split8s(vi, -fs, nfih[tj], nfil[tj]);
     }
 
     const int nc = L.nc[b];

@@ -40,10 +40,12 @@
 // clamped to row nz - 1. P2 lo (and P1 lo when LDS is short) is read from L2, a z group ahead.
 #include "msx_internal.h"
 #include "msx_mfma.h"
+#include "msx_split.h"
+
+#include <type_traits>
 
 namespace msx {
 
-typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 
 constexpr int H3_WAVES = 8;
 constexpr int H3_ZG = 2;  // z tiles per group of the second product
@@ -59,17 +61,6 @@ __device__ __forceinline__ float h3_scale(float m) {
     int e = 0;
     (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
     return ldexpf(1.0f, 15 - max(e, -100));
-}
-
-// hi + lo = v to 2^-22 relative: hi = fp16(v) (round to nearest), lo = fp16(v - hi) (v - hi is
-// exact in FP32).
-__device__ __forceinline__ void split8(const float (&v)[8], h8 &hi, h8 &lo) {
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-        const _Float16 h = (_Float16)v[u];
-        hi[u] = h;
-        lo[u] = (_Float16)(v[u] - (float)h);
-    }
 }
 
 __device__ __forceinline__ float wave_max4(float v) {
@@ -153,63 +144,75 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
     // tile t to wavefront slot t: the first gridDim.x tiles to wavefront 0 of every
     // workgroup, the next to wavefront 1, ...: a launch with few active tiles gives each its
     // own SIMD
-#pragma unroll 1
-    for (int t = wave * gridDim.x + blockIdx.x; t < n_tiles; t += gridDim.x * H3_WAVES) {
+    const int t_step = gridDim.x * H3_WAVES;
+
+    // One tile: its instance's channel count and this lane's channel; element indices in 32
+    // bits (the launcher checks that the batch's vectors fit), nodes past nz clamped to the
+    // channel's last node (a clamp of the index, which depends on the tile, rather than of z,
+    // which the compiler would hoist)
+    struct Tile {
+        int nc, cidx, vb, vlast;
+        bool live;
+        double ej;
+        cx<float> cj;
+        float facN;
+    };
+    float xr[KT][8], xi[KT][8];
+    // Sets up tile t and issues its loads: the lane's channel scalars and X at z = 32 s + 8 g + u
+    // (past nz: the last node, which P1's zero columns cancel). The loads of the next tile are
+    // issued from inside the current one, after its last products and before its last stores,
+    // so that their wait includes neither those stores nor the setup.
+    auto begin_tile = [&](int t, Tile &T) {
         int lo = 0, hi = n_inst - 1;  // last instance with tile_start <= t
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
             if (tile_start[mid] <= t) lo = mid;
             else hi = mid - 1;
         }
-        const int b = lo;
-        const int j0 = (t - tile_start[b]) * 16;
-        const int nc = inst_nc[b];
-        if (j0 >= nc) continue;  // never taken
+        const int j0 = (t - tile_start[lo]) * 16;
+        T.nc = inst_nc[lo];
         const int jc = j0 + l16;
-        const bool live = jc < nc;
-        // element indices in 32 bits (the launcher checks that the batch's vectors fit);
-        // nodes past nz read the channel's last node instead (a clamp of the index, which
-        // depends on the tile, rather than of z, which the compiler would hoist)
-        const int cidx = inst_off[b] + (live ? jc : 0);
-        const int vb = inst_off[b] * nz + (live ? jc : 0);
-        const int vlast = vb + (nz - 1) * nc;
-        auto vi = [&](int z) -> int { return min(vb + z * nc, vlast); };
-        const double ej = e[cidx];
-        const cx<float> cj = c[cidx];
-        const float facN = fac[vlast];
-
-        // ---- the tile's X, z = 32 s + 8 g + u (past nz: the last node, which P1's zero
-        // columns cancel), column maxima and split ----
-        float xr[KT][8], xi[KT][8];
+        T.live = jc < T.nc;
+        T.cidx = inst_off[lo] + (T.live ? jc : 0);
+        T.vb = inst_off[lo] * nz + (T.live ? jc : 0);
+        T.vlast = T.vb + (nz - 1) * T.nc;
+        T.ej = e[T.cidx];
+        T.cj = c[T.cidx];
+        T.facN = fac[T.vlast];
 #pragma unroll
         for (int s = 0; s < KT; ++s)
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int z = 32 * s + 8 * g + u;
-                const cx<float> v = x[vi(z)];
+                const cx<float> v = x[min(T.vb + z * T.nc, T.vlast)];
                 xr[s][u] = v.re;
                 xi[s][u] = v.im;
             }
+    };
+
+    int t = wave * gridDim.x + blockIdx.x;
+    Tile cur, nxt;
+    if (t < n_tiles) begin_tile(t, cur);
+#pragma unroll 1
+    while (t < n_tiles) {
+        const int t_next = t + t_step;
+        auto vi = [&](int z) -> int { return min(cur.vb + z * cur.nc, cur.vlast); };
+
+        // ---- column maxima and split of the tile's X ----
         float mr = 0.f, mi = 0.f;
 #pragma unroll
         for (int s = 0; s < KT; ++s)
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                mr = fmaxf(mr, fabsf(xr[s][u]));
-                mi = fmaxf(mi, fabsf(xi[s][u]));
+            for (int u = 0; u < 8; u += 2) {
+                mr = max3abs(xr[s][u], xr[s][u + 1], mr);
+                mi = max3abs(xi[s][u], xi[s][u + 1], mi);
             }
         const float sxr = h3_scale(wave_max4(mr)), sxi = h3_scale(wave_max4(mi));
         h8 xhr[KT], xlr[KT], xhi[KT], xli[KT];
 #pragma unroll
         for (int s = 0; s < KT; ++s) {
-            float ar[8], ai[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                ar[u] = xr[s][u] * sxr;
-                ai[u] = xi[s][u] * sxi;
-            }
-            split8(ar, xhr[s], xlr[s]);
-            split8(ai, xhi[s], xli[s]);
+            split8s(xr[s], sxr, xhr[s], xlr[s]);
+            split8s(xi[s], sxi, xhi[s], xli[s]);
         }
 
         // ---- W^T = Q^T X^T ----
@@ -249,27 +252,31 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
         const float ixr = 1.0f / (sxr * H3_QSCALE), ixi = 1.0f / (sxi * H3_QSCALE);
         float yn0 = 0.f, yn1 = 0.f, wr = 0.f, wi = 0.f;
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
+        for (int mt = 0; mt < MT; ++mt) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int m = 16 * mt + 4 * g + v;
                 // unconditional loads (lam is 0 past nz), then a select: a condition around
                 // the load made every element a branch
-                const float r = __builtin_amdgcn_rcpf((float)(ej + lam[m]));
+                const float r = __builtin_amdgcn_rcpf((float)(cur.ej + lam[m]));
                 const float sc = m < nz ? r : 0.f;
                 const float q = qn[m];
                 acc1[mt][0][v] *= sc * ixr;
                 acc1[mt][1][v] *= sc * ixi;
                 yn0 += q * acc1[mt][0][v];
                 yn1 += q * acc1[mt][1][v];
-                wr = fmaxf(wr, fabsf(acc1[mt][0][v]));
-                wi = fmaxf(wi, fabsf(acc1[mt][1][v]));
             }
+#pragma unroll
+            for (int v = 0; v < 4; v += 2) {
+                wr = max3abs(acc1[mt][0][v], acc1[mt][0][v + 1], wr);
+                wi = max3abs(acc1[mt][1][v], acc1[mt][1][v + 1], wi);
+            }
+        }
         yn0 = wave_sum4(yn0);
         yn1 = wave_sum4(yn1);
         // D^-1 x = D0^-1 x + corr f_j, corr = c y0(N) / (1 - c f_j(N))
         cx<float> corr = mk(0.f, 0.f);
-        if (live) corr = cdiv(mk(yn0, yn1) * cj, mk(1.0f, 0.0f) - cj * facN);
+        if (cur.live) corr = cdiv(mk(yn0, yn1) * cur.cj, mk(1.0f, 0.0f) - cur.cj * cur.facN);
         const float swr = h3_scale(wave_max4(wr)), swi = h3_scale(wave_max4(wi));
         h8 whr[KT], wlr[KT], whi[KT], wli[KT];
 #pragma unroll
@@ -277,13 +284,13 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
             float ar[8], ai[8];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                ar[v] = acc1[2 * p][0][v] * swr;
-                ar[4 + v] = acc1[2 * p + 1][0][v] * swr;
-                ai[v] = acc1[2 * p][1][v] * swi;
-                ai[4 + v] = acc1[2 * p + 1][1][v] * swi;
+                ar[v] = acc1[2 * p][0][v];
+                ar[4 + v] = acc1[2 * p + 1][0][v];
+                ai[v] = acc1[2 * p][1][v];
+                ai[4 + v] = acc1[2 * p + 1][1][v];
             }
-            split8(ar, whr[p], wlr[p]);
-            split8(ai, whi[p], wli[p]);
+            split8s(ar, swr, whr[p], wlr[p]);
+            split8s(ai, swi, whi[p], wli[p]);
         }
         const float iwr = 1.0f / (swr * H3_QSCALE), iwi = 1.0f / (swi * H3_QSCALE);
 
@@ -301,9 +308,10 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
                     if constexpr (BASE) bz[zg][v] = base[vi(z)];
                 }
         };
-        load_group(0);
-#pragma unroll 1
-        for (int zt0 = 0; zt0 < MT; zt0 += H3_ZG) {
+        // one group: its products, then (the last group) the next tile's loads, then its
+        // results and stores, then (the others) the next group's factor and base loads
+        auto group = [&](int zt0, auto last_tag) {
+            constexpr bool LAST = decltype(last_tag)::value;
             int zrow[H3_ZG];
 #pragma unroll
             for (int zg = 0; zg < H3_ZG; ++zg) zrow[zg] = min(16 * (zt0 + zg) + l16, nz - 1);
@@ -327,13 +335,23 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
                     acc2[zg][0] = mfma_h(a2l[p][zg], whr[p], acc2[zg][0]);
                     acc2[zg][1] = mfma_h(a2l[p][zg], whi[p], acc2[zg][1]);
                 }
-            // the next group's P2 lo fragments (past the last group: clamped rows, unused)
+            if constexpr (LAST) {
+                // after every product of the tile (the scheduler would otherwise move the
+                // loads above the products, where W and the new X would both need registers)
+                __builtin_amdgcn_sched_barrier(0);
+                // unconditional (past the last tile: the last tile again, unused), so that the
+                // current X's registers are free for the whole tile: under a condition the old
+                // values would have to survive until it
+                begin_tile(min(t_next, n_tiles - 1), nxt);
+            } else {
+                // the next group's P2 lo fragments
 #pragma unroll
-            for (int zg = 0; zg < H3_ZG; ++zg) {
-                const int zr = min(16 * (zt0 + H3_ZG + zg) + l16, nz - 1);
+                for (int zg = 0; zg < H3_ZG; ++zg) {
+                    const int zr = min(16 * (zt0 + H3_ZG + zg) + l16, nz - 1);
 #pragma unroll
-                for (int p = 0; p < KT; ++p)
-                    a2l[p][zg] = *reinterpret_cast<const h8 *>(g2l + zr * S + 32 * p + 8 * g);
+                    for (int p = 0; p < KT; ++p)
+                        a2l[p][zg] = *reinterpret_cast<const h8 *>(g2l + zr * S + 32 * p + 8 * g);
+                }
             }
             cx<float> out[H3_ZG][4];
 #pragma unroll
@@ -345,7 +363,7 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
                     out[zg][v] = BASE ? bz[zg][v] + val * sign : val;
                 }
             if (16 * (zt0 + H3_ZG) <= nz) {
-                if (live) {
+                if (cur.live) {
 #pragma unroll
                     for (int zg = 0; zg < H3_ZG; ++zg)
 #pragma unroll
@@ -357,12 +375,20 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int z = 16 * (zt0 + zg) + 4 * g + v;
-                        if (live && zt0 + zg < MT && z < nz) y[vi(z)] = out[zg][v];
+                        if (cur.live && zt0 + zg < MT && z < nz) y[vi(z)] = out[zg][v];
                     }
             }
-            load_group(zt0 + H3_ZG);  // past the last group: clamped, unused
+            if constexpr (!LAST) load_group(zt0 + H3_ZG);
+            // keeps the loads here: without a possible write after them, the compiler merges
+            // them with the loads before the loop into one load at the point of use
             asm volatile("" ::: "memory");
-        }
+        };
+        load_group(0);
+#pragma unroll 1
+        for (int zt0 = 0; zt0 < MT - H3_ZG; zt0 += H3_ZG) group(zt0, std::false_type{});
+        group(MT - H3_ZG, std::true_type{});
+        cur = nxt;
+        t = t_next;
     }
 }
 
