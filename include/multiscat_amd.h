@@ -148,8 +148,9 @@ int msx_debug_apply_coupling(int nkx, int nky, int nz, const double *potential, 
  * (multiscat/multiscat/_scipy.py:249-354). potential00: complex[nz] (V_00(z)); kinetic:
  * nz x nz row-major; states (nc, nz) row-major complex. variant 0 = matrix-core GEMMs
  * (nz <= 320), 1 = vector ALU, 2 = matrix-core GEMMs with 8 channels per wavefront (FP64,
- * nz <= 160; variant 0 takes 16 there), 3 = matrix-core GEMMs on split-FP16 operands (FP32
- * inverse only, nz <= 160); precision 0 = FP64, 1 = FP32. */
+ * nz <= 160; variant 0 takes 16 there), 3 and 4 = matrix-core GEMMs on split-FP16 operands
+ * with 16 and 8 channels per wavefront (FP32 inverse only, nz <= 160); precision 0 = FP64,
+ * 1 = FP32. */
 int msx_debug_apply_diagonal(int nz, const double *potential00, const double *kinetic, int nc,
                              const double *e, const double *wave_c, const double *state_in,
                              double *state_out, int inverse, int precision, int variant);

@@ -392,6 +392,256 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
     }
 }
 
+// The same D^-1 with 8 channels per wavefront (H8_WAVES = 12 wavefronts a workgroup, three per
+// SIMD): one 16-column tile whose column l16 is the real (l16 even) or imaginary (odd) part of
+// channel l16 / 2, so the accumulators, operands and W take half the registers of the
+// 16-channel form (the planes in LDS feed half as many products per read). Loads and stores
+// are 4-byte parts of the complex elements; the two parts of a channel meet by a lane swap for
+// the Sherman-Morrison correction.
+constexpr int H8_WAVES = 12;
+constexpr int H8_ZG = 1;
+
+template <int KT, bool P1LO, bool BASE>
+__global__ __launch_bounds__(64 * H8_WAVES) void diag_h8_kernel(
+    Layout L, H3Planes H, const double *__restrict__ lam_g, const double *__restrict__ e,
+    const cx<float> *__restrict__ c, const float *__restrict__ fac, const cx<float> *x,
+    const cx<float> *base, cx<float> *y, float sign, const int *__restrict__ active) {
+    constexpr int NQ = 32 * KT, MT = 2 * KT, S = NQ + 8;
+    extern __shared__ double h3_smem[];
+    __shared__ int wave_sum[H8_WAVES];
+    const int nz = L.nz, n_inst = L.n_inst;
+    double *lam = h3_smem;                                    // [NQ], 0 past nz
+    float *qn = reinterpret_cast<float *>(lam + NQ);           // [NQ]: Q[nz - 1][m]
+    _Float16 *p1h = reinterpret_cast<_Float16 *>(qn + NQ);    // [nz][S]
+    _Float16 *p2h = p1h + nz * S;                              // [nz][S]
+    _Float16 *p1l = p2h + nz * S;                              // [nz][S] (P1LO)
+    int *tile_start = reinterpret_cast<int *>(p1l + (P1LO ? nz * S : 0));  // [n_inst + 1]
+    int *inst_nc = tile_start + n_inst + 1;                    // [n_inst]
+    int *inst_off = inst_nc + n_inst;                          // [n_inst]
+    const _Float16 *g1l = H.planes + (size_t)NQ * S;           // P1 lo in global memory
+    const _Float16 *g2l = H.planes + (size_t)3 * NQ * S;       // P2 lo in global memory
+    {
+        // exclusive scan of the active instances' 8-channel tiles (as the 16-channel kernel)
+        const int per = (n_inst + blockDim.x - 1) / blockDim.x;
+        const int i0 = threadIdx.x * per, i1 = min(n_inst, i0 + per);
+        int local = 0;
+        for (int i = i0; i < i1; ++i) local += (active && !active[i]) ? 0 : (L.nc[i] + 7) / 8;
+        int incl = local;
+        const int ln = threadIdx.x & 63;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o);
+            if (ln >= o) incl += t;
+        }
+        if (ln == 63) wave_sum[threadIdx.x >> 6] = incl;
+        __syncthreads();
+        int before = 0;
+        for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) before += wave_sum[w];
+        int run = before + incl - local;
+        for (int i = i0; i < i1; ++i) {
+            const int nci = L.nc[i];
+            tile_start[i] = run;
+            inst_nc[i] = nci;
+            inst_off[i] = (int)L.off[i];
+            run += (active && !active[i]) ? 0 : (nci + 7) / 8;
+        }
+        if (threadIdx.x == blockDim.x - 1) tile_start[n_inst] = before + incl;
+    }
+    for (int m = threadIdx.x; m < NQ; m += blockDim.x) {
+        lam[m] = m < nz ? lam_g[m] : 0.0;
+        qn[m] = m < nz ? H.qlast[m] : 0.0f;
+    }
+    {
+        const int n16 = nz * S / 8;
+        const uint4 *s1h = reinterpret_cast<const uint4 *>(H.planes);
+        const uint4 *s1l = reinterpret_cast<const uint4 *>(g1l);
+        const uint4 *s2h = reinterpret_cast<const uint4 *>(H.planes + (size_t)2 * NQ * S);
+        for (int i = threadIdx.x; i < n16; i += blockDim.x) {
+            reinterpret_cast<uint4 *>(p1h)[i] = s1h[i];
+            reinterpret_cast<uint4 *>(p2h)[i] = s2h[i];
+            if (P1LO) reinterpret_cast<uint4 *>(p1l)[i] = s1l[i];
+        }
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int l16 = lane & 15, g = lane >> 4;
+    const int jl = l16 >> 1, part = l16 & 1;  // column l16: channel jl's real or imaginary part
+    const int n_tiles = tile_start[n_inst];
+    const float *xf = reinterpret_cast<const float *>(x);
+    const float *bf = reinterpret_cast<const float *>(base);
+    float *yf = reinterpret_cast<float *>(y);
+#pragma unroll 1
+    for (int t = wave * gridDim.x + blockIdx.x; t < n_tiles; t += gridDim.x * H8_WAVES) {
+        int lo = 0, hi = n_inst - 1;  // last instance with tile_start <= t
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tile_start[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const int nc = inst_nc[lo];
+        const int jc = (t - tile_start[lo]) * 8 + jl;
+        const bool live = jc < nc;
+        const int cidx = inst_off[lo] + (live ? jc : 0);
+        const int vb = inst_off[lo] * nz + (live ? jc : 0);
+        const int vlast = vb + (nz - 1) * nc;
+        // float index of this lane's part of element (channel, z); nodes past nz: the last node
+        auto vf = [&](int z) -> int { return 2 * min(vb + z * nc, vlast) + part; };
+        const double ej = e[cidx];
+        const cx<float> cj = c[cidx];
+        const float facN = fac[vlast];
+
+        // ---- the tile's X (this lane's part), z = 32 s + 8 g + u; column maximum, split ----
+        float xv[KT][8];
+#pragma unroll
+        for (int s = 0; s < KT; ++s)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) xv[s][u] = xf[vf(32 * s + 8 * g + u)];
+        float mx = 0.f;
+#pragma unroll
+        for (int s = 0; s < KT; ++s)
+#pragma unroll
+            for (int u = 0; u < 8; u += 2) mx = max3abs(xv[s][u], xv[s][u + 1], mx);
+        const float sx = h3_scale(wave_max4(mx));
+        h8 xh[KT], xl[KT];
+#pragma unroll
+        for (int s = 0; s < KT; ++s) split8s(xv[s], sx, xh[s], xl[s]);
+
+        // ---- W^T = Q^T X^T ----
+        v4f acc1[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) acc1[mt] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < KT; ++s)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int row = min(16 * mt + l16, nz - 1);
+                const h8 ah = *reinterpret_cast<const h8 *>(p1h + row * S + 32 * s + 8 * g);
+                const h8 al = P1LO ? *reinterpret_cast<const h8 *>(p1l + row * S + 32 * s + 8 * g)
+                                   : *reinterpret_cast<const h8 *>(g1l + row * S + 32 * s + 8 * g);
+                acc1[mt] = mfma_h(ah, xh[s], acc1[mt]);
+                acc1[mt] = mfma_h(ah, xl[s], acc1[mt]);
+                acc1[mt] = mfma_h(al, xh[s], acc1[mt]);
+            }
+
+        // P2 lo fragments of the first z group, read during the scaling and the split of W
+        h8 a2l[KT][H8_ZG];
+#pragma unroll
+        for (int zg = 0; zg < H8_ZG; ++zg) {
+            const int zr = min(16 * zg + l16, nz - 1);
+#pragma unroll
+            for (int p = 0; p < KT; ++p)
+                a2l[p][zg] = *reinterpret_cast<const h8 *>(g2l + zr * S + 32 * p + 8 * g);
+        }
+
+        // ---- W' = W / (e_j + lam_m) with the column scale removed; y0(N) = Q[N-1][:] W' ----
+        const float ix = 1.0f / (sx * H3_QSCALE);
+        float yn = 0.f, wm = 0.f;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = 16 * mt + 4 * g + v;
+                const float r = __builtin_amdgcn_rcpf((float)(ej + lam[m]));
+                const float sc = m < nz ? r : 0.f;
+                acc1[mt][v] *= sc * ix;
+                yn += qn[m] * acc1[mt][v];
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v += 2) wm = max3abs(acc1[mt][v], acc1[mt][v + 1], wm);
+        }
+        yn = wave_sum4(yn);
+        // the channel's other part sits in the neighbouring lane (l16 ^ 1)
+        const float yo = __int_as_float(
+            __builtin_amdgcn_mov_dpp(__float_as_int(yn), 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
+        const cx<float> y0 = part ? mk(yo, yn) : mk(yn, yo);
+        // D^-1 x = D0^-1 x + corr f_j, corr = c y0(N) / (1 - c f_j(N)); this lane's part of it
+        cx<float> corr = mk(0.f, 0.f);
+        if (live) corr = cdiv(y0 * cj, mk(1.0f, 0.0f) - cj * facN);
+        const float cp = part ? corr.im : corr.re;
+        const float sw = h3_scale(wave_max4(wm));
+        h8 wh[KT], wl[KT];
+#pragma unroll
+        for (int p = 0; p < KT; ++p) {
+            float a[8];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                a[v] = acc1[2 * p][v];
+                a[4 + v] = acc1[2 * p + 1][v];
+            }
+            split8s(a, sw, wh[p], wl[p]);
+        }
+        const float iw = 1.0f / (sw * H3_QSCALE);
+
+        // ---- Y^T = Q W' by groups of H8_ZG z tiles; factors and base parts one group ahead
+        // at clamped addresses (unused values are never stored) ----
+        float fz[H8_ZG][4], bz[H8_ZG][4];
+        auto load_group = [&](int zt) {
+#pragma unroll
+            for (int zg = 0; zg < H8_ZG; ++zg)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int z = 16 * (zt + zg) + 4 * g + v;
+                    fz[zg][v] = fac[min(vb + z * nc, vlast)];
+                    if constexpr (BASE) bz[zg][v] = bf[vf(z)];
+                }
+        };
+        load_group(0);
+#pragma unroll 1
+        for (int zt0 = 0; zt0 < MT; zt0 += H8_ZG) {
+            int zrow[H8_ZG];
+#pragma unroll
+            for (int zg = 0; zg < H8_ZG; ++zg) zrow[zg] = min(16 * (zt0 + zg) + l16, nz - 1);
+            v4f acc2[H8_ZG];
+#pragma unroll
+            for (int zg = 0; zg < H8_ZG; ++zg) acc2[zg] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int p = 0; p < KT; ++p)
+#pragma unroll
+                for (int zg = 0; zg < H8_ZG; ++zg) {
+                    const h8 ah = *reinterpret_cast<const h8 *>(p2h + zrow[zg] * S + 32 * p + 8 * g);
+                    acc2[zg] = mfma_h(ah, wh[p], acc2[zg]);
+                    acc2[zg] = mfma_h(ah, wl[p], acc2[zg]);
+                    acc2[zg] = mfma_h(a2l[p][zg], wh[p], acc2[zg]);
+                }
+            // the next group's P2 lo fragments (past the last group: clamped rows, unused)
+#pragma unroll
+            for (int zg = 0; zg < H8_ZG; ++zg) {
+                const int zr = min(16 * (zt0 + H8_ZG + zg) + l16, nz - 1);
+#pragma unroll
+                for (int p = 0; p < KT; ++p)
+                    a2l[p][zg] = *reinterpret_cast<const h8 *>(g2l + zr * S + 32 * p + 8 * g);
+            }
+            float out[H8_ZG][4];
+#pragma unroll
+            for (int zg = 0; zg < H8_ZG; ++zg)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const float val = fmaf(cp, fz[zg][v], acc2[zg][v] * iw);
+                    out[zg][v] = BASE ? fmaf(sign, val, bz[zg][v]) : val;
+                }
+            if (16 * (zt0 + H8_ZG) <= nz) {
+                if (live) {
+#pragma unroll
+                    for (int zg = 0; zg < H8_ZG; ++zg)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) yf[vf(16 * (zt0 + zg) + 4 * g + v)] = out[zg][v];
+                }
+            } else {
+#pragma unroll
+                for (int zg = 0; zg < H8_ZG; ++zg)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int z = 16 * (zt0 + zg) + 4 * g + v;
+                        if (live && zt0 + zg < MT && z < nz) yf[vf(z)] = out[zg][v];
+                    }
+            }
+            load_group(zt0 + H8_ZG);  // past the last group: clamped, unused
+            // keeps the loads here (without a possible write after them, the compiler merges
+            // them with the loads before the loop into one load at the point of use)
+            asm volatile("" ::: "memory");
+        }
+    }
+}
+
 // Planes from the FP64 eigenvectors Q[z][m] (nz x nz row-major): 2^15 Q as P1 hi, P1 lo,
 // P2 hi, P2 lo, each nq rows of stride nq + 8 halves, zero outside the nz x nz block.
 __global__ void h3_planes_kernel(int nz, int nq, const double *__restrict__ Q, _Float16 *planes) {
@@ -451,7 +701,17 @@ int cu_count() {
 template <int KT, bool P1LO, bool BASE>
 void launch_h3(const Layout &L, const H3Planes &H, const double *lam, const double *e,
                const cx<float> *c, const float *fac, const cx<float> *x, const cx<float> *base,
-               cx<float> *y, float sign, const int *active, size_t shm, hipStream_t s) {
+               cx<float> *y, float sign, const int *active, size_t shm, hipStream_t s, int cpw) {
+    if (cpw == 8) {
+        set_dynamic_lds((const void *)diag_h8_kernel<KT, P1LO, BASE>, shm);
+        const long long tiles_max = (long long)L.n_inst * ((L.max_nc + 7) / 8);
+        const int grid = (int)std::min<long long>(cu_count(), (tiles_max + H8_WAVES - 1) / H8_WAVES);
+        hipLaunchKernelGGL((diag_h8_kernel<KT, P1LO, BASE>), dim3(std::max(grid, 1)),
+                           dim3(64 * H8_WAVES), shm, s, L, H, lam, e, c, fac, x, base, y, sign,
+                           active);
+        MSX_LAUNCH_CHECK();
+        return;
+    }
     set_dynamic_lds((const void *)diag_h3_kernel<KT, P1LO, BASE>, shm);
     const long long tiles_max = (long long)L.n_inst * ((L.max_nc + 15) / 16);
     const int grid = (int)std::min<long long>(cu_count(), (tiles_max + H3_WAVES - 1) / H3_WAVES);
@@ -472,19 +732,21 @@ bool diag_h3_fits(const Layout &L, const H3Planes &H) {
 void launch_diag_h3(const Layout &L, const H3Planes &H, const double *lam, const double *e,
                     const cx<float> *c, const float *fac, const cx<float> *x,
                     const cx<float> *base, cx<float> *y, float sign, const int *active,
-                    hipStream_t s) {
+                    hipStream_t s, int cpw) {
     if (!diag_h3_fits(L, H))
         throw Error(3, "split-FP16 diagonal blocks: shape or LDS size outside the kernel");
+    if (cpw != 0 && cpw != 8 && cpw != 16)
+        throw Error(3, "split-FP16 diagonal blocks: 8 or 16 channels per wavefront");
     const bool p1lo = h3_lds_bytes(H.nq, L.nz, L.n_inst, true) <= lds_limit();
     const size_t shm = h3_lds_bytes(H.nq, L.nz, L.n_inst, p1lo);
 #define MSX_H3(KT_)                                                                            \
     if (H.nq == 32 * KT_) {                                                                    \
         if (p1lo) {                                                                            \
-            if (base) return launch_h3<KT_, true, true>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s); \
-            return launch_h3<KT_, true, false>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s);         \
+            if (base) return launch_h3<KT_, true, true>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s, cpw); \
+            return launch_h3<KT_, true, false>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s, cpw);         \
         }                                                                                      \
-        if (base) return launch_h3<KT_, false, true>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s);    \
-        return launch_h3<KT_, false, false>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s);            \
+        if (base) return launch_h3<KT_, false, true>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s, cpw);    \
+        return launch_h3<KT_, false, false>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s, cpw);            \
     }
     MSX_H3(1) MSX_H3(2) MSX_H3(3) MSX_H3(4) MSX_H3(5)
 #undef MSX_H3

@@ -208,11 +208,12 @@ size_t h3_plane_halves(int nq);
 void launch_h3_planes(int nz, int nq, const double *Q, _Float16 *planes, hipStream_t s);
 // Whether the split-FP16 kernel takes this layout (node-major, nz <= 160, LDS large enough).
 bool diag_h3_fits(const Layout &L, const H3Planes &H);
-// y = D^-1 x, or y = base + sign D^-1 x, node-major FP32 vectors and factors.
+// y = D^-1 x, or y = base + sign D^-1 x, node-major FP32 vectors and factors; cpw: channels
+// per wavefront, 16 (or 0) or 8.
 void launch_diag_h3(const Layout &L, const H3Planes &H, const double *lam, const double *e,
                     const cx<float> *c, const float *fac, const cx<float> *x,
                     const cx<float> *base, cx<float> *y, float sign, const int *active,
-                    hipStream_t s);
+                    hipStream_t s, int cpw = 0);
 
 // MFMA form of the diagonal blocks (inverse = 1: D^-1, inverse = 0: D), nz <= 320.
 // channels_per_wave (FP64, nzp <= 160): 8 or 16 channels per wavefront, 0 = the built-in 16.

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <sstream>
 #include <memory>
 #include <cstdlib>
 #include <fstream>
@@ -566,6 +567,7 @@ struct msx_batch {
         int dft_npw32 = 0, dft_npw64 = 0;  // DFT coupling: nodes per wavefront (0: built-in 4)
         long long chunk32 = 2048, chunk64 = 2048;  // Arnoldi vector kernels: elements per block
         int diag64_cpw = 0;                // FP64 diagonal blocks: channels per wavefront
+        int diag32_cpw = 0;                // split-FP16 FP32 D^-1: channels per wavefront (0: 16)
         bool done = false;
         std::string key, report;
     } tuning;
@@ -686,11 +688,11 @@ struct Ops {
                                 const void *c) const {
         char buf[512];
         std::snprintf(buf, sizeof buf,
-                      "%s|%p|%p|%p|V%p|Vf%p|D%p|nm%d|neu%d|ord%d|dir%d|vd%d|h3%d|npw%d,%d|cpw%d|fac%p",
+                      "%s|%p|%p|%p|V%p|Vf%p|D%p|nm%d|neu%d|ord%d|dir%d|vd%d|h3%d|npw%d,%d|cpw%d,%d|fac%p",
                       what, a, b, c, (const void *)V, (const void *)Vf, (const void *)dft,
                       L32.nm, neumann ? 1 : 0, order, direct ? 1 : 0, vector_diag ? 1 : 0,
                       split_f16 ? 1 : 0,
-                      B.tuning.dft_npw32, B.tuning.dft_npw64, B.tuning.diag64_cpw,
+                      B.tuning.dft_npw32, B.tuning.dft_npw64, B.tuning.diag64_cpw, B.tuning.diag32_cpw,
                       (const void *)(L32.nm ? B.facf_nm_d.p : B.facf_d.p));
         return buf;
     }
@@ -803,7 +805,7 @@ struct Ops {
             if (mfma && split_f16 && diag_h3_fits(lay<float>(), t.h3())) {
                 ProfScope ps("diag_h3_f32", s, fl, by);
                 launch_diag_h3(lay<float>(), t.h3(), t.lam.p, B.e_d.p, c<float>(), fac<float>(), in,
-                               base, out, sign, act, s);
+                               base, out, sign, act, s, B.tuning.diag32_cpw);
                 return;
             }
             if (split_f16_required)
@@ -986,7 +988,15 @@ void read_tune_file() {
     std::ifstream in(path);
     std::string key;
     msx_batch::Tuning t;
-    while (in >> key >> t.dft_npw32 >> t.dft_npw64 >> t.chunk32 >> t.chunk64 >> t.diag64_cpw) {
+    // one shape a line: key and the choices; a line without the last field (written before
+    // it existed) leaves it at its built-in value
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        msx_batch::Tuning t;
+        if (!(ls >> key >> t.dft_npw32 >> t.dft_npw64 >> t.chunk32 >> t.chunk64 >> t.diag64_cpw))
+            continue;
+        if (!(ls >> t.diag32_cpw)) t.diag32_cpw = 0;
         t.done = true;
         t.key = key;
         g_tune_cache[key] = t;
@@ -998,7 +1008,7 @@ void append_tune_file(const msx_batch::Tuning &t) {
     if (!path || !*path) return;
     std::ofstream out(path, std::ios::app);
     out << t.key << ' ' << t.dft_npw32 << ' ' << t.dft_npw64 << ' ' << t.chunk32 << ' '
-        << t.chunk64 << ' ' << t.diag64_cpw << '\n';
+        << t.chunk64 << ' ' << t.diag64_cpw << ' ' << t.diag32_cpw << '\n';
 }
 
 // Index of the shape to keep among measured times ts: the built-in one (index def) unless
@@ -1104,7 +1114,8 @@ void autotune(msx_batch &B, Ops &ops, const msx_solve_options &o) {
                               std::to_string(B.tuning.dft_npw64) + ", \"arnoldi_block_f32\": " +
                               std::to_string(B.tuning.chunk32) + ", \"arnoldi_block_f64\": " +
                               std::to_string(B.tuning.chunk64) + ", \"diag_f64_channels_per_wave\": " +
-                              std::to_string(B.tuning.diag64_cpw) + "}";
+                              std::to_string(B.tuning.diag64_cpw) + ", \"diag_split_channels_per_wave\": " +
+                              std::to_string(B.tuning.diag32_cpw ? B.tuning.diag32_cpw : 16) + "}";
             return;
         }
     }
@@ -1162,13 +1173,33 @@ void autotune(msx_batch &B, Ops &ops, const msx_solve_options &o) {
         T.diag64_cpw = cpw_c[pick_with_margin(ts, 1)];
         rep += "}, ";
     }
+    // the split-FP16 FP32 D^-1 (mixed precision, node-major vectors): 16 or 8 channels per
+    // wavefront (two or four wavefronts per SIMD)
+    if (mixed && ops.split_f16 && !ops.vector_diag && ops.L32.nm &&
+        diag_h3_fits(ops.L32, B.t->h3())) {
+        msx_batch::need(B.f_tmp1, n);
+        msx_batch::need(B.f_tmp2, n);
+        const int cpw_c[] = {16, 8};
+        std::vector<double> ts;
+        rep += "\"diag_split_us_f32\": {";
+        for (int i = 0; i < 2; ++i) {
+            ts.push_back(tm.best_us([&] {
+                launch_diag_h3(ops.L32, B.t->h3(), B.t->lam.p, B.e_d.p, B.cf_d.p, B.facf_nm_d.p,
+                               B.f_tmp1.p, nullptr, B.f_tmp2.p, 1.0f, nullptr, s, cpw_c[i]);
+            }));
+            rep += (i ? ", \"" : "\"") + std::to_string(cpw_c[i]) + "\": " + std::to_string(ts[i]);
+        }
+        T.diag32_cpw = cpw_c[pick_with_margin(ts, 0)];
+        rep += "}, ";
+    }
     if (mixed) T.chunk32 = tune_chunk<float>(B, tm, rep);
     else T.chunk64 = tune_chunk<double>(B, tm, rep);
     rep += "\"dft_nodes_per_wave_f32\": " + std::to_string(T.dft_npw32) +
            ", \"dft_nodes_per_wave_f64\": " + std::to_string(T.dft_npw64) +
            ", \"arnoldi_block_f32\": " + std::to_string(T.chunk32) +
            ", \"arnoldi_block_f64\": " + std::to_string(T.chunk64) +
-           ", \"diag_f64_channels_per_wave\": " + std::to_string(T.diag64_cpw) + "}";
+           ", \"diag_f64_channels_per_wave\": " + std::to_string(T.diag64_cpw) +
+           ", \"diag_split_channels_per_wave\": " + std::to_string(T.diag32_cpw ? T.diag32_cpw : 16) + "}";
     T.done = true;
     T.report = rep;
     B.tuning = T;
@@ -2161,11 +2192,14 @@ int msx_debug_apply_diagonal(int nz, const double *potential00, const double *ki
         if (nz <= 0 || nc <= 0) return MSX_ERR_DIMS;
         if (!potential00 || !kinetic || !e || !wave_c || !state_in || !state_out)
             throw Error(MSX_ERR_INVALID, "null argument");
-        if (precision < 0 || precision > 1 || variant < 0 || variant > 3 || inverse < 0 ||
+        if (precision < 0 || precision > 1 || variant < 0 || variant > 4 || inverse < 0 ||
             inverse > 1)
             throw Error(MSX_ERR_INVALID, "bad inverse/precision/variant");
+        // variants 3 and 4: split FP16 with 16 and 8 channels per wavefront
+        const int h3_cpw = variant == 4 ? 8 : 16;
+        if (variant == 4) variant = 3;
         if (variant == 3 && (precision != 1 || inverse != 1))
-            throw Error(MSX_ERR_INVALID, "variant 3 (split FP16) is the FP32 inverse only");
+            throw Error(MSX_ERR_INVALID, "variants 3, 4 (split FP16) are the FP32 inverse only");
         if (variant == 3 && h3_padded_size(nz) == 0)
             throw Error(MSX_ERR_INVALID, "split-FP16 diagonal blocks need nz <= 160");
         // variant 2: the matrix-core form with 8 channels per wavefront (FP64)
@@ -2219,7 +2253,7 @@ int msx_debug_apply_diagonal(int nz, const double *potential00, const double *ki
             if (!diag_h3_fits(Ln, t.h3()))
                 throw Error(MSX_ERR_INVALID, "split-FP16 diagonal blocks: shape outside the kernel");
             launch_diag_h3(Ln, t.h3(), t.lam.p, e_d.p, cf.p, facn.p, xf.p, nullptr, yf.p, 1.0f,
-                           nullptr, s);
+                           nullptr, s, h3_cpw);
             yf.download(hx.data(), n, s);
             MSX_HIP(hipStreamSynchronize(s));
             for (int j = 0; j < nc; ++j)

@@ -83,14 +83,14 @@ def test_split_fp16_diagonal_blocks(nz, nc):
         ref[j] = np.linalg.solve(D, x[j])
     cond = max(np.linalg.cond(D) for D in Ds)
     out = {}
-    for variant in (0, 3):
+    for variant in (0, 3, 4):  # FP32 operands; split FP16 with 16 and 8 channels per wavefront
         y = np.zeros_like(x)
         rc = _lib.lib().msx_debug_apply_diagonal(nz, _lib.ptr(V00), _lib.ptr(T), nc, _lib.ptr(e),
                                                  _lib.ptr(c), _lib.ptr(x), _lib.ptr(y), 1, 1,
                                                  variant)
         _lib.check(rc, "msx_debug_apply_diagonal")
         out[variant] = y
-    assert not out[3][nc // 2].any()
+    assert not out[3][nc // 2].any() and not out[4][nc // 2].any()
     fwd = {}
     for variant, y in out.items():
         per = [np.abs(y[j] - ref[j]).max() / np.abs(ref[j]).max()
@@ -102,6 +102,7 @@ def test_split_fp16_diagonal_blocks(nz, nc):
         assert bwd < 1e-5, (variant, bwd)
         assert fwd[variant] < 2e-5 * max(1.0, cond / 1e7), (variant, fwd[variant], cond)
     assert fwd[3] < 4.0 * fwd[0] + 1e-6, fwd
+    assert fwd[4] < 4.0 * fwd[0] + 1e-6, fwd
 
 
 def test_split_fp16_rejects_shapes_outside_the_kernel():
@@ -112,7 +113,7 @@ def test_split_fp16_rejects_shapes_outside_the_kernel():
     c = np.ones(nc, np.complex128)
     x = np.ones((nc, nz), np.complex128)
     y = np.zeros_like(x)
-    for prec, variant in ((1, 3), (0, 3)):
+    for prec, variant in ((1, 3), (0, 3), (1, 4)):
         rc = _lib.lib().msx_debug_apply_diagonal(nz, _lib.ptr(V00), _lib.ptr(T), nc, _lib.ptr(e),
                                                  _lib.ptr(c), _lib.ptr(x), _lib.ptr(y), 1, prec,
                                                  variant)
