@@ -221,6 +221,12 @@ int msx_table_create(int nkx, int nky, int nz, const double *potential, const do
 /* Recomputes H0 and its eigensystem from the table already in HBM (the per-potential set-up
  * of multiscat_gmres.f90:226-273, done once for every batch that shares the table). */
 int msx_table_prepare(msx_table *table);
+/* Replaces the table's potential with `potential` (complex, C order (nkx, nky, nz), natural
+ * units; host memory, or device memory of this process when on_device = 1) and prepares the
+ * table again; batches on the table use it from their next solve (the channel factors are
+ * rebuilt at every solve). For callers that update the potential in a loop (gradient descent
+ * through the autograd bridge) without a new table and batch per step. */
+int msx_table_set_potential(msx_table *table, const double *potential, int on_device);
 int msx_table_destroy(msx_table *table);
 /* Copies the eigensystem to the host: lam[nz] ascending, Q row-major (Q[z*nz + m]). */
 int msx_table_eigensystem(const msx_table *table, double *lam, double *Q);

@@ -34,7 +34,7 @@ EXPORTED = (
     "msx_debug_build_preconditioner", "msx_debug_apply_upper_block",
     "msx_debug_solve_lower_block", "msx_debug_apply_coupling", "msx_debug_apply_diagonal",
     "msx_default_options", "msx_table_create",
-    "msx_table_prepare", "msx_table_destroy", "msx_table_eigensystem", "msx_table_eig_phases",
+    "msx_table_prepare", "msx_table_set_potential", "msx_table_destroy", "msx_table_eigensystem", "msx_table_eig_phases",
     "msx_symmetric_eigensystem", "msx_symmetric_eigensystem_noalloc", "msx_batch_create",
     "msx_batch_destroy", "msx_batch_solve", "msx_batch_state", "msx_batch_set_state",
     "msx_batch_intensities",
@@ -102,6 +102,7 @@ def _declare(lib):
         "msx_default_options": (None, [C.POINTER(SolveOptions), i32]),
         "msx_table_create": (i32, [i32, i32, i32, vp, vp, pp]),
         "msx_table_prepare": (i32, [vp]),
+        "msx_table_set_potential": (i32, [vp, vp, i32]),
         "msx_table_destroy": (i32, [vp]),
         "msx_table_eigensystem": (i32, [vp, vp, vp]),
         "msx_table_eig_phases": (i32, [vp, vp]),

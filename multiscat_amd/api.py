@@ -21,7 +21,9 @@ the host. Backends:
 
 from __future__ import annotations
 
+import hashlib
 import math
+from collections import OrderedDict
 from dataclasses import dataclass, field
 from typing import Literal, Sequence
 
@@ -209,9 +211,38 @@ def _intensities_from_state(psi: np.ndarray, inputs: B.SolverInputs) -> np.ndarr
     return (np.abs(S) ** 2).astype(np.complex128)
 
 
+# Prepared tables of recent (potential, kinetic) pairs, keyed by a digest of their bytes: the
+# reference's API solves one condition per call, and a sweep of conditions over one potential
+# then computes the table's eigensystem once instead of per call.
+_TABLES: "OrderedDict[bytes, object]" = OrderedDict()
+_TABLES_KEPT = 2
+
+
+def _shared_table(V: np.ndarray, T: np.ndarray):
+    from multiscat_amd import batch as BT
+
+    V = np.ascontiguousarray(V, dtype=np.complex128)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    h = hashlib.blake2b(digest_size=20)
+    h.update(np.asarray(V.shape + T.shape, dtype=np.int64).tobytes())
+    h.update(memoryview(V).cast("B"))
+    h.update(memoryview(T).cast("B"))
+    key = h.digest()
+    table = _TABLES.get(key)
+    if table is not None:
+        _TABLES.move_to_end(key)
+        return table
+    table = BT.Table(V, T)
+    _TABLES[key] = table
+    while len(_TABLES) > _TABLES_KEPT:
+        _TABLES.popitem(last=False)[1].close()
+    return table
+
+
 class _OneCondition:
-    """A table and a one-condition batch on the GPU for `inputs` over the channel set
-    `channels` (solve order), closed on exit."""
+    """A one-condition batch on the GPU for `inputs` over the channel set `channels` (solve
+    order), on the prepared table of its potential (shared between calls); the batch is
+    closed on exit."""
 
     def __init__(self, inputs: B.SolverInputs, channels):
         from multiscat_amd import batch as BT
@@ -221,7 +252,7 @@ class _OneCondition:
             raise ValueError("n_channels too small: the specular channel is not included")
         self.channels = channels
         self.shape = inputs.potential_values.shape
-        self.table = BT.Table(inputs.potential_values, inputs.parallel_kinetic_energy)
+        self.table = _shared_table(inputs.potential_values, inputs.parallel_kinetic_energy)
         self.batch = BT.Batch(self.table, [dict(
             e=inputs.perpendicular_kinetic_difference, wave_a=inputs.wave_a,
             wave_b=inputs.wave_b, wave_c=inputs.wave_c, channels=channels)])
@@ -231,7 +262,6 @@ class _OneCondition:
 
     def __exit__(self, *exc):
         self.batch.close()
-        self.table.close()
 
     def restrict(self, full: np.ndarray) -> np.ndarray:
         return np.asarray(full, dtype=np.complex128).reshape(self.shape)[self.channels]

@@ -10,6 +10,7 @@ adjoint solve gives d(loss)/dV_G(z). Channel subsets follow the reference's SciP
 from __future__ import annotations
 
 import ctypes as C
+import sys
 from dataclasses import dataclass
 
 import numpy as np
@@ -116,6 +117,26 @@ class Table:
 
     def prepare(self) -> None:
         _lib.check(_lib.lib().msx_table_prepare(self._h), "msx_table_prepare")
+
+    def set_potential(self, potential) -> None:
+        """Replace V (same shape) and prepare the table again. `potential` is a NumPy array or
+        a complex128 torch tensor; a contiguous tensor on the GPU is read in place (after its
+        stream's pending work), anything else through host memory."""
+        torch = sys.modules.get("torch")
+        if torch is not None and isinstance(potential, torch.Tensor):
+            if tuple(potential.shape) != tuple(self.shape) or potential.dtype != torch.complex128:
+                raise ValueError(f"potential must be complex128 of shape {self.shape}")
+            if potential.is_cuda and potential.is_contiguous():
+                torch.cuda.current_stream(potential.device).synchronize()
+                _lib.check(_lib.lib().msx_table_set_potential(
+                    self._h, C.c_void_p(potential.data_ptr()), 1), "msx_table_set_potential")
+                return
+            potential = potential.detach().cpu().numpy()
+        V = np.ascontiguousarray(potential, dtype=np.complex128)
+        if V.shape != tuple(self.shape):
+            raise ValueError(f"potential must have shape {self.shape}, not {V.shape}")
+        _lib.check(_lib.lib().msx_table_set_potential(self._h, _lib.ptr(V), 0),
+                   "msx_table_set_potential")
 
     def eigensystem(self):
         nz = self.shape[2]

@@ -1614,6 +1614,21 @@ int msx_table_prepare(msx_table *t) {
     }
 }
 
+int msx_table_set_potential(msx_table *t, const double *potential, int on_device) {
+    try {
+        if (!t || !potential) throw Error(MSX_ERR_INVALID, "null argument");
+        if (on_device != 0 && on_device != 1) throw Error(MSX_ERR_INVALID, "on_device is 0 or 1");
+        const size_t n = t->nG() * t->nz;
+        MSX_HIP(hipMemcpyAsync(t->V.p, potential, sizeof(cx<double>) * n,
+                               on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                               t->stream));
+        t->prepare();
+        return MSX_OK;
+    } catch (const Error &e) {
+        return fail(e);
+    }
+}
+
 int msx_table_eig_phases(const msx_table *t, double *ms) {
     try {
         if (!t || !ms) throw Error(MSX_ERR_INVALID, "null argument");

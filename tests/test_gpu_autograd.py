@@ -74,3 +74,51 @@ def test_autograd_mixed_forward_default_adjoint_and_retained_graph():
     bt.solve(fp64)
     direct = bt.adjoint(2.0 * (bt.intensities_flat() - target.numpy()), fp64)
     assert np.abs(g1 - direct).max() <= 1e-6 * np.abs(direct).max()
+
+
+def test_autograd_reuses_its_batch_across_calls_and_reads_v_on_the_gpu():
+    """Calls with the same conditions list share one table and batch (the potential replaced
+    in place, from device memory when V lives on the GPU): a descent step's gradients equal
+    those of fresh tables, and a backward whose forward state a later forward replaced solves
+    its forward again."""
+    w = W.hex_batch(n_conditions=3, nz=24, nk=7, cond_seed=11, channel_counts=[12, 20, 30])
+    opts = BT.SolveOptions(method="jacobi", eps=5e-14)
+    conds = list(w.conditions)
+
+    def fresh_grad(Vn, target):
+        tab = BT.Table(Vn, w.kinetic)
+        bt = BT.Batch(tab, conds)
+        bt.solve(opts)
+        return bt.adjoint(2.0 * (bt.intensities_flat() - target), opts), bt.intensities_flat()
+
+    rng = np.random.default_rng(4)
+    V0 = w.potential.copy()
+    _, P0 = fresh_grad(V0, np.zeros(1))
+    target = P0 * (1 + 0.05 * rng.standard_normal(P0.size))
+    for dev in ("cpu", "cuda"):
+        Vn = V0.copy()
+        slots = set()
+        for step in range(3):
+            V = torch.tensor(Vn, dtype=torch.complex128, device=dev, requires_grad=True)
+            P = AG.scattering_intensities(V, conds, w.kinetic, opts, adjoint_options=opts)
+            slots.add(id(AG._SLOT[0]))
+            loss = ((P - torch.from_numpy(target).to(dev)) ** 2).sum()
+            loss.backward()
+            g = V.grad.cpu().numpy()
+            ref, Pref = fresh_grad(Vn, target)
+            np.testing.assert_allclose(P.detach().cpu().numpy(), Pref, rtol=0, atol=1e-13)
+            np.testing.assert_allclose(g, ref, rtol=1e-12, atol=1e-14 * np.abs(ref).max())
+            Vn = Vn - 1e-3 * g / np.abs(g).max() * np.abs(V0).max()
+        assert len(slots) == 1, slots
+    # two forwards, then the first one's backward: its state was replaced, so it solves again
+    Va = torch.tensor(V0, dtype=torch.complex128, requires_grad=True)
+    Vb = torch.tensor(V0 * 1.01, dtype=torch.complex128, requires_grad=True)
+    la = ((AG.scattering_intensities(Va, conds, w.kinetic, opts, adjoint_options=opts)
+           - torch.from_numpy(target)) ** 2).sum()
+    lb = ((AG.scattering_intensities(Vb, conds, w.kinetic, opts, adjoint_options=opts)
+           - torch.from_numpy(target)) ** 2).sum()
+    la.backward()
+    lb.backward()
+    for Vt, Vn in ((Va, V0), (Vb, V0 * 1.01)):
+        ref, _ = fresh_grad(Vn, target)
+        np.testing.assert_allclose(Vt.grad.numpy(), ref, rtol=1e-12, atol=1e-14 * np.abs(ref).max())
