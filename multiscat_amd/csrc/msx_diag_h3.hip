@@ -20,8 +20,12 @@
 //
 // A wavefront owns 16 channels: their real and imaginary parts are two 16-column tiles.
 //   W^T = Q^T X^T   A = Q^T (m x z, from LDS), B = X^T (z x column), K = z
-//   W'  = W^T scaled by 1/(e_j + lam_m) in FP32 (e and lam summed in FP64), 0 for m >= nz
-//   Y^T = Q W'      A = Q (z x m), B = W' (m x column), K = m
+//   W'  = W^T scaled by 1/(e_j + lam_m) in FP32 (e and lam summed in FP64), 0 for m >= nz,
+//         plus the Sherman-Morrison term corr_j Q^T f_j (Q^T f_j = Q[N-1][m] / (e_j + lam_m),
+//         corr_j from g_j(N) = Q[N-1][:] W' and the table's f_j(N)), so that
+//   Y^T = Q W'      A = Q (z x m), B = W' (m x column), K = m, is D^-1 x itself: the
+//         factor vectors f_j are not read (the 8-channel form below adds corr_j f_j after
+//         the second product instead)
 // K runs in blocks of 32; lane group g (lane / 16) supplies the 8 K values of its operands.
 // Product 1 takes z = 32 s + 8 g + u in block s. The B operand of product 2 is formed in place
 // from product 1's accumulators: lane group g of m tile mt holds rows 16 mt + 4 g + v, so in
@@ -70,6 +74,20 @@ __device__ __forceinline__ float wave_max4(float v) {
 __device__ __forceinline__ float wave_sum4(float v) {
     v += __shfl_xor(v, 16);
     return v + __shfl_xor(v, 32);
+}
+
+// First tile of wavefront `wave` of a persistent launch with `waves` wavefronts per workgroup,
+// which takes gridDim.x * waves consecutive tiles per round. With a tile for every wavefront,
+// a workgroup takes consecutive tiles and consecutive workgroups of one XCD (workgroups are
+// dealt to the 8 XCDs in turn) take consecutive runs of them: the rows of neighbouring tiles
+// share cache lines (a tile's row of one node is 64 or 128 bytes at any alignment), which are
+// then fetched into one L2 rather than two. With fewer tiles, tile t goes to wavefront
+// t / gridDim.x of workgroup t % gridDim.x, which gives each tile its own SIMD.
+__device__ __forceinline__ int first_tile(int n_tiles, int waves, int wave) {
+    const int G = gridDim.x;
+    if (n_tiles < G * waves) return wave * G + blockIdx.x;
+    const int b = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
+    return b * waves + wave;
 }
 
 // P1LO: P1 lo in LDS (else read from L2 like P2 lo). BASE: y = base + sign D^-1 x.
@@ -141,9 +159,6 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int l16 = lane & 15, g = lane >> 4;
     const int n_tiles = tile_start[n_inst];
-    // tile t to wavefront slot t: the first gridDim.x tiles to wavefront 0 of every
-    // workgroup, the next to wavefront 1, ...: a launch with few active tiles gives each its
-    // own SIMD
     const int t_step = gridDim.x * H3_WAVES;
 
     // One tile: its instance's channel count and this lane's channel; element indices in 32
@@ -190,7 +205,7 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
             }
     };
 
-    int t = wave * gridDim.x + blockIdx.x;
+    int t = first_tile(n_tiles, H3_WAVES, wave);
     Tile cur, nxt;
     if (t < n_tiles) begin_tile(t, cur);
 #pragma unroll 1
@@ -236,21 +251,12 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
             }
         }
 
-        // P2 lo fragments of the first z group, read during the scaling and the split of W
-        h8 a2l[KT][H3_ZG];
-#pragma unroll
-        for (int zg = 0; zg < H3_ZG; ++zg) {
-            const int zr = min(16 * zg + l16, nz - 1);
-#pragma unroll
-            for (int p = 0; p < KT; ++p)
-                a2l[p][zg] = *reinterpret_cast<const h8 *>(g2l + zr * S + 32 * p + 8 * g);
-        }
-
         // ---- W' = W / (e_j + lam_m) with the column scales removed; y0(N) = Q[N-1][:] W';
         // 1 / (e_j + lam_m) from the FP64 sum by the hardware reciprocal, 0 past nz ----
         // powers of two (exact): the column scales and the planes' 2^15
         const float ixr = 1.0f / (sxr * H3_QSCALE), ixi = 1.0f / (sxi * H3_QSCALE);
         float yn0 = 0.f, yn1 = 0.f, wr = 0.f, wi = 0.f;
+        float qs[MT][4];  // Q[N-1][m] / (e_j + lam_m), the Q^T of f_j = D0^-1 e_N
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
 #pragma unroll
@@ -261,10 +267,25 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
                 const float r = __builtin_amdgcn_rcpf((float)(cur.ej + lam[m]));
                 const float sc = m < nz ? r : 0.f;
                 const float q = qn[m];
+                qs[mt][v] = q * sc;
                 acc1[mt][0][v] *= sc * ixr;
                 acc1[mt][1][v] *= sc * ixi;
                 yn0 += q * acc1[mt][0][v];
                 yn1 += q * acc1[mt][1][v];
+            }
+        }
+        yn0 = wave_sum4(yn0);
+        yn1 = wave_sum4(yn1);
+        // D^-1 x = D0^-1 x + corr f_j, corr = c y0(N) / (1 - c f_j(N)), f_j = Q (Q^T f_j):
+        // the correction joins W' before the second product, which then gives D^-1 x itself
+        cx<float> corr = mk(0.f, 0.f);
+        if (cur.live) corr = cdiv(mk(yn0, yn1) * cur.cj, mk(1.0f, 0.0f) - cur.cj * cur.facN);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                acc1[mt][0][v] = fmaf(corr.re, qs[mt][v], acc1[mt][0][v]);
+                acc1[mt][1][v] = fmaf(corr.im, qs[mt][v], acc1[mt][1][v]);
             }
 #pragma unroll
             for (int v = 0; v < 4; v += 2) {
@@ -272,11 +293,15 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
                 wi = max3abs(acc1[mt][1][v], acc1[mt][1][v + 1], wi);
             }
         }
-        yn0 = wave_sum4(yn0);
-        yn1 = wave_sum4(yn1);
-        // D^-1 x = D0^-1 x + corr f_j, corr = c y0(N) / (1 - c f_j(N))
-        cx<float> corr = mk(0.f, 0.f);
-        if (cur.live) corr = cdiv(mk(yn0, yn1) * cur.cj, mk(1.0f, 0.0f) - cur.cj * cur.facN);
+        // P2 lo fragments of the first z group, read during the split of W'
+        h8 a2l[KT][H3_ZG];
+#pragma unroll
+        for (int zg = 0; zg < H3_ZG; ++zg) {
+            const int zr = min(16 * zg + l16, nz - 1);
+#pragma unroll
+            for (int p = 0; p < KT; ++p)
+                a2l[p][zg] = *reinterpret_cast<const h8 *>(g2l + zr * S + 32 * p + 8 * g);
+        }
         const float swr = h3_scale(wave_max4(wr)), swi = h3_scale(wave_max4(wi));
         h8 whr[KT], wlr[KT], whi[KT], wli[KT];
 #pragma unroll
@@ -294,19 +319,16 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
         }
         const float iwr = 1.0f / (swr * H3_QSCALE), iwi = 1.0f / (swi * H3_QSCALE);
 
-        // ---- Y^T = Q W' by groups of H3_ZG z tiles; factors and base values one group
-        // ahead at clamped addresses (unused values are never stored) ----
-        float fz[H3_ZG][4];
+        // ---- Y^T = Q W' by groups of H3_ZG z tiles; base values one group ahead at clamped
+        // addresses (unused values are never stored) ----
         cx<float> bz[H3_ZG][4];
         auto load_group = [&](int zt) {
+            if constexpr (BASE) {
 #pragma unroll
-            for (int zg = 0; zg < H3_ZG; ++zg)
+                for (int zg = 0; zg < H3_ZG; ++zg)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int z = 16 * (zt + zg) + 4 * g + v;
-                    fz[zg][v] = fac[vi(z)];
-                    if constexpr (BASE) bz[zg][v] = base[vi(z)];
-                }
+                    for (int v = 0; v < 4; ++v) bz[zg][v] = base[vi(16 * (zt + zg) + 4 * g + v)];
+            }
         };
         // one group: its products, then (the last group) the next tile's loads, then its
         // results and stores, then (the others) the next group's factor and base loads
@@ -358,8 +380,7 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
             for (int zg = 0; zg < H3_ZG; ++zg)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    cx<float> val = mk(acc2[zg][0][v] * iwr, acc2[zg][1][v] * iwi);
-                    val = val + corr * fz[zg][v];
+                    const cx<float> val = mk(acc2[zg][0][v] * iwr, acc2[zg][1][v] * iwi);
                     out[zg][v] = BASE ? bz[zg][v] + val * sign : val;
                 }
             if (16 * (zt0 + H3_ZG) <= nz) {
@@ -470,7 +491,7 @@ __global__ __launch_bounds__(64 * H8_WAVES) void diag_h8_kernel(
     const float *bf = reinterpret_cast<const float *>(base);
     float *yf = reinterpret_cast<float *>(y);
 #pragma unroll 1
-    for (int t = wave * gridDim.x + blockIdx.x; t < n_tiles; t += gridDim.x * H8_WAVES) {
+    for (int t = first_tile(n_tiles, H8_WAVES, wave); t < n_tiles; t += gridDim.x * H8_WAVES) {
         int lo = 0, hi = n_inst - 1;  // last instance with tile_start <= t
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;

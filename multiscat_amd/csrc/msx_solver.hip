@@ -793,16 +793,23 @@ struct Ops {
     template <typename T>
     void dinv(const cx<T> *in, const cx<T> *base, cx<T> *out, T sign, const int *act) {
         const bool mfma = !vector_diag && diag_mfma_supported(L.nz);
+        bool h3 = false;
+        if constexpr (sizeof(T) == 4) h3 = mfma && split_f16 && diag_h3_fits(lay<float>(), t.h3());
         Work fl, by;
         if (profiling()) {  // two real nz x nz products on the re and im rows of each channel
+            // bytes: x, y (and base) per element, the factor vectors f_j per element except in
+            // the split kernel (which forms corr_j Q^T f_j in registers), e_j, c_j and f_j(N)
+            // per channel, and the eigenvector tables (the split kernel's four FP16 planes)
             const double nz = L.nz;
+            const double per_elem = (base ? 3.0 : 2.0) * sizeof(cx<T>) + (h3 ? 0.0 : sizeof(T));
+            const double per_ch = sizeof(double) + sizeof(cx<T>) + sizeof(T);
+            const double nq = t.h3().nq;
             fl = over_active(act, [&](int nc) { return nc * (8.0 * nz * nz + 12.0 * nz); });
-            by = over_active(act, [&](int nc) {
-                     return nc * nz * (3.0 * sizeof(cx<T>) + sizeof(T));
-                 }) + 2.0 * nz * nz * sizeof(T);
+            by = over_active(act, [&](int nc) { return nc * (nz * per_elem + per_ch); }) +
+                 (h3 ? 4.0 * nq * (nq + 8) * sizeof(_Float16) : 2.0 * nz * nz * sizeof(T));
         }
         if constexpr (sizeof(T) == 4) {
-            if (mfma && split_f16 && diag_h3_fits(lay<float>(), t.h3())) {
+            if (h3) {
                 ProfScope ps("diag_h3_f32", s, fl, by);
                 launch_diag_h3(lay<float>(), t.h3(), t.lam.p, B.e_d.p, c<float>(), fac<float>(), in,
                                base, out, sign, act, s, B.tuning.diag32_cpw);
