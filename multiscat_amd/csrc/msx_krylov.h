@@ -36,19 +36,26 @@ struct KState {
 // dst[b] = src[b * stride].re for b < n (dst may be pinned host memory mapped for the device)
 void k_gather_re(int n, int stride, const cx<double> *src, double *dst, hipStream_t s);
 
+// Chunk partials of V^H w (and |w|^2 with with_norm) in partial, reduced into h; with h null
+// the partials are left for a consumer that sums them itself (pin of k_axpy, k_axpy_dots).
 template <typename T>
 void k_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<T> *w, int with_norm,
             double *partial, cx<double> *h, int accumulate, hipStream_t s);
 // with_norm: chunk partials of |w|^2 (after the update) in slot kcount, reduced into
-// hnorm_dst[kcount] if hnorm_dst is given
+// hnorm_dst[kcount] if hnorm_dst is given. pin (kcount <= 16): the coefficients are the sums
+// of these chunk partials (slots 0 .. kcount-1), formed by every workgroup, and also stored to
+// hout; h is then unused. partial may be pin's buffer (the norm goes to slot kcount).
 template <typename T>
 void k_axpy(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<double> *h, cx<T> *w,
-            int with_norm, double *partial, cx<double> *hnorm_dst, hipStream_t s);
+            int with_norm, double *partial, cx<double> *hnorm_dst, hipStream_t s,
+            const double *pin = nullptr, cx<double> *hout = nullptr);
 // w -= sum_{j<kcount} v_j h_j, then h2 = V^H w, in one pass over the basis (kcount <= 16;
-// false, with nothing launched, above that).
+// false, with nothing launched, above that). pin / hout as for k_axpy (pin must not be
+// partial: this pass writes its own partials there); h2 null leaves the new partials unreduced.
 template <typename T>
 bool k_axpy_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<double> *h,
-                 cx<T> *w, double *partial, cx<double> *h2, hipStream_t s);
+                 cx<T> *w, double *partial, cx<double> *h2, hipStream_t s,
+                 const double *pin = nullptr, cx<double> *hout = nullptr);
 template <typename T, typename TX>
 void k_update(const KLayout &K, const cx<T> *const *vptr, const int *kk, const cx<double> *y,
               cx<TX> *x, const int *mask, hipStream_t s);

@@ -183,6 +183,46 @@ __global__ __launch_bounds__(64 * RP_WAVES) void reduce_partials_kernel(
     }
 }
 
+// The coefficients of an update for instance b, in LDS as T: h[slot + j], or (pin given) the
+// sums of the chunk partials pin[(slot + j) nchunk + c] formed here, in the order of
+// reduce_partials_kernel (lane-strided, then the same butterfly), so that every workgroup of the
+// instance holds the same values and they equal the reduction kernel's bit for bit; workgroup 0
+// also stores the sums to hout[slot + j] for the Givens kernel. This spares a reduction launch
+// (and the gap after it) between a projection and the update that consumes it.
+template <typename T>
+__device__ __forceinline__ void load_coefficients(const KLayout &K, long long slot, int kcount,
+                                                  const cx<double> *__restrict__ h,
+                                                  const double *__restrict__ pin,
+                                                  cx<double> *__restrict__ hout, cx<T> *hs) {
+    if (pin) {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int j = wave; j < kcount; j += KR_THREADS / 64) {
+            const double2 *p = reinterpret_cast<const double2 *>(pin + (slot + j) * K.nchunk * 2);
+            double sr = 0, si = 0;
+            for (int c = lane; c < K.nchunk; c += 64) {
+                const double2 v = p[c];
+                sr += v.x;
+                si += v.y;
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                sr += __shfl_xor(sr, o);
+                si += __shfl_xor(si, o);
+            }
+            if (lane == 0) {
+                hs[j] = mk((T)sr, (T)si);
+                if (hout && blockIdx.x == 0) hout[slot + j] = mk(sr, si);
+            }
+        }
+    } else {
+        for (int j = threadIdx.x; j < kcount; j += KR_THREADS) {
+            const cx<double> hv = h[slot + j];
+            hs[j] = mk((T)hv.re, (T)hv.im);
+        }
+    }
+    __syncthreads();
+}
+
 // w -= sum_{j<kcount} v_j h_j ; optionally norm partials of the updated w in slot kcount
 template <typename T>
 __global__ __launch_bounds__(KR_THREADS) void axpy_kernel(KLayout K, const cx<T> *const *vptr,
@@ -234,18 +274,16 @@ __global__ __launch_bounds__(KR_THREADS) void axpy_reg_kernel(KLayout K, const c
                                                               int kcount,
                                                               const cx<double> *__restrict__ h,
                                                               cx<T> *__restrict__ w, int with_norm,
-                                                              double *__restrict__ partial) {
+                                                              double *__restrict__ partial,
+                                                              const double *__restrict__ pin,
+                                                              cx<double> *__restrict__ hout) {
     __shared__ cx<T> hs[KM];
     __shared__ double red[2 * (KR_THREADS / 64)];
     const int b = blockIdx.y;
     if (K.active && !K.active[b]) return;
     const long long n = K.len[b], base = K.start[b];
     const long long slot = (long long)b * (K.kmax + 1);
-    for (int j = threadIdx.x; j < kcount; j += KR_THREADS) {
-        const cx<double> hv = h[slot + j];
-        hs[j] = mk((T)hv.re, (T)hv.im);
-    }
-    __syncthreads();
+    load_coefficients<T>(K, slot, kcount, h, pin, hout, hs);
     const cx<T> *vb[KM];
 #pragma unroll
     for (int j = 0; j < KM; ++j) vb[j] = j < kcount ? vptr[j] + base : nullptr;
@@ -290,18 +328,16 @@ __global__ __launch_bounds__(KR_THREADS) void axpy_dots_kernel(KLayout K,
                                                                int kcount,
                                                                const cx<double> *__restrict__ h,
                                                                cx<T> *__restrict__ w,
-                                                               double *__restrict__ partial) {
+                                                               double *__restrict__ partial,
+                                                               const double *__restrict__ pin,
+                                                               cx<double> *__restrict__ hout) {
     __shared__ cx<T> hs[KM];
     __shared__ double red[(KR_THREADS / 64) * (KM + 1) * 2];
     const int b = blockIdx.y;
     if (K.active && !K.active[b]) return;
     const long long n = K.len[b], base = K.start[b];
     const long long slot = (long long)b * (K.kmax + 1);
-    for (int j = threadIdx.x; j < kcount; j += KR_THREADS) {
-        const cx<double> hv = h[slot + j];
-        hs[j] = mk((T)hv.re, (T)hv.im);
-    }
-    __syncthreads();
+    load_coefficients<T>(K, slot, kcount, h, pin, hout, hs);
     const cx<T> *vb[KM];
 #pragma unroll
     for (int j = 0; j < KM; ++j) vb[j] = j < kcount ? vptr[j] + base : nullptr;
@@ -652,6 +688,7 @@ void k_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<T> 
         hipLaunchKernelGGL((dots_kernel<T>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr, kcount, w,
                            with_norm, partial);
     MSX_LAUNCH_CHECK();
+    if (!h) return;  // the consumer sums the partials itself
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(K.n_inst), dim3(64 * RP_WAVES), 0, s, K, 0,
                        kcount + (with_norm ? 1 : 0), partial, h, accumulate);
     MSX_LAUNCH_CHECK();
@@ -659,14 +696,16 @@ void k_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<T> 
 
 template <typename T>
 void k_axpy(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<double> *h,
-            cx<T> *w, int with_norm, double *partial, cx<double> *hnorm_dst, hipStream_t s) {
+            cx<T> *w, int with_norm, double *partial, cx<double> *hnorm_dst, hipStream_t s,
+            const double *pin, cx<double> *hout) {
     if (kcount > KR_MAX_BASIS) throw Error(3, "Krylov dimension exceeds the kernel limit");
+    if (pin && kcount > 16) throw Error(3, "in-kernel coefficient sums cover 16 basis vectors");
     if (kcount <= 8)
         hipLaunchKernelGGL((axpy_reg_kernel<T, 8>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr,
-                           kcount, h, w, with_norm, partial);
+                           kcount, h, w, with_norm, partial, pin, hout);
     else if (kcount <= 16)
         hipLaunchKernelGGL((axpy_reg_kernel<T, 16>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr,
-                           kcount, h, w, with_norm, partial);
+                           kcount, h, w, with_norm, partial, pin, hout);
     else
         hipLaunchKernelGGL((axpy_kernel<T>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr, kcount, h,
                            w, with_norm, partial);
@@ -682,16 +721,18 @@ void k_axpy(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<dou
 
 template <typename T>
 bool k_axpy_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<double> *h,
-                 cx<T> *w, double *partial, cx<double> *h2, hipStream_t s) {
+                 cx<T> *w, double *partial, cx<double> *h2, hipStream_t s, const double *pin,
+                 cx<double> *hout) {
     if (kcount <= 8)
         hipLaunchKernelGGL((axpy_dots_kernel<T, 8>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr,
-                           kcount, h, w, partial);
+                           kcount, h, w, partial, pin, hout);
     else if (kcount <= 16)
         hipLaunchKernelGGL((axpy_dots_kernel<T, 16>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr,
-                           kcount, h, w, partial);
+                           kcount, h, w, partial, pin, hout);
     else
         return false;
     MSX_LAUNCH_CHECK();
+    if (!h2) return true;  // the consumer sums the partials itself
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(K.n_inst), dim3(64 * RP_WAVES), 0, s, K, 0, kcount,
                        partial, h2, 0);
     MSX_LAUNCH_CHECK();
@@ -758,10 +799,11 @@ void k_cycle_start(const KLayout &K, const KState &S, const cx<double> *nrm2, in
     template void k_dots<T>(const KLayout &, const cx<T> *const *, int, const cx<T> *, int,      \
                             double *, cx<double> *, int, hipStream_t);                          \
     template void k_axpy<T>(const KLayout &, const cx<T> *const *, int, const cx<double> *,     \
-                            cx<T> *, int, double *, cx<double> *, hipStream_t);                 \
+                            cx<T> *, int, double *, cx<double> *, hipStream_t, const double *,  \
+                            cx<double> *);                                                      \
     template bool k_axpy_dots<T>(const KLayout &, const cx<T> *const *, int,                    \
                                  const cx<double> *, cx<T> *, double *, cx<double> *,           \
-                                 hipStream_t);                                                  \
+                                 hipStream_t, const double *, cx<double> *);                    \
     template void k_scale<T>(const KLayout &, const cx<T> *, cx<T> *, const double *,           \
                              const int *, hipStream_t);                                         \
     template void k_axpby<T>(const KLayout &, const cx<T> *, const cx<T> *, T, cx<T> *,          \

@@ -60,7 +60,9 @@ struct Krylov {
     struct IntView {
         int *p = nullptr;
     } active, count;
-    DBuf<double> beta, beta0, resid, hnorm, partial;
+    // chunk partials of the projections; partial2 holds the second CGS2 pass's while the
+    // first pass's are still being summed by their consumer
+    DBuf<double> beta, beta0, resid, hnorm, partial, partial2;
     DBuf<cx<double>> H, g, cs, sn, y, hcol, hcol2;
     std::vector<DBuf<cx<T>>> blocks;
     std::vector<cx<T> *> vp;
@@ -117,6 +119,7 @@ struct Krylov {
         for (auto *b : {&beta, &beta0, &resid, &hnorm}) b->alloc(n_inst);
         const size_t slots = (size_t)n_inst * (kmax + 1);
         partial.alloc(slots * K.nchunk * 2);
+        partial2.alloc(slots * K.nchunk * 2);
         H.alloc(slots * (kmax + 1));
         for (auto *b : {&g, &cs, &sn, &y, &hcol, &hcol2}) b->alloc(slots);
         vp_d.alloc(kmax + 1);
@@ -321,6 +324,7 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
             // the fixed kernel sequence of iteration k: operator, Gram-Schmidt passes, Givens
             // update (writing the flags to slot next_slot), normalisation
             auto iteration = [&]() {
+            const double *norm_src = kr.partial.p;  // where the Givens kernel finds |w|^2 partials
             if (scaled_in) {
                 (*op_scaled)(kr.w.p, kr.hnorm.p, kr.vp[k], kr.w.p, kr.active.p);
             } else {
@@ -340,8 +344,20 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
                                      ? (sizeof(T) == 4 ? "arnoldi_cgs2_f32" : "arnoldi_cgs2_f64")
                                      : (sizeof(T) == 4 ? "arnoldi_cgs1_f32" : "arnoldi_cgs1_f64");
                 ProfScope ps(nm, s, fl, by);
-                k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol.p, 0, s);
-                if (orth_passes >= 2) {
+                if (orth_passes >= 2 && k + 1 <= 16) {
+                    // each pass's chunk partials summed by the kernel that consumes them (no
+                    // reduction launches): dots -> partial; the first update sums them (and
+                    // stores hcol) and writes the second projection's partials to partial2;
+                    // the second update sums those (and stores hcol2) and adds the norm
+                    // partials in slot k + 1 of partial2, which the Givens kernel sums
+                    k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, nullptr, 0, s);
+                    k_axpy_dots<T>(kr.K, kr.vp_d.p, k + 1, nullptr, kr.w.p, kr.partial2.p, nullptr,
+                                   s, kr.partial.p, kr.hcol.p);
+                    k_axpy<T>(kr.K, kr.vp_d.p, k + 1, nullptr, kr.w.p, 1, kr.partial2.p, nullptr, s,
+                              kr.partial2.p, kr.hcol2.p);
+                    norm_src = kr.partial2.p;
+                } else if (orth_passes >= 2) {
+                    k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol.p, 0, s);
                     // the first update and the second projection share one pass over V
                     if (!k_axpy_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol.p, kr.w.p, kr.partial.p,
                                         kr.hcol2.p, s)) {
@@ -352,13 +368,14 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
                     k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol2.p, kr.w.p, 1, kr.partial.p,
                               nullptr, s);
                 } else {
+                    k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, kr.hcol.p, 0, s);
                     k_axpy<T>(kr.K, kr.vp_d.p, k + 1, kr.hcol.p, kr.w.p, 1, kr.partial.p, nullptr,
                               s);
                 }
             }
             // the Givens kernel sums the norm partials of the last update itself
             k_givens(kr.K, k, kr.hcol.p, orth_passes >= 2 ? kr.hcol2.p : nullptr, kr.S, kr.count.p, s,
-                     kr.next_flags(), kr.partial.p);
+                     kr.next_flags(), norm_src);
             // v_{k+1} = w / h_{k+1,k}: by the next operator call when it can, else here (a
             // vector the loop leaves unnormalised is never read: the update uses v_0 .. v_k)
             if (!op_scaled) k_scale<T>(kr.K, kr.w.p, kr.vp[k + 1], kr.hnorm.p, kr.active.p, s);
