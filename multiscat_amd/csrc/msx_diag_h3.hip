@@ -66,6 +66,25 @@ __device__ __forceinline__ float h3_scale(float m) {
     (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
     return ldexpf(1.0f, 15 - max(e, -100));
 }
+// The same scale as an exponent k (scale 2^k), whose removal together with the planes' 2^15
+// is the exact 2^(-k-15)
+__device__ __forceinline__ int h3_scale_exp(float m) {
+    int e = 0;
+    (void)frexpf(m, &e);
+    return 15 - max(e, -100);
+}
+
+// a / b for the Sherman-Morrison quotient: Smith's form (the larger of |b.re|, |b.im| divides)
+// with hardware reciprocals and selects instead of branches and IEEE divisions
+__device__ __forceinline__ cx<float> cdiv_rcp(cx<float> a, cx<float> b) {
+    const bool sw = fabsf(b.re) < fabsf(b.im);
+    const float p = sw ? b.im : b.re, q = sw ? b.re : b.im;
+    const float u = sw ? a.im : a.re, w = sw ? a.re : a.im;
+    const float r = q * __builtin_amdgcn_rcpf(p);
+    const float id = __builtin_amdgcn_rcpf(fmaf(q, r, p));
+    const float re = fmaf(w, r, u) * id, im = fmaf(-u, r, w) * id;
+    return mk(re, sw ? -im : im);
+}
 
 __device__ __forceinline__ float wave_max4(float v) {
     v = fmaxf(v, __shfl_xor(v, 16));
@@ -193,13 +212,13 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
         T.vlast = T.vb + (nz - 1) * T.nc;
         T.ej = e[T.cidx];
         T.cj = c[T.cidx];
-        T.facN = fac[T.vlast];
+        T.facN = fac[(unsigned)T.vlast];
 #pragma unroll
         for (int s = 0; s < KT; ++s)
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int z = 32 * s + 8 * g + u;
-                const cx<float> v = x[min(T.vb + z * T.nc, T.vlast)];
+                const cx<float> v = x[(unsigned)min(T.vb + z * T.nc, T.vlast)];
                 xr[s][u] = v.re;
                 xi[s][u] = v.im;
             }
@@ -211,7 +230,9 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
 #pragma unroll 1
     while (t < n_tiles) {
         const int t_next = t + t_step;
-        auto vi = [&](int z) -> int { return min(cur.vb + z * cur.nc, cur.vlast); };
+        auto vi = [&](int z) -> unsigned { return (unsigned)min(cur.vb + z * cur.nc, cur.vlast); };
+        // (nodes known to lie below nz: no clamp)
+        auto vu = [&](int z) -> unsigned { return (unsigned)(cur.vb + z * cur.nc); };
 
         // ---- column maxima and split of the tile's X ----
         float mr = 0.f, mi = 0.f;
@@ -222,7 +243,8 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
                 mr = max3abs(xr[s][u], xr[s][u + 1], mr);
                 mi = max3abs(xi[s][u], xi[s][u + 1], mi);
             }
-        const float sxr = h3_scale(wave_max4(mr)), sxi = h3_scale(wave_max4(mi));
+        const int exr = h3_scale_exp(wave_max4(mr)), exi = h3_scale_exp(wave_max4(mi));
+        const float sxr = ldexpf(1.0f, exr), sxi = ldexpf(1.0f, exi);
         h8 xhr[KT], xlr[KT], xhi[KT], xli[KT];
 #pragma unroll
         for (int s = 0; s < KT; ++s) {
@@ -254,7 +276,7 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
         // ---- W' = W / (e_j + lam_m) with the column scales removed; y0(N) = Q[N-1][:] W';
         // 1 / (e_j + lam_m) from the FP64 sum by the hardware reciprocal, 0 past nz ----
         // powers of two (exact): the column scales and the planes' 2^15
-        const float ixr = 1.0f / (sxr * H3_QSCALE), ixi = 1.0f / (sxi * H3_QSCALE);
+        const float ixr = ldexpf(1.0f, -exr - 15), ixi = ldexpf(1.0f, -exi - 15);
         float yn0 = 0.f, yn1 = 0.f, wr = 0.f, wi = 0.f;
         float qs[MT][4];  // Q[N-1][m] / (e_j + lam_m), the Q^T of f_j = D0^-1 e_N
 #pragma unroll
@@ -279,7 +301,7 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
         // D^-1 x = D0^-1 x + corr f_j, corr = c y0(N) / (1 - c f_j(N)), f_j = Q (Q^T f_j):
         // the correction joins W' before the second product, which then gives D^-1 x itself
         cx<float> corr = mk(0.f, 0.f);
-        if (cur.live) corr = cdiv(mk(yn0, yn1) * cur.cj, mk(1.0f, 0.0f) - cur.cj * cur.facN);
+        if (cur.live) corr = cdiv_rcp(mk(yn0, yn1) * cur.cj, mk(1.0f, 0.0f) - cur.cj * cur.facN);
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
 #pragma unroll
@@ -302,7 +324,8 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
             for (int p = 0; p < KT; ++p)
                 a2l[p][zg] = *reinterpret_cast<const h8 *>(g2l + zr * S + 32 * p + 8 * g);
         }
-        const float swr = h3_scale(wave_max4(wr)), swi = h3_scale(wave_max4(wi));
+        const int ewr = h3_scale_exp(wave_max4(wr)), ewi = h3_scale_exp(wave_max4(wi));
+        const float swr = ldexpf(1.0f, ewr), swi = ldexpf(1.0f, ewi);
         h8 whr[KT], wlr[KT], whi[KT], wli[KT];
 #pragma unroll
         for (int p = 0; p < KT; ++p) {
@@ -317,7 +340,7 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
             split8s(ar, swr, whr[p], wlr[p]);
             split8s(ai, swi, whi[p], wli[p]);
         }
-        const float iwr = 1.0f / (swr * H3_QSCALE), iwi = 1.0f / (swi * H3_QSCALE);
+        const float iwr = ldexpf(1.0f, -ewr - 15), iwi = ldexpf(1.0f, -ewi - 15);
 
         // ---- Y^T = Q W' by groups of H3_ZG z tiles; base values one group ahead at clamped
         // addresses (unused values are never stored) ----
@@ -388,7 +411,7 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
 #pragma unroll
                     for (int zg = 0; zg < H3_ZG; ++zg)
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) y[vi(16 * (zt0 + zg) + 4 * g + v)] = out[zg][v];
+                        for (int v = 0; v < 4; ++v) y[vu(16 * (zt0 + zg) + 4 * g + v)] = out[zg][v];
                 }
             } else {
 #pragma unroll
