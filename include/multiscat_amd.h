@@ -149,8 +149,8 @@ int msx_debug_apply_coupling(int nkx, int nky, int nz, const double *potential, 
  * nz x nz row-major; states (nc, nz) row-major complex. variant 0 = matrix-core GEMMs
  * (nz <= 320), 1 = vector ALU, 2 = matrix-core GEMMs with 8 channels per wavefront (FP64,
  * nz <= 160; variant 0 takes 16 there), 3 and 4 = matrix-core GEMMs on split-FP16 operands
- * with 16 and 8 channels per wavefront (FP32 inverse only, nz <= 160); precision 0 = FP64,
- * 1 = FP32. */
+ * with 16 and 8 channels per wavefront (FP32 inverse only, nz <= 224; above 160 the planes are
+ * read from L2 and both take 8 channels per wavefront); precision 0 = FP64, 1 = FP32. */
 int msx_debug_apply_diagonal(int nz, const double *potential00, const double *kinetic, int nc,
                              const double *e, const double *wave_c, const double *state_in,
                              double *state_out, int inverse, int precision, int variant);
@@ -202,8 +202,9 @@ typedef struct {
                          batch; 0: every kernel launched one by one. Off while profiling.      */
     int inner_products; /* mixed precision: operands of the FP32 inner operator's products on
                          the matrix cores (D^-1 and the DFT coupling). 0 = auto (split FP16
-                         where the kernels cover the shape: node-major vectors; D^-1 nz <= 160
-                         with enough LDS, the coupling square tables up to 32 x 32; else FP32),
+                         where the kernels cover the shape: node-major vectors; D^-1 nz <= 224
+                         with LDS for the tile scan, the coupling square tables up to 32 x 32;
+                         else FP32),
                          1 = FP32 operands, 2 = split FP16 (an error where the D^-1 kernel does
                          not cover the shape). Split FP16: each FP32 operand as a pair hi + lo
                          of FP16 values under a power-of-two scale (22 significant bits), three

@@ -444,21 +444,29 @@ __global__ __launch_bounds__(64 * H3_WAVES) void diag_h3_kernel(
 // the Sherman-Morrison correction.
 constexpr int H8_WAVES = 12;
 constexpr int H8_ZG = 1;
+// nq 192 and 224 (nz up to 224): the four planes do not fit LDS, so all of them are read from
+// L2 (each A fragment once per tile), and the workgroup holds 8 wavefronts of up to 256 VGPRs
+// (189 and 213 used; nq 256 would spill)
+constexpr int H8L_WAVES = 8;
 
-template <int KT, bool P1LO, bool BASE>
-__global__ __launch_bounds__(64 * H8_WAVES) void diag_h8_kernel(
+// L2P: every plane read from L2 (P1LO false; LDS holds lam, the Q row and the tile scan only).
+template <int KT, bool P1LO, bool BASE, bool L2P = false, int WAVES = H8_WAVES>
+__global__ __launch_bounds__(64 * WAVES) void diag_h8_kernel(
     Layout L, H3Planes H, const double *__restrict__ lam_g, const double *__restrict__ e,
     const cx<float> *__restrict__ c, const float *__restrict__ fac, const cx<float> *x,
     const cx<float> *base, cx<float> *y, float sign, const int *__restrict__ active) {
+    static_assert(!(L2P && P1LO), "with the planes in L2, P1 lo is there too");
     constexpr int NQ = 32 * KT, MT = 2 * KT, S = NQ + 8;
     extern __shared__ double h3_smem[];
-    __shared__ int wave_sum[H8_WAVES];
+    __shared__ int wave_sum[WAVES];
     const int nz = L.nz, n_inst = L.n_inst;
     double *lam = h3_smem;                                    // [NQ], 0 past nz
     float *qn = reinterpret_cast<float *>(lam + NQ);           // [NQ]: Q[nz - 1][m]
-    _Float16 *p1h = reinterpret_cast<_Float16 *>(qn + NQ);    // [nz][S]
-    _Float16 *p2h = p1h + nz * S;                              // [nz][S]
-    _Float16 *p1l = p2h + nz * S;                              // [nz][S] (P1LO)
+    _Float16 *lp = reinterpret_cast<_Float16 *>(qn + NQ);     // LDS planes (not with L2P)
+    const int lrows = L2P ? 0 : nz;
+    _Float16 *p1l = lp + 2 * lrows * S;                        // [nz][S] (P1LO)
+    const _Float16 *p1h = L2P ? H.planes : lp;                             // [nz][S]
+    const _Float16 *p2h = L2P ? H.planes + (size_t)2 * NQ * S : lp + lrows * S;  // [nz][S]
     int *tile_start = reinterpret_cast<int *>(p1l + (P1LO ? nz * S : 0));  // [n_inst + 1]
     int *inst_nc = tile_start + n_inst + 1;                    // [n_inst]
     int *inst_off = inst_nc + n_inst;                          // [n_inst]
@@ -494,14 +502,14 @@ __global__ __launch_bounds__(64 * H8_WAVES) void diag_h8_kernel(
         lam[m] = m < nz ? lam_g[m] : 0.0;
         qn[m] = m < nz ? H.qlast[m] : 0.0f;
     }
-    {
+    if constexpr (!L2P) {
         const int n16 = nz * S / 8;
         const uint4 *s1h = reinterpret_cast<const uint4 *>(H.planes);
         const uint4 *s1l = reinterpret_cast<const uint4 *>(g1l);
         const uint4 *s2h = reinterpret_cast<const uint4 *>(H.planes + (size_t)2 * NQ * S);
         for (int i = threadIdx.x; i < n16; i += blockDim.x) {
-            reinterpret_cast<uint4 *>(p1h)[i] = s1h[i];
-            reinterpret_cast<uint4 *>(p2h)[i] = s2h[i];
+            reinterpret_cast<uint4 *>(lp)[i] = s1h[i];
+            reinterpret_cast<uint4 *>(lp + lrows * S)[i] = s2h[i];
             if (P1LO) reinterpret_cast<uint4 *>(p1l)[i] = s1l[i];
         }
     }
@@ -514,7 +522,7 @@ __global__ __launch_bounds__(64 * H8_WAVES) void diag_h8_kernel(
     const float *bf = reinterpret_cast<const float *>(base);
     float *yf = reinterpret_cast<float *>(y);
 #pragma unroll 1
-    for (int t = first_tile(n_tiles, H8_WAVES, wave); t < n_tiles; t += gridDim.x * H8_WAVES) {
+    for (int t = first_tile(n_tiles, WAVES, wave); t < n_tiles; t += gridDim.x * WAVES) {
         int lo = 0, hi = n_inst - 1;  // last instance with tile_start <= t
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -707,7 +715,7 @@ __global__ void h3_planes_kernel(int nz, int nq, const double *__restrict__ Q, _
     }
 }
 
-int h3_padded_size(int nz) { return nz <= 160 ? (nz + 31) / 32 * 32 : 0; }
+int h3_padded_size(int nz) { return nz <= 224 ? (nz + 31) / 32 * 32 : 0; }
 
 size_t h3_plane_halves(int nq) { return 4 * (size_t)nq * (nq + 8); }
 
@@ -721,6 +729,10 @@ size_t h3_lds_bytes(int nq, int nz, int n_inst, bool p1lo) {
     const size_t S = (size_t)nq + 8;
     return 12 * (size_t)nq + (p1lo ? 3 : 2) * (size_t)nz * S * 2 +
            (3 * (size_t)n_inst + 1) * sizeof(int);
+}
+// LDS of the planes-in-L2 form (nq > 160): lam, the Q row and the tile scan
+size_t h8l_lds_bytes(int nq, int n_inst) {
+    return 12 * (size_t)nq + (3 * (size_t)n_inst + 1) * sizeof(int);
 }
 size_t lds_limit() {
     static size_t lim = 0;
@@ -763,13 +775,27 @@ void launch_h3(const Layout &L, const H3Planes &H, const double *lam, const doub
                        dim3(64 * H3_WAVES), shm, s, L, H, lam, e, c, fac, x, base, y, sign, active);
     MSX_LAUNCH_CHECK();
 }
+
+template <int KT, bool BASE>
+void launch_h8l(const Layout &L, const H3Planes &H, const double *lam, const double *e,
+                const cx<float> *c, const float *fac, const cx<float> *x, const cx<float> *base,
+                cx<float> *y, float sign, const int *active, size_t shm, hipStream_t s) {
+    const void *kern = (const void *)diag_h8_kernel<KT, false, BASE, true, H8L_WAVES>;
+    set_dynamic_lds(kern, shm);
+    const long long tiles_max = (long long)L.n_inst * ((L.max_nc + 7) / 8);
+    const int grid = (int)std::min<long long>(cu_count(), (tiles_max + H8L_WAVES - 1) / H8L_WAVES);
+    hipLaunchKernelGGL((diag_h8_kernel<KT, false, BASE, true, H8L_WAVES>), dim3(std::max(grid, 1)),
+                       dim3(64 * H8L_WAVES), shm, s, L, H, lam, e, c, fac, x, base, y, sign, active);
+    MSX_LAUNCH_CHECK();
+}
 }  // namespace
 
 bool diag_h3_fits(const Layout &L, const H3Planes &H) {
-    if (!H.planes || H.nq == 0 || !L.nm || L.nz > H.nq || H.nq > 160) return false;
+    if (!H.planes || H.nq == 0 || !L.nm || L.nz > H.nq || H.nq > 224) return false;
     // 32-bit element indices, including vb + z nc for the padded nodes z < nq before the clamp
     const long long total = L.total > 0 ? L.total : (long long)L.n_inst * L.max_nc;
     if (total * L.nz + (long long)H.nq * L.max_nc >= (1LL << 31)) return false;
+    if (H.nq > 160) return h8l_lds_bytes(H.nq, L.n_inst) <= lds_limit();
     return h3_lds_bytes(H.nq, L.nz, L.n_inst, false) <= lds_limit();
 }
 
@@ -781,6 +807,18 @@ void launch_diag_h3(const Layout &L, const H3Planes &H, const double *lam, const
         throw Error(3, "split-FP16 diagonal blocks: shape or LDS size outside the kernel");
     if (cpw != 0 && cpw != 8 && cpw != 16)
         throw Error(3, "split-FP16 diagonal blocks: 8 or 16 channels per wavefront");
+    if (H.nq > 160) {
+        // planes in L2, 8 channels per wavefront whatever cpw asks
+        const size_t shm = h8l_lds_bytes(H.nq, L.n_inst);
+#define MSX_H8L(KT_)                                                                           \
+    if (H.nq == 32 * KT_) {                                                                    \
+        if (base) return launch_h8l<KT_, true>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s); \
+        return launch_h8l<KT_, false>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s);        \
+    }
+        MSX_H8L(6) MSX_H8L(7)
+#undef MSX_H8L
+        throw Error(3, "split-FP16 diagonal blocks: padded size not instantiated");
+    }
     const bool p1lo = h3_lds_bytes(H.nq, L.nz, L.n_inst, true) <= lds_limit();
     const size_t shm = h3_lds_bytes(H.nq, L.nz, L.n_inst, p1lo);
 #define MSX_H3(KT_)                                                                            \

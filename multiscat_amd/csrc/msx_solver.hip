@@ -480,7 +480,7 @@ struct msx_table {
     DBuf<double> T, A, Vm, lam, Q, QT, Qp, QTp, dcw;
     DBuf<double> H0, H0p;  // symmetrised H0 and its zero-padded copy (nzp x nzp)
     DBuf<float> Qf, QTf, Qpf, QTpf;
-    DBuf<_Float16> H3;  // split-FP16 planes of Q^T and Q (nz <= 160)
+    DBuf<_Float16> H3;  // split-FP16 planes of Q^T and Q (nz <= 224)
     int nq3 = 0;
     DBuf<int> info;
     DBuf<long long> ticks;  // phase stamps of the last eigensolve
@@ -671,10 +671,16 @@ struct Ops {
     }
     // FP32 vectors node-major when every FP32 kernel of this configuration reads and writes
     // that order: the DFT coupling of tables up to 32 x 32 and the diagonal blocks with Q^T in
-    // LDS (nzp <= 160)
+    // LDS (nzp <= 160) or the split-FP16 ones with their planes in L2 (nz <= 224)
     void set_fp32_order() {
         L32 = L;
-        L32.nm = (!direct && !vector_diag && t.nzp > 0 && t.nzp <= 160 && dft->np <= 32) ? 1 : 0;
+        bool nm_diag = t.nzp > 0 && t.nzp <= 160;
+        if (!nm_diag && split_f16 && t.nzp <= 224) {
+            Layout Ln = L;
+            Ln.nm = 1;
+            nm_diag = diag_h3_fits(Ln, t.h3());
+        }
+        L32.nm = (!direct && !vector_diag && nm_diag && dft->np <= 32) ? 1 : 0;
         if (L32.nm && !B.facf_nm_ready) {  // the FP32 factors in the same order as the vectors
             msx_batch::need(B.facf_nm_d, (size_t)B.n_elem());
             launch_to_float_real_nm(L32, B.fac_d.p, B.facf_nm_d.p, s);
@@ -835,7 +841,7 @@ struct Ops {
             if (split_f16_required)
                 throw Error(MSX_ERR_INVALID,
                             "inner_products = 2: split-FP16 diagonal blocks need node-major FP32 "
-                            "vectors, nz <= 160 and LDS for the batch's tile scan");
+                            "vectors, nz <= 224 and LDS for the batch's tile scan");
         }
         const char *name = sizeof(T) == 4 ? (mfma ? "diag_mfma_f32" : "diag_vector_f32")
                                           : (mfma ? "diag_mfma_f64" : "diag_vector_f64");
@@ -2240,7 +2246,7 @@ int msx_debug_apply_diagonal(int nz, const double *potential00, const double *ki
         if (variant == 3 && (precision != 1 || inverse != 1))
             throw Error(MSX_ERR_INVALID, "variants 3, 4 (split FP16) are the FP32 inverse only");
         if (variant == 3 && h3_padded_size(nz) == 0)
-            throw Error(MSX_ERR_INVALID, "split-FP16 diagonal blocks need nz <= 160");
+            throw Error(MSX_ERR_INVALID, "split-FP16 diagonal blocks need nz <= 224");
         // variant 2: the matrix-core form with 8 channels per wavefront (FP64)
         const int cpw = variant == 2 ? 8 : 0;
         if (variant == 2) variant = 0;

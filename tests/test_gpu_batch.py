@@ -111,6 +111,29 @@ def test_split_fp16_with_planes_in_l2():
     assert worst < 1e-6
 
 
+@pytest.mark.parametrize("nz", [170, 200])
+def test_split_fp16_with_all_planes_in_l2(nz):
+    """nz above 160: the split-FP16 diagonal blocks read all four planes from L2 (8 channels
+    per wavefront), with node-major FP32 vectors and the split-FP16 coupling; required
+    (inner_products = split_fp16) and checked against the oracle, and against the FP32-operand
+    path (channel-major vectors there) to the same bar."""
+    w = W.hex_batch(n_conditions=4, nz=nz, nk=9, cond_seed=21, channel_counts=[14, 33, 25, 40])
+    tab = BT.Table(w.potential, w.kinetic)
+    bt = BT.Batch(tab, w.conditions)
+    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11,
+                             inner_products="split_fp16"))
+    P_split = bt.intensities()
+    bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11, inner_products="fp32"))
+    P_fp32 = bt.intensities()
+    worst = 0.0
+    for c, P, Pf in zip(w.conditions, P_split, P_fp32):
+        psi = oracle.solve(1, 0, w.potential, c.e, c.wave_a, c.wave_b, c.wave_c, w.kinetic,
+                           channels=c.channels, eps=5e-13)
+        Pr = oracle.intensities(psi, c.wave_a, c.wave_b)[c.channels[0], c.channels[1]]
+        worst = max(worst, float(np.abs(P - Pr).max()), float(np.abs(Pf - Pr).max()))
+    assert worst < 1e-6
+
+
 @pytest.mark.parametrize("degree", [0, 2, 3, 4, 7])
 def test_neumann_degrees(small_batch, degree):
     """Block Jacobi with no Neumann factor and with the series to degree 2..7: the same

@@ -57,7 +57,8 @@ def test_diagonal_blocks(nz, nc, inverse):
                 assert bwd < bwd_tol, (prec, variant, bwd)
 
 
-@pytest.mark.parametrize("nz,nc", [(24, 16), (40, 51), (96, 7), (150, 40), (160, 33)])
+@pytest.mark.parametrize("nz,nc", [(24, 16), (40, 51), (96, 7), (150, 40), (160, 33), (170, 21),
+                                   (200, 45), (224, 9)])
 def test_split_fp16_diagonal_blocks(nz, nc):
     """D^-1 with split-FP16 operands (each FP32 operand a scaled pair of FP16 values, three
     FP16 products accumulated in FP32; variant 3): within the FP32 bars against the dense FP64
@@ -106,7 +107,7 @@ def test_split_fp16_diagonal_blocks(nz, nc):
 
 
 def test_split_fp16_rejects_shapes_outside_the_kernel():
-    nz, nc = 200, 4
+    nz, nc = 240, 4
     T = np.ascontiguousarray(B.kinetic_matrix(nz, -1.5, 8.0))
     V00 = np.zeros(nz, np.complex128)
     e = np.ones(nc)
