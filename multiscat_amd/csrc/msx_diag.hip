@@ -694,14 +694,14 @@ __global__ __launch_bounds__(64 * DG_WAVES, 2) void split_residual_kernel(
     }
 }
 
-bool split_residual_supported(int nzp) { return nzp % 16 == 0 && nzp <= 160; }
+bool split_residual_supported(int nzp) { return nzp % 16 == 0 && nzp <= 224; }
 
 void launch_split_residual(const Layout &L, const Layout &Lo, const double *H0p, int nzp,
                            const double *e, const cx<double> *c, const int *spec,
                            const cx<double> *bsrc, const cx<double> *x, const cx<double> *vx,
                            cx<float> *rho, const int *active, hipStream_t s) {
     if (!split_residual_supported(nzp) || nzp < L.nz)
-        throw Error(3, "split residual: nz <= 160 required");
+        throw Error(3, "split residual: nz <= 224 required");
     // wavefronts: (channel tiles) x (NT / RT row groups), DG_WAVES per workgroup
     auto grid = [&](int halves) {
         const long long waves = (long long)((L.max_nc + 15) / 16) * halves;
@@ -710,8 +710,11 @@ void launch_split_residual(const Layout &L, const Layout &Lo, const double *H0p,
     if (nzp <= 64)
         hipLaunchKernelGGL((split_residual_kernel<4, 4>), grid(1), dim3(64 * DG_WAVES), 0, s, L,
                            Lo, H0p, nzp, e, c, spec, bsrc, x, vx, rho, active);
-    else
+    else if (nzp <= 160)
         hipLaunchKernelGGL((split_residual_kernel<10, 5>), grid(2), dim3(64 * DG_WAVES), 0, s, L,
+                           Lo, H0p, nzp, e, c, spec, bsrc, x, vx, rho, active);
+    else
+        hipLaunchKernelGGL((split_residual_kernel<14, 7>), grid(2), dim3(64 * DG_WAVES), 0, s, L,
                            Lo, H0p, nzp, e, c, spec, bsrc, x, vx, rho, active);
     MSX_LAUNCH_CHECK();
 }
