@@ -566,8 +566,10 @@ static void launch_diag_lds_any(const Layout &L, const EigTables<float> &E, cons
 
 // Unpreconditioned residual of the scattering system for the mixed solver's refinement:
 //   rho_j = s_j - D_j x_j - (V x)_j,   D_j x_j = (H0 + e_j) x_j - c_j x_j(N) e_N,
-// s the scattering source (b at node N of each instance's specular channel), V x given (the
-// FP64 coupling), x and V x in the vector order of L, written in FP32 in the vector order of
+// s the scattering source (b at node N of each instance's specular channel; src null) or a
+// given source vector src (the adjoint's), V x given (the FP64 coupling, of the flipped table
+// for the transposed system: D is complex symmetric), x, V x and src in the vector order of L,
+// written in FP32 in the vector order of
 // Lo (the inner solver's). H0 x_j is one FP64 product on the matrix cores (D^-1 takes two):
 // 16 channels (real and imaginary parts as two column tiles) share RT row tiles of Y^T = H0 X^T
 // per wavefront, so NT / RT wavefronts cover a channel tile (for NT = 10, two of five tiles:
@@ -580,8 +582,8 @@ __global__ __launch_bounds__(64 * DG_WAVES, 2) void split_residual_kernel(
     Layout L, Layout Lo, const double *__restrict__ H0p, int nzp, const double *__restrict__ e,
     const cx<double> *__restrict__ c, const int *__restrict__ spec,
     const cx<double> *__restrict__ bsrc, const cx<double> *__restrict__ x,
-    const cx<double> *__restrict__ vx, cx<float> *__restrict__ rho,
-    const int *__restrict__ active) {
+    const cx<double> *__restrict__ vx, const cx<double> *__restrict__ src,
+    cx<float> *__restrict__ rho, const int *__restrict__ active) {
     static_assert(NT % RT == 0, "row tiles split evenly over the wavefronts of a channel tile");
     constexpr int NHALF = NT / RT;  // wavefronts per channel tile
     using M = Mfma<double>;
@@ -656,7 +658,7 @@ __global__ __launch_bounds__(64 * DG_WAVES, 2) void split_residual_kernel(
     // tile's before this tile's stores), and a tile inside [0, nz) stored with no condition.
     // With loads and stores under per-element conditions the compiler waited for every
     // outstanding memory operation before each store: 40 round trips per tile.
-    cx<double> xz[4], vz[4];
+    cx<double> xz[4], vz[4], sz[4];
     auto load_t = [&](int mt) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -664,6 +666,7 @@ __global__ __launch_bounds__(64 * DG_WAVES, 2) void split_residual_kernel(
             const long long i = vi(mt < NTL && z < nz ? z : 0);
             xz[v] = x[i];
             vz[v] = vx[i];
+            sz[v] = src ? src[i] : mk(0.0, 0.0);
         }
     };
     load_t(mt0);
@@ -676,8 +679,8 @@ __global__ __launch_bounds__(64 * DG_WAVES, 2) void split_residual_kernel(
             const int z = 16 * mt + M::row(g, v);
             cx<double> dx = mk(acc[t][0][v] + ej * xz[v].re, acc[t][1][v] + ej * xz[v].im);
             if (z == nz - 1) dx = dx - cj * xN;
-            cx<double> r = mk(0.0, 0.0) - dx - vz[v];
-            if (is_spec && z == nz - 1) r = r + bs;
+            cx<double> r = sz[v] - dx - vz[v];
+            if (!src && is_spec && z == nz - 1) r = r + bs;
             out[v] = mk((float)r.re, (float)r.im);
         }
         if (t + 1 < RT) load_t(mt + 1);
@@ -699,7 +702,8 @@ bool split_residual_supported(int nzp) { return nzp % 16 == 0 && nzp <= 224; }
 void launch_split_residual(const Layout &L, const Layout &Lo, const double *H0p, int nzp,
                            const double *e, const cx<double> *c, const int *spec,
                            const cx<double> *bsrc, const cx<double> *x, const cx<double> *vx,
-                           cx<float> *rho, const int *active, hipStream_t s) {
+                           cx<float> *rho, const int *active, hipStream_t s,
+                           const cx<double> *src) {
     if (!split_residual_supported(nzp) || nzp < L.nz)
         throw Error(3, "split residual: nz <= 224 required");
     // wavefronts: (channel tiles) x (NT / RT row groups), DG_WAVES per workgroup
@@ -709,13 +713,13 @@ void launch_split_residual(const Layout &L, const Layout &Lo, const double *H0p,
     };
     if (nzp <= 64)
         hipLaunchKernelGGL((split_residual_kernel<4, 4>), grid(1), dim3(64 * DG_WAVES), 0, s, L,
-                           Lo, H0p, nzp, e, c, spec, bsrc, x, vx, rho, active);
+                           Lo, H0p, nzp, e, c, spec, bsrc, x, vx, src, rho, active);
     else if (nzp <= 160)
         hipLaunchKernelGGL((split_residual_kernel<10, 5>), grid(2), dim3(64 * DG_WAVES), 0, s, L,
-                           Lo, H0p, nzp, e, c, spec, bsrc, x, vx, rho, active);
+                           Lo, H0p, nzp, e, c, spec, bsrc, x, vx, src, rho, active);
     else
         hipLaunchKernelGGL((split_residual_kernel<14, 7>), grid(2), dim3(64 * DG_WAVES), 0, s, L,
-                           Lo, H0p, nzp, e, c, spec, bsrc, x, vx, rho, active);
+                           Lo, H0p, nzp, e, c, spec, bsrc, x, vx, src, rho, active);
     MSX_LAUNCH_CHECK();
 }
 

@@ -297,12 +297,14 @@ void launch_add_ints(int n, const int *src, int *dst, hipStream_t s);
 void launch_to_float(long long n, const cx<double> *src, cx<float> *dst, hipStream_t s);
 // rho = s - D x - V x (FP64 sums, FP32 result in the vector order of Lo): the residual of the
 // scattering system before the block-Jacobi preconditioner, with D_j x_j from H0p (the
-// symmetrised H0 zero-padded to nzp, FP64 matrix cores) and V x given (msx_diag.hip).
+// symmetrised H0 zero-padded to nzp, FP64 matrix cores) and V x given (msx_diag.hip); s the
+// scattering source (src null) or the vector src in the order of L.
 bool split_residual_supported(int nzp);
 void launch_split_residual(const Layout &L, const Layout &Lo, const double *H0p, int nzp,
                            const double *e, const cx<double> *c, const int *spec,
                            const cx<double> *bsrc, const cx<double> *x, const cx<double> *vx,
-                           cx<float> *rho, const int *active, hipStream_t s);
+                           cx<float> *rho, const int *active, hipStream_t s,
+                           const cx<double> *src = nullptr);
 // FP64 channel-major -> FP32 in the vector order of L (a tiled transpose when L.nm); n is the
 // element count of the batch.
 void launch_to_float_vec(const Layout &L, long long n, const cx<double> *src, cx<float> *dst,

@@ -1261,17 +1261,20 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     ops.split_f16_required = o.inner_products == 2;
     ops.set_fp32_order();
     // The mixed solver's refinement residual as D^-1 (s - D x - V x) with D^-1 in FP32
-    // (split_residual) instead of d0 - x - D^-1 V x in FP64: for the scattering source and
-    // tolerances down to 1e-9, where the FP64 cancellation in s - D x - V x (about 1e-16 |D| |x|)
-    // stays far below the residuals the test has to resolve
+    // (split_residual) instead of d0 - x - D^-1 V x in FP64: for the scattering source or a
+    // given one (the adjoint's: D is complex symmetric, and the transposed coupling is the
+    // flipped table's) at tolerances down to 1e-9, where the FP64 cancellation in s - D x - V x
+    // (about 1e-16 |D| |x|) stays far below the residuals the test has to resolve
     const bool mixed = o.precision == MSX_PRECISION_MIXED;
-    const bool split = mixed && scatter_source && !transposed && o.eps >= 1e-9 &&
-                       split_residual_supported(t.nzp) && t.nzp >= t.nz;
-    // With the split residual and node-major FP32 vectors, the FP64 refinement vectors (the
-    // state, D^-1 s, the coupling of the state) are node-major too: the FP64 coupling, the
-    // split residual and the solution update then read and write whole lines. The batch's
-    // state is left in that order (psi_nm).
-    const bool nm64 = split && ops.L32.nm && x == B.psi_d.p;
+    const bool split = mixed && (scatter_source || src_in) && o.eps >= 1e-9 &&
+                       split_residual_supported(t.nzp) && t.nzp >= t.nz &&
+                       (scatter_source || o.method == MSX_METHOD_JACOBI);
+    // With the split residual, the scattering source and node-major FP32 vectors, the FP64
+    // refinement vectors (the state, D^-1 s, the coupling of the state) are node-major too: the
+    // FP64 coupling, the split residual and the solution update then read and write whole
+    // lines. The batch's state is left in that order (psi_nm). A given source stays in the
+    // batch order, and so does everything of its solve.
+    const bool nm64 = split && scatter_source && !transposed && ops.L32.nm && x == B.psi_d.p;
     if (nm64) ops.L64.nm = 1;
     if (o.autotune == 1 && o.method == MSX_METHOD_JACOBI &&
         tune_key(B, ops, o) != B.tuning.key)
@@ -1409,11 +1412,13 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
                 if (profiling()) {
                     const double nzd = t.nz;
                     fl = ops.over_active(outer.active.p, [&](int nc) { return nc * (4.0 * nzd * nzd + 16.0 * nzd); });
-                    by = ops.over_active(outer.active.p, [&](int nc) { return nc * nzd * 40.0; });
+                    const double per = scatter_source ? 40.0 : 56.0;  // + the source vector
+                    by = ops.over_active(outer.active.p, [&](int nc) { return nc * nzd * per; });
                 }
                 ProfScope ps("residual_split_f64", s, fl, by);
                 launch_split_residual(ops.L64, ops.L32, t.H0p.p, t.nzp, B.e_d.p, B.c_d.p, B.spec_d.p,
-                                      B.bsrc_d.p, x, B.tmp1.p, B.f_tmp1.p, outer.active.p, s);
+                                      B.bsrc_d.p, x, B.tmp1.p, B.f_tmp1.p, outer.active.p, s,
+                                      scatter_source ? nullptr : src);
             }
             ops.dinv<float>(B.f_tmp1.p, nullptr, B.f_tmp2.p, 1.0f, outer.active.p);
             inner.set_all_active(act.data());
