@@ -523,52 +523,23 @@ __device__ __forceinline__ void tred_step(int n, int k, double *M, int b0, int l
     if (tid == 0) x[0] = u0;  // the reflector is stored in row k, which no later step reads
 }
 
-// A (n x n row-major, symmetric; overwritten) -> d, e (e[i] couples rows i and i + 1), the
-// reflectors u_k in row k of A (columns k+1 .. n-1) and h (h[k] = 0: no reflector). lds_rows
-// rows of stride lds_ld fit the dynamic LDS after the partial sums and q.
+// Steps 0 .. k_end - 1 of the reduction of A (n x n row-major, symmetric, both triangles
+// stored; overwritten) in place in global memory, for n > 173, where the trailing block of
+// the first steps does not fit LDS: reflector u_k into row k of A (columns k+1 .. n-1), h[k]
+// (0: no reflector) and e[k]; rows and columns k_end .. n-1 are left to
+// eig_tred_packed_kernel.
 __global__ __launch_bounds__(1024) void eig_tred_kernel(int n, double *A, double *h_out,
-                                                        double *d_out, double *e_out,
-                                                        int lds_rows, int lds_ld,
-                                                        long long *ticks) {
+                                                        double *e_out, long long *ticks,
+                                                        int k_end) {
     extern __shared__ double tl[];
-    double *pp = tl, *q = pp + TRED_WAVES * 64, *Al = q + n;
-    const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
-    if (ticks && tid == 0) ticks[0] = wall_clock64();
-    int b0 = -1;  // first row held in LDS (rows and columns b0 .. n-1), -1 while none
-    for (int k = 0; k + 2 < n; ++k) {
-        if (b0 < 0 && n - k <= lds_rows) {  // rows k .. n-1 fit from here on
-            for (int r = k + wave; r < n; r += TRED_WAVES)
-                for (int c = k + lane; c < n; c += 64)
-                    Al[(r - k) * lds_ld + (c - k)] = A[(long long)r * n + c];
-            __syncthreads();
-            b0 = k;
-        }
-        if (b0 >= 0) tred_step(n, k, Al, b0, lds_ld, pp, q, h_out, e_out);
-        else tred_step(n, k, A, 0, n, pp, q, h_out, e_out);
-    }
-    __syncthreads();
-    // the LDS rows back into A (their reflectors and the final 2 x 2 block), then d and e
-    if (b0 >= 0)
-        for (int r = b0 + wave; r < n; r += TRED_WAVES)
-            for (int c = r + lane; c < n; c += 64)
-                A[(long long)r * n + c] = Al[(r - b0) * lds_ld + (c - b0)];
-    __syncthreads();
-    for (int r = tid; r < n; r += blockDim.x) d_out[r] = A[(long long)r * n + r];
-    if (tid == 0) {
-        if (n >= 2) {
-            e_out[n - 2] = A[(long long)(n - 2) * n + n - 1];
-            h_out[n - 2] = 0.0;
-        }
-        e_out[n - 1] = 0.0;
-        h_out[n - 1] = 0.0;
-        if (ticks) ticks[1] = ticks[2] = wall_clock64();
-    }
+    double *pp = tl, *q = pp + TRED_WAVES * 64;
+    if (ticks && threadIdx.x == 0) ticks[0] = wall_clock64();
+    for (int k = 0; k + 2 < n && k < k_end; ++k) tred_step(n, k, A, 0, n, pp, q, h_out, e_out);
 }
 
-// Packed form, for n <= 173: the lower triangle of A packed by rows in LDS for the whole
-// reduction (element (r, c), c <= r, at r (r + 1) / 2 + c: 90 KB at n = 150, where the full
-// matrix would not fit and the first steps ran from global memory). Every wavefront forms the
+// Packed form: the lower triangle of A packed by rows in LDS for the whole reduction when
+// n <= 173 (element (r, c), c <= r, at r (r + 1) / 2 + c: 90 KB at n = 150, where the full
+// matrix would not fit), and for its last n - k0 rows (n - k0 <= 189) above that. Every wavefront forms the
 // reflector itself from column k (u in registers, lane j holding coordinates j, j + 64,
 // j + 128, and in a private LDS copy that the product reads as broadcasts), and after the
 // partial sums of p = A22 u every wavefront completes p, K and q for all coordinates the same
@@ -618,21 +589,33 @@ __device__ __forceinline__ double coord(const double (&v)[TP_CH], int r) {
     return s == 0 ? a : (s == 1 ? b : c);
 }
 
+//
+// k0 > 0: steps 0 .. k0 - 1 ran in global memory (eig_tred_kernel with k_end = k0), and the
+// triangle holds rows and columns k0 .. n-1 only (element (r, c) at P(r, c) =
+// (r - k0)(r - k0 + 1) / 2 + c - k0). SHU: one copy of u shared by the wavefronts instead of one
+// a wavefront (every wavefront writes the same values before it reads them, and the step's
+// barriers keep a step's reads from the next step's writes), which leaves LDS for a larger
+// triangle.
+template <bool SHU>
 __global__ __launch_bounds__(1024) void eig_tred_packed_kernel(int n, double *A, double *h_out,
                                                                double *d_out, double *e_out,
-                                                               long long *ticks) {
+                                                               long long *ticks, int k0) {
     extern __shared__ double tl[];
-    // partial sums [TRED_WAVES][64], per-wavefront copies of u [TRED_WAVES][64 TP_CH], triangle
-    double *pp = tl, *Lp = pp + TRED_WAVES * 64 * (1 + TP_CH);
+    constexpr int UCOPIES = SHU ? 1 : TRED_WAVES;
+    // partial sums [TRED_WAVES][64], copies of u [UCOPIES][64 TP_CH], triangle
+    double *pp = tl, *Lp = pp + TRED_WAVES * 64 + UCOPIES * 64 * TP_CH;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    double *uw = pp + TRED_WAVES * 64 + wave * 64 * TP_CH;
-    if (ticks && tid == 0) ticks[0] = wall_clock64();
-    auto tri = [](int r) { return (r * (r + 1)) >> 1; };
-    for (int r = wave; r < n; r += TRED_WAVES)
-        for (int c = lane; c <= r; c += 64) Lp[tri(r) + c] = A[(long long)r * n + c];
+    double *uw = pp + TRED_WAVES * 64 + (SHU ? 0 : wave * 64 * TP_CH);
+    if (ticks && tid == 0 && k0 == 0) ticks[0] = wall_clock64();
+    auto P = [k0](int r, int c) {
+        const int rr = r - k0;
+        return ((rr * (rr + 1)) >> 1) + c - k0;
+    };
+    for (int r = k0 + wave; r < n; r += TRED_WAVES)
+        for (int c = k0 + lane; c <= r; c += 64) Lp[P(r, c)] = A[(long long)r * n + c];
     __syncthreads();
-    for (int k = 0; k + 2 < n; ++k) {
+    for (int k = k0; k + 2 < n; ++k) {
         const int m = n - k - 1, b = k + 1;  // block coordinates t = 0 .. m-1: rows b + t
         // x[t] = A[b + t][k] (column k below the diagonal), u = x - g e_0
         double u[TP_CH];
@@ -640,7 +623,7 @@ __global__ __launch_bounds__(1024) void eig_tred_packed_kernel(int n, double *A,
 #pragma unroll
         for (int s = 0; s < TP_CH; ++s) {
             const int t = lane + 64 * s;
-            u[s] = t < m ? Lp[tri(b + t) + k] : 0.0;
+            u[s] = t < m ? Lp[P(b + t, k)] : 0.0;
             if (t >= 1) loc = fma(u[s], u[s], loc);
         }
         const double sigma = wave_sum_dpp(loc);
@@ -677,7 +660,7 @@ __global__ __launch_bounds__(1024) void eig_tred_packed_kernel(int n, double *A,
         const int cs = wave % CS, rgp = wave / CS;
         if (rgp < RGP) {
             const int c = 64 * cs + lane, cc = c < m ? c : m - 1;  // past m: clamped, unused
-            const double *colp = Lp + tri(b + cc) + b;  // colp[r] = A22[r][cc] for r <= cc
+            const double *colp = Lp + P(b + cc, b);  // colp[r] = A22[r][cc] for r <= cc
             const int rb = (m * rgp) / RGP, re = (m * (rgp + 1)) / RGP;
             const int c0 = 64 * cs, c1 = c0 + 64;  // rows [c0, c1): mixed
             double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -694,8 +677,8 @@ __global__ __launch_bounds__(1024) void eig_tred_packed_kernel(int n, double *A,
             };
             run(rb, min(re, c0), [&](int r) { return colp[r]; });
             run(max(rb, c0), min(re, c1),
-                [&](int r) { return r >= cc ? Lp[tri(b + r) + b + cc] : colp[r]; });
-            run(max(rb, c1), re, [&](int r) { return Lp[tri(b + r) + b + cc]; });
+                [&](int r) { return r >= cc ? Lp[P(b + r, b + cc)] : colp[r]; });
+            run(max(rb, c1), re, [&](int r) { return Lp[P(b + r, b + cc)]; });
             pp[wave * 64 + lane] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) +
                                    ((acc[4] + acc[5]) + (acc[6] + acc[7]));
         }
@@ -719,8 +702,8 @@ __global__ __launch_bounds__(1024) void eig_tred_packed_kernel(int n, double *A,
         // stored (four at a time measured slower)
         for (int r = wave; r < m; r += 2 * TRED_WAVES) {
             const int r2 = r + TRED_WAVES;
-            double *row = Lp + tri(b + r) + b;
-            double *row2 = Lp + tri(b + (r2 < m ? r2 : r)) + b;
+            double *row = Lp + P(b + r, b);
+            double *row2 = Lp + P(b + (r2 < m ? r2 : r), b);
             double a1[TP_CH], a2[TP_CH];
 #pragma unroll
             for (int s = 0; s < TP_CH; ++s) {
@@ -739,10 +722,10 @@ __global__ __launch_bounds__(1024) void eig_tred_packed_kernel(int n, double *A,
         }
         __syncthreads();
     }
-    for (int r = tid; r < n; r += blockDim.x) d_out[r] = Lp[tri(r) + r];
+    for (int r = tid; r < n; r += blockDim.x) d_out[r] = r < k0 ? A[(long long)r * n + r] : Lp[P(r, r)];
     if (tid == 0) {
         if (n >= 2) {
-            e_out[n - 2] = Lp[tri(n - 1) + n - 2];
+            e_out[n - 2] = Lp[P(n - 1, n - 2)];
             h_out[n - 2] = 0.0;
         }
         e_out[n - 1] = 0.0;
@@ -850,29 +833,34 @@ void launch_eigensystem_dc(int n, double *A, double *work, double *lam_sorted,
     double *h = lam + n;
     if (ticks) MSX_HIP(hipMemsetAsync(ticks, 0, 8 * sizeof(long long), s));
     // Householder stage: the packed lower triangle in LDS throughout when it fits 150 KiB
-    // (n <= 173); otherwise rows b0 .. n-1 of stride lds_ld (odd) fit 150 KiB of LDS beside the
-    // partial sums and q, and the steps before that work in place in A
-    const size_t packed =
-        sizeof(double) * ((size_t)TRED_WAVES * 64 * (1 + TP_CH) + (size_t)n * (n + 1) / 2);
-    if (n <= 64 * TP_CH && packed <= 150 * 1024) {
-        set_dynamic_lds((const void *)eig_tred_packed_kernel, packed);
-        hipLaunchKernelGGL(eig_tred_packed_kernel, dim3(1), dim3(64 * TRED_WAVES), packed, s, n, A,
-                           h, d, e, ticks);
+    // (n <= 173); up to n = 1024, the first k0 steps work in place in A and the packed kernel
+    // takes the last n - k0 rows (at most 189: the triangle with one shared copy of u fits
+    // 150 KiB, and a lane's three chunks cover the 188 block coordinates)
+    constexpr size_t lds_cap = 150 * 1024;
+    auto packed_bytes = [](int rows, int ucopies) {
+        return sizeof(double) * ((size_t)TRED_WAVES * 64 + (size_t)ucopies * 64 * TP_CH +
+                                 (size_t)rows * (rows + 1) / 2);
+    };
+    if (n <= 64 * TP_CH && packed_bytes(n, TRED_WAVES) <= lds_cap) {
+        const size_t shm = packed_bytes(n, TRED_WAVES);
+        set_dynamic_lds((const void *)eig_tred_packed_kernel<false>, shm);
+        hipLaunchKernelGGL(eig_tred_packed_kernel<false>, dim3(1), dim3(64 * TRED_WAVES), shm, s,
+                           n, A, h, d, e, ticks, 0);
         MSX_LAUNCH_CHECK();
     } else {
-        const size_t fixed = sizeof(double) * ((size_t)TRED_WAVES * 64 + n);
-        const size_t budget = 150 * 1024 - fixed;
-        int rows = std::min(n, 1);
-        while (rows < n) {
-            const int r = rows + 1, ld = r | 1;
-            if (sizeof(double) * (size_t)r * ld > budget) break;
-            rows = r;
+        int k0 = std::max(0, n - 1 - 64 * TP_CH);
+        while (packed_bytes(n - k0, 1) > lds_cap) ++k0;
+        if (k0 > 0) {
+            const size_t fixed = sizeof(double) * ((size_t)TRED_WAVES * 64 + n);
+            set_dynamic_lds((const void *)eig_tred_kernel, fixed);
+            hipLaunchKernelGGL(eig_tred_kernel, dim3(1), dim3(64 * TRED_WAVES), fixed, s, n, A,
+                               h, e, ticks, k0);
+            MSX_LAUNCH_CHECK();
         }
-        const int ld = rows | 1;
-        const size_t shm = fixed + sizeof(double) * (size_t)rows * ld;
-        set_dynamic_lds((const void *)eig_tred_kernel, shm);
-        hipLaunchKernelGGL(eig_tred_kernel, dim3(1), dim3(64 * TRED_WAVES), shm, s, n, A, h, d, e,
-                           rows, ld, ticks);
+        const size_t shm = packed_bytes(n - k0, 1);
+        set_dynamic_lds((const void *)eig_tred_packed_kernel<true>, shm);
+        hipLaunchKernelGGL(eig_tred_packed_kernel<true>, dim3(1), dim3(64 * TRED_WAVES), shm, s,
+                           n, A, h, d, e, ticks, k0);
         MSX_LAUNCH_CHECK();
     }
     const unsigned leaf_blocks = (unsigned)std::min<size_t>((n2 + 255) / 256, 4096);

@@ -28,8 +28,9 @@ def _orthogonal(n, rng):
 def _matrices():
     rng = np.random.default_rng(7)
     out = {}
-    # 173 and 174: the largest size whose packed triangle fits LDS, and the first that does not
-    for n in (2, 3, 5, 17, 64, 150, 173, 174, 300):
+    # 173: the largest size whose packed triangle (with a copy of u per wavefront) fits LDS;
+    # 174 .. 189 with one shared copy; from 190 the first n - 189 steps run in global memory
+    for n in (2, 3, 5, 17, 64, 150, 173, 174, 189, 190, 200, 300):
         a = rng.standard_normal((n, n))
         out[f"random{n}"] = a + a.T
     m = 10
@@ -45,7 +46,7 @@ def _matrices():
     out["triple_eigenvalues90"] = (q * np.repeat([-2.0, 0.5, 3.0], 30)) @ q.T
     out["graded80"] = _tridiag(10.0 ** np.linspace(-10, 10, 80), 10.0 ** np.linspace(-10, 10, 79))
     out["tiny_coupling100"] = _tridiag(np.arange(100.0), np.full(99, 1e-12))
-    for nz in (100, 150, 300):
+    for nz in (100, 150, 200, 300):
         w = W.hex_batch(n_conditions=1, nz=nz, nk=3, cond_seed=2, channel_counts=[5])
         out[f"H0_nz{nz}"] = w.kinetic + np.diag(w.potential[0, 0].real)
     return out
