@@ -18,6 +18,7 @@
 #include <memory>
 #include <cstdlib>
 #include <fstream>
+#include <list>
 #include <map>
 #include <mutex>
 #include <set>
@@ -85,12 +86,31 @@ struct Krylov {
     // the last fall forward once.
     int hint_count = -1;
     // Arnoldi iterations captured as HIP graphs, keyed by iteration, operator form, flag slot
-    // and launch shape; valid while the operator's buffers and options match graph_sig
-    std::map<long long, hipGraphExec_t> graphs;
-    std::string graph_sig;
+    // and launch shape, in one set per signature (the operator's buffers and options): the
+    // GRAPH_SETS signatures used last keep theirs, so solves that alternate (a forward solve
+    // and its adjoint on the same workspace) replay instead of capturing again
+    static constexpr int GRAPH_SETS = 4;
+    std::list<std::pair<std::string, std::map<long long, hipGraphExec_t>>> graph_sets;  // newest first
+    static void destroy_set(std::map<long long, hipGraphExec_t> &g) {
+        for (auto &kv : g) (void)hipGraphExecDestroy(kv.second);
+        g.clear();
+    }
+    std::map<long long, hipGraphExec_t> &graphs_for(const std::string &sig) {
+        for (auto it = graph_sets.begin(); it != graph_sets.end(); ++it)
+            if (it->first == sig) {
+                graph_sets.splice(graph_sets.begin(), graph_sets, it);
+                return graph_sets.front().second;
+            }
+        if ((int)graph_sets.size() >= GRAPH_SETS) {
+            destroy_set(graph_sets.back().second);
+            graph_sets.pop_back();
+        }
+        graph_sets.emplace_front(sig, std::map<long long, hipGraphExec_t>());
+        return graph_sets.front().second;
+    }
     void clear_graphs() {
-        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
-        graphs.clear();
+        for (auto &g : graph_sets) destroy_set(g.second);
+        graph_sets.clear();
     }
 
     Krylov(const std::vector<long long> &start, const std::vector<long long> &len, int kmax_,
@@ -249,7 +269,8 @@ struct Krylov {
 // normalised by the next operator call instead of a separate pass.
 // graph_sig (optional): replay each Arnoldi iteration from a HIP graph captured the first
 // time that (iteration, operator form, flag slot, launch shape) comes up; graph_sig names the
-// operator's buffers and options, and a different one drops the captured graphs. shape_key
+// operator's buffers and options, and selects the set of captured graphs (the workspace keeps
+// the sets of its last few signatures). shape_key
 // gives the host-side launch choices that depend on the active count (part of the key).
 // stop_below: once the host sees at most this many instances iterating, the solve stops after
 // the iteration already queued and every instance keeps the solution of its current subspace.
@@ -262,17 +283,14 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
            const std::function<int()> *shape_key = nullptr, int stop_below = 0) {
     hipStream_t s = kr.s;
     const bool use_graphs = graph_sig != nullptr && !profiling();
+    std::map<long long, hipGraphExec_t> *gset = nullptr;  // this signature's captured iterations
     if (use_graphs) {
         // the recurrence state goes to the Givens kernel by value: its tolerance, window and
         // per-instance tolerances are part of what a captured iteration replays
         char extra[160];
         std::snprintf(extra, sizeof extra, "|eps%.17g|win%d|einst%p|orth%d", eps, window,
                       (const void *)eps_inst, orth_passes);
-        const std::string sig = *graph_sig + extra;
-        if (sig != kr.graph_sig) {
-            kr.clear_graphs();
-            kr.graph_sig = sig;
-        }
+        gset = &kr.graphs_for(*graph_sig + extra);
         // every flag slot posted by an earlier solve has been waited for: start at slot 0, so
         // iteration k of a cycle always writes the same slot
         kr.next_slot = 0;
@@ -384,9 +402,9 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
                 const long long shape = shape_key ? (long long)(*shape_key)() : 0;
                 const long long key =
                     ((((long long)k * 2 + (scaled_in ? 1 : 0)) * 2 + kr.next_slot) << 20) + shape;
-                auto it = kr.graphs.find(key);
+                auto it = gset->find(key);
                 hipGraphExec_t exec = nullptr;
-                if (it == kr.graphs.end()) {
+                if (it == gset->end()) {
                     hipGraph_t graph = nullptr;
                     MSX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                     try {
@@ -401,7 +419,7 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
                     const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
                     (void)hipGraphDestroy(graph);
                     MSX_HIP(e);
-                    kr.graphs[key] = exec;
+                    (*gset)[key] = exec;
                 } else {
                     exec = it->second;
                 }
@@ -588,6 +606,10 @@ struct msx_batch {
         bool done = false;
         std::string key, report;
     } tuning;
+    // Arnoldi block lengths chosen at the batch's first tuned solve of each precision (0: none
+    // yet); later keys of the batch (another vector order: a forward solve and its adjoint)
+    // keep them, so the Krylov workspaces sized for them are not rebuilt
+    long long kept_chunk32 = 0, kept_chunk64 = 0;
 
     long long n_elem() const { return total_ch * t->nz; }
     Layout layout() const {
@@ -1131,6 +1153,19 @@ long long tune_chunk(msx_batch &B, LaunchTimer &tm, std::string &rep) {
     return cands[pick_with_margin(ts, 1)];
 }
 
+// The batch's Arnoldi block lengths: the first value tuned for each precision is kept and
+// replaces what a later key of the same batch brings (the block length depends on the batch's
+// vector lengths only, not on the vector order or the operator form); the precision a key does
+// not tune takes the kept value too, if there is one.
+void keep_chunks(msx_batch &B, bool mixed) {
+    long long &kept = mixed ? B.kept_chunk32 : B.kept_chunk64;
+    long long &cur = mixed ? B.tuning.chunk32 : B.tuning.chunk64;
+    if (kept) cur = kept;
+    else kept = cur;
+    const long long other = mixed ? B.kept_chunk64 : B.kept_chunk32;
+    if (other) (mixed ? B.tuning.chunk64 : B.tuning.chunk32) = other;
+}
+
 void autotune(msx_batch &B, Ops &ops, const msx_solve_options &o) {
     const std::string key = tune_key(B, ops, o);
     {
@@ -1139,6 +1174,7 @@ void autotune(msx_batch &B, Ops &ops, const msx_solve_options &o) {
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) {
             B.tuning = it->second;
+            keep_chunks(B, o.precision == MSX_PRECISION_MIXED);
             B.tuning.report = "{\"source\": \"cache\", \"key\": \"" + key + "\", \"dft_nodes_per_wave_f32\": " +
                               std::to_string(B.tuning.dft_npw32) + ", \"dft_nodes_per_wave_f64\": " +
                               std::to_string(B.tuning.dft_npw64) + ", \"arnoldi_block_f32\": " +
@@ -1222,8 +1258,8 @@ void autotune(msx_batch &B, Ops &ops, const msx_solve_options &o) {
         T.diag32_cpw = cpw_c[pick_with_margin(ts, 0)];
         rep += "}, ";
     }
-    if (mixed) T.chunk32 = tune_chunk<float>(B, tm, rep);
-    else T.chunk64 = tune_chunk<double>(B, tm, rep);
+    if (mixed) T.chunk32 = B.kept_chunk32 ? B.kept_chunk32 : tune_chunk<float>(B, tm, rep);
+    else T.chunk64 = B.kept_chunk64 ? B.kept_chunk64 : tune_chunk<double>(B, tm, rep);
     rep += "\"dft_nodes_per_wave_f32\": " + std::to_string(T.dft_npw32) +
            ", \"dft_nodes_per_wave_f64\": " + std::to_string(T.dft_npw64) +
            ", \"arnoldi_block_f32\": " + std::to_string(T.chunk32) +
@@ -1233,6 +1269,7 @@ void autotune(msx_batch &B, Ops &ops, const msx_solve_options &o) {
     T.done = true;
     T.report = rep;
     B.tuning = T;
+    keep_chunks(B, mixed);
     std::lock_guard<std::mutex> lock(g_tune_mu);
     g_tune_cache[key] = T;
     append_tune_file(T);
