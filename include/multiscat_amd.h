@@ -189,8 +189,11 @@ typedef struct {
     int warm_start;   /* 1: GMRES starts from the batch's current state (msx_batch_set_state, or the
                          last solve) instead of 0; tolerance still relative to the source, as an
                          initial guess does in the reference's SciPy path (_gmres.py:106-114) */
-    int inner_orth;   /* Gram-Schmidt passes of the FP32 inner Arnoldi (mixed precision): 0 = default,
-                         1 = classical single pass, 2 = classical with one reorthogonalisation */
+    int inner_orth;   /* Gram-Schmidt passes of the FP32 inner Arnoldi (mixed precision): 0 = default
+                         (Neumann degree <= 2: one classical pass, and a second for the conditions
+                         whose new vector kept a projection on the basis above 1e-4 of its norm;
+                         higher degrees: two), 1 = classical single pass, 2 = classical with one
+                         reorthogonalisation */
     int autotune;     /* 1 (block Jacobi): at the first solve of a batch shape, time the launch
                          shapes of the DFT coupling (nodes per wavefront), the Arnoldi vector
                          kernels (block length) and the FP64 diagonal blocks (channels per

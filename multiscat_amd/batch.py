@@ -34,7 +34,7 @@ class SolveOptions:
     coupling: str = "auto"  # or "direct"
     diag: str = "auto"  # "auto" (fastest), "vector" or "mfma" (matrix cores, nz <= 320)
     warm_start: bool = False  # start from the batch's current state (Batch.set_state)
-    inner_orth: int = 0  # FP32 inner Arnoldi: 0 default, 1 single classical pass, 2 CGS2
+    inner_orth: int = 0  # FP32 inner Arnoldi: 0 one pass + selective second, 1 one pass, 2 CGS2
     autotune: bool = True  # time the main kernels' launch shapes at a batch shape's first solve
     graphs: bool = True  # replay each Arnoldi iteration from a captured HIP graph
     # mixed precision, operands of the inner operator's products on the matrix cores (D^-1

@@ -31,6 +31,9 @@ struct KState {
     double eps;
     int window;
     const double *eps_inst = nullptr;  // [n_inst] per-instance tolerance, overrides eps if set
+    // Arnoldi on F with A = hshift I + hscale F (the Krylov spaces of A and F are the same):
+    // each Hessenberg column of F enters as hshift e_k + hscale h (1 and 0: F is A itself)
+    double hscale = 1.0, hshift = 0.0;
 };
 
 // dst[b] = src[b * stride].re for b < n (dst may be pinned host memory mapped for the device)
@@ -52,10 +55,15 @@ void k_axpy(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<dou
 // w -= sum_{j<kcount} v_j h_j, then h2 = V^H w, in one pass over the basis (kcount <= 16;
 // false, with nothing launched, above that). pin / hout as for k_axpy (pin must not be
 // partial: this pass writes its own partials there); h2 null leaves the new partials unreduced.
+// with_norm: |w|^2 partials of the updated w in slot kcount too.
 template <typename T>
 bool k_axpy_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<double> *h,
                  cx<T> *w, double *partial, cx<double> *h2, hipStream_t s,
-                 const double *pin = nullptr, cx<double> *hout = nullptr);
+                 const double *pin = nullptr, cx<double> *hout = nullptr, int with_norm = 0);
+// From k_axpy_dots' partials (with_norm): mask[b] = 1 where an iterating instance's second
+// projection exceeds tau |w| (it takes the second update), else 0 with h2 stored as zero.
+void k_reorth_select(const KLayout &K, int kcount, const double *partial, int *mask,
+                     cx<double> *h2, double tau, hipStream_t s);
 template <typename T, typename TX>
 void k_update(const KLayout &K, const cx<T> *const *vptr, const int *kk, const cx<double> *y,
               cx<TX> *x, const int *mask, hipStream_t s);

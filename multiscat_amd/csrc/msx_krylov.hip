@@ -321,7 +321,8 @@ __global__ __launch_bounds__(KR_THREADS) void axpy_reg_kernel(KLayout K, const c
 //   w <- w - sum_{j<kcount} v_j h_j,  then partial sums of conj(v_j) w (as dots_kernel writes
 // them), with the kcount <= KM basis values of each element held in registers between the two.
 // Same per-thread element order and block reductions as axpy_kernel then dots_kernel, so the
-// results are bitwise those of the two separate passes.
+// results are bitwise those of the two separate passes. with_norm: |w|^2 partials of the
+// updated w in slot kcount as well.
 template <typename T, int KM>
 __global__ __launch_bounds__(KR_THREADS) void axpy_dots_kernel(KLayout K,
                                                                const cx<T> *const *vptr,
@@ -330,7 +331,8 @@ __global__ __launch_bounds__(KR_THREADS) void axpy_dots_kernel(KLayout K,
                                                                cx<T> *__restrict__ w,
                                                                double *__restrict__ partial,
                                                                const double *__restrict__ pin,
-                                                               cx<double> *__restrict__ hout) {
+                                                               cx<double> *__restrict__ hout,
+                                                               int with_norm) {
     __shared__ cx<T> hs[KM];
     __shared__ double red[(KR_THREADS / 64) * (KM + 1) * 2];
     const int b = blockIdx.y;
@@ -343,7 +345,7 @@ __global__ __launch_bounds__(KR_THREADS) void axpy_dots_kernel(KLayout K,
     for (int j = 0; j < KM; ++j) vb[j] = j < kcount ? vptr[j] + base : nullptr;
     const long long cbeg = (long long)blockIdx.x * K.chunk;
     const long long cend = cbeg + K.chunk < n ? cbeg + K.chunk : n;
-    double sr[KM], si[KM];
+    double sr[KM], si[KM], nr = 0.0;
 #pragma unroll
     for (int j = 0; j < KM; ++j) sr[j] = si[j] = 0.0;
     for (long long i = cbeg + threadIdx.x; i < cend; i += KR_THREADS) {
@@ -366,8 +368,49 @@ __global__ __launch_bounds__(KR_THREADS) void axpy_dots_kernel(KLayout K,
                 sr[j] += (double)v[j].re * acc.re + (double)v[j].im * acc.im;
                 si[j] += (double)v[j].re * acc.im - (double)v[j].im * acc.re;
             }
+        if (with_norm) nr += (double)acc.re * acc.re + (double)acc.im * acc.im;
     }
-    write_pair_sums<KM>(sr, si, kcount, false, 0.0, red, partial, slot, K.nchunk, cbeg < n);
+    write_pair_sums<KM>(sr, si, kcount, with_norm != 0, nr, red, partial, slot, K.nchunk, cbeg < n);
+}
+
+// Selective second Gram-Schmidt pass: after the first pass, partial holds the chunk partials
+// of the second projection h2 = V^H w (slots 0 .. kcount-1) and of |w|^2 (slot kcount). An
+// iterating instance takes the second update (mask[b] = 1) only if |h2| > tau |w|, i.e. the
+// first pass left the new vector's projection on the basis above tau; otherwise mask[b] = 0
+// and its h2 is stored as zero (the update and the Hessenberg column then use h1 alone).
+__global__ __launch_bounds__(64 * RP_WAVES) void reorth_select_kernel(
+    KLayout K, int kcount, const double *__restrict__ partial, int *__restrict__ mask,
+    cx<double> *__restrict__ h2, double tau2) {
+    __shared__ double acc[RP_WAVES];
+    __shared__ double nrm;
+    const int b = blockIdx.x;
+    if (!K.active[b]) {
+        if (threadIdx.x == 0) mask[b] = 0;
+        return;
+    }
+    const long long slot = (long long)b * (K.kmax + 1);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double s2 = 0.0;
+    for (int j = wave; j <= kcount; j += RP_WAVES) {
+        const double2 *p = reinterpret_cast<const double2 *>(partial + (slot + j) * K.nchunk * 2);
+        double sr = 0, si = 0;
+        for (int c = lane; c < K.nchunk; c += 64) {
+            const double2 v = p[c];
+            sr += v.x;
+            si += v.y;
+        }
+        sr = wave_sum(sr);
+        si = wave_sum(si);
+        if (j < kcount) s2 += sr * sr + si * si;
+        else if (lane == 0) nrm = sr;
+    }
+    if (lane == 0) acc[wave] = s2;
+    __syncthreads();
+    const double tot = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    const int take = tot > tau2 * nrm ? 1 : 0;
+    if (threadIdx.x == 0) mask[b] = take;
+    if (!take)
+        for (int j = threadIdx.x; j < kcount; j += blockDim.x) h2[slot + j] = mk(0.0, 0.0);
 }
 
 // x += sum_{j<kk_b} v_j y_j  (kk_b per instance)
@@ -533,7 +576,11 @@ __global__ void givens_kernel(KLayout K, int k, const cx<double> *__restrict__ h
         const long long slot = (long long)b * (K.kmax + 1);
         cx<double> *H = S.H + (long long)b * (K.kmax + 1) * (K.kmax + 1);  // column-major
 #define HH(i, j) H[(long long)(j) * (K.kmax + 1) + (i)]
-        for (int i = 0; i <= k; ++i) HH(i, k) = hcol2 ? hcol[slot + i] + hcol2[slot + i] : hcol[slot + i];
+        for (int i = 0; i <= k; ++i) {
+            const cx<double> h = hcol2 ? hcol[slot + i] + hcol2[slot + i] : hcol[slot + i];
+            HH(i, k) = mk(S.hscale * h.re, S.hscale * h.im);
+        }
+        HH(k, k).re += S.hshift;
         // |w|^2: from the chunk partials of slot k + 1 (summed here, in chunk order, as the
         // reduction kernel would), or already reduced into hcol
         double n2;
@@ -554,7 +601,7 @@ __global__ void givens_kernel(KLayout K, int k, const cx<double> *__restrict__ h
             n2 = hcol[slot + k + 1].re;
         }
         const double hn = sqrt(n2);
-        HH(k + 1, k) = mk(hn, 0.0);
+        HH(k + 1, k) = mk(S.hscale * hn, 0.0);
         cx<double> *cs = S.cs + slot, *sn = S.sn + slot, *g = S.g + slot;
         for (int i = 0; i < k; ++i) {
             const cx<double> t = cs[i] * HH(i, k) + conj(sn[i]) * HH(i + 1, k);
@@ -719,16 +766,23 @@ void k_axpy(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<dou
     }
 }
 
+void k_reorth_select(const KLayout &K, int kcount, const double *partial, int *mask,
+                     cx<double> *h2, double tau, hipStream_t s) {
+    hipLaunchKernelGGL(reorth_select_kernel, dim3(K.n_inst), dim3(64 * RP_WAVES), 0, s, K, kcount,
+                       partial, mask, h2, tau * tau);
+    MSX_LAUNCH_CHECK();
+}
+
 template <typename T>
 bool k_axpy_dots(const KLayout &K, const cx<T> *const *vptr, int kcount, const cx<double> *h,
                  cx<T> *w, double *partial, cx<double> *h2, hipStream_t s, const double *pin,
-                 cx<double> *hout) {
+                 cx<double> *hout, int with_norm) {
     if (kcount <= 8)
         hipLaunchKernelGGL((axpy_dots_kernel<T, 8>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr,
-                           kcount, h, w, partial, pin, hout);
+                           kcount, h, w, partial, pin, hout, with_norm);
     else if (kcount <= 16)
         hipLaunchKernelGGL((axpy_dots_kernel<T, 16>), kgrid(K), dim3(KR_THREADS), 0, s, K, vptr,
-                           kcount, h, w, partial, pin, hout);
+                           kcount, h, w, partial, pin, hout, with_norm);
     else
         return false;
     MSX_LAUNCH_CHECK();
@@ -803,7 +857,7 @@ void k_cycle_start(const KLayout &K, const KState &S, const cx<double> *nrm2, in
                             cx<double> *);                                                      \
     template bool k_axpy_dots<T>(const KLayout &, const cx<T> *const *, int,                    \
                                  const cx<double> *, cx<T> *, double *, cx<double> *,           \
-                                 hipStream_t, const double *, cx<double> *);                    \
+                                 hipStream_t, const double *, cx<double> *, int);               \
     template void k_scale<T>(const KLayout &, const cx<T> *, cx<T> *, const double *,           \
                              const int *, hipStream_t);                                         \
     template void k_axpby<T>(const KLayout &, const cx<T> *, const cx<T> *, T, cx<T> *,          \

@@ -55,6 +55,7 @@ struct Krylov {
     KState S{};
     DBuf<long long> start_d, len_d;
     DBuf<int> enabled, kconv, iters, cycle_iters, converged;
+    DBuf<int> reorth;  // instances taking the second Gram-Schmidt update (selective form)
     // active flags and the count of iterating instances in one buffer, so the host reads
     // both with a single copy
     DBuf<int> flags;
@@ -132,7 +133,7 @@ struct Krylov {
         len_d.upload(len.data(), n_inst, s);
         K.start = start_d.p;
         K.len = len_d.p;
-        for (auto *b : {&enabled, &kconv, &iters, &cycle_iters, &converged}) b->alloc(n_inst);
+        for (auto *b : {&enabled, &kconv, &iters, &cycle_iters, &converged, &reorth}) b->alloc(n_inst);
         flags.alloc((size_t)n_inst + 1);
         active.p = flags.p;
         count.p = flags.p + n_inst;
@@ -275,8 +276,11 @@ struct Krylov {
 // stop_below: once the host sees at most this many instances iterating, the solve stops after
 // the iteration already queued and every instance keeps the solution of its current subspace.
 // Profiling needs the eager launches (its events bracket single kernels), so it turns this off.
+// orth_passes: Gram-Schmidt passes (1, 2), or 3: one pass and the second only for the
+// instances whose first left the new vector's projection on the basis above 1e-4 (up to
+// k + 1 = 16; two passes above). Returns the number of iterations queued.
 template <typename T, typename TX>
-void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_zero,
+int gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_zero,
            double eps, int window, int maxiter, bool allow_restart, int orth_passes = 2,
            const Layout *basis_order = nullptr, const double *eps_inst = nullptr,
            const OpScaled<T> *op_scaled = nullptr, const std::string *graph_sig = nullptr,
@@ -288,8 +292,8 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
         // the recurrence state goes to the Givens kernel by value: its tolerance, window and
         // per-instance tolerances are part of what a captured iteration replays
         char extra[160];
-        std::snprintf(extra, sizeof extra, "|eps%.17g|win%d|einst%p|orth%d", eps, window,
-                      (const void *)eps_inst, orth_passes);
+        std::snprintf(extra, sizeof extra, "|eps%.17g|win%d|einst%p|orth%d|h%g,%g", eps, window,
+                      (const void *)eps_inst, orth_passes, kr.S.hscale, kr.S.hshift);
         gset = &kr.graphs_for(*graph_sig + extra);
         // every flag slot posted by an earlier solve has been waited for: start at slot 0, so
         // iteration k of a cycle always writes the same slot
@@ -352,17 +356,34 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
                 // CGS2: two passes of (k+1 dot products, then w -= V h); HBM traffic of the
                 // vector kernels is 4 (k+1) + 6 vectors of the iterating instances, 3 (k+1) + 6
                 // when the first update and the second projection share a pass (k+1 <= 16);
-                // one pass: 2 (k+1) + 3
-                const double P = orth_passes >= 2 ? 2.0 : 1.0;
-                const double vecs = orth_passes >= 2 ? (k + 1 <= 16 ? 3.0 * (k + 1) + 6.0 : 4.0 * (k + 1) + 6.0)
-                                                     : 2.0 * (k + 1) + 3.0;
+                // one pass: 2 (k+1) + 3; the selective form (3): 2 (k+1) + 3 plus the second
+                // update's k + 3 where an instance takes it (counted here as if none did)
+                const bool selective = orth_passes == 3 && k + 1 <= 16;
+                const double P = orth_passes >= 2 && !selective ? 2.0 : 1.0;
+                const double vecs = selective ? 2.0 * (k + 1) + 3.0
+                                    : orth_passes >= 2 ? (k + 1 <= 16 ? 3.0 * (k + 1) + 6.0 : 4.0 * (k + 1) + 6.0)
+                                                       : 2.0 * (k + 1) + 3.0;
                 const Work fl = kr.per_instance(P * 16.0 * (k + 1));
                 const Work by = kr.per_instance(vecs * sizeof(cx<T>));
-                const char *nm = orth_passes >= 2
+                const char *nm = selective ? (sizeof(T) == 4 ? "arnoldi_cgs1s_f32" : "arnoldi_cgs1s_f64")
+                                 : orth_passes >= 2
                                      ? (sizeof(T) == 4 ? "arnoldi_cgs2_f32" : "arnoldi_cgs2_f64")
                                      : (sizeof(T) == 4 ? "arnoldi_cgs1_f32" : "arnoldi_cgs1_f64");
                 ProfScope ps(nm, s, fl, by);
-                if (orth_passes >= 2 && k + 1 <= 16) {
+                if (selective) {
+                    // one pass, and the second only for the instances whose first left the new
+                    // vector's projection on the basis above 1e-4 of its norm (the second
+                    // projection and |w|^2 come with the first update's pass over V)
+                    k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, nullptr, 0, s);
+                    k_axpy_dots<T>(kr.K, kr.vp_d.p, k + 1, nullptr, kr.w.p, kr.partial2.p, nullptr,
+                                   s, kr.partial.p, kr.hcol.p, 1);
+                    k_reorth_select(kr.K, k + 1, kr.partial2.p, kr.reorth.p, kr.hcol2.p, 1e-4, s);
+                    KLayout K2 = kr.K;
+                    K2.active = kr.reorth.p;
+                    k_axpy<T>(K2, kr.vp_d.p, k + 1, nullptr, kr.w.p, 1, kr.partial2.p, nullptr, s,
+                              kr.partial2.p, kr.hcol2.p);
+                    norm_src = kr.partial2.p;
+                } else if (orth_passes >= 2 && k + 1 <= 16) {
                     // each pass's chunk partials summed by the kernel that consumes them (no
                     // reduction launches): dots -> partial; the first update sums them (and
                     // stores hcol) and writes the second projection's partials to partial2;
@@ -460,6 +481,7 @@ void gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_z
         if (alive == 0 || done >= maxiter || !allow_restart) break;
         k_reactivate(kr.K, kr.S, kr.enabled.p, s);
     }
+    return done;
 }
 
 }  // namespace msx
@@ -878,8 +900,11 @@ struct Ops {
     // raw / raw_norm (DFT form only): in = raw / raw_norm[b] is formed (and stored to in) by
     // the first coupling; raw may alias out
     template <typename T>
+    // shifted: out = F in with A = I + shifted_scale() F (F = K, -/+K^(order+1)): no identity
+    // term, so the Arnoldi on F keeps the bits that A = I + small would cancel, and no D^-1
+    // launch reads a base vector
     void jac(const cx<T> *in, cx<T> *out, const int *act, cx<T> *u, cx<T> *w1buf,
-             const cx<T> *raw = nullptr, const double *raw_norm = nullptr) {
+             const cx<T> *raw = nullptr, const double *raw_norm = nullptr, bool shifted = false) {
         auto first_coupling = [&](const cx<T> *src, cx<T> *dst) {
             if (raw) couple_full(raw, dst, act, raw_norm, const_cast<cx<T> *>(in));
             else couple_full(src, dst, act);
@@ -887,10 +912,11 @@ struct Ops {
         if (!neumann || order <= 1) {
             cx<T> *w1 = neumann ? w1buf : out;
             first_coupling(in, u);
-            dinv<T>(u, in, w1, T(1), act);
+            dinv<T>(u, shifted ? nullptr : in, w1, T(1), act);
             if (neumann) {
                 couple_full(w1, u, act);
-                dinv<T>(u, w1, out, T(-1), act);
+                if (shifted) dinv<T>(u, nullptr, out, T(1), act);
+                else dinv<T>(u, w1, out, T(-1), act);
             }
             return;
         }
@@ -903,8 +929,10 @@ struct Ops {
             t = w1buf;
         }
         couple_full(t, u, act);
-        dinv<T>(u, in, out, (order & 1) ? T(-1) : T(1), act);
+        if (shifted) dinv<T>(u, nullptr, out, T(1), act);
+        else dinv<T>(u, in, out, (order & 1) ? T(-1) : T(1), act);
     }
+    double shifted_scale() const { return neumann && (order & 1) ? -1.0 : 1.0; }
     // s = M r = sum_{i<=order} (-K)^i r by Horner (s <- r - K s); `first` is the first
     // coupling product V r (null: computed here). s must not alias r.
     template <typename T>
@@ -1406,12 +1434,15 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     if (!B.kr32 || B.kr32->kmax != inner_dim || B.kr32->K.chunk != B.tuning.chunk32)
         B.kr32.reset(new Krylov<float>(B.kstart(), B.klen(), inner_dim, s, B.tuning.chunk32));
     Krylov<float> &inner = *B.kr32;
+    // the inner Arnoldi runs on F = (A - I) / scale (the operator without its identity term)
+    inner.S.hscale = ops.shifted_scale();
+    inner.S.hshift = 1.0;
     Op<float> op32 = [&](const cx<float> *in, cx<float> *out, const int *act) {
-        ops.jac<float>(in, out, act, B.f_tmp1.p, B.f_tmp2.p);
+        ops.jac<float>(in, out, act, B.f_tmp1.p, B.f_tmp2.p, nullptr, nullptr, true);
     };
     const OpScaled<float> op32s = [&](const cx<float> *raw, const double *nrm, cx<float> *in,
                                       cx<float> *out, const int *act) {
-        ops.jac<float>(in, out, act, B.f_tmp1.p, B.f_tmp2.p, raw, nrm);
+        ops.jac<float>(in, out, act, B.f_tmp1.p, B.f_tmp2.p, raw, nrm, true);
     };
     // |D^-1 s|^2 per instance: on a cold start it is also the first residual's
     const std::vector<double> rhs_n2 = outer.norms2(d0);
@@ -1430,6 +1461,15 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
     std::vector<double> eps_in(B.n_inst, inner_eps);
     msx_batch::need(B.eps_in_d, (size_t)B.n_inst);
     constexpr double kSweepMargin = 0.03;
+    // Gram-Schmidt passes of the inner Arnoldi: by default one classical pass, and the second
+    // only for the instances whose first pass left the new vector's projection on the basis
+    // above 1e-4 of its norm (measured from the second projection, which the first update's
+    // pass over the basis forms anyway), for Neumann degrees up to 2; two passes above (the
+    // high powers of D^-1 V spread the operator's spectrum: at degree 7 the selective form left
+    // a condition unconverged); 1: one pass; 2: two
+    const int orth = o.inner_orth == 1   ? 1
+                     : o.inner_orth == 2 ? 2
+                     : (!ops.neumann || ops.order <= 2) ? 3 : 2;
     for (int sweep = 0;; ++sweep) {
         outer.set_all_active(act.data());
         // a cold start's first residual is d0 itself
@@ -1509,8 +1549,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
                                ops.neumann ? B.f_tmp1.p : nullptr);
         inner.set_all_active(act.data());
         ops.snap_src = &inner.snap; ops.count_src = &inner.hint_count;
-        const std::string sig32 = ops.graph_signature(
-            o.inner_orth == 1 ? "mixed32-cgs1" : "mixed32", B.f_tmp1.p, B.f_tmp2.p, B.eps_in_d.p);
+        const std::string sig32 = ops.graph_signature("mixed32", B.f_tmp1.p, B.f_tmp2.p, B.eps_in_d.p);
         const std::function<int()> shape32 = [&] { return ops.launch_shape_key(); };
         // The first sweep's inner solve ends for everyone once at most a tenth of a large batch
         // still iterates: its last iterations carry a few conditions at latency-bound cost, and
@@ -1519,7 +1558,7 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         // 2.6e-8 -> 8e-8 against the tight solve)
         const int stop_below = sweep == 0 && n_act >= 32 ? n_act / 10 : 0;
         gmres<float, double>(inner, op32, B.f_rhs.p, x, true, inner_eps, 1,
-                             std::min(o.maxiter, inner_dim), false, o.inner_orth == 1 ? 1 : 2,
+                             std::min(o.maxiter, inner_dim), false, orth,
                              nm64 ? nullptr : &ops.L32, B.eps_in_d.p, ops.direct ? nullptr : &op32s,
                              o.graphs == 1 ? &sig32 : nullptr, &shape32, stop_below);
         launch_add_ints(B.n_inst, inner.iters.p, B.iters_acc.p, s);

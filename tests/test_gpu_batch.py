@@ -72,6 +72,25 @@ def test_mixed_batch_within_bar(small_batch, coupling, diag):
         assert abs(P[open_].sum() - 1.0) < 1e-6
 
 
+@pytest.mark.parametrize("orth,neumann", [(0, 0), (0, 1), (0, 2), (0, 7), (1, 1), (2, 1),
+                                          (2, 2)])
+def test_mixed_inner_gram_schmidt_passes(small_batch, orth, neumann):
+    """The FP32 inner Arnoldi runs on the operator without its identity term (F with
+    A = I + s F, the Hessenberg columns mapped back to A's): the default (one classical pass
+    and a second where the first left a projection above 1e-4), one pass (inner_orth = 1) and
+    two passes (2) reach the oracle to the outer tolerance, with and without the Neumann factor
+    (s = -1 for odd degrees, +1 for none and even degrees), in about as many inner iterations
+    as two passes."""
+    w, _, bt, ref = small_batch
+    its = {}
+    for o in sorted({orth, 2}):
+        bt.solve(BT.SolveOptions(method="jacobi", precision="mixed", eps=5e-11, neumann=neumann,
+                                 inner_orth=o))
+        assert _maxdiff(bt.intensities(), ref) < 1e-6
+        its[o] = int(bt.stats()["iterations"].sum())
+    assert its[orth] <= 1.25 * its[2] + 2, its
+
+
 @pytest.mark.parametrize("products", ["auto", "fp32", "split_fp16"])
 def test_mixed_inner_products(small_batch, products):
     """The inner operator's D^-1 products with FP32 operands or split-FP16 pairs: both reach
