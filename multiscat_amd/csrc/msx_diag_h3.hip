@@ -449,24 +449,27 @@ constexpr int H8_ZG = 1;
 // (189 and 213 used; nq 256 would spill)
 constexpr int H8L_WAVES = 8;
 
-// L2P: every plane read from L2 (P1LO false; LDS holds lam, the Q row and the tile scan only).
-template <int KT, bool P1LO, bool BASE, bool L2P = false, int WAVES = H8_WAVES>
+// L2P: every plane read from L2 (P1LO false; LDS holds lam, the Q row and the tile scan only),
+// or with P1HL all but P1 hi, which LDS holds (nz rows: 93 KB at nz = 200), so a tile reads
+// three planes from L2 instead of four.
+template <int KT, bool P1LO, bool BASE, bool L2P = false, int WAVES = H8_WAVES, bool P1HL = false>
 __global__ __launch_bounds__(64 * WAVES) void diag_h8_kernel(
     Layout L, H3Planes H, const double *__restrict__ lam_g, const double *__restrict__ e,
     const cx<float> *__restrict__ c, const float *__restrict__ fac, const cx<float> *x,
     const cx<float> *base, cx<float> *y, float sign, const int *__restrict__ active) {
     static_assert(!(L2P && P1LO), "with the planes in L2, P1 lo is there too");
+    static_assert(L2P || !P1HL, "P1HL is a form of the planes-in-L2 kernel");
+    constexpr int LPLANES = L2P ? (P1HL ? 1 : 0) : 2;  // hi planes held in LDS
     constexpr int NQ = 32 * KT, MT = 2 * KT, S = NQ + 8;
     extern __shared__ double h3_smem[];
     __shared__ int wave_sum[WAVES];
     const int nz = L.nz, n_inst = L.n_inst;
     double *lam = h3_smem;                                    // [NQ], 0 past nz
     float *qn = reinterpret_cast<float *>(lam + NQ);           // [NQ]: Q[nz - 1][m]
-    _Float16 *lp = reinterpret_cast<_Float16 *>(qn + NQ);     // LDS planes (not with L2P)
-    const int lrows = L2P ? 0 : nz;
-    _Float16 *p1l = lp + 2 * lrows * S;                        // [nz][S] (P1LO)
-    const _Float16 *p1h = L2P ? H.planes : lp;                             // [nz][S]
-    const _Float16 *p2h = L2P ? H.planes + (size_t)2 * NQ * S : lp + lrows * S;  // [nz][S]
+    _Float16 *lp = reinterpret_cast<_Float16 *>(qn + NQ);     // LDS planes
+    _Float16 *p1l = lp + LPLANES * nz * S;                     // [nz][S] (P1LO)
+    const _Float16 *p1h = LPLANES >= 1 ? lp : H.planes;                                  // [nz][S]
+    const _Float16 *p2h = LPLANES == 2 ? lp + nz * S : H.planes + (size_t)2 * NQ * S;  // [nz][S]
     int *tile_start = reinterpret_cast<int *>(p1l + (P1LO ? nz * S : 0));  // [n_inst + 1]
     int *inst_nc = tile_start + n_inst + 1;                    // [n_inst]
     int *inst_off = inst_nc + n_inst;                          // [n_inst]
@@ -502,14 +505,14 @@ __global__ __launch_bounds__(64 * WAVES) void diag_h8_kernel(
         lam[m] = m < nz ? lam_g[m] : 0.0;
         qn[m] = m < nz ? H.qlast[m] : 0.0f;
     }
-    if constexpr (!L2P) {
+    if constexpr (LPLANES > 0) {
         const int n16 = nz * S / 8;
         const uint4 *s1h = reinterpret_cast<const uint4 *>(H.planes);
         const uint4 *s1l = reinterpret_cast<const uint4 *>(g1l);
         const uint4 *s2h = reinterpret_cast<const uint4 *>(H.planes + (size_t)2 * NQ * S);
         for (int i = threadIdx.x; i < n16; i += blockDim.x) {
             reinterpret_cast<uint4 *>(lp)[i] = s1h[i];
-            reinterpret_cast<uint4 *>(lp + lrows * S)[i] = s2h[i];
+            if (LPLANES == 2) reinterpret_cast<uint4 *>(lp + nz * S)[i] = s2h[i];
             if (P1LO) reinterpret_cast<uint4 *>(p1l)[i] = s1l[i];
         }
     }
@@ -730,9 +733,11 @@ size_t h3_lds_bytes(int nq, int nz, int n_inst, bool p1lo) {
     return 12 * (size_t)nq + (p1lo ? 3 : 2) * (size_t)nz * S * 2 +
            (3 * (size_t)n_inst + 1) * sizeof(int);
 }
-// LDS of the planes-in-L2 form (nq > 160): lam, the Q row and the tile scan
-size_t h8l_lds_bytes(int nq, int n_inst) {
-    return 12 * (size_t)nq + (3 * (size_t)n_inst + 1) * sizeof(int);
+// LDS of the planes-in-L2 form (nq > 160): lam, the Q row and the tile scan, and with p1h
+// P1 hi (nz rows)
+size_t h8l_lds_bytes(int nq, int n_inst, int nz = 0, bool p1h = false) {
+    return 12 * (size_t)nq + (p1h ? (size_t)nz * (nq + 8) * 2 : 0) +
+           (3 * (size_t)n_inst + 1) * sizeof(int);
 }
 size_t lds_limit() {
     static size_t lim = 0;
@@ -776,16 +781,17 @@ void launch_h3(const Layout &L, const H3Planes &H, const double *lam, const doub
     MSX_LAUNCH_CHECK();
 }
 
-template <int KT, bool BASE>
+template <int KT, bool BASE, bool P1HL>
 void launch_h8l(const Layout &L, const H3Planes &H, const double *lam, const double *e,
                 const cx<float> *c, const float *fac, const cx<float> *x, const cx<float> *base,
                 cx<float> *y, float sign, const int *active, size_t shm, hipStream_t s) {
-    const void *kern = (const void *)diag_h8_kernel<KT, false, BASE, true, H8L_WAVES>;
+    const void *kern = (const void *)diag_h8_kernel<KT, false, BASE, true, H8L_WAVES, P1HL>;
     set_dynamic_lds(kern, shm);
     const long long tiles_max = (long long)L.n_inst * ((L.max_nc + 7) / 8);
     const int grid = (int)std::min<long long>(cu_count(), (tiles_max + H8L_WAVES - 1) / H8L_WAVES);
-    hipLaunchKernelGGL((diag_h8_kernel<KT, false, BASE, true, H8L_WAVES>), dim3(std::max(grid, 1)),
-                       dim3(64 * H8L_WAVES), shm, s, L, H, lam, e, c, fac, x, base, y, sign, active);
+    hipLaunchKernelGGL((diag_h8_kernel<KT, false, BASE, true, H8L_WAVES, P1HL>),
+                       dim3(std::max(grid, 1)), dim3(64 * H8L_WAVES), shm, s, L, H, lam, e, c, fac,
+                       x, base, y, sign, active);
     MSX_LAUNCH_CHECK();
 }
 }  // namespace
@@ -808,12 +814,17 @@ void launch_diag_h3(const Layout &L, const H3Planes &H, const double *lam, const
     if (cpw != 0 && cpw != 8 && cpw != 16)
         throw Error(3, "split-FP16 diagonal blocks: 8 or 16 channels per wavefront");
     if (H.nq > 160) {
-        // planes in L2, 8 channels per wavefront whatever cpw asks
-        const size_t shm = h8l_lds_bytes(H.nq, L.n_inst);
+        // planes in L2 (P1 hi in LDS where it fits), 8 channels per wavefront whatever cpw asks
+        const bool p1h = h8l_lds_bytes(H.nq, L.n_inst, L.nz, true) <= lds_limit();
+        const size_t shm = h8l_lds_bytes(H.nq, L.n_inst, L.nz, p1h);
 #define MSX_H8L(KT_)                                                                           \
     if (H.nq == 32 * KT_) {                                                                    \
-        if (base) return launch_h8l<KT_, true>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s); \
-        return launch_h8l<KT_, false>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s);        \
+        if (p1h) {                                                                             \
+            if (base) return launch_h8l<KT_, true, true>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s); \
+            return launch_h8l<KT_, false, true>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s);        \
+        }                                                                                      \
+        if (base) return launch_h8l<KT_, true, false>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s); \
+        return launch_h8l<KT_, false, false>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s);        \
     }
         MSX_H8L(6) MSX_H8L(7)
 #undef MSX_H8L
