@@ -450,8 +450,9 @@ constexpr int H8_ZG = 1;
 constexpr int H8L_WAVES = 8;
 
 // L2P: every plane read from L2 (P1LO false; LDS holds lam, the Q row and the tile scan only),
-// or with P1HL all but P1 hi, which LDS holds (nz rows: 93 KB at nz = 200), so a tile reads
-// three planes from L2 instead of four.
+// or with P1HL all but P1 hi, which LDS holds (nz rows: 93 KB at nz = 200), and the first
+// H.p2_lds_rows rows of P2 hi (a multiple of 16: whole row tiles of the second product), so a
+// tile reads less than three planes from L2 instead of four.
 template <int KT, bool P1LO, bool BASE, bool L2P = false, int WAVES = H8_WAVES, bool P1HL = false>
 __global__ __launch_bounds__(64 * WAVES) void diag_h8_kernel(
     Layout L, H3Planes H, const double *__restrict__ lam_g, const double *__restrict__ e,
@@ -467,7 +468,9 @@ __global__ __launch_bounds__(64 * WAVES) void diag_h8_kernel(
     double *lam = h3_smem;                                    // [NQ], 0 past nz
     float *qn = reinterpret_cast<float *>(lam + NQ);           // [NQ]: Q[nz - 1][m]
     _Float16 *lp = reinterpret_cast<_Float16 *>(qn + NQ);     // LDS planes
-    _Float16 *p1l = lp + LPLANES * nz * S;                     // [nz][S] (P1LO)
+    const int p2r = P1HL ? H.p2_lds_rows : 0;
+    _Float16 *p2part = lp + LPLANES * nz * S;                  // [p2r][S]: P2 hi rows (P1HL)
+    _Float16 *p1l = p2part + p2r * S;                          // [nz][S] (P1LO)
     const _Float16 *p1h = LPLANES >= 1 ? lp : H.planes;                                  // [nz][S]
     const _Float16 *p2h = LPLANES == 2 ? lp + nz * S : H.planes + (size_t)2 * NQ * S;  // [nz][S]
     int *tile_start = reinterpret_cast<int *>(p1l + (P1LO ? nz * S : 0));  // [n_inst + 1]
@@ -515,6 +518,9 @@ __global__ __launch_bounds__(64 * WAVES) void diag_h8_kernel(
             if (LPLANES == 2) reinterpret_cast<uint4 *>(lp + nz * S)[i] = s2h[i];
             if (P1LO) reinterpret_cast<uint4 *>(p1l)[i] = s1l[i];
         }
+        if (P1HL)
+            for (int i = threadIdx.x; i < p2r * S / 8; i += blockDim.x)
+                reinterpret_cast<uint4 *>(p2part)[i] = s2h[i];
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -648,15 +654,21 @@ __global__ __launch_bounds__(64 * WAVES) void diag_h8_kernel(
             v4f acc2[H8_ZG];
 #pragma unroll
             for (int zg = 0; zg < H8_ZG; ++zg) acc2[zg] = v4f{0.f, 0.f, 0.f, 0.f};
+            auto product2 = [&](const _Float16 *src) {
 #pragma unroll
-            for (int p = 0; p < KT; ++p)
+                for (int p = 0; p < KT; ++p)
 #pragma unroll
-                for (int zg = 0; zg < H8_ZG; ++zg) {
-                    const h8 ah = *reinterpret_cast<const h8 *>(p2h + zrow[zg] * S + 32 * p + 8 * g);
-                    acc2[zg] = mfma_h(ah, wh[p], acc2[zg]);
-                    acc2[zg] = mfma_h(ah, wl[p], acc2[zg]);
-                    acc2[zg] = mfma_h(a2l[p][zg], wh[p], acc2[zg]);
-                }
+                    for (int zg = 0; zg < H8_ZG; ++zg) {
+                        const h8 ah = *reinterpret_cast<const h8 *>(src + zrow[zg] * S + 32 * p + 8 * g);
+                        acc2[zg] = mfma_h(ah, wh[p], acc2[zg]);
+                        acc2[zg] = mfma_h(ah, wl[p], acc2[zg]);
+                        acc2[zg] = mfma_h(a2l[p][zg], wh[p], acc2[zg]);
+                    }
+            };
+            // P2 hi rows of this group from LDS when they are all there (the group index is
+            // the same in every lane)
+            if (P1HL && 16 * (zt0 + H8_ZG) <= p2r) product2(p2part);
+            else product2(p2h);
             // the next group's P2 lo fragments (past the last group: clamped rows, unused)
 #pragma unroll
             for (int zg = 0; zg < H8_ZG; ++zg) {
@@ -734,9 +746,9 @@ size_t h3_lds_bytes(int nq, int nz, int n_inst, bool p1lo) {
            (3 * (size_t)n_inst + 1) * sizeof(int);
 }
 // LDS of the planes-in-L2 form (nq > 160): lam, the Q row and the tile scan, and with p1h
-// P1 hi (nz rows)
-size_t h8l_lds_bytes(int nq, int n_inst, int nz = 0, bool p1h = false) {
-    return 12 * (size_t)nq + (p1h ? (size_t)nz * (nq + 8) * 2 : 0) +
+// P1 hi (nz rows) and p2_rows rows of P2 hi
+size_t h8l_lds_bytes(int nq, int n_inst, int nz = 0, bool p1h = false, int p2_rows = 0) {
+    return 12 * (size_t)nq + (p1h ? (size_t)(nz + p2_rows) * (nq + 8) * 2 : 0) +
            (3 * (size_t)n_inst + 1) * sizeof(int);
 }
 size_t lds_limit() {
@@ -816,15 +828,22 @@ void launch_diag_h3(const Layout &L, const H3Planes &H, const double *lam, const
     if (H.nq > 160) {
         // planes in L2 (P1 hi in LDS where it fits), 8 channels per wavefront whatever cpw asks
         const bool p1h = h8l_lds_bytes(H.nq, L.n_inst, L.nz, true) <= lds_limit();
-        const size_t shm = h8l_lds_bytes(H.nq, L.n_inst, L.nz, p1h);
+        // and as many 16-row tiles of P2 hi as the rest of LDS holds
+        int p2_rows = 0;
+        while (p1h && p2_rows + 16 <= L.nz &&
+               h8l_lds_bytes(H.nq, L.n_inst, L.nz, true, p2_rows + 16) + 256 <= lds_limit())
+            p2_rows += 16;
+        const size_t shm = h8l_lds_bytes(H.nq, L.n_inst, L.nz, p1h, p2_rows);
+        H3Planes Hp = H;
+        Hp.p2_lds_rows = p2_rows;
 #define MSX_H8L(KT_)                                                                           \
     if (H.nq == 32 * KT_) {                                                                    \
         if (p1h) {                                                                             \
-            if (base) return launch_h8l<KT_, true, true>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s); \
-            return launch_h8l<KT_, false, true>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s);        \
+            if (base) return launch_h8l<KT_, true, true>(L, Hp, lam, e, c, fac, x, base, y, sign, active, shm, s); \
+            return launch_h8l<KT_, false, true>(L, Hp, lam, e, c, fac, x, base, y, sign, active, shm, s);        \
         }                                                                                      \
-        if (base) return launch_h8l<KT_, true, false>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s); \
-        return launch_h8l<KT_, false, false>(L, H, lam, e, c, fac, x, base, y, sign, active, shm, s);        \
+        if (base) return launch_h8l<KT_, true, false>(L, Hp, lam, e, c, fac, x, base, y, sign, active, shm, s); \
+        return launch_h8l<KT_, false, false>(L, Hp, lam, e, c, fac, x, base, y, sign, active, shm, s);        \
     }
         MSX_H8L(6) MSX_H8L(7)
 #undef MSX_H8L

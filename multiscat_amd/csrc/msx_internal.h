@@ -200,8 +200,9 @@ struct EigTables {
 // halves apiece, and Q[nz-1][m] in FP32 (the Sherman-Morrison row).
 struct H3Planes {
     const _Float16 *planes = nullptr;  // [4][nq][nq + 8]: P1 hi, P1 lo, P2 hi, P2 lo
-    int nq = 0;                        // 32 ceil(nz / 32), 0 when nz > 160
+    int nq = 0;                        // 32 ceil(nz / 32), 0 when nz > 224
     const float *qlast = nullptr;      // [nz]
+    int p2_lds_rows = 0;  // planes-in-L2 kernel: leading rows of P2 hi held in LDS (its launcher)
 };
 int h3_padded_size(int nz);
 size_t h3_plane_halves(int nq);
