@@ -372,12 +372,14 @@ int gmres(Krylov<T> &kr, const Op<T> &op, const cx<T> *rhs, cx<TX> *x, bool x_ze
                 ProfScope ps(nm, s, fl, by);
                 if (selective) {
                     // one pass, and the second only for the instances whose first left the new
-                    // vector's projection on the basis above 1e-4 of its norm (the second
+                    // vector's projection on the basis above 1e-4 of its norm in FP32, 1e-8
+                    // (about the square root of the unit roundoff) in FP64 (the second
                     // projection and |w|^2 come with the first update's pass over V)
                     k_dots<T>(kr.K, kr.vp_d.p, k + 1, kr.w.p, 0, kr.partial.p, nullptr, 0, s);
                     k_axpy_dots<T>(kr.K, kr.vp_d.p, k + 1, nullptr, kr.w.p, kr.partial2.p, nullptr,
                                    s, kr.partial.p, kr.hcol.p, 1);
-                    k_reorth_select(kr.K, k + 1, kr.partial2.p, kr.reorth.p, kr.hcol2.p, 1e-4, s);
+                    k_reorth_select(kr.K, k + 1, kr.partial2.p, kr.reorth.p, kr.hcol2.p,
+                                    sizeof(T) == 4 ? 1e-4 : 1e-8, s);
                     KLayout K2 = kr.K;
                     K2.active = kr.reorth.p;
                     k_axpy<T>(K2, kr.vp_d.p, k + 1, nullptr, kr.w.p, 1, kr.partial2.p, nullptr, s,
@@ -1398,7 +1400,10 @@ void solve_system(msx_batch &B, const msx_solve_options &o, const cx<double> *sr
         const std::string sig64 = ops.graph_signature(
             o.method == MSX_METHOD_JACOBI ? "jacobi64" : "gs64", B.tmp1.p, B.tmp2.p, B.rhs.p);
         const std::function<int()> shape64 = [&] { return ops.launch_shape_key(); };
-        gmres<double, double>(kr, op, B.rhs.p, x, !warm, o.eps, o.window, o.maxiter, true, 2,
+        // block Jacobi: the selective second Gram-Schmidt pass; Gauss-Seidel keeps two passes
+        // (its iteration counts are the reference's)
+        gmres<double, double>(kr, op, B.rhs.p, x, !warm, o.eps, o.window, o.maxiter, true,
+                              o.method == MSX_METHOD_JACOBI ? 3 : 2,
                               nullptr, nullptr, fused ? &op_s : nullptr,
                               o.graphs == 1 ? &sig64 : nullptr, &shape64);
         ops.snap_src = nullptr; ops.count_src = nullptr;
