@@ -553,6 +553,36 @@ def test_graph_replay_matches_eager_launches(method, precision):
         np.testing.assert_array_equal(bt.stats()["iterations"], it_loose)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_forward_and_adjoint_alternating_replay_their_own_graphs(precision):
+    """A gradient loop alternates a forward solve and its adjoint (the flipped table, another
+    vector order and tuning key) on one batch: each keeps its captured graphs and the batch
+    its Arnoldi block length, and every replayed pair gives the eager pair's intensities,
+    gradient and iteration counts bit for bit."""
+    w = W.hex_batch(n_conditions=4, nz=40, nk=15, cond_seed=12, channel_counts=[40, 85, 60, 110])
+    tab = BT.Table(w.potential, w.kinetic)
+    bt = BT.Batch(tab, w.conditions)
+    opts = dict(method="jacobi", precision=precision, eps=5e-11 if precision == "mixed" else 5e-12)
+    g = np.random.default_rng(3).standard_normal(bt.total_channels)
+
+    def pair(graphs):
+        o = BT.SolveOptions(graphs=graphs, **opts)
+        bt.solve(o)
+        P, it = bt.intensities_flat(), bt.stats()["iterations"].copy()
+        grad = bt.adjoint(g, o)
+        return P, it, grad, bt.stats()["iterations"].copy(), bt.tuning()
+
+    P_e, it_e, grad_e, ait_e, tune_e = pair(False)
+    for _ in range(3):
+        P, it, grad, ait, tune = pair(True)
+        np.testing.assert_array_equal(P, P_e)
+        np.testing.assert_array_equal(it, it_e)
+        np.testing.assert_array_equal(grad, grad_e)
+        np.testing.assert_array_equal(ait, ait_e)
+        key = "arnoldi_block_f32" if precision == "mixed" else "arnoldi_block_f64"
+        assert tune[key] == tune_e[key]
+
+
 @pytest.mark.parametrize("precision,eps,tol", [("fp64", 5e-13, 1e-10), ("mixed", 5e-10, 1e-7)])
 def test_large_table_dft_solve_matches_oracle(precision, eps, tol):
     """A 45 x 45 table (the DFT coupling on a 48-point padded grid, workgroup kernel): block-
